@@ -1,0 +1,131 @@
+// Vector kernels (BLAS-1) and deterministic dot products.
+//
+// These are the device forms of the reference's Vector/ParVector helpers
+// (linalg/src/vector.rs:39-70, parallel/src/par_vector.rs:182-218,
+// parallel/src/par_solver.rs:15-78).  Dots accumulate in double for both f64
+// and f32 vectors.
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace femgpu {
+
+static inline unsigned grid_for(size_t n, int block = 256) {
+    size_t g = (n + block - 1) / block;
+    if (g > 65535u * 16u) g = 65535u * 16u;
+    return (unsigned)(g ? g : 1);
+}
+
+template <class T>
+__global__ void k_fill(T *__restrict__ y, T v, size_t n, const int *skip) {
+    if (skip && *skip) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = v;
+}
+
+template <class T>
+__global__ void k_copy(T *__restrict__ y, const T *__restrict__ x, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+
+template <class T>
+__global__ void k_axpby(T *__restrict__ y, double a, const T *__restrict__ x, double b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+template <class T>
+__global__ void k_inv_diag(T *__restrict__ inv, const T *__restrict__ d, size_t n, double tiny) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const T v = d[i];
+        inv[i] = (fabs((double)v) > tiny) ? (T)(1.0 / (double)v) : (T)1;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_dot_partial(const T *__restrict__ a,
+                                                     const T *__restrict__ b, size_t n,
+                                                     double *__restrict__ partials) {
+    double v[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        v[0] += (double)a[i] * (double)b[i];
+    block_sum<1, 256>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(1024) void k_finalize1(const double *__restrict__ partials, int nb,
+                                                    double *__restrict__ out) {
+    double v[1] = {threadIdx.x < (unsigned)nb ? partials[threadIdx.x] : 0.0};
+    block_sum<1, 1024>(v);
+    if (threadIdx.x == 0) out[0] = v[0];
+}
+
+__global__ void k_f64_to_f32(float *__restrict__ y, const double *__restrict__ x, size_t n,
+                             double scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (float)(x[i] * scale);
+}
+
+__global__ void k_f32_axpy_f64(double *__restrict__ y, const float *__restrict__ x, size_t n,
+                               double scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] += scale * (double)x[i];
+}
+
+template <class T> void fill(hipStream_t s, T *y, T v, size_t n, const int *skip) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(256), 0, s, y, v, n, skip);
+}
+template <class T> void copy(hipStream_t s, T *y, const T *x, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(256), 0, s, y, x, n);
+}
+template <class T> void axpby(hipStream_t s, T *y, double a, const T *x, double b, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_axpby<T>, dim3(grid_for(n)), dim3(256), 0, s, y, a, x, b, n);
+}
+template <class T> void inv_diag(hipStream_t s, T *inv, const T *d, size_t n, double tiny) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_inv_diag<T>, dim3(grid_for(n)), dim3(256), 0, s, inv, d, n, tiny);
+}
+template <class T> void dot(femgpu_ctx *c, const T *a, const T *b, size_t n, double *out) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot_partial<T>, dim3(nb), dim3(256), 0, c->stream, a, b, n,
+                       c->partials.as<double>());
+    hipLaunchKernelGGL(k_finalize1, dim3(1), dim3(1024), 0, c->stream,
+                       (const double *)c->partials.as<double>(), nb, out);
+}
+
+double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dtype dt) {
+    double *out = c->scalars.as<double>() + 63;
+    if (dt == FEMGPU_F64) dot<double>(c, (const double *)a, (const double *)b, n, out);
+    else dot<float>(c, (const float *)a, (const float *)b, n, out);
+    FG_HIP(hipMemcpyAsync(c->pinned + 63, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    return c->pinned[63];
+}
+
+void convert_f64_to_f32(hipStream_t s, float *y, const double *x, size_t n, double scale) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(n)), dim3(256), 0, s, y, x, n, scale);
+}
+void convert_f32_to_f64_axpy(hipStream_t s, double *y, const float *x, size_t n, double scale) {
+    hipLaunchKernelGGL(k_f32_axpy_f64, dim3(grid_for(n)), dim3(256), 0, s, y, x, n, scale);
+}
+
+#define INST(T)                                                                          \
+    template void fill<T>(hipStream_t, T *, T, size_t, const int *);                     \
+    template void copy<T>(hipStream_t, T *, const T *, size_t);                          \
+    template void axpby<T>(hipStream_t, T *, double, const T *, double, size_t);         \
+    template void inv_diag<T>(hipStream_t, T *, const T *, size_t, double);              \
+    template void dot<T>(femgpu_ctx *, const T *, const T *, size_t, double *);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace femgpu

@@ -1,0 +1,235 @@
+// CsrMatrix<T> on the device and its SpMV (linalg/src/csr.rs).
+//
+// Layout is the reference's own: row_ptr u64[n+1], col_idx u32[nnz],
+// values T[nnz].  Two kernels:
+//  - row-per-thread, used for short rows (the assembled P1/P2 matrices): the
+//    f64 row sum runs in blocks of four exactly as csr_row_dot_f64
+//    (csr.rs:38-64) and f32 sums sequentially as spmv_serial (csr.rs:195-205),
+//    with FMA contraction off, so results are bitwise those of the reference;
+//  - row-per-subgroup (8..64 lanes, shuffle-reduced), used when rows are long
+//    (assembled high-order hex matrices); equal within rounding.
+#include "internal.hpp"
+#include "reduce.hpp"
+
+#include <cstdlib>
+
+namespace femgpu {
+
+template <class T>
+__device__ __forceinline__ T row_dot_ref(const uint64_t *__restrict__ rp,
+                                         const uint32_t *__restrict__ ci,
+                                         const T *__restrict__ v, const T *__restrict__ x,
+                                         size_t row);
+
+template <>
+__device__ __forceinline__ double row_dot_ref<double>(const uint64_t *__restrict__ rp,
+                                                      const uint32_t *__restrict__ ci,
+                                                      const double *__restrict__ v,
+                                                      const double *__restrict__ x, size_t row) {
+#pragma clang fp contract(off)
+    const uint64_t s = rp[row], e = rp[row + 1];
+    const uint64_t e4 = s + (e - s) / 4 * 4;
+    double sum = 0.0;
+    uint64_t k = s;
+    for (; k < e4; k += 4)
+        sum += v[k] * x[ci[k]] + v[k + 1] * x[ci[k + 1]] + v[k + 2] * x[ci[k + 2]] +
+               v[k + 3] * x[ci[k + 3]];
+    for (; k < e; ++k) sum += v[k] * x[ci[k]];
+    return sum;
+}
+
+template <>
+__device__ __forceinline__ float row_dot_ref<float>(const uint64_t *__restrict__ rp,
+                                                    const uint32_t *__restrict__ ci,
+                                                    const float *__restrict__ v,
+                                                    const float *__restrict__ x, size_t row) {
+#pragma clang fp contract(off)
+    float sum = 0.0f;
+    for (uint64_t k = rp[row]; k < rp[row + 1]; ++k) sum += v[k] * x[ci[k]];
+    return sum;
+}
+
+// mode 0: y = Ax; 1: y += Ax; 2: y = alpha*Ax + beta*y
+template <class T>
+__global__ __launch_bounds__(256) void k_spmv_row(size_t n, const uint64_t *__restrict__ rp,
+                                                  const uint32_t *__restrict__ ci,
+                                                  const T *__restrict__ v,
+                                                  const T *__restrict__ x, T *__restrict__ y,
+                                                  int mode, T alpha, T beta, const int *skip) {
+#pragma clang fp contract(off)
+    if (skip && *skip) return;
+    const size_t r = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (r >= n) return;
+    const T s = row_dot_ref<T>(rp, ci, v, x, r);
+    if (mode == 0) y[r] = s;
+    else if (mode == 1) y[r] += s;
+    else y[r] = alpha * s + beta * y[r];
+}
+
+template <class T, int W>
+__global__ __launch_bounds__(256) void k_spmv_sub(size_t n, const uint64_t *__restrict__ rp,
+                                                  const uint32_t *__restrict__ ci,
+                                                  const T *__restrict__ v,
+                                                  const T *__restrict__ x, T *__restrict__ y,
+                                                  int mode, T alpha, T beta, const int *skip) {
+    if (skip && *skip) return;
+    const size_t r = (blockIdx.x * (size_t)256 + threadIdx.x) / W;
+    const int lane = threadIdx.x % W;
+    T s = 0;
+    if (r < n) {
+        for (uint64_t k = rp[r] + lane; k < rp[r + 1]; k += W) s += v[k] * x[ci[k]];
+    }
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+    if (r < n && lane == 0) {
+        if (mode == 0) y[r] = s;
+        else if (mode == 1) y[r] += s;
+        else y[r] = alpha * s + beta * y[r];
+    }
+}
+
+template <class T>
+__global__ void k_csr_diag(size_t n, const uint64_t *__restrict__ rp,
+                           const uint32_t *__restrict__ ci, const T *__restrict__ v,
+                           T *__restrict__ d) {
+    const size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T out = 0;
+    for (uint64_t k = rp[r]; k < rp[r + 1]; ++k)
+        if (ci[k] == r) { out = v[k]; break; }
+    d[r] = out;
+}
+
+struct CsrOp final : femgpu_op {
+    uint64_t ncols = 0, nnz = 0;
+    DevBuf rp, ci, vals;
+    int sub = 1;  // lanes per row (1 = reference-order row kernel)
+
+    const char *name() const override { return "csr"; }
+
+    template <class T>
+    void launch(const T *x, T *y, int mode, T alpha, T beta, const int *skip) {
+        hipStream_t s = ctx->stream;
+        const uint64_t *r = rp.as<uint64_t>();
+        const uint32_t *c = ci.as<uint32_t>();
+        const T *v = vals.as<T>();
+        if (n == 0) return;
+        if (sub == 1) {
+            hipLaunchKernelGGL(k_spmv_row<T>, dim3((n + 255) / 256), dim3(256), 0, s, n, r, c, v, x,
+                               y, mode, alpha, beta, skip);
+            return;
+        }
+        const size_t threads = n * (size_t)sub;
+        const dim3 grid((threads + 255) / 256);
+        switch (sub) {
+        case 8: hipLaunchKernelGGL((k_spmv_sub<T, 8>), grid, dim3(256), 0, s, n, r, c, v, x, y, mode, alpha, beta, skip); break;
+        case 16: hipLaunchKernelGGL((k_spmv_sub<T, 16>), grid, dim3(256), 0, s, n, r, c, v, x, y, mode, alpha, beta, skip); break;
+        case 32: hipLaunchKernelGGL((k_spmv_sub<T, 32>), grid, dim3(256), 0, s, n, r, c, v, x, y, mode, alpha, beta, skip); break;
+        default: hipLaunchKernelGGL((k_spmv_sub<T, 64>), grid, dim3(256), 0, s, n, r, c, v, x, y, mode, alpha, beta, skip); break;
+        }
+    }
+
+    void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
+        if (dtype == FEMGPU_F64)
+            launch<double>((const double *)x, (double *)y, acc ? 1 : 0, 1.0, 0.0, skip);
+        else
+            launch<float>((const float *)x, (float *)y, acc ? 1 : 0, 1.f, 0.f, skip);
+    }
+
+    void diagonal_dev(void *d) override {
+        if (!n) return;
+        if (dtype == FEMGPU_F64)
+            hipLaunchKernelGGL(k_csr_diag<double>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                               n, rp.as<uint64_t>(), ci.as<uint32_t>(), vals.as<double>(),
+                               (double *)d);
+        else
+            hipLaunchKernelGGL(k_csr_diag<float>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                               n, rp.as<uint64_t>(), ci.as<uint32_t>(), vals.as<float>(),
+                               (float *)d);
+    }
+};
+
+}  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+int femgpu_csr_create(femgpu_ctx *c, uint64_t nrows, uint64_t ncols, const uint64_t *row_ptr,
+                      const uint32_t *col_idx, const void *values, femgpu_dtype dtype,
+                      femgpu_op **out) {
+    return guard([&] {
+        FG_REQUIRE(c && row_ptr && out, FEMGPU_ERR_INVALID_ARG, "csr_create: NULL argument");
+        FG_REQUIRE(dtype == FEMGPU_F64 || dtype == FEMGPU_F32, FEMGPU_ERR_INVALID_ARG,
+                   "csr_create: bad dtype");
+        FG_REQUIRE(row_ptr[0] == 0, FEMGPU_ERR_INVALID_ARG, "csr_create: row_ptr[0] != 0");
+        const uint64_t nnz = row_ptr[nrows];
+        for (uint64_t r = 0; r < nrows; ++r)
+            FG_REQUIRE(row_ptr[r] <= row_ptr[r + 1], FEMGPU_ERR_INVALID_ARG,
+                       "csr_create: row_ptr not monotone at row " + std::to_string(r));
+        FG_REQUIRE(nnz == 0 || (col_idx && values), FEMGPU_ERR_INVALID_ARG,
+                   "csr_create: NULL col_idx/values");
+        for (uint64_t k = 0; k < nnz; ++k)
+            FG_REQUIRE(col_idx[k] < ncols, FEMGPU_ERR_INVALID_ARG,
+                       "csr_create: col_idx[" + std::to_string(k) + "] out of range");
+        FG_REQUIRE(nrows == ncols, FEMGPU_ERR_DIM_MISMATCH,
+                   "csr_create: operator must be square (" + std::to_string(nrows) + "x" +
+                       std::to_string(ncols) + ")");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        auto op = new CsrOp();
+        try {
+            op->ctx = c; op->dtype = dtype; op->n = nrows; op->ncols = ncols; op->nnz = nnz;
+            const size_t es = op->esize();
+            op->rp.alloc((nrows + 1) * 8);
+            op->ci.alloc(nnz * 4);
+            op->vals.alloc(nnz * es);
+            FG_HIP(hipMemcpyAsync(op->rp.ptr, row_ptr, (nrows + 1) * 8, hipMemcpyHostToDevice, c->stream));
+            if (nnz) {
+                FG_HIP(hipMemcpyAsync(op->ci.ptr, col_idx, nnz * 4, hipMemcpyHostToDevice, c->stream));
+                FG_HIP(hipMemcpyAsync(op->vals.ptr, values, nnz * es, hipMemcpyHostToDevice, c->stream));
+            }
+            FG_HIP(hipStreamSynchronize(c->stream));
+            const double avg = nrows ? (double)nnz / nrows : 0.0;
+            op->sub = avg <= 16 ? 1 : avg <= 48 ? 8 : avg <= 128 ? 16 : avg <= 400 ? 32 : 64;
+            if (const char *f = std::getenv("FEMGPU_CSR_LANES")) op->sub = std::atoi(f);
+            if (op->sub != 1 && op->sub != 8 && op->sub != 16 && op->sub != 32) op->sub = 64;
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+        return 0;
+    });
+}
+
+int femgpu_csr_spmv_add(femgpu_op *op_, double alpha, const void *x, double beta, void *y,
+                        femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<CsrOp *>(op_);
+        FG_REQUIRE(op && x && y, FEMGPU_ERR_INVALID_ARG, "csr_spmv_add: not a CSR operator");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * op->esize();
+        const void *xd = x;
+        void *yd = y;
+        if (mem == FEMGPU_HOST) {
+            op->stage_x.ensure(bytes); op->stage_y.ensure(bytes);
+            FG_HIP(hipMemcpyAsync(op->stage_x.ptr, x, bytes, hipMemcpyHostToDevice, c->stream));
+            FG_HIP(hipMemcpyAsync(op->stage_y.ptr, y, bytes, hipMemcpyHostToDevice, c->stream));
+            xd = op->stage_x.ptr; yd = op->stage_y.ptr;
+        }
+        if (op->dtype == FEMGPU_F64)
+            op->launch<double>((const double *)xd, (double *)yd, 2, alpha, beta, nullptr);
+        else
+            op->launch<float>((const float *)xd, (float *)yd, 2, (float)alpha, (float)beta, nullptr);
+        FG_HIP(hipGetLastError());
+        if (mem == FEMGPU_HOST)
+            FG_HIP(hipMemcpyAsync(y, yd, bytes, hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// Internal declarations of libfemgpu (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/femgpu.h"
+
+namespace femgpu {
+
+// ------------------------------------------------------------- errors --
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define FG_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::femgpu::Error(FEMGPU_ERR_HIP, std::string(#call) + ": " +               \
+                                                      hipGetErrorString(e_));               \
+    } while (0)
+
+#define FG_REQUIRE(cond, code, msg)                                                         \
+    do {                                                                                    \
+        if (!(cond)) throw ::femgpu::Error((code), (msg));                                  \
+    } while (0)
+
+// Wraps a C-ABI body: converts exceptions to status codes + last error.
+template <class F>
+int guard(F &&f) {
+    try {
+        set_last_error("");
+        return f();
+    } catch (const Error &e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(FEMGPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(FEMGPU_ERR_INVALID_ARG, e.what());
+    }
+}
+
+// ------------------------------------------------------- device buffer --
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t b) { alloc(b); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        release();
+        ptr = o.ptr; bytes = o.bytes; o.ptr = nullptr; o.bytes = 0;
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t b) {
+        release();
+        if (b == 0) return;
+        hipError_t e = hipMalloc(&ptr, b);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw Error(FEMGPU_ERR_OUT_OF_MEMORY,
+                        "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+        }
+        bytes = b;
+    }
+    void ensure(size_t b) { if (b > bytes) alloc(b); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr; bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Reduction partials: a fixed grid gives a deterministic summation order.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+}  // namespace femgpu
+
+// ------------------------------------------------------------- context --
+struct femgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    femgpu::DevBuf partials;  // kRedBlocks * 8 doubles
+    femgpu::DevBuf scalars;   // solver scalars (see solvers)
+    femgpu::DevBuf flags;     // int flags: [0] stop (kernels skip work when set)
+    double *pinned = nullptr; // host-visible scalars for read-back
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+// ------------------------------------------------------------ operator --
+struct femgpu_op {
+    femgpu_ctx *ctx = nullptr;
+    femgpu_dtype dtype = FEMGPU_F64;
+    uint64_t n = 0;
+    // Staging for host-memory calls.
+    femgpu::DevBuf stage_x, stage_y;
+    // Solver workspace vectors, grown on demand and reused across solves.
+    std::vector<femgpu::DevBuf> ws;
+    bool diag_ready = false;  // workspace slot of the diagonal holds diag(A)
+    void *wsvec(size_t i, size_t bytes) {
+        if (ws.size() <= i) ws.resize(i + 1);
+        ws[i].ensure(bytes);
+        return ws[i].ptr;
+    }
+
+    // Optional profiling: when set, apply_dev records these events around its
+    // main compute kernel (femgpu_op_time_apply).
+    hipEvent_t prof_begin = nullptr, prof_end = nullptr;
+
+    virtual ~femgpu_op() = default;
+    // y (+)= A x on device pointers of type dtype, enqueued on ctx->stream.
+    // skip: optional device flag; when it reads non-zero the kernels do no work.
+    virtual void apply_dev(const void *x, void *y, bool accumulate, const int *skip) = 0;
+    virtual void diagonal_dev(void *d) = 0;
+    virtual const char *name() const = 0;
+    size_t esize() const { return dtype == FEMGPU_F64 ? 8 : 4; }
+};
+
+namespace femgpu {
+
+// blas.hip
+template <class T> void fill(hipStream_t s, T *y, T v, size_t n, const int *skip = nullptr);
+template <class T> void copy(hipStream_t s, T *y, const T *x, size_t n);
+// y = a*x + b*y
+template <class T> void axpby(hipStream_t s, T *y, double a, const T *x, double b, size_t n);
+// inv[i] = 1/d[i] (1 if |d| <= tiny)
+template <class T> void inv_diag(hipStream_t s, T *inv, const T *d, size_t n, double tiny);
+// Deterministic dot products in double: writes partials then finalises into out[0].
+template <class T> void dot(femgpu_ctx *c, const T *a, const T *b, size_t n, double *out_dev);
+double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dtype dt);
+void convert_f64_to_f32(hipStream_t s, float *y, const double *x, size_t n, double scale);
+void convert_f32_to_f64_axpy(hipStream_t s, double *y, const float *x, size_t n, double scale);
+
+// Helpers for host/device vector arguments.
+struct VecArg {
+    void *dev = nullptr;   // device pointer to use
+    void *host = nullptr;  // host pointer (if staged)
+    size_t bytes = 0;
+};
+
+// hex mesh generator (host) shared by the C ABI and set-up code
+void hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, double *coords);
+double philox_uniform(uint64_t seed, uint32_t stream, uint64_t index);
+
+}  // namespace femgpu

@@ -1,0 +1,730 @@
+// Sum-factorised partial-assembly (PA) mass / diffusion apply on structured
+// hexahedral Q_p meshes for gfx950.
+//
+// Reference behaviour: PADiffusionOperator / PAMassOperator::apply
+// (assembly/src/partial.rs:85-130, 154-208), y_e += sum_q w|J| kappa grad phi_i .
+// grad u (+ w|J| rho phi_i u), which the reference evaluates element by element
+// on simplices only.  Here the element operator is factored through 1-D
+// tables: with B = l_i(x_q), G = l_i'(x_q)
+//   grad_ref u = (B(x)B(x)G, B(x)G(x)B, G(x)B(x)B) u,  v = D grad_ref u,
+//   y_e = (B^T B^T G^T, B^T G^T B^T, G^T B^T B^T) v + (B^T)^3 (M u)
+// where D = w kappa |J| J^-1 J^-T and M = w rho |J| are the quadrature data
+// ("qdata") computed once at set-up from the trilinear (HexQ1) geometry.
+//
+// Kernel layout (one workgroup = NE elements x Q*Q lanes, 64-wide waves):
+//   X  : lane (j,k) reads its x-line of the element from HBM and contracts it
+//        in registers -> LDS
+//   Y  : lane (qx,k) contracts along y in registers -> LDS
+//   Z  : lane (qx,qy) contracts along z, streams its qdata column from HBM,
+//        applies D / M and immediately contracts back along z, all in
+//        registers -> LDS (in place, no hazard: each lane owns its column)
+//   Y^T, X^T as Y, X; X^T scatters with hardware f64/f32 atomics.
+// Every contraction of a line is done in registers, so LDS carries ~10 values
+// per element point while the 1-D tables come from the scalar cache
+// (compile-time-indexed __constant__ memory).
+//
+// qdata is stored plane-major: plane (comp, qz) holds [E][Q*Q] values, so the
+// Z phase of a wavefront reads fully contiguous, coalesced runs.
+#include "basis.hpp"
+#include "internal.hpp"
+#include "philox.hpp"
+
+#include <cmath>
+#include <mutex>
+
+namespace femgpu {
+
+constexpr int kTab = 64;
+constexpr int combo(int n1, int q) { return (n1 - 2) * 3 + (q - n1); }
+constexpr int kCombos = 15;  // n1 = 2..6, q = n1..n1+2
+
+__constant__ double c_tab64[kCombos][2][kTab];
+__constant__ float c_tab32[kCombos][2][kTab];
+
+template <class T> struct Tab;
+template <> struct Tab<double> {
+    static __device__ __forceinline__ const double *B(int c) { return c_tab64[c][0]; }
+    static __device__ __forceinline__ const double *G(int c) { return c_tab64[c][1]; }
+};
+template <> struct Tab<float> {
+    static __device__ __forceinline__ const float *B(int c) { return c_tab32[c][0]; }
+    static __device__ __forceinline__ const float *G(int c) { return c_tab32[c][1]; }
+};
+
+static std::mutex g_tab_mu;
+static bool g_tab_done[64];
+
+// Fill the 1-D basis tables for every supported (n1, q) on the current device.
+static void ensure_tables(int device) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    if (device < 64 && g_tab_done[device]) return;
+    static double h64[kCombos][2][kTab];
+    static float h32[kCombos][2][kTab];
+    std::memset(h64, 0, sizeof h64);
+    std::memset(h32, 0, sizeof h32);
+    for (int n1 = 2; n1 <= 6; ++n1)
+        for (int q = n1; q <= n1 + 2 && q <= 6; ++q) {
+            double x[8], w[8], B[64], G[64];
+            gauss_legendre(q, x, w);
+            lagrange_1d(n1 - 1, q, x, B, G);
+            const int c = combo(n1, q);
+            for (int i = 0; i < q * n1; ++i) {
+                h64[c][0][i] = B[i]; h64[c][1][i] = G[i];
+                h32[c][0][i] = (float)B[i]; h32[c][1][i] = (float)G[i];
+            }
+        }
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), h64, sizeof h64));
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), h32, sizeof h32));
+    if (device < 64) g_tab_done[device] = true;
+}
+
+__host__ __device__ constexpr int ne_of(int q) { return 256 / (q * q) > 0 ? 256 / (q * q) : 1; }
+
+struct HexGrid {
+    int nx, ny, nz;
+    long long E;
+    long long Nx, Ny, Nz;
+};
+
+__device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long gy, long long gz) {
+    return gx == 0 || gx == g.Nx - 1 || gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1;
+}
+
+template <int N1, int Q, class T, int FORM>
+__global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
+    const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm,
+    const T *__restrict__ x, T *__restrict__ y, const int dirichlet, const int *skip) {
+    constexpr int P = N1 - 1;
+    constexpr int T2 = Q * Q;
+    constexpr int NE = ne_of(Q);
+    constexpr int C = combo(N1, Q);
+    constexpr bool DIFF = (FORM & 2) != 0;
+    constexpr bool MASS = (FORM & 1) != 0;
+    constexpr int ND = DIFF ? NE : 1;
+    __shared__ T sA[NE][N1][N1][Q];   // X out (B path), later F_B
+    __shared__ T sAg[ND][N1][N1][Q];  // X out (G path), later F_G
+    __shared__ T sBB[NE][N1][Q][Q];   // Y out BB, later C_zeta+mass
+    __shared__ T sBG[ND][N1][Q][Q];   // Y out BG, later C_xi
+    __shared__ T sGB[ND][N1][Q][Q];   // Y out GB, later C_eta
+
+    if (skip && *skip) return;
+    const T *Bt = Tab<T>::B(C);
+    const T *Gt = Tab<T>::G(C);
+    const int le = threadIdx.x / T2;
+    const int t = threadIdx.x - le * T2;
+    const long long e = (long long)blockIdx.x * NE + le;
+    const bool valid = e < g.E;
+    int ex = 0, ey = 0, ez = 0;
+    if (valid) {
+        ex = (int)(e % g.nx);
+        const long long r = e / g.nx;
+        ey = (int)(r % g.ny);
+        ez = (int)(r / g.ny);
+    }
+
+    // ---- X: lane (j,k) owns an x-line
+    if (valid && t < N1 * N1) {
+        const int j = t % N1, k = t / N1;
+        const long long gy = (long long)ey * P + j, gz = (long long)ez * P + k;
+        const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
+        const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+        T u[N1];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            const long long gx = (long long)ex * P + i;
+            const bool b = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
+            u[i] = b ? T(0) : x[base + i];
+        }
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            T s = 0, sg = 0;
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                s += Bt[a * N1 + i] * u[i];
+                if constexpr (DIFF) sg += Gt[a * N1 + i] * u[i];
+            }
+            sA[le][k][j][a] = s;
+            if constexpr (DIFF) sAg[le][k][j][a] = sg;
+        }
+    }
+    __syncthreads();
+
+    // ---- Y: lane (qx,k)
+    if (valid && t < Q * N1) {
+        const int a = t % Q, k = t / Q;
+        T A_[N1], Ag_[N1];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            A_[j] = sA[le][k][j][a];
+            if constexpr (DIFF) Ag_[j] = sAg[le][k][j][a];
+        }
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            T bb = 0, bg = 0, gb = 0;
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                bb += Bt[b * N1 + j] * A_[j];
+                if constexpr (DIFF) {
+                    bg += Bt[b * N1 + j] * Ag_[j];
+                    gb += Gt[b * N1 + j] * A_[j];
+                }
+            }
+            sBB[le][k][b][a] = bb;
+            if constexpr (DIFF) { sBG[le][k][b][a] = bg; sGB[le][k][b][a] = gb; }
+        }
+    }
+    __syncthreads();
+
+    // ---- Z + pointwise + Z^T: lane (qx,qy) owns a z-column
+    if (valid) {
+        const int a = t % Q, b = t / Q;
+        T bb[N1], bg[N1], gb[N1];
+        T c0[N1], c1[N1], c2[N1];
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            bb[k] = sBB[le][k][b][a];
+            if constexpr (DIFF) { bg[k] = sBG[le][k][b][a]; gb[k] = sGB[le][k][b][a]; }
+            c0[k] = c1[k] = c2[k] = 0;
+        }
+        const size_t plane = (size_t)g.E * T2;
+        const size_t off = (size_t)e * T2 + t;
+#pragma unroll
+        for (int c = 0; c < Q; ++c) {
+            T v0 = 0, v1 = 0, v2 = 0, vm = 0;
+            if constexpr (DIFF) {
+                T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    g0 += Bt[c * N1 + k] * bg[k];
+                    g1 += Bt[c * N1 + k] * gb[k];
+                    g2 += Gt[c * N1 + k] * bb[k];
+                }
+                const T d0 = qd[(size_t)(0 * Q + c) * plane + off];
+                const T d1 = qd[(size_t)(1 * Q + c) * plane + off];
+                const T d2 = qd[(size_t)(2 * Q + c) * plane + off];
+                const T d3 = qd[(size_t)(3 * Q + c) * plane + off];
+                const T d4 = qd[(size_t)(4 * Q + c) * plane + off];
+                const T d5 = qd[(size_t)(5 * Q + c) * plane + off];
+                v0 = d0 * g0 + d1 * g1 + d2 * g2;
+                v1 = d1 * g0 + d3 * g1 + d4 * g2;
+                v2 = d2 * g0 + d4 * g1 + d5 * g2;
+            }
+            if constexpr (MASS) {
+                T uu = 0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k) uu += Bt[c * N1 + k] * bb[k];
+                vm = qm[(size_t)c * plane + off] * uu;
+            }
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                const T bz = Bt[c * N1 + k];
+                if constexpr (DIFF) {
+                    c0[k] += bz * v0;
+                    c1[k] += bz * v1;
+                    c2[k] += Gt[c * N1 + k] * v2;
+                }
+                if constexpr (MASS) c2[k] += bz * vm;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N1; ++k) {
+            sBB[le][k][b][a] = c2[k];
+            if constexpr (DIFF) { sBG[le][k][b][a] = c0[k]; sGB[le][k][b][a] = c1[k]; }
+        }
+    }
+    __syncthreads();
+
+    // ---- Y^T: lane (qx,k)
+    if (valid && t < Q * N1) {
+        const int a = t % Q, k = t / Q;
+        T C0[Q], C1[Q], C2[Q];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            C2[b] = sBB[le][k][b][a];
+            if constexpr (DIFF) { C0[b] = sBG[le][k][b][a]; C1[b] = sGB[le][k][b][a]; }
+        }
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            T fg = 0, fb = 0;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                fb += Bt[b * N1 + j] * C2[b];
+                if constexpr (DIFF) {
+                    fg += Bt[b * N1 + j] * C0[b];
+                    fb += Gt[b * N1 + j] * C1[b];
+                }
+            }
+            sA[le][k][j][a] = fb;
+            if constexpr (DIFF) sAg[le][k][j][a] = fg;
+        }
+    }
+    __syncthreads();
+
+    // ---- X^T + scatter: lane (j,k)
+    if (valid && t < N1 * N1) {
+        const int j = t % N1, k = t / N1;
+        T FG[Q], FB[Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            FB[a] = sA[le][k][j][a];
+            if constexpr (DIFF) FG[a] = sAg[le][k][j][a];
+        }
+        const long long gy = (long long)ey * P + j, gz = (long long)ez * P + k;
+        const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
+        const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            T s = 0;
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                s += Bt[a * N1 + i] * FB[a];
+                if constexpr (DIFF) s += Gt[a * N1 + i] * FG[a];
+            }
+            const long long gx = (long long)ex * P + i;
+            const bool bd = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
+            if (!bd) unsafeAtomicAdd(&y[base + i], s);
+        }
+    }
+}
+
+// mode 0: y = bdr ? x : 0   (apply_zero)
+// mode 1: y += bdr ? x : 0  (accumulate, only with essential DOFs)
+// mode 2: y = bdr ? 1 : y   (diagonal fix-up)
+// mode 3: y = bdr ? 0 : y   (right-hand side masking)
+template <class T>
+__global__ void k_hex_bc(const HexGrid g, const T *__restrict__ x, T *__restrict__ y, int mode,
+                         int dirichlet, const int *skip) {
+    if (skip && *skip) return;
+    const size_t N = (size_t)g.Nx * g.Ny * g.Nz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const long long gx = (long long)(i % g.Nx);
+        const long long r = (long long)(i / g.Nx);
+        const long long gy = r % g.Ny, gz = r / g.Ny;
+        const bool b = dirichlet && on_bdr(g, gx, gy, gz);
+        if (mode == 0) y[i] = b ? x[i] : T(0);
+        else if (mode == 1) { if (b) y[i] += x[i]; }
+        else if (mode == 2) { if (b) y[i] = T(1); }
+        else { if (b) y[i] = T(0); }
+    }
+}
+
+// Quadrature data from the trilinear geometry; one thread per (element, point).
+// Computed in double, stored in the operator's precision.
+struct CoefDev {
+    int kind;
+    double value;
+    int nf;
+    double scale, lo, hi;
+    const double *wf, *bf, *qp;
+};
+
+__device__ inline double coef_eval(const CoefDev &c, const double xp[3], long long e, int qp, int Q3) {
+    switch (c.kind) {
+    case FEMGPU_COEF_CONST: return c.value;
+    case FEMGPU_COEF_SINPROD:
+        return 1.0 + 0.5 * sin(2.0 * M_PI * xp[0]) * sin(2.0 * M_PI * xp[1]) * sin(2.0 * M_PI * xp[2]);
+    case FEMGPU_COEF_GRF: {
+        double s = 0.0;
+        for (int m = 0; m < c.nf; ++m)
+            s += cos(c.wf[3 * m] * xp[0] + c.wf[3 * m + 1] * xp[1] + c.wf[3 * m + 2] * xp[2] + c.bf[m]);
+        double k = exp(c.scale * s);
+        return fmin(fmax(k, c.lo), c.hi);
+    }
+    default: return c.qp[(size_t)e * Q3 + qp];
+    }
+}
+
+template <class T>
+__global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ xq,
+                            const double *__restrict__ wq, const double *__restrict__ coords,
+                            CoefDev coef, double rho, int form, T *__restrict__ qd,
+                            T *__restrict__ qm, int mass_only_weight) {
+    const int Q3 = Q * Q * Q, T2 = Q * Q;
+    const size_t total = (size_t)g.E * Q3;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
+         id += (size_t)gridDim.x * blockDim.x) {
+        const long long e = (long long)(id / Q3);
+        const int qp = (int)(id - (size_t)e * Q3);
+        const int qx = qp % Q, qy = (qp / Q) % Q, qz = qp / T2;
+        const int ex = (int)(e % g.nx);
+        const int ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+        const double xi[3] = {xq[qx], xq[qy], xq[qz]};
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, xp[3] = {0, 0, 0};
+        // HexQ1 vertex order (lagrange/hex.rs:27-36)
+        const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                  sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const long long vi = ex + (sx[v] > 0), vj = ey + (sy[v] > 0), vk = ez + (sz[v] > 0);
+            const size_t gv = (size_t)vi + (size_t)(g.nx + 1) * ((size_t)vj + (size_t)(g.ny + 1) * vk);
+            const double X0 = coords[3 * gv], X1 = coords[3 * gv + 1], X2 = coords[3 * gv + 2];
+            const double fx = 1.0 + sx[v] * xi[0], fy = 1.0 + sy[v] * xi[1], fz = 1.0 + sz[v] * xi[2];
+            const double phi = 0.125 * fx * fy * fz;
+            const double gr[3] = {0.125 * sx[v] * fy * fz, 0.125 * sy[v] * fx * fz, 0.125 * sz[v] * fx * fy};
+            const double X[3] = {X0, X1, X2};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                xp[i] += phi * X[i];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) J[i][d] += X[i] * gr[d];
+            }
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        const double wd = wq[qx] * wq[qy] * wq[qz] * fabs(det);
+        const size_t plane = (size_t)g.E * T2;
+        const size_t off = (size_t)e * T2 + (qx + Q * qy);
+        if ((form & 1) && qm) qm[(size_t)qz * plane + off] = (T)(mass_only_weight ? wd : wd * rho);
+        if ((form & 2) && qd) {
+            const double id = 1.0 / det;
+            double I[3][3];
+            I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+            I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+            I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+            I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+            I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+            I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+            I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+            I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+            I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+            const double s = wd * coef_eval(coef, xp, e, qp, Q3);
+            const double D[6] = {
+                s * (I[0][0] * I[0][0] + I[0][1] * I[0][1] + I[0][2] * I[0][2]),
+                s * (I[0][0] * I[1][0] + I[0][1] * I[1][1] + I[0][2] * I[1][2]),
+                s * (I[0][0] * I[2][0] + I[0][1] * I[2][1] + I[0][2] * I[2][2]),
+                s * (I[1][0] * I[1][0] + I[1][1] * I[1][1] + I[1][2] * I[1][2]),
+                s * (I[1][0] * I[2][0] + I[1][1] * I[2][1] + I[1][2] * I[2][2]),
+                s * (I[2][0] * I[2][0] + I[2][1] * I[2][1] + I[2][2] * I[2][2])};
+#pragma unroll
+            for (int comp = 0; comp < 6; ++comp) qd[(size_t)(comp * Q + qz) * plane + off] = (T)D[comp];
+        }
+    }
+}
+
+// Assembled diagonal: one thread per (element, local DOF); setup-time only.
+template <class T>
+__global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restrict__ B,
+                           const double *__restrict__ G, const T *__restrict__ qd,
+                           const T *__restrict__ qm, int form, int dirichlet, T *__restrict__ diag) {
+    const int L = N1 * N1 * N1, T2 = Q * Q, P = N1 - 1;
+    const size_t total = (size_t)g.E * L;
+    const size_t plane = (size_t)g.E * T2;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
+         id += (size_t)gridDim.x * blockDim.x) {
+        const long long e = (long long)(id / L);
+        const int l = (int)(id - (size_t)e * L);
+        const int i = l % N1, j = (l / N1) % N1, k = l / (N1 * N1);
+        const int ex = (int)(e % g.nx);
+        const int ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+        const long long gx = (long long)ex * P + i, gy = (long long)ey * P + j, gz = (long long)ez * P + k;
+        if (dirichlet && on_bdr(g, gx, gy, gz)) continue;
+        double s = 0.0;
+        for (int c = 0; c < Q; ++c)
+            for (int b = 0; b < Q; ++b)
+                for (int a = 0; a < Q; ++a) {
+                    const size_t off = (size_t)e * T2 + a + Q * b;
+                    const double Bi = B[a * N1 + i], Bj = B[b * N1 + j], Bk = B[c * N1 + k];
+                    if (form & 2) {
+                        const double g0 = G[a * N1 + i] * Bj * Bk, g1 = Bi * G[b * N1 + j] * Bk,
+                                     g2 = Bi * Bj * G[c * N1 + k];
+                        double d[6];
+                        for (int m = 0; m < 6; ++m) d[m] = (double)qd[(size_t)(m * Q + c) * plane + off];
+                        s += d[0] * g0 * g0 + d[3] * g1 * g1 + d[5] * g2 * g2 +
+                             2.0 * (d[1] * g0 * g1 + d[2] * g0 * g2 + d[4] * g1 * g2);
+                    }
+                    if (form & 1) {
+                        const double ph = Bi * Bj * Bk;
+                        s += (double)qm[(size_t)c * plane + off] * ph * ph;
+                    }
+                }
+        unsafeAtomicAdd(&diag[gx + g.Nx * (gy + g.Ny * gz)], (T)s);
+    }
+}
+
+template <class T>
+__global__ void k_random_field(size_t n, uint64_t seed, T *__restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (T)(2.0 * philox_u01(seed, 7, i) - 1.0);
+}
+
+static unsigned grid1(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 1u << 20) b = 1u << 20;
+    return (unsigned)(b ? b : 1);
+}
+
+template <int N1, int Q, class T>
+static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, const T *x, T *y,
+                        int dir, const int *skip, hipStream_t s) {
+    constexpr int NE = ne_of(Q);
+    const dim3 grid((unsigned)((g.E + NE - 1) / NE)), block(NE * Q * Q);
+    if (form == 1) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 1>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
+    else if (form == 2) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
+    else hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
+}
+
+template <class T>
+static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm,
+                      const T *x, T *y, int dir, const int *skip, hipStream_t s) {
+#define CASE(A, B)                                                                 \
+    if (n1 == A && q == B) { launch_form<A, B, T>(g, form, qd, qm, x, y, dir, skip, s); return true; }
+    CASE(2, 2) CASE(2, 3) CASE(3, 3) CASE(3, 4) CASE(4, 4) CASE(4, 5) CASE(5, 5) CASE(5, 6)
+    CASE(6, 6)
+#undef CASE
+    return false;
+}
+
+static bool supported(int n1, int q) {
+    return (n1 >= 2 && n1 <= 6) && (q == n1 || (q == n1 + 1 && n1 <= 5));
+}
+
+struct HexOp final : femgpu_op {
+    femgpu_hex_desc desc{};
+    HexGrid g{};
+    DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
+
+    const char *name() const override { return "pa_hex"; }
+
+    template <class T> void apply_t(const T *x, T *y, bool acc, const int *skip) {
+        hipStream_t s = ctx->stream;
+        const int dir = desc.dirichlet ? 1 : 0;
+        if (!acc) hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, x, y, 0, dir, skip);
+        else if (dir) hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, x, y, 1, dir, skip);
+        if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
+        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, y, dir, skip, s);
+        if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
+    }
+
+    void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
+        if (dtype == FEMGPU_F64) apply_t<double>((const double *)x, (double *)y, acc, skip);
+        else apply_t<float>((const float *)x, (float *)y, acc, skip);
+    }
+
+    template <class T> void diag_t(T *d) {
+        hipStream_t s = ctx->stream;
+        fill<T>(s, d, T(0), n);
+        hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1((size_t)g.E * (desc.p + 1) * (desc.p + 1) * (desc.p + 1))),
+                           dim3(256), 0, s, g, desc.p + 1, desc.q, Bd.as<double>(), Gd.as<double>(),
+                           qd.as<T>(), qm.as<T>(), desc.form, desc.dirichlet, d);
+        hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, (const T *)d, d, 2,
+                           desc.dirichlet, nullptr);
+    }
+
+    void diagonal_dev(void *d) override {
+        if (dtype == FEMGPU_F64) diag_t<double>((double *)d);
+        else diag_t<float>((float *)d);
+    }
+
+    // Builds qdata of the given form into (qd_, qm_) in type T.
+    template <class T> void setup_t(int form, double rho, bool mass_weight_only, T *qd_, T *qm_) {
+        CoefDev c{};
+        const femgpu_coef &k = desc.kappa;
+        c.kind = k.kind; c.value = k.value; c.nf = k.n_features; c.lo = k.lo; c.hi = k.hi;
+        c.scale = k.kind == FEMGPU_COEF_GRF ? k.sigma * std::sqrt(2.0 / k.n_features) : 0.0;
+        c.wf = grf_w.as<double>(); c.bf = grf_b.as<double>(); c.qp = kqp.as<double>();
+        const size_t total = (size_t)g.E * desc.q * desc.q * desc.q;
+        hipLaunchKernelGGL(k_hex_setup<T>, dim3(grid1(total)), dim3(256), 0, ctx->stream, g, desc.q,
+                           xq.as<double>(), wq.as<double>(), coords.as<double>(), c, rho, form,
+                           qd_, qm_, mass_weight_only ? 1 : 0);
+        FG_HIP(hipGetLastError());
+    }
+
+    void build() {
+        const int n1 = desc.p + 1, q = desc.q;
+        double x[8], w[8], B[64], G[64];
+        gauss_legendre(q, x, w);
+        lagrange_1d(desc.p, q, x, B, G);
+        xq.alloc(8 * q); wq.alloc(8 * q); Bd.alloc(8 * q * n1); Gd.alloc(8 * q * n1);
+        hipStream_t s = ctx->stream;
+        FG_HIP(hipMemcpyAsync(xq.ptr, x, 8 * q, hipMemcpyHostToDevice, s));
+        FG_HIP(hipMemcpyAsync(wq.ptr, w, 8 * q, hipMemcpyHostToDevice, s));
+        FG_HIP(hipMemcpyAsync(Bd.ptr, B, 8 * q * n1, hipMemcpyHostToDevice, s));
+        FG_HIP(hipMemcpyAsync(Gd.ptr, G, 8 * q * n1, hipMemcpyHostToDevice, s));
+        const size_t nq = (size_t)g.E * q * q * q;
+        const size_t es = esize();
+        if (desc.form & 2) qd.alloc(6 * nq * es);
+        if (desc.form & 1) qm.alloc(nq * es);
+        if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
+        else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
+        FG_HIP(hipStreamSynchronize(s));
+    }
+};
+
+static void grf_features_host(const femgpu_coef &c, std::vector<double> &w, std::vector<double> &b) {
+    // Random Fourier features of a squared-exponential covariance (Rahimi &
+    // Recht 2007): w ~ N(0, I / length^2) by Box-Muller, b ~ U[0, 2 pi).
+    w.assign(3 * c.n_features, 0.0);
+    b.assign(c.n_features, 0.0);
+    for (int m = 0; m < c.n_features; ++m) {
+        for (int k = 0; k < 3; ++k) {
+            const double u1 = philox_u01(c.seed, 10 + 2 * k, (uint64_t)m);
+            const double u2 = philox_u01(c.seed, 11 + 2 * k, (uint64_t)m);
+            const double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+            w[3 * m + k] = r * std::cos(2.0 * M_PI * u2) / c.length;
+        }
+        b[m] = 2.0 * M_PI * philox_u01(c.seed, 20, (uint64_t)m);
+    }
+}
+
+static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *coords_host,
+                       const void *coords_dev, const void *kqp_dev) {
+    FG_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, FEMGPU_ERR_INVALID_ARG, "pa_create_hex: empty mesh");
+    FG_REQUIRE(d.p >= 1 && supported(d.p + 1, d.q), FEMGPU_ERR_UNSUPPORTED,
+               "pa_create_hex: unsupported (p, q) = (" + std::to_string(d.p) + ", " +
+                   std::to_string(d.q) + "); supported q = p+1 (p <= 5) or q = p+2 (p <= 4)");
+    FG_REQUIRE(d.form >= 1 && d.form <= 3, FEMGPU_ERR_INVALID_ARG, "pa_create_hex: bad form");
+    FG_REQUIRE(d.dtype == FEMGPU_F64 || d.dtype == FEMGPU_F32, FEMGPU_ERR_INVALID_ARG,
+               "pa_create_hex: bad dtype");
+    if (d.form & 2) {
+        FG_REQUIRE(d.kappa.kind >= 0 && d.kappa.kind <= 3, FEMGPU_ERR_INVALID_ARG,
+                   "pa_create_hex: bad coefficient kind");
+        if (d.kappa.kind == FEMGPU_COEF_GRF)
+            FG_REQUIRE(d.kappa.n_features > 0 && d.kappa.length > 0, FEMGPU_ERR_INVALID_ARG,
+                       "pa_create_hex: GRF needs n_features > 0 and length > 0");
+        if (d.kappa.kind == FEMGPU_COEF_QP_ARRAY)
+            FG_REQUIRE(d.kappa.qp_values || kqp_dev, FEMGPU_ERR_INVALID_ARG,
+                       "pa_create_hex: QP_ARRAY needs qp_values");
+    }
+    FG_HIP(hipSetDevice(c->device));
+    ensure_tables(c->device);
+    auto *op = new HexOp();
+    try {
+        op->ctx = c; op->dtype = d.dtype; op->desc = d; op->desc.kappa.qp_values = nullptr;
+        HexGrid &g = op->g;
+        g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+        g.E = (long long)d.nx * d.ny * d.nz;
+        g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
+        op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
+        FG_REQUIRE(op->n < (1ull << 40), FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
+        const size_t nv = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+        hipStream_t s = c->stream;
+        op->coords.alloc(nv * 3 * 8);
+        if (coords_host) {
+            FG_HIP(hipMemcpyAsync(op->coords.ptr, coords_host, nv * 24, hipMemcpyHostToDevice, s));
+        } else if (coords_dev) {
+            FG_HIP(hipMemcpyAsync(op->coords.ptr, coords_dev, nv * 24, hipMemcpyDeviceToDevice, s));
+        } else {
+            std::vector<double> h(nv * 3);
+            hex_vertices(d.nx, d.ny, d.nz, 0.0, 0, h.data());
+            FG_HIP(hipMemcpyAsync(op->coords.ptr, h.data(), nv * 24, hipMemcpyHostToDevice, s));
+            FG_HIP(hipStreamSynchronize(s));
+        }
+        if ((d.form & 2) && d.kappa.kind == FEMGPU_COEF_QP_ARRAY) {
+            const size_t nq = (size_t)g.E * d.q * d.q * d.q;
+            op->kqp.alloc(nq * 8);
+            if (d.kappa.qp_values)
+                FG_HIP(hipMemcpyAsync(op->kqp.ptr, d.kappa.qp_values, nq * 8, hipMemcpyHostToDevice, s));
+            else
+                FG_HIP(hipMemcpyAsync(op->kqp.ptr, kqp_dev, nq * 8, hipMemcpyDeviceToDevice, s));
+        }
+        if ((d.form & 2) && d.kappa.kind == FEMGPU_COEF_GRF) {
+            std::vector<double> w, b;
+            grf_features_host(d.kappa, w, b);
+            op->grf_w.alloc(w.size() * 8);
+            op->grf_b.alloc(b.size() * 8);
+            FG_HIP(hipMemcpyAsync(op->grf_w.ptr, w.data(), w.size() * 8, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemcpyAsync(op->grf_b.ptr, b.data(), b.size() * 8, hipMemcpyHostToDevice, s));
+            FG_HIP(hipStreamSynchronize(s));
+        }
+        op->build();
+    } catch (...) {
+        delete op;
+        throw;
+    }
+    return op;
+}
+
+}  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+int femgpu_pa_create_hex(femgpu_ctx *c, const femgpu_hex_desc *d, const double *coords,
+                         femgpu_op **out) {
+    return guard([&] {
+        FG_REQUIRE(c && d && out, FEMGPU_ERR_INVALID_ARG, "pa_create_hex: NULL argument");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        *out = make_hex(c, *d, coords, nullptr, nullptr);
+        return 0;
+    });
+}
+
+int femgpu_pa_hex_convert(femgpu_op *op_, femgpu_dtype dtype, femgpu_op **out) {
+    return guard([&] {
+        auto *op = dynamic_cast<HexOp *>(op_);
+        FG_REQUIRE(op && out, FEMGPU_ERR_INVALID_ARG, "pa_hex_convert: not a hex operator");
+        std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
+        femgpu_hex_desc d = op->desc;
+        d.dtype = dtype;
+        *out = make_hex(op->ctx, d, nullptr, op->coords.ptr, op->kqp.ptr);
+        return 0;
+    });
+}
+
+int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<HexOp *>(op_);
+        FG_REQUIRE(op && b, FEMGPU_ERR_INVALID_ARG, "pa_hex_rhs_unit: not a hex operator");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t nq = (size_t)op->g.E * op->desc.q * op->desc.q * op->desc.q;
+        const size_t bytes = op->n * op->esize();
+        DevBuf tq(nq * op->esize()), ones(bytes), out(mem == FEMGPU_DEVICE ? 0 : bytes);
+        void *dst = mem == FEMGPU_DEVICE ? b : out.ptr;
+        hipStream_t s = c->stream;
+        const int n1 = op->desc.p + 1, q = op->desc.q;
+        if (op->dtype == FEMGPU_F64) {
+            op->setup_t<double>(1, 1.0, true, nullptr, tq.as<double>());
+            fill<double>(s, ones.as<double>(), 1.0, op->n);
+            fill<double>(s, (double *)dst, 0.0, op->n);
+            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), ones.as<double>(), (double *)dst, 0, nullptr, s);
+            hipLaunchKernelGGL(k_hex_bc<double>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
+                               (const double *)dst, (double *)dst, 3, op->desc.dirichlet, nullptr);
+        } else {
+            op->setup_t<float>(1, 1.0, true, nullptr, tq.as<float>());
+            fill<float>(s, ones.as<float>(), 1.f, op->n);
+            fill<float>(s, (float *)dst, 0.f, op->n);
+            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), ones.as<float>(), (float *)dst, 0, nullptr, s);
+            hipLaunchKernelGGL(k_hex_bc<float>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
+                               (const float *)dst, (float *)dst, 3, op->desc.dirichlet, nullptr);
+        }
+        FG_HIP(hipGetLastError());
+        if (mem != FEMGPU_DEVICE) FG_HIP(hipMemcpyAsync(b, dst, bytes, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+int femgpu_pa_hex_rhs_random(femgpu_op *op_, uint64_t seed, void *b, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<HexOp *>(op_);
+        FG_REQUIRE(op && b, FEMGPU_ERR_INVALID_ARG, "pa_hex_rhs_random: not a hex operator");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * op->esize();
+        DevBuf out(mem == FEMGPU_DEVICE ? 0 : bytes);
+        void *dst = mem == FEMGPU_DEVICE ? b : out.ptr;
+        hipStream_t s = c->stream;
+        if (op->dtype == FEMGPU_F64) {
+            hipLaunchKernelGGL(k_random_field<double>, dim3(grid1(op->n)), dim3(256), 0, s, (size_t)op->n, seed, (double *)dst);
+            hipLaunchKernelGGL(k_hex_bc<double>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
+                               (const double *)dst, (double *)dst, 3, op->desc.dirichlet, nullptr);
+        } else {
+            hipLaunchKernelGGL(k_random_field<float>, dim3(grid1(op->n)), dim3(256), 0, s, (size_t)op->n, seed, (float *)dst);
+            hipLaunchKernelGGL(k_hex_bc<float>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
+                               (const float *)dst, (float *)dst, 3, op->desc.dirichlet, nullptr);
+        }
+        FG_HIP(hipGetLastError());
+        if (mem != FEMGPU_DEVICE) FG_HIP(hipMemcpyAsync(b, dst, bytes, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+}  // extern "C"

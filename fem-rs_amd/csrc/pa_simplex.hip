@@ -1,0 +1,346 @@
+// Partial assembly on simplex meshes (Tri3 / Tet4, P1..P3) and the lumped
+// mass operator.
+//
+// Reference: PAMassOperator / PADiffusionOperator::apply
+// (assembly/src/partial.rs:85-130, 154-208) and LumpedMassOperator
+// (partial.rs:217-274).  The reference re-evaluates bases, Jacobians and the
+// coefficient for every element on every apply; here the affine geometry is
+// folded at set-up into per-element quadrature data
+//   diffusion: D_e = coef |det J| J^-1 J^-T  (dim x dim, symmetric)
+//   mass:      m_e = coef |det J|
+// and the apply is y_e[i] = sum_q w_q G_q[i] . (D_e G_q^T x_e)  (diffusion) or
+// y_e[i] = sum_q w_q m_e phi_q[i] (phi_q . x_e)  (mass), with the reference
+// basis tables (phi_q, G_q) in LDS.  One thread per element; scatter with
+// hardware atomics.
+#include "basis.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace femgpu {
+
+constexpr int kMaxNl = 20, kMaxQp = 36;
+
+template <class T, int DIM>
+__global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, int form,
+                                                    const uint32_t *__restrict__ edofs,
+                                                    const T *__restrict__ qdata,
+                                                    const double *__restrict__ tab,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    const int *skip) {
+    // tab: [nq] weights, [nq][nl] phi, [nq][nl][DIM] grad (reference element)
+    __shared__ T s_tab[kMaxQp * (1 + kMaxNl * (1 + 3))];
+    if (skip && *skip) return;
+    const int tab_len = nq * (1 + nl * (1 + DIM));
+    for (int i = threadIdx.x; i < tab_len; i += blockDim.x) s_tab[i] = (T)tab[i];
+    __syncthreads();
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const T *W = s_tab, *PHI = s_tab + nq, *GR = s_tab + nq + nq * nl;
+    T xe[kMaxNl], ye[kMaxNl];
+    for (int l = 0; l < nl; ++l) { xe[l] = x[edofs[e * nl + l]]; ye[l] = 0; }
+    constexpr int NS = DIM == 2 ? 3 : 6;
+    if (form == FEMGPU_DIFFUSION) {
+        T D[NS];
+        for (int c = 0; c < NS; ++c) D[c] = qdata[e * NS + c];
+        for (int q = 0; q < nq; ++q) {
+            const T *G = GR + (size_t)q * nl * DIM;
+            T gu[3] = {0, 0, 0};
+            for (int l = 0; l < nl; ++l)
+                for (int d = 0; d < DIM; ++d) gu[d] += xe[l] * G[l * DIM + d];
+            T v[3];
+            if (DIM == 2) {
+                v[0] = D[0] * gu[0] + D[1] * gu[1];
+                v[1] = D[1] * gu[0] + D[2] * gu[1];
+            } else {
+                v[0] = D[0] * gu[0] + D[1] * gu[1] + D[2] * gu[2];
+                v[1] = D[1] * gu[0] + D[3] * gu[1] + D[4] * gu[2];
+                v[2] = D[2] * gu[0] + D[4] * gu[1] + D[5] * gu[2];
+            }
+            for (int i = 0; i < nl; ++i) {
+                T s = 0;
+                for (int d = 0; d < DIM; ++d) s += G[i * DIM + d] * v[d];
+                ye[i] += W[q] * s;
+            }
+        }
+    } else {
+        const T m = qdata[e];
+        for (int q = 0; q < nq; ++q) {
+            const T *ph = PHI + (size_t)q * nl;
+            T u = 0;
+            for (int l = 0; l < nl; ++l) u += ph[l] * xe[l];
+            const T wu = W[q] * m * u;
+            for (int i = 0; i < nl; ++i) ye[i] += wu * ph[i];
+        }
+    }
+    for (int l = 0; l < nl; ++l) unsafeAtomicAdd(&y[edofs[e * nl + l]], ye[l]);
+}
+
+template <class T>
+__global__ void k_diag_scale(size_t n, const T *__restrict__ d, const T *__restrict__ x,
+                             T *__restrict__ y, int mode, const int *skip) {
+    if (skip && *skip) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (mode == 0) y[i] = d[i] * x[i];
+        else if (mode == 1) y[i] += d[i] * x[i];
+        else y[i] = fabs((double)d[i]) > 1e-14 ? x[i] / d[i] : T(0);
+    }
+}
+
+static unsigned blocks_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > (1u << 20)) b = 1u << 20;
+    return (unsigned)(b ? b : 1);
+}
+
+// Host-side reference-element tables and per-element geometry.
+struct SimplexSetup {
+    int dim, order, nl, nq;
+    std::vector<double> tab;  // see kernel
+    std::vector<double> w, phi, grad;
+
+    SimplexSetup(int dim_, int order_, int quad_order) : dim(dim_), order(order_) {
+        FG_REQUIRE(dim == 2 || dim == 3, FEMGPU_ERR_INVALID_ARG, "simplex: dim must be 2 or 3");
+        FG_REQUIRE(order >= 1 && order <= 3, FEMGPU_ERR_UNSUPPORTED, "simplex: order must be 1..3");
+        nl = simplex_nlocal(dim, order);
+        double pts[120], wts[40];
+        nq = dim == 2 ? tri_rule(quad_order, pts, wts) : tet_rule(quad_order, pts, wts);
+        w.assign(wts, wts + nq);
+        phi.assign((size_t)nq * nl, 0.0);
+        grad.assign((size_t)nq * nl * dim, 0.0);
+        for (int q = 0; q < nq; ++q)
+            simplex_basis(dim, order, pts + dim * q, &phi[(size_t)q * nl], &grad[(size_t)q * nl * dim]);
+        tab = w;
+        tab.insert(tab.end(), phi.begin(), phi.end());
+        tab.insert(tab.end(), grad.begin(), grad.end());
+    }
+
+    // Affine map of ElementTransformation::from_simplex_nodes
+    // (mesh/src/transformation.rs:34-64): J columns x_k - x_0.
+    double jac(const double *coords, const uint32_t *nodes, double J[3][3], double I[3][3]) const {
+        for (int c = 0; c < dim; ++c)
+            for (int r = 0; r < dim; ++r)
+                J[r][c] = coords[dim * nodes[c + 1] + r] - coords[dim * nodes[0] + r];
+        double det;
+        if (dim == 2) {
+            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            I[0][0] = J[1][1] / det; I[0][1] = -J[0][1] / det;
+            I[1][0] = -J[1][0] / det; I[1][1] = J[0][0] / det;
+        } else {
+            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            const double id = 1.0 / det;
+            I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+            I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+            I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+            I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+            I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+            I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+            I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+            I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+            I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+        }
+        return det;
+    }
+};
+
+struct SimplexOp final : femgpu_op {
+    int dim = 2, form = FEMGPU_DIFFUSION, nl = 0, nq = 0;
+    size_t ne = 0;
+    DevBuf edofs, qdata, tab, diag;
+
+    const char *name() const override { return "pa_simplex"; }
+
+    template <class T> void apply_t(const T *x, T *y, bool acc, const int *skip) {
+        hipStream_t s = ctx->stream;
+        if (!acc) fill<T>(s, y, T(0), n, skip);
+        if (!ne) return;
+        if (dim == 2)
+            hipLaunchKernelGGL((k_pa_simplex<T, 2>), dim3((ne + 255) / 256), dim3(256), 0, s, ne, nl, nq,
+                               form, edofs.as<uint32_t>(), qdata.as<T>(), tab.as<double>(), x, y, skip);
+        else
+            hipLaunchKernelGGL((k_pa_simplex<T, 3>), dim3((ne + 255) / 256), dim3(256), 0, s, ne, nl, nq,
+                               form, edofs.as<uint32_t>(), qdata.as<T>(), tab.as<double>(), x, y, skip);
+    }
+    void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
+        if (dtype == FEMGPU_F64) apply_t<double>((const double *)x, (double *)y, acc, skip);
+        else apply_t<float>((const float *)x, (float *)y, acc, skip);
+    }
+    void diagonal_dev(void *d) override {
+        FG_HIP(hipMemcpyAsync(d, diag.ptr, n * esize(), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+};
+
+struct LumpedOp final : femgpu_op {
+    DevBuf diag;
+    const char *name() const override { return "lumped_mass"; }
+    template <class T> void run(const T *x, T *y, int mode, const int *skip) {
+        hipLaunchKernelGGL(k_diag_scale<T>, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, (size_t)n,
+                           diag.as<T>(), x, y, mode, skip);
+    }
+    void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
+        if (dtype == FEMGPU_F64) run<double>((const double *)x, (double *)y, acc ? 1 : 0, skip);
+        else run<float>((const float *)x, (float *)y, acc ? 1 : 0, skip);
+    }
+    void diagonal_dev(void *d) override {
+        FG_HIP(hipMemcpyAsync(d, diag.ptr, n * esize(), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+};
+
+static void check_mesh(int dim, int order, uint64_t n_nodes, const double *coords, uint64_t n_elem,
+                       const uint32_t *nodes, const uint32_t *edofs, uint64_t n_dofs, int nl) {
+    FG_REQUIRE(coords && nodes && edofs, FEMGPU_ERR_INVALID_ARG, "simplex: NULL mesh array");
+    (void)order; (void)dim;
+    for (uint64_t i = 0; i < n_elem * (uint64_t)(dim + 1); ++i)
+        FG_REQUIRE(nodes[i] < n_nodes, FEMGPU_ERR_INVALID_ARG,
+                   "simplex: elem_nodes[" + std::to_string(i) + "] out of range");
+    for (uint64_t i = 0; i < n_elem * (uint64_t)nl; ++i)
+        FG_REQUIRE(edofs[i] < n_dofs, FEMGPU_ERR_INVALID_ARG,
+                   "simplex: elem_dofs[" + std::to_string(i) + "] out of range");
+}
+
+template <class T> static void upload(femgpu_ctx *c, DevBuf &b, const std::vector<double> &h) {
+    std::vector<T> t(h.begin(), h.end());
+    b.alloc(t.size() * sizeof(T));
+    if (!t.empty()) FG_HIP(hipMemcpyAsync(b.ptr, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                             const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                             const uint32_t *edofs, uint64_t n_dofs, int form, double coef,
+                             int quad_order, femgpu_op **out) {
+    return guard([&] {
+        FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "pa_create_simplex: NULL argument");
+        FG_REQUIRE(form == FEMGPU_MASS || form == FEMGPU_DIFFUSION, FEMGPU_ERR_INVALID_ARG,
+                   "pa_create_simplex: form must be MASS or DIFFUSION");
+        SimplexSetup S(dim, order, quad_order);
+        FG_REQUIRE(S.nq <= kMaxQp, FEMGPU_ERR_UNSUPPORTED, "pa_create_simplex: too many points");
+        check_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        auto *op = new SimplexOp();
+        try {
+            op->ctx = c; op->dtype = FEMGPU_F64; op->n = n_dofs; op->dim = dim; op->form = form;
+            op->nl = S.nl; op->nq = S.nq; op->ne = n_elem;
+            const int ns = dim == 2 ? 3 : 6;
+            std::vector<double> qd((size_t)n_elem * (form == FEMGPU_DIFFUSION ? ns : 1));
+            std::vector<double> dg(n_dofs, 0.0);
+            for (uint64_t e = 0; e < n_elem; ++e) {
+                double J[3][3], I[3][3];
+                const double det = S.jac(coords, nodes + e * (dim + 1), J, I);
+                const double s = coef * std::fabs(det);
+                double D[6] = {0, 0, 0, 0, 0, 0};
+                if (form == FEMGPU_DIFFUSION) {
+                    int k = 0;
+                    for (int a = 0; a < dim; ++a)
+                        for (int b = a; b < dim; ++b) {
+                            double v = 0.0;
+                            for (int m = 0; m < dim; ++m) v += I[a][m] * I[b][m];
+                            D[k] = s * v;
+                            qd[e * ns + k] = D[k];
+                            ++k;
+                        }
+                } else {
+                    qd[e] = s;
+                }
+                // assembled diagonal for Jacobi
+                for (int q = 0; q < S.nq; ++q)
+                    for (int i = 0; i < S.nl; ++i) {
+                        double v;
+                        if (form == FEMGPU_DIFFUSION) {
+                            const double *g = &S.grad[((size_t)q * S.nl + i) * dim];
+                            if (dim == 2) v = D[0] * g[0] * g[0] + 2 * D[1] * g[0] * g[1] + D[2] * g[1] * g[1];
+                            else v = D[0] * g[0] * g[0] + D[3] * g[1] * g[1] + D[5] * g[2] * g[2] +
+                                     2 * (D[1] * g[0] * g[1] + D[2] * g[0] * g[2] + D[4] * g[1] * g[2]);
+                        } else {
+                            const double ph = S.phi[(size_t)q * S.nl + i];
+                            v = s * ph * ph;
+                        }
+                        dg[edofs[e * S.nl + i]] += S.w[q] * v;
+                    }
+            }
+            upload<double>(c, op->qdata, qd);
+            upload<double>(c, op->tab, S.tab);
+            upload<double>(c, op->diag, dg);
+            op->edofs.alloc(n_elem * S.nl * 4);
+            if (n_elem)
+                FG_HIP(hipMemcpyAsync(op->edofs.ptr, edofs, n_elem * S.nl * 4, hipMemcpyHostToDevice, c->stream));
+            FG_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+        return 0;
+    });
+}
+
+int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                              const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                              const uint32_t *edofs, uint64_t n_dofs, double rho, int quad_order,
+                              femgpu_op **out) {
+    return guard([&] {
+        FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "lumped_mass_create: NULL argument");
+        SimplexSetup S(dim, order, quad_order);
+        check_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        // LumpedMassOperator::assemble (partial.rs:228-262): M_ii = sum_q w|J| rho phi_i sum_j phi_j
+        std::vector<double> dg(n_dofs, 0.0);
+        for (uint64_t e = 0; e < n_elem; ++e) {
+            double J[3][3], I[3][3];
+            const double det = S.jac(coords, nodes + e * (dim + 1), J, I);
+            for (int q = 0; q < S.nq; ++q) {
+                const double w = S.w[q] * std::fabs(det);
+                const double *ph = &S.phi[(size_t)q * S.nl];
+                double sum = 0.0;
+                for (int i = 0; i < S.nl; ++i) sum += ph[i];
+                for (int i = 0; i < S.nl; ++i) dg[edofs[e * S.nl + i]] += w * rho * ph[i] * sum;
+            }
+        }
+        auto *op = new LumpedOp();
+        try {
+            op->ctx = c; op->dtype = FEMGPU_F64; op->n = n_dofs;
+            upload<double>(c, op->diag, dg);
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+        return 0;
+    });
+}
+
+int femgpu_lumped_mass_apply_inverse(femgpu_op *op_, const void *x, void *y, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<LumpedOp *>(op_);
+        FG_REQUIRE(op && x && y, FEMGPU_ERR_INVALID_ARG, "lumped_mass_apply_inverse: not a lumped mass");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * 8;
+        const void *xd = x;
+        void *yd = y;
+        if (mem == FEMGPU_HOST) {
+            op->stage_x.ensure(bytes); op->stage_y.ensure(bytes);
+            FG_HIP(hipMemcpyAsync(op->stage_x.ptr, x, bytes, hipMemcpyHostToDevice, c->stream));
+            xd = op->stage_x.ptr; yd = op->stage_y.ptr;
+        }
+        op->run<double>((const double *)xd, (double *)yd, 2, nullptr);
+        FG_HIP(hipGetLastError());
+        if (mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(y, yd, bytes, hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+// Context, memory, errors and the operator-generic C entry points.
+#include "internal.hpp"
+#include "philox.hpp"
+
+#include <cstdlib>
+#include <cstring>
+
+namespace femgpu {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    if (std::getenv("FEMGPU_VERBOSE")) std::fprintf(stderr, "femgpu: %s\n", msg.c_str());
+    return code;
+}
+
+// Counter (index lo, index hi, stream, 0), key = seed; 53 random bits.
+double philox_uniform(uint64_t seed, uint32_t stream, uint64_t index) {
+    return philox_u01(seed, stream, index);
+}
+
+void hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, double *c) {
+    const double hx = 1.0 / nx, hy = 1.0 / ny, hz = 1.0 / nz;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const size_t v = (size_t)i + (size_t)(nx + 1) * ((size_t)j + (size_t)(ny + 1) * k);
+                double x = i * hx, y = j * hy, z = k * hz;
+                const bool interior = i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
+                if (interior && amp != 0.0) {
+                    x += amp * hx * (2.0 * philox_uniform(seed, 1, v) - 1.0);
+                    y += amp * hy * (2.0 * philox_uniform(seed, 2, v) - 1.0);
+                    z += amp * hz * (2.0 * philox_uniform(seed, 3, v) - 1.0);
+                }
+                c[3 * v] = x; c[3 * v + 1] = y; c[3 * v + 2] = z;
+            }
+}
+
+}  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+const char *femgpu_last_error(void) { return g_last_error.c_str(); }
+
+const char *femgpu_version(void) { return "femgpu 0.1 (gfx950)"; }
+
+int femgpu_ctx_create(int device, femgpu_ctx **out) {
+    return guard([&] {
+        FG_REQUIRE(out, FEMGPU_ERR_INVALID_ARG, "ctx_create: out is NULL");
+        int ndev = 0;
+        FG_HIP(hipGetDeviceCount(&ndev));
+        FG_REQUIRE(device >= 0 && device < ndev, FEMGPU_ERR_INVALID_ARG,
+                   "ctx_create: device " + std::to_string(device) + " not present (" +
+                       std::to_string(ndev) + " devices)");
+        FG_HIP(hipSetDevice(device));
+        auto *c = new femgpu_ctx();
+        c->device = device;
+        try {
+            FG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+            c->partials.alloc(sizeof(double) * kRedBlocks * 8);
+            c->scalars.alloc(sizeof(double) * 64);
+            FG_HIP(hipMemset(c->scalars.ptr, 0, c->scalars.bytes));
+            c->flags.alloc(sizeof(int) * 16);
+            FG_HIP(hipMemset(c->flags.ptr, 0, c->flags.bytes));
+            FG_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * 64, hipHostMallocDefault));
+            FG_HIP(hipEventCreate(&c->t0));
+            FG_HIP(hipEventCreate(&c->t1));
+        } catch (...) {
+            femgpu_ctx_destroy(c);
+            throw;
+        }
+        *out = c;
+        return 0;
+    });
+}
+
+int femgpu_ctx_destroy(femgpu_ctx *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->partials.release();
+    c->scalars.release();
+    c->flags.release();
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int femgpu_ctx_synchronize(femgpu_ctx *c) {
+    return guard([&] {
+        FG_REQUIRE(c, FEMGPU_ERR_INVALID_ARG, "ctx is NULL");
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int femgpu_malloc(femgpu_ctx *c, size_t bytes, void **dptr) {
+    return guard([&] {
+        FG_REQUIRE(c && dptr, FEMGPU_ERR_INVALID_ARG, "malloc: NULL argument");
+        FG_HIP(hipSetDevice(c->device));
+        hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+        FG_REQUIRE(e == hipSuccess, FEMGPU_ERR_OUT_OF_MEMORY,
+                   "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return 0;
+    });
+}
+
+int femgpu_free(femgpu_ctx *c, void *p) {
+    return guard([&] {
+        FG_REQUIRE(c, FEMGPU_ERR_INVALID_ARG, "free: ctx is NULL");
+        if (p) {
+            FG_HIP(hipStreamSynchronize(c->stream));
+            FG_HIP(hipFree(p));
+        }
+        return 0;
+    });
+}
+
+int femgpu_memcpy(femgpu_ctx *c, void *dst, const void *src, size_t bytes, int kind) {
+    return guard([&] {
+        FG_REQUIRE(c && (bytes == 0 || (dst && src)), FEMGPU_ERR_INVALID_ARG, "memcpy: NULL");
+        hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                        : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int femgpu_timer_start(femgpu_ctx *c) {
+    return guard([&] {
+        FG_REQUIRE(c, FEMGPU_ERR_INVALID_ARG, "ctx is NULL");
+        FG_HIP(hipEventRecord(c->t0, c->stream));
+        return 0;
+    });
+}
+
+int femgpu_timer_stop(femgpu_ctx *c, double *ms) {
+    return guard([&] {
+        FG_REQUIRE(c && ms, FEMGPU_ERR_INVALID_ARG, "timer_stop: NULL");
+        FG_HIP(hipEventRecord(c->t1, c->stream));
+        FG_HIP(hipEventSynchronize(c->t1));
+        float f = 0.f;
+        FG_HIP(hipEventElapsedTime(&f, c->t0, c->t1));
+        *ms = f;
+        return 0;
+    });
+}
+
+int femgpu_hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, double *coords) {
+    return guard([&] {
+        FG_REQUIRE(nx > 0 && ny > 0 && nz > 0 && coords, FEMGPU_ERR_INVALID_ARG,
+                   "hex_vertices: bad arguments");
+        hex_vertices(nx, ny, nz, amp, seed, coords);
+        return 0;
+    });
+}
+
+// --------------------------------------------------- generic operators --
+int femgpu_op_apply(femgpu_op *op, const void *x, void *y, int accumulate, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(op && x && y, FEMGPU_ERR_INVALID_ARG, "op_apply: NULL argument");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * op->esize();
+        if (mem == FEMGPU_DEVICE) {
+            op->apply_dev(x, y, accumulate != 0, nullptr);
+            FG_HIP(hipGetLastError());
+            FG_HIP(hipStreamSynchronize(c->stream));
+            return 0;
+        }
+        op->stage_x.ensure(bytes);
+        op->stage_y.ensure(bytes);
+        FG_HIP(hipMemcpyAsync(op->stage_x.ptr, x, bytes, hipMemcpyHostToDevice, c->stream));
+        if (accumulate)
+            FG_HIP(hipMemcpyAsync(op->stage_y.ptr, y, bytes, hipMemcpyHostToDevice, c->stream));
+        op->apply_dev(op->stage_x.ptr, op->stage_y.ptr, accumulate != 0, nullptr);
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipMemcpyAsync(y, op->stage_y.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int femgpu_op_n_dofs(const femgpu_op *op, uint64_t *n) {
+    return guard([&] {
+        FG_REQUIRE(op && n, FEMGPU_ERR_INVALID_ARG, "op_n_dofs: NULL");
+        *n = op->n;
+        return 0;
+    });
+}
+
+int femgpu_op_dtype(const femgpu_op *op, femgpu_dtype *d) {
+    return guard([&] {
+        FG_REQUIRE(op && d, FEMGPU_ERR_INVALID_ARG, "op_dtype: NULL");
+        *d = op->dtype;
+        return 0;
+    });
+}
+
+int femgpu_op_diagonal(femgpu_op *op, void *d, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(op && d, FEMGPU_ERR_INVALID_ARG, "op_diagonal: NULL");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * op->esize();
+        if (mem == FEMGPU_DEVICE) {
+            op->diagonal_dev(d);
+        } else {
+            op->stage_y.ensure(bytes);
+            op->diagonal_dev(op->stage_y.ptr);
+            FG_HIP(hipMemcpyAsync(d, op->stage_y.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+        }
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int femgpu_op_time_apply(femgpu_op *op, const void *x, void *y, int reps, double *ms_apply,
+                         double *ms_kernel) {
+    return guard([&] {
+        FG_REQUIRE(op && x && y && reps > 0, FEMGPU_ERR_INVALID_ARG, "op_time_apply: bad argument");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        std::vector<hipEvent_t> ev(2 * reps);
+        for (auto &e : ev) FG_HIP(hipEventCreate(&e));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        FG_HIP(hipEventRecord(c->t0, c->stream));
+        for (int r = 0; r < reps; ++r) {
+            op->prof_begin = ev[2 * r];
+            op->prof_end = ev[2 * r + 1];
+            op->apply_dev(x, y, false, nullptr);
+        }
+        op->prof_begin = op->prof_end = nullptr;
+        FG_HIP(hipEventRecord(c->t1, c->stream));
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipEventSynchronize(c->t1));
+        float tot = 0.f;
+        FG_HIP(hipEventElapsedTime(&tot, c->t0, c->t1));
+        double kern = 0.0;
+        bool have_kernel = true;
+        for (int r = 0; r < reps; ++r) {
+            float f = 0.f;
+            if (hipEventElapsedTime(&f, ev[2 * r], ev[2 * r + 1]) != hipSuccess) { have_kernel = false; break; }
+            kern += f;
+        }
+        (void)hipGetLastError();
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (ms_apply) *ms_apply = tot / reps;
+        if (ms_kernel) *ms_kernel = have_kernel ? kern / reps : -1.0;
+        return 0;
+    });
+}
+
+int femgpu_op_destroy(femgpu_op *op) {
+    if (!op) return 0;
+    (void)hipSetDevice(op->ctx->device);
+    (void)hipStreamSynchronize(op->ctx->stream);
+    delete op;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,605 @@
+"""Python mirror of the fem-rs solve-path API over the libfemgpu C ABI.
+
+Every name here follows the reference crate it stands in for (fem-rs, Rust):
+
+  CsrMatrix.spmv / spmv_add / diagonal     linalg/src/csr.rs:140-252
+  PAMassOperator, PADiffusionOperator      assembly/src/partial.rs:45-209
+  LumpedMassOperator                       assembly/src/partial.rs:217-274
+  SolverConfig, SolveResult, SolverError   solver/src/lib.rs:47-142
+  solve_cg_operator, solve_pcg_jacobi      solver/src/lib.rs:198-297
+  chebyshev_smooth, estimate_spectral_radius  amg/src/lib.rs:53-123
+
+All arithmetic runs in the HIP kernels of ``libfemgpu.so`` (gfx950); this
+module only marshals arguments through ctypes.  There is no CPU fallback:
+importing fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfemgpu.so")
+
+F64, F32 = 0, 1
+HOST, DEVICE = 0, 1
+MASS, DIFFUSION, DIFF_MASS = 1, 2, 3
+COEF_CONST, COEF_SINPROD, COEF_GRF, COEF_QP_ARRAY = 0, 1, 2, 3
+
+OK, ERR_INVALID_ARG, ERR_DIM_MISMATCH, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_HIP, \
+    ERR_UNSUPPORTED, ERR_OOM = range(8)
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libfemgpu.so not found at {LIB_PATH}: build it with `make -C fem-rs_amd` "
+        "(or __graft_entry__.build()); there is no CPU fallback")
+
+lib = C.CDLL(LIB_PATH)
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+U64 = C.c_uint64
+Z = C.c_size_t
+
+
+class _Coef(C.Structure):
+    _fields_ = [("kind", I), ("value", D), ("n_features", I), ("sigma", D), ("length", D),
+                ("lo", D), ("hi", D), ("seed", U64), ("qp_values", P)]
+
+
+class _HexDesc(C.Structure):
+    _fields_ = [("p", I), ("q", I), ("nx", I), ("ny", I), ("nz", I), ("form", I), ("dtype", I),
+                ("dirichlet", I), ("rho", D), ("kappa", _Coef)]
+
+
+class _SolverConfig(C.Structure):
+    _fields_ = [("rtol", D), ("atol", D), ("max_iter", U64), ("verbose", I), ("print_level", I)]
+
+
+class _SolveResult(C.Structure):
+    _fields_ = [("converged", I), ("iterations", U64), ("final_residual", D),
+                ("final_residual_is_relative", I), ("error_estimate", D)]
+
+
+class _ChebConfig(C.Structure):
+    _fields_ = [("degree", I), ("ratio", D), ("safety", D), ("power_iters", I), ("lmax", D)]
+
+
+class _IrConfig(C.Structure):
+    _fields_ = [("inner_rtol", D), ("inner_max_iter", U64), ("max_outer", U64),
+                ("cheb", _ChebConfig)]
+
+
+def _sig(name, args, res=I):
+    f = getattr(lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("femgpu_last_error", [], C.c_char_p)
+_sig("femgpu_version", [], C.c_char_p)
+_sig("femgpu_ctx_create", [I, C.POINTER(P)])
+_sig("femgpu_ctx_destroy", [P])
+_sig("femgpu_ctx_synchronize", [P])
+_sig("femgpu_malloc", [P, Z, C.POINTER(P)])
+_sig("femgpu_free", [P, P])
+_sig("femgpu_memcpy", [P, P, P, Z, I])
+_sig("femgpu_timer_start", [P])
+_sig("femgpu_timer_stop", [P, C.POINTER(D)])
+_sig("femgpu_op_apply", [P, P, P, I, I])
+_sig("femgpu_op_n_dofs", [P, C.POINTER(U64)])
+_sig("femgpu_op_dtype", [P, C.POINTER(I)])
+_sig("femgpu_op_diagonal", [P, P, I])
+_sig("femgpu_op_destroy", [P])
+_sig("femgpu_op_time_apply", [P, P, P, I, C.POINTER(D), C.POINTER(D)])
+_sig("femgpu_csr_create", [P, U64, U64, P, P, P, I, C.POINTER(P)])
+_sig("femgpu_csr_spmv_add", [P, D, P, D, P, I])
+_sig("femgpu_pa_create_hex", [P, C.POINTER(_HexDesc), P, C.POINTER(P)])
+_sig("femgpu_hex_vertices", [I, I, I, D, U64, P])
+_sig("femgpu_pa_hex_rhs_unit", [P, P, I])
+_sig("femgpu_pa_hex_rhs_random", [P, U64, P, I])
+_sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
+_sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
+_sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
+_sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
+_sig("femgpu_solve_cg_operator", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
+_sig("femgpu_solve_pcg_jacobi", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
+_sig("femgpu_solve_pcg_chebyshev", [P, C.POINTER(_ChebConfig), P, P, I, C.POINTER(_SolverConfig),
+                                    C.POINTER(_SolveResult), C.POINTER(D)])
+_sig("femgpu_chebyshev_smooth", [P, P, P, D, D, I, I])
+_sig("femgpu_estimate_spectral_radius", [P, I, C.POINTER(D)])
+_sig("femgpu_solve_ir_mixed", [P, P, C.POINTER(_IrConfig), P, P, I, C.POINTER(_SolverConfig),
+                               C.POINTER(_SolveResult), C.POINTER(U64)])
+
+
+# ---------------------------------------------------------------- errors --
+class FemGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[femgpu {code}] {msg}")
+        self.code = code
+
+
+class SolverError(FemGpuError):
+    """solver/src/lib.rs:47-55"""
+
+
+class ConvergenceFailed(SolverError):
+    def __init__(self, max_iter, residual, result=None):
+        RuntimeError.__init__(self, f"solver did not converge in {max_iter} iterations "
+                                    f"(residual = {residual:.3e})")
+        self.code = ERR_NOT_CONVERGED
+        self.max_iter, self.residual, self.result = max_iter, residual, result
+
+
+class DimensionMismatch(SolverError):
+    def __init__(self, rows, cols, rhs):
+        RuntimeError.__init__(self, f"dimension mismatch: matrix is {rows}x{cols}, rhs has length {rhs}")
+        self.code = ERR_DIM_MISMATCH
+        self.rows, self.cols, self.rhs = rows, cols, rhs
+
+
+class Linger(SolverError):
+    def __init__(self, msg):
+        RuntimeError.__init__(self, f"linger error: {msg}")
+        self.code = ERR_BREAKDOWN
+        self.msg = msg
+
+
+def _check(rc):
+    if rc != OK:
+        raise FemGpuError(rc, lib.femgpu_last_error().decode())
+
+
+def version():
+    return lib.femgpu_version().decode()
+
+
+# --------------------------------------------------------------- context --
+class Context:
+    """One HIP stream + solver scratch on one GPU (femgpu_ctx)."""
+
+    def __init__(self, device=0):
+        h = P()
+        _check(lib.femgpu_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def synchronize(self):
+        _check(lib.femgpu_ctx_synchronize(self.h))
+
+    def timer_start(self):
+        _check(lib.femgpu_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = D()
+        _check(lib.femgpu_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.femgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("FEMGPU_DEVICE", "0")))
+    return _default_ctx
+
+
+class DeviceArray:
+    """A 1-D vector resident in HBM (for FEMGPU_DEVICE calls)."""
+
+    def __init__(self, ctx, n, dtype=np.float64, data=None):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        h = P()
+        _check(lib.femgpu_malloc(ctx.h, self.nbytes, C.byref(h)))
+        self.ptr = h
+        if data is not None:
+            self.upload(data)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        _check(lib.femgpu_memcpy(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes, 0))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib.femgpu_memcpy(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def copy_from(self, other):
+        _check(lib.femgpu_memcpy(self.ctx.h, self.ptr, other.ptr, self.nbytes, 2))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib.femgpu_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _npdt(dtype):
+    return np.float64 if dtype == F64 else np.float32
+
+
+def _vec(a, dtype, n, what):
+    if isinstance(a, DeviceArray):
+        if a.n != n:
+            raise DimensionMismatch(n, n, a.n)
+        return a.ptr, DEVICE
+    if a.dtype != _npdt(dtype) or not a.flags.c_contiguous:
+        raise TypeError(f"{what}: expected contiguous {np.dtype(_npdt(dtype))} array")
+    return a.ctypes.data, HOST
+
+
+# ------------------------------------------------------------- operators --
+class Operator:
+    """A square device operator (femgpu_op)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+        n = U64()
+        _check(lib.femgpu_op_n_dofs(handle, C.byref(n)))
+        dt = I()
+        _check(lib.femgpu_op_dtype(handle, C.byref(dt)))
+        self._n, self.dtype = n.value, dt.value
+
+    def n_dofs(self):
+        return self._n
+
+    @property
+    def nrows(self):
+        return self._n
+
+    @property
+    def ncols(self):
+        return self._n
+
+    def _apply(self, x, y, accumulate):
+        if not isinstance(x, DeviceArray) and len(x) != self._n:
+            raise DimensionMismatch(self._n, self._n, len(x))
+        if not isinstance(y, DeviceArray) and len(y) != self._n:
+            raise DimensionMismatch(self._n, self._n, len(y))
+        xp, mx = _vec(x, self.dtype, self._n, "x")
+        yp, my = _vec(y, self.dtype, self._n, "y")
+        if mx != my:
+            raise TypeError("x and y must both be host arrays or both DeviceArrays")
+        _check(lib.femgpu_op_apply(self.h, xp, yp, 1 if accumulate else 0, mx))
+
+    def apply(self, x, y):
+        """MatFreeOperator::apply: y += A x (partial.rs:45-47)."""
+        self._apply(x, y, True)
+
+    def apply_zero(self, x, y):
+        """MatFreeOperator::apply_zero: y = A x (partial.rs:50-53)."""
+        self._apply(x, y, False)
+
+    def matvec(self, x):
+        y = np.zeros(self._n, _npdt(self.dtype))
+        self.apply_zero(np.ascontiguousarray(x, _npdt(self.dtype)), y)
+        return y
+
+    def diagonal(self):
+        d = np.zeros(self._n, _npdt(self.dtype))
+        _check(lib.femgpu_op_diagonal(self.h, d.ctypes.data, HOST))
+        return d
+
+    def time_apply(self, x, y, reps=20):
+        """(ms per apply, ms per main compute kernel) over reps device applies."""
+        a, k = D(), D()
+        _check(lib.femgpu_op_time_apply(self.h, x.ptr, y.ptr, reps, C.byref(a), C.byref(k)))
+        return a.value, k.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.femgpu_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CsrMatrix(Operator):
+    """CsrMatrix<T> (linalg/src/csr.rs:81-93) resident on the GPU."""
+
+    def __init__(self, row_ptr, col_idx, values, ncols=None, dtype=F64, ctx=None):
+        ctx = ctx or default_context()
+        rp = np.ascontiguousarray(row_ptr, np.uint64)
+        ci = np.ascontiguousarray(col_idx, np.uint32)
+        v = np.ascontiguousarray(values, _npdt(dtype))
+        nrows = len(rp) - 1
+        ncols = nrows if ncols is None else ncols
+        h = P()
+        _check(lib.femgpu_csr_create(ctx.h, nrows, ncols, rp.ctypes.data, ci.ctypes.data,
+                                     v.ctypes.data, dtype, C.byref(h)))
+        super().__init__(ctx, h)
+        self._nnz = int(rp[-1])
+
+    def nnz(self):
+        return self._nnz
+
+    def spmv(self, x, y):
+        """y = A x (csr.rs:140-164); a length mismatch raises (the reference panics)."""
+        self.apply_zero(x, y)
+
+    def spmv_add(self, alpha, x, beta, y):
+        """y = alpha A x + beta y (csr.rs:167-193)."""
+        if not isinstance(x, DeviceArray) and (len(x) != self._n or len(y) != self._n):
+            raise DimensionMismatch(self._n, self._n, len(x))
+        xp, m = _vec(x, self.dtype, self._n, "x")
+        yp, _ = _vec(y, self.dtype, self._n, "y")
+        _check(lib.femgpu_csr_spmv_add(self.h, alpha, xp, beta, yp, m))
+
+
+def _coef_struct(kappa, qp_keep):
+    if isinstance(kappa, _Coef):
+        return kappa
+    if kappa is None:
+        return _Coef(COEF_CONST, 1.0, 0, 0.0, 0.0, 0.0, 0.0, 0, None)
+    if np.isscalar(kappa):
+        return _Coef(COEF_CONST, float(kappa), 0, 0.0, 0.0, 0.0, 0.0, 0, None)
+    if isinstance(kappa, str) and kappa == "sinprod":
+        return _Coef(COEF_SINPROD, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0, None)
+    if isinstance(kappa, dict) and kappa.get("kind") == "grf":
+        return _Coef(COEF_GRF, 0.0, int(kappa.get("n_features", 256)), float(kappa.get("sigma", 1.5)),
+                     float(kappa.get("length", 0.1)), float(kappa.get("lo", 1e-2)),
+                     float(kappa.get("hi", 1e2)), int(kappa.get("seed", 0)), None)
+    arr = np.ascontiguousarray(kappa, np.float64)
+    qp_keep.append(arr)
+    return _Coef(COEF_QP_ARRAY, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0, arr.ctypes.data)
+
+
+def grf(seed, n_features=256, sigma=1.5, length=0.1, lo=1e-2, hi=1e2):
+    """Gaussian-random-field coefficient kappa = clamp(exp(g), lo, hi)."""
+    return dict(kind="grf", seed=seed, n_features=n_features, sigma=sigma, length=length, lo=lo, hi=hi)
+
+
+def hex_vertices(nx, ny, nz, amp=0.0, seed=0):
+    c = np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1))
+    _check(lib.femgpu_hex_vertices(nx, ny, nz, amp, seed, c.ctypes.data))
+    return c
+
+
+class HexPAOperator(Operator):
+    """Sum-factorised PA mass / diffusion / diffusion+mass on a structured hex
+    Q_p mesh (the tensor-product form of PADiffusionOperator / PAMassOperator)."""
+
+    def __init__(self, p, q, n, form=DIFFUSION, kappa=1.0, rho=1.0, dirichlet=False, dtype=F64,
+                 vertices=None, ctx=None, _handle=None):
+        ctx = ctx or default_context()
+        nx, ny, nz = (n, n, n) if np.isscalar(n) else n
+        self.p, self.q, self.shape, self.form = p, q, (nx, ny, nz), form
+        if _handle is None:
+            keep = []
+            desc = _HexDesc(p, q, nx, ny, nz, form, dtype, 1 if dirichlet else 0, float(rho),
+                            _coef_struct(kappa, keep))
+            v = None if vertices is None else np.ascontiguousarray(vertices, np.float64)
+            h = P()
+            _check(lib.femgpu_pa_create_hex(ctx.h, C.byref(desc), None if v is None else v.ctypes.data,
+                                             C.byref(h)))
+            _handle = h
+        super().__init__(ctx, _handle)
+
+    @property
+    def n_elements(self):
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def convert(self, dtype):
+        h = P()
+        _check(lib.femgpu_pa_hex_convert(self.h, dtype, C.byref(h)))
+        return HexPAOperator(self.p, self.q, self.shape, self.form, ctx=self.ctx, _handle=h)
+
+    def rhs_unit(self, out=None):
+        out = np.zeros(self._n, _npdt(self.dtype)) if out is None else out
+        ptr, m = _vec(out, self.dtype, self._n, "b")
+        _check(lib.femgpu_pa_hex_rhs_unit(self.h, ptr, m))
+        return out
+
+    def rhs_random(self, seed, out=None):
+        out = np.zeros(self._n, _npdt(self.dtype)) if out is None else out
+        ptr, m = _vec(out, self.dtype, self._n, "b")
+        _check(lib.femgpu_pa_hex_rhs_random(self.h, seed, ptr, m))
+        return out
+
+
+def _simplex_create(fn, ctx, dim, order, coords, elem_nodes, elem_dofs, n_dofs, *rest):
+    coords = np.ascontiguousarray(coords, np.float64)
+    en = np.ascontiguousarray(elem_nodes, np.uint32)
+    ed = np.ascontiguousarray(elem_dofs, np.uint32)
+    h = P()
+    _check(fn(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data, len(en) // (dim + 1),
+              en.ctypes.data, ed.ctypes.data, n_dofs, *rest, C.byref(h)))
+    return h
+
+
+class PAMassOperator(Operator):
+    """PAMassOperator::new(space, rho, quad_order) (partial.rs:65-131), simplex P1..P3."""
+
+    def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, rho=1.0, quad_order=3,
+                 ctx=None):
+        ctx = ctx or default_context()
+        h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
+                            elem_dofs, n_dofs, MASS, float(rho), quad_order)
+        super().__init__(ctx, h)
+
+
+class PADiffusionOperator(Operator):
+    """PADiffusionOperator::new(space, kappa, quad_order) (partial.rs:138-209), simplex P1..P3."""
+
+    def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, kappa=1.0, quad_order=3,
+                 ctx=None):
+        ctx = ctx or default_context()
+        h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
+                            elem_dofs, n_dofs, DIFFUSION, float(kappa), quad_order)
+        super().__init__(ctx, h)
+
+
+class LumpedMassOperator(Operator):
+    """LumpedMassOperator::assemble(space, rho, quad_order) (partial.rs:217-274)."""
+
+    def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, rho=1.0, quad_order=3,
+                 ctx=None):
+        ctx = ctx or default_context()
+        h = _simplex_create(lib.femgpu_lumped_mass_create, ctx, dim, order, coords, elem_nodes,
+                            elem_dofs, n_dofs, float(rho), quad_order)
+        super().__init__(ctx, h)
+
+    @property
+    def diag(self):
+        return self.diagonal()
+
+    def apply_inverse(self, x, y):
+        xp, m = _vec(x, self.dtype, self._n, "x")
+        yp, _ = _vec(y, self.dtype, self._n, "y")
+        _check(lib.femgpu_lumped_mass_apply_inverse(self.h, xp, yp, m))
+
+
+# --------------------------------------------------------------- solvers --
+@dataclass
+class SolverConfig:
+    """solver/src/lib.rs:97-113"""
+    rtol: float = 1e-8
+    atol: float = 0.0
+    max_iter: int = 1000
+    verbose: bool = False
+    print_level: int = 0
+
+    def _c(self):
+        return _SolverConfig(self.rtol, self.atol, self.max_iter, 1 if self.verbose else 0,
+                             self.print_level)
+
+
+@dataclass
+class SolveResult:
+    """solver/src/lib.rs:71-75"""
+    converged: bool
+    iterations: int
+    final_residual: float
+
+
+def _result(r):
+    return SolveResult(bool(r.converged), int(r.iterations), float(r.final_residual))
+
+
+def _solve_vectors(op, b, x):
+    bp, mb = _vec(b, op.dtype, op.n_dofs(), "b")
+    xp, mx = _vec(x, op.dtype, op.n_dofs(), "x")
+    if mb != mx:
+        raise TypeError("b and x must both be host arrays or both DeviceArrays")
+    return bp, xp, mb
+
+
+def _len(v):
+    return v.n if isinstance(v, DeviceArray) else len(v)
+
+
+def solve_cg_operator(nrows, ncols, op, b, x, cfg=None):
+    """solve_cg_operator (solver/src/lib.rs:198-278): x is the initial guess
+    and is overwritten with the solution; returns SolveResult or raises
+    ConvergenceFailed / Linger (breakdown) / DimensionMismatch."""
+    cfg = cfg or SolverConfig()
+    if nrows != ncols or _len(b) != nrows or _len(x) != ncols or op.n_dofs() != nrows:
+        raise DimensionMismatch(nrows, ncols, _len(b))
+    bp, xp, m = _solve_vectors(op, b, x)
+    r = _SolveResult()
+    rc = lib.femgpu_solve_cg_operator(op.h, bp, xp, m, C.byref(cfg._c()), C.byref(r))
+    if rc == ERR_NOT_CONVERGED:
+        raise ConvergenceFailed(cfg.max_iter, r.final_residual, _result(r))
+    if rc == ERR_BREAKDOWN:
+        raise Linger("CG operator breakdown: p^T A p is near zero")
+    _check(rc)
+    return _result(r)
+
+
+def _check_dims(a, b, x):
+    # check_dims (solver/src/lib.rs:1074-1083)
+    if a.nrows != _len(b) or a.ncols != _len(x):
+        raise DimensionMismatch(a.nrows, a.ncols, _len(b))
+
+
+def solve_pcg_jacobi(a, b, x, cfg=None):
+    """solve_pcg_jacobi (solver/src/lib.rs:281-297) with par_solve_pcg_jacobi
+    semantics (parallel/src/par_solver.rs:262-346)."""
+    cfg = cfg or SolverConfig()
+    _check_dims(a, b, x)
+    bp, xp, m = _solve_vectors(a, b, x)
+    r = _SolveResult()
+    rc = lib.femgpu_solve_pcg_jacobi(a.h, bp, xp, m, C.byref(cfg._c()), C.byref(r))
+    if rc == ERR_NOT_CONVERGED:
+        return _result(r)  # par_solve_pcg_jacobi: Ok(converged = false)
+    _check(rc)
+    return _result(r)
+
+
+def solve_pcg_chebyshev(a, b, x, cfg=None, degree=4, ratio=30.0, safety=1.1, power_iters=20,
+                        lmax=0.0):
+    """PCG with a textbook Chebyshev-Jacobi preconditioner; returns
+    (SolveResult, lmax used)."""
+    cfg = cfg or SolverConfig()
+    _check_dims(a, b, x)
+    bp, xp, m = _solve_vectors(a, b, x)
+    r = _SolveResult()
+    lm = D()
+    cc = _ChebConfig(degree, ratio, safety, power_iters, lmax)
+    rc = lib.femgpu_solve_pcg_chebyshev(a.h, C.byref(cc), bp, xp, m, C.byref(cfg._c()),
+                                        C.byref(r), C.byref(lm))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        _check(rc)
+    return _result(r), lm.value
+
+
+def chebyshev_smooth(a, x, b, lambda_min, lambda_max, n_iter):
+    """chebyshev_smooth (amg/src/lib.rs:53-103); x updated in place."""
+    xp, m = _vec(x, a.dtype, a.n_dofs(), "x")
+    bp, _ = _vec(b, a.dtype, a.n_dofs(), "b")
+    _check(lib.femgpu_chebyshev_smooth(a.h, xp, bp, lambda_min, lambda_max, n_iter, m))
+
+
+def estimate_spectral_radius(a, n_iter):
+    """estimate_spectral_radius (amg/src/lib.rs:108-123)."""
+    rho = D()
+    _check(lib.femgpu_estimate_spectral_radius(a.h, n_iter, C.byref(rho)))
+    return rho.value
+
+
+def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0, max_outer=50,
+                   degree=4, ratio=30.0, safety=1.1, power_iters=20):
+    """FP32-inner / FP64-outer iterative refinement; returns (SolveResult with
+    outer iterations, total inner iterations)."""
+    cfg = cfg or SolverConfig()
+    _check_dims(op64, b, x)
+    bp, xp, m = _solve_vectors(op64, b, x)
+    r = _SolveResult()
+    inner = U64()
+    irc = _IrConfig(inner_rtol, inner_max_iter, max_outer,
+                    _ChebConfig(degree, ratio, safety, power_iters, 0.0))
+    rc = lib.femgpu_solve_ir_mixed(op64.h, op32.h, C.byref(irc), bp, xp, m, C.byref(cfg._c()),
+                                   C.byref(r), C.byref(inner))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        _check(rc)
+    return _result(r), inner.value

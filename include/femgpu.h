@@ -1,0 +1,255 @@
+/*
+ * femgpu.h — C ABI of the MI355X (gfx950) solve path for fem-rs.
+ *
+ * This is the drop-in boundary: every entry point takes plain pointers and
+ * sizes, returns an femgpu_status, and mirrors one reference (fem-rs, Rust)
+ * interface, cited as crate-relative file:line.  A Rust `fem-gpu-sys` crate
+ * (or any other FFI) binds these symbols directly; the bindings are shown in
+ * INTEGRATION.md.
+ *
+ * Conventions
+ *  - Vectors passed to apply/solve calls live in host memory (FEMGPU_HOST;
+ *    the library stages them) or in device memory of the context's GPU
+ *    (FEMGPU_DEVICE; e.g. from femgpu_malloc).  Their element type is the
+ *    handle's dtype.
+ *  - Handles copy every array given at creation time.
+ *  - A handle serialises its own work on its context's stream; separate
+ *    contexts run concurrently.  Calls on one context from several threads
+ *    are serialised by the context's mutex (the reference's Send + Sync).
+ *  - On error the message for the calling thread is available from
+ *    femgpu_last_error().
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- status ---- */
+/* Error mapping to the reference's SolverError (solver/src/lib.rs:47-55):
+ * DIM_MISMATCH -> DimensionMismatch{rows,cols,rhs}; NOT_CONVERGED ->
+ * ConvergenceFailed{max_iter,residual}; BREAKDOWN -> Linger(String).
+ * CsrMatrix::spmv panics on a length mismatch (csr.rs:140-142); the C ABI
+ * returns DIM_MISMATCH instead. */
+typedef enum {
+    FEMGPU_OK = 0,
+    FEMGPU_ERR_INVALID_ARG = 1,
+    FEMGPU_ERR_DIM_MISMATCH = 2,
+    FEMGPU_ERR_NOT_CONVERGED = 3,
+    FEMGPU_ERR_BREAKDOWN = 4,
+    FEMGPU_ERR_HIP = 5,
+    FEMGPU_ERR_UNSUPPORTED = 6,
+    FEMGPU_ERR_OUT_OF_MEMORY = 7
+} femgpu_status;
+
+typedef enum { FEMGPU_F64 = 0, FEMGPU_F32 = 1 } femgpu_dtype;
+typedef enum { FEMGPU_HOST = 0, FEMGPU_DEVICE = 1 } femgpu_mem;
+
+typedef struct femgpu_ctx femgpu_ctx;
+typedef struct femgpu_op femgpu_op;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *femgpu_last_error(void);
+/* Library version string. */
+const char *femgpu_version(void);
+
+/* ------------------------------------------------------------ context ---- */
+/* One context per GPU and caller; owns a HIP stream and solver scratch. */
+int femgpu_ctx_create(int device, femgpu_ctx **out);
+int femgpu_ctx_destroy(femgpu_ctx *ctx);
+int femgpu_ctx_synchronize(femgpu_ctx *ctx);
+/* Device memory helpers (for FEMGPU_DEVICE vectors). kind: 0 = H2D, 1 = D2H,
+ * 2 = D2D.  Copies are ordered on the context stream and complete on return. */
+int femgpu_malloc(femgpu_ctx *ctx, size_t bytes, void **dptr);
+int femgpu_free(femgpu_ctx *ctx, void *dptr);
+int femgpu_memcpy(femgpu_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+/* HIP event timing of everything enqueued between start and stop on the
+ * context stream (milliseconds). */
+int femgpu_timer_start(femgpu_ctx *ctx);
+int femgpu_timer_stop(femgpu_ctx *ctx, double *ms);
+
+/* ---------------------------------------------------------- operators ---- */
+/* Every operator below is a square n x n linear operator y = A x.
+ *
+ * femgpu_op_apply: accumulate = 1 is MatFreeOperator::apply (y += A x,
+ * assembly/src/partial.rs:45-47); accumulate = 0 is apply_zero / LinearOperator
+ * ::apply / CsrMatrix::spmv (y = A x; partial.rs:50-53, backend.rs:44-45,
+ * linalg/src/csr.rs:140-164). */
+int femgpu_op_apply(femgpu_op *op, const void *x, void *y, int accumulate, femgpu_mem mem);
+/* MatFreeOperator::n_dofs (partial.rs:56) / LinearOperator::nrows. */
+int femgpu_op_n_dofs(const femgpu_op *op, uint64_t *n);
+int femgpu_op_dtype(const femgpu_op *op, femgpu_dtype *dtype);
+/* d_i = A_ii (CsrMatrix::diagonal, csr.rs:246-252; for PA operators the
+ * assembled diagonal, computed on the device). */
+int femgpu_op_diagonal(femgpu_op *op, void *d, femgpu_mem mem);
+int femgpu_op_destroy(femgpu_op *op);
+/* Profiling (no reference counterpart; the reference only times with
+ * std::time::Instant in benchmarks): runs reps overwrite applies on device
+ * vectors and
+ * returns the mean time per apply and, for operators with one main compute
+ * kernel (hex PA), the mean duration of that kernel from HIP events recorded
+ * around it on the context stream (-1 if not available). */
+int femgpu_op_time_apply(femgpu_op *op, const void *x, void *y, int reps, double *ms_apply,
+                         double *ms_kernel);
+
+/* CsrMatrix<T> (linalg/src/csr.rs:81-93): row_ptr u64[nrows+1], col_idx
+ * u32[nnz], values T[nnz] (T = f64 or f32 per dtype), exactly the reference
+ * layout.  Arrays are host pointers and are copied. */
+int femgpu_csr_create(femgpu_ctx *ctx, uint64_t nrows, uint64_t ncols, const uint64_t *row_ptr,
+                      const uint32_t *col_idx, const void *values, femgpu_dtype dtype,
+                      femgpu_op **out);
+/* CsrMatrix::spmv_add: y = alpha*(A x) + beta*y (csr.rs:167-193). */
+int femgpu_csr_spmv_add(femgpu_op *csr, double alpha, const void *x, double beta, void *y,
+                        femgpu_mem mem);
+
+/* Coefficient for the hex PA operators (CoeffCtx/ScalarCoeff,
+ * assembly/src/coefficient.rs:67-132, evaluated once at set-up into the
+ * quadrature data instead of per apply). */
+typedef enum {
+    FEMGPU_COEF_CONST = 0,    /* value */
+    FEMGPU_COEF_SINPROD = 1,  /* 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z) */
+    FEMGPU_COEF_GRF = 2,      /* clamp(exp(g), lo, hi); g a Gaussian random field
+                                 (random Fourier features, squared-exponential) */
+    FEMGPU_COEF_QP_ARRAY = 3  /* values given per element and quadrature point */
+} femgpu_coef_kind;
+
+typedef struct {
+    int kind;             /* femgpu_coef_kind */
+    double value;         /* CONST */
+    int n_features;       /* GRF */
+    double sigma, length, lo, hi;
+    uint64_t seed;
+    const double *qp_values; /* QP_ARRAY: host array [E][q^3], qp = qx + q(qy + q qz),
+                                e = ex + nx(ey + ny ez) */
+} femgpu_coef;
+
+typedef enum { FEMGPU_MASS = 1, FEMGPU_DIFFUSION = 2, FEMGPU_DIFF_MASS = 3 } femgpu_form;
+
+/* Structured hexahedral Q_p mesh of [0,1]^3 with nx*ny*nz elements and
+ * trilinear (HexQ1, element/src/lagrange/hex.rs:8-36) geometry; DOFs are the
+ * equispaced tensor nodes in lexicographic order, dof(gx,gy,gz) = gx + Nx*(gy
+ * + Ny*gz) with Nd = n*p + 1.  q Gauss-Legendre points per direction. */
+typedef struct {
+    int p, q;
+    int nx, ny, nz;
+    int form;           /* femgpu_form */
+    femgpu_dtype dtype; /* arithmetic and storage precision of the operator */
+    int dirichlet;      /* 1: every boundary DOF is essential: x is zeroed there
+                           and y = x there (examples/src/maxwell.rs:142-197) */
+    double rho;         /* mass (reaction) coefficient, constant */
+    femgpu_coef kappa;  /* diffusion coefficient */
+} femgpu_hex_desc;
+
+/* PADiffusionOperator::new / PAMassOperator::new (partial.rs:65-79, 138-148)
+ * for hex Q_p: the reference supports simplices only (partial.rs:278-288);
+ * this is the tensor-product, sum-factorised form.  vertex_coords: host
+ * f64[(nx+1)(ny+1)(nz+1)][3], vertex v = i + (nx+1)(j + (ny+1)k). */
+int femgpu_pa_create_hex(femgpu_ctx *ctx, const femgpu_hex_desc *desc,
+                         const double *vertex_coords, femgpu_op **out);
+/* Structured vertex grid with interior vertices moved by amp*h*(2u-1), u
+ * uniform from Philox4x32-10 keyed by seed (the perturbed-mesh generator). */
+int femgpu_hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, double *coords);
+/* b_i = integral of phi_i (load of f = 1), 0 on essential DOFs. */
+int femgpu_pa_hex_rhs_unit(femgpu_op *op, void *b, femgpu_mem mem);
+/* Seeded uniform [-1,1] nodal field (0 on essential DOFs when dirichlet). */
+int femgpu_pa_hex_rhs_random(femgpu_op *op, uint64_t seed, void *b, femgpu_mem mem);
+/* Same-mesh copy of a hex operator in another precision, reusing nothing but
+ * the description (for the FP32 inner / FP64 outer solver). */
+int femgpu_pa_hex_convert(femgpu_op *op, femgpu_dtype dtype, femgpu_op **out);
+
+/* Simplex PA: PAMassOperator / PADiffusionOperator::apply
+ * (partial.rs:85-130, 154-208) on Tri3/Tet4 meshes, P1..P3 (TriP1..P3,
+ * TetP1..P3).  coords f64[n_nodes*dim]; elem_nodes u32[n_elem*(dim+1)];
+ * elem_dofs u32[n_elem*n_local] in the reference element's local order
+ * (FESpace::element_dofs, space/src/fe_space.rs:28-65); coef constant;
+ * quad_order as in tri_rule/tet_rule (element/src/quadrature.rs:169-330). */
+int femgpu_pa_create_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                             const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                             const uint32_t *elem_dofs, uint64_t n_dofs, int form, double coef,
+                             int quad_order, femgpu_op **out);
+/* LumpedMassOperator::assemble (partial.rs:217-262): y = diag(m) x. */
+int femgpu_lumped_mass_create(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                              const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                              const uint32_t *elem_dofs, uint64_t n_dofs, double rho,
+                              int quad_order, femgpu_op **out);
+/* LumpedMassOperator::apply_inverse (partial.rs:269-273). */
+int femgpu_lumped_mass_apply_inverse(femgpu_op *op, const void *x, void *y, femgpu_mem mem);
+
+/* ------------------------------------------------------------ solvers ---- */
+/* SolverConfig (solver/src/lib.rs:97-113): defaults rtol 1e-8, atol 0,
+ * max_iter 1000. */
+typedef struct {
+    double rtol;
+    double atol;
+    uint64_t max_iter;
+    int verbose;
+    int print_level;
+} femgpu_solver_config;
+
+/* SolveResult (solver/src/lib.rs:71-75).  final_residual is absolute for
+ * femgpu_solve_cg_operator (as solve_cg_operator) and relative otherwise
+ * (as par_solve_pcg_jacobi); final_residual_is_relative says which. */
+typedef struct {
+    int converged;
+    uint64_t iterations;
+    double final_residual;
+    int final_residual_is_relative;
+    double error_estimate; /* reserved */
+} femgpu_solve_result;
+
+/* solve_cg_operator (solver/src/lib.rs:198-278): unpreconditioned CG with the
+ * operator's overwrite apply.  Errors: BREAKDOWN if |p^T A p| < 1e-32,
+ * NOT_CONVERGED on max_iter (result still filled). */
+int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
+                             const femgpu_solver_config *cfg, femgpu_solve_result *res);
+/* solve_pcg_jacobi (solver/src/lib.rs:281-297), with the semantics of the
+ * in-tree PCG-Jacobi par_solve_pcg_jacobi (parallel/src/par_solver.rs:262-346):
+ * inv_diag = 1/d (1 if |d| <= 1e-30), stop when |r|/|b| < rtol or |r| < atol.
+ * Works for any operator (CSR or PA).  NOT_CONVERGED on max_iter. */
+int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
+                            const femgpu_solver_config *cfg, femgpu_solve_result *res);
+/* PCG as above with a textbook Chebyshev-Jacobi polynomial preconditioner
+ * z = p(D^-1 A) D^-1 r of the given degree on [lmax/ratio, lmax], lmax =
+ * safety * (power-iteration estimate of the largest eigenvalue of D^-1 A,
+ * power_iters steps).  No reference counterpart beyond chebyshev_smooth
+ * (amg/src/lib.rs:53-103). */
+typedef struct {
+    int degree;        /* >= 1 */
+    double ratio;      /* lmax / lmin */
+    double safety;     /* multiplies the estimate */
+    int power_iters;
+    double lmax;       /* > 0: use this instead of estimating */
+} femgpu_cheb_config;
+int femgpu_solve_pcg_chebyshev(femgpu_op *op, const femgpu_cheb_config *cheb, const void *b,
+                               void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
+                               femgpu_solve_result *res, double *lmax_used);
+/* chebyshev_smooth (amg/src/lib.rs:53-103), the exact reference recurrence
+ * (d0 = (2/alpha) D^-1 r0, rho_k = 1/(1 - (delta/2alpha)^2 rho_{k-1}),
+ * d_k = rho_k((2/alpha) D^-1 r_k + (rho_k - 1) d_{k-1})) on any operator. */
+int femgpu_chebyshev_smooth(femgpu_op *op, void *x, const void *b, double lambda_min,
+                            double lambda_max, int n_iter, femgpu_mem mem);
+/* estimate_spectral_radius (amg/src/lib.rs:108-123). */
+int femgpu_estimate_spectral_radius(femgpu_op *op, int n_iter, double *rho);
+
+/* Mixed-precision iterative refinement (no reference counterpart): outer
+ * FP64 residual r = b - A64 u, update u += e and the norms in FP64 on the
+ * GPU; inner Chebyshev-Jacobi PCG in FP32 (op32: same operator in FP32) with
+ * FP64 dot products, to inner_rtol.  Converges when |r|/|b| < cfg->rtol. */
+typedef struct {
+    double inner_rtol;
+    uint64_t inner_max_iter;
+    uint64_t max_outer;
+    femgpu_cheb_config cheb;
+} femgpu_ir_config;
+int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *ir,
+                          const void *b, void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
+                          femgpu_solve_result *res, uint64_t *inner_iterations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

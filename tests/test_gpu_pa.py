@@ -1,0 +1,169 @@
+"""GPU parity: hex sum-factorised PA and simplex PA through the C ABI against
+the oracle.  Bar: 1e-12 relative (max-norm) in FP64, 1e-5 in FP32."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HEX_CASES = [(1, 2), (2, 3), (3, 4), (4, 5), (1, 3), (2, 4), (3, 5), (4, 6), (5, 6)]
+
+
+def _hex(fg, fr, ctx, p, q, n, form, kappa="sinprod", rho=1.3, dirichlet=False, amp=0.2, seed=3,
+         dtype=None):
+    dtype = fg.F64 if dtype is None else dtype
+    nx, ny, nz = n
+    c = fg.hex_vertices(nx, ny, nz, amp, seed)
+    assert np.array_equal(c, fr.hex_vertices(nx, ny, nz, amp, seed))  # same generator
+    if kappa == "sinprod":
+        kg, kr = "sinprod", fr.coef_sinprod()
+    elif isinstance(kappa, dict):
+        kg, kr = kappa, fr.coef_grf(kappa["seed"])
+    else:
+        kg, kr = float(kappa), fr.coef_const(kappa)
+    op = fg.HexPAOperator(p, q, n, form=form, kappa=kg, rho=rho, dirichlet=dirichlet, dtype=dtype,
+                          vertices=c, ctx=ctx)
+    d = fr.hex_desc(p, q, n, kind=form, kappa=kr, rho=rho, dirichlet=dirichlet)
+    Dq, Mq = fr.hex_qdata(d, c)
+    return op, d, c, Dq, Mq
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("p,q", HEX_CASES)
+@pytest.mark.parametrize("form", [1, 2, 3])
+def test_hex_apply_f64(fg, fr, ctx, p, q, form):
+    n = (3, 2, 4)
+    op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, form)
+    N = op.n_dofs()
+    assert N == fr.hex_ndofs(d)
+    x = np.random.default_rng(p * 10 + q).standard_normal(N)
+    y = np.zeros(N)
+    op.apply_zero(x, y)
+    ref = fr.hex_apply_sumfac(d, Dq, Mq, x)
+    assert _rel(y, ref) < 1e-12
+    # MatFreeOperator::apply accumulates
+    y0 = np.random.default_rng(5).standard_normal(N)
+    y1 = y0.copy()
+    op.apply(x, y1)
+    assert np.abs(y1 - (y0 + ref)).max() < 1e-12 * np.abs(ref).max() + 1e-12 * np.abs(y0).max()
+
+
+@pytest.mark.parametrize("p,q", [(2, 3), (3, 4), (4, 5)])
+def test_hex_dirichlet_diag_rhs(fg, fr, ctx, p, q):
+    n = (4, 3, 3)
+    op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, 3, dirichlet=True)
+    x = np.random.default_rng(1).standard_normal(op.n_dofs())
+    assert _rel(op.matvec(x), fr.hex_apply_sumfac(d, Dq, Mq, x)) < 1e-12
+    assert _rel(op.diagonal(), fr.hex_diagonal(d, Dq, Mq)) < 1e-12
+    assert _rel(op.rhs_unit(), fr.hex_rhs_unit(d, c)) < 1e-12
+    # reference element loop (partial.rs:151-208 generalised) agrees too
+    assert _rel(op.matvec(x), fr.hex_apply_refloop(d, c, x)) < 1e-12
+
+
+def test_hex_grf_and_qp_array(fg, fr, ctx):
+    n = (3, 3, 3)
+    op, d, c, Dq, Mq = _hex(fg, fr, ctx, 3, 4, n, 2, kappa=dict(kind="grf", seed=11))
+    x = np.random.default_rng(2).standard_normal(op.n_dofs())
+    assert _rel(op.matvec(x), fr.hex_apply_sumfac(d, Dq, Mq, x)) < 1e-12
+    # QP_ARRAY: per-point kappa values (e, qp) from the oracle's sin-product
+    E, Q3 = 27, 64
+    kq = np.random.default_rng(4).random((E, Q3)) + 0.5
+    op2 = fg.HexPAOperator(3, 4, n, form=2, kappa=kq.ravel(), vertices=c, ctx=ctx)
+    dk = fr.hex_desc(3, 4, n, kind=2, kappa=fr.coef_const(1.0))
+    D1, M1 = fr.hex_qdata(dk, c)
+    D1 = D1 * kq[:, :, None]
+    assert _rel(op2.matvec(x), fr.hex_apply_sumfac(dk, D1, M1, x)) < 1e-12
+
+
+@pytest.mark.parametrize("p,q", [(2, 3), (3, 4), (4, 5)])
+def test_hex_f32(fg, fr, ctx, p, q):
+    n = (4, 4, 3)
+    op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, 3, dtype=fg.F32)
+    x = np.random.default_rng(7).standard_normal(op.n_dofs()).astype(np.float32)
+    y = np.zeros_like(x)
+    op.apply_zero(x, y)
+    ref64 = fr.hex_apply_sumfac(d, Dq, Mq, x.astype(np.float64))
+    assert _rel(y.astype(np.float64), ref64) < 1e-5
+    ref32 = fr.hex_apply_sumfac_f32(d, Dq, Mq, x)
+    assert _rel(y, ref32) < 1e-5
+
+
+def test_hex_full_size_properties(fg, ctx):
+    """Q2 on 64^3 (2.1M DOFs, the config-2 operator): K 1 = 0, 1^T M 1 = 1,
+    symmetry x^T A y = y^T A x, linearity."""
+    n = 64
+    K = fg.HexPAOperator(2, 3, n, form=fg.DIFFUSION, kappa="sinprod", ctx=ctx)
+    M = fg.HexPAOperator(2, 3, n, form=fg.MASS, rho=1.0, ctx=ctx)
+    N = K.n_dofs()
+    assert N == 129 ** 3
+    ones = np.ones(N)
+    assert np.abs(K.matvec(ones)).max() < 1e-12
+    assert abs(M.matvec(ones).sum() - 1.0) < 1e-12
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(N), rng.standard_normal(N)
+    Kx, Ky = K.matvec(x), K.matvec(y)
+    assert abs(x @ Ky - y @ Kx) < 1e-12 * abs(x @ Ky)
+    assert _rel(K.matvec(2.0 * x - 3.0 * y), 2.0 * Kx - 3.0 * Ky) < 1e-12
+
+
+# ---------------------------------------------------------------- simplex
+def _tri_mesh(fr, n, order):
+    coords, conn = fr.unit_square_tri(n)
+    nd, ed = fr.simplex_dofs(2, order, len(coords) // 2, conn)
+    return coords, conn, nd, ed
+
+
+def _tet_mesh(n):
+    """Structured unit cube split into 6 tets per cube (Kuhn split)."""
+    g = np.linspace(0, 1, n + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    coords = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).ravel()
+    vid = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k
+    conn = []
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for pm in perms:
+                    cur = [i, j, k]
+                    path = [vid(*cur)]
+                    for ax in pm:
+                        cur[ax] += 1
+                        path.append(vid(*cur))
+                    conn += path
+    return coords, np.array(conn, np.uint32)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_simplex_pa(fg, fr, ctx, dim, order, kind):
+    if dim == 2:
+        coords, conn, nd, ed = _tri_mesh(fr, 6, order)
+    else:
+        coords, conn = _tet_mesh(3)
+        nd, ed = fr.simplex_dofs(3, order, len(coords) // 3, conn)
+    qo = 2 * order + 1
+    cls = fg.PAMassOperator if kind == 1 else fg.PADiffusionOperator
+    op = cls(dim, order, coords, conn, ed, nd, 1.7, qo, ctx=ctx)
+    x = np.random.default_rng(order).standard_normal(nd)
+    ref = fr.simplex_pa_apply(dim, order, coords, conn, ed, kind, 1.7, qo, x)
+    assert _rel(op.matvec(x), ref) < 1e-12
+    # matches the assembled matrix (the reference's own PA test, partial.rs:323-366)
+    rp, ci, v = fr.simplex_assemble_csr(dim, order, coords, conn, ed, nd, kind, 1.7, qo)
+    assert _rel(op.matvec(x), fr.CsrRef(rp, ci, v).spmv(x)) < 1e-12
+    assert _rel(op.diagonal(), fr.CsrRef(rp, ci, v).diagonal()) < 1e-12
+
+
+def test_lumped_mass(fg, fr, ctx):
+    coords, conn, nd, ed = _tri_mesh(fr, 4, 1)
+    lm = fg.LumpedMassOperator(2, 1, coords, conn, ed, nd, 1.0, 3, ctx=ctx)
+    assert abs(lm.diag.sum() - 1.0) < 1e-11  # partial.rs:370-378
+    ref = fr.simplex_lumped_mass(2, 1, coords, conn, ed, nd, 1.0, 3)
+    assert _rel(lm.diag, ref) < 1e-14
+    x = np.random.default_rng(0).standard_normal(nd)
+    y = np.zeros(nd)
+    lm.apply_inverse(x, y)
+    assert _rel(y, x / ref) < 1e-14

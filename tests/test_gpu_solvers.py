@@ -1,0 +1,170 @@
+"""GPU parity of the device solvers against the oracle and the reference's
+own known-answer tests (solver/src/lib.rs:1197-1221, amg/src/lib.rs:254-286,
+examples/mfem_ex1_poisson.rs:184-237)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _lap1d(n):
+    rp = [0]; ci = []; v = []
+    for i in range(n):
+        if i > 0: ci.append(i - 1); v.append(-1.0)
+        ci.append(i); v.append(2.0)
+        if i < n - 1: ci.append(i + 1); v.append(-1.0)
+        rp.append(len(ci))
+    return np.array(rp, np.uint64), np.array(ci, np.uint32), np.array(v)
+
+
+def test_cg_laplacian(fg, fr, ctx):
+    # lib.rs:1197-1210 (CG on the 1-D Laplacian, |Ax - b| < 1e-6)
+    rp, ci, v = _lap1d(50)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    b = np.ones(50); x = np.zeros(50)
+    res = fg.solve_cg_operator(50, 50, A, b, x, fg.SolverConfig())
+    assert res.converged
+    ref = fr.CsrRef(rp, ci, v)
+    assert np.linalg.norm(ref.spmv(x) - b) < 1e-6
+    xo, ro = fr.solve_cg_operator(fr._Op(50, csr=ref), b, np.zeros(50))
+    assert res.iterations == ro.iterations  # in exact arithmetic <= 25 (symmetry)
+    assert res.iterations <= 26
+    assert np.abs(x - xo).max() < 1e-10 * np.abs(xo).max()
+    assert abs(res.final_residual - ro.final_residual) <= 1e-6 * max(ro.final_residual, 1e-300) + 1e-14
+
+
+def test_pcg_jacobi_laplacian(fg, ctx):
+    # lib.rs:1212-1221: converged, iterations < 60
+    A = fg.CsrMatrix(*_lap1d(50), ctx=ctx)
+    x = np.zeros(50)
+    res = fg.solve_pcg_jacobi(A, np.ones(50), x)
+    assert res.converged and res.iterations < 60
+
+
+def test_config1_poisson(fg, fr, ctx):
+    """Config 1: 256^2 P1 Poisson, PCG-Jacobi to 1e-10 (mfem_ex1_poisson.rs:71-114)."""
+    rp, ci, v, rhs, coords, conn = fr.poisson_p1(256)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    x = np.zeros(len(rhs))
+    res = fg.solve_pcg_jacobi(A, rhs, x, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    ref = fr.CsrRef(rp, ci, v)
+    xo, ro = fr.pcg_jacobi(fr._Op(len(rhs), csr=ref), ref.diagonal(), rhs, np.zeros(len(rhs)),
+                           rtol=1e-10, max_iter=5000)
+    assert res.converged and ro.converged
+    assert abs(res.iterations - ro.iterations) <= 2
+    assert res.final_residual < 1e-10
+    assert np.abs(x - xo).max() < 1e-8 * np.abs(xo).max()
+    # discretisation error vs sin(pi x) sin(pi y): nodal max error ~ O(h^2)
+    X, Y = coords[0::2], coords[1::2]
+    assert np.abs(x - np.sin(np.pi * X) * np.sin(np.pi * Y)).max() < 5e-5
+
+
+def test_ex1_small_golden(fg, fr, ctx):
+    z = np.load(os.path.join(GOLD, "csr_fixtures.npz"))
+    A = fg.CsrMatrix(z["p8_rp"], z["p8_ci"], z["p8_v"], ctx=ctx)
+    x = np.zeros(len(z["p8_b"]))
+    res = fg.solve_pcg_jacobi(A, z["p8_b"], x, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    assert res.converged and res.final_residual < 1e-8  # ex1 test thresholds
+    assert res.iterations == int(z["p8_iters"][0])
+    assert np.abs(x - z["p8_x"]).max() < 1e-12
+
+
+def test_chebyshev_reference_mode(fg, fr, ctx):
+    """chebyshev_smooth / estimate_spectral_radius (amg/src/lib.rs:254-286)."""
+    z = np.load(os.path.join(GOLD, "csr_fixtures.npz"))
+    rp, ci, v = _lap1d(50)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    rho30 = fg.estimate_spectral_radius(A, 30)
+    assert 3.5 < rho30 < 4.1
+    assert abs(rho30 - z["cheb_rho30"][0]) < 1e-14
+    rho = fg.estimate_spectral_radius(A, 20)
+    assert abs(rho - z["cheb_rho20"][0]) < 1e-14
+    lmax = 1.1 * rho; lmin = lmax / 30
+    x = np.zeros(50)
+    fg.chebyshev_smooth(A, x, np.ones(50), lmin, lmax, 20)
+    assert np.abs(x - z["cheb_x20"]).max() <= 1e-12 * np.abs(z["cheb_x20"]).max()
+    r = np.ones(50) - fr.CsrRef(rp, ci, v).spmv(x)
+    assert np.linalg.norm(r) / math.sqrt(50) < 0.9
+
+
+def test_cg_errors(fg, ctx):
+    A = fg.CsrMatrix(*_lap1d(20), ctx=ctx)
+    with pytest.raises(fg.DimensionMismatch):
+        fg.solve_cg_operator(20, 20, A, np.ones(19), np.zeros(20))
+    with pytest.raises(fg.ConvergenceFailed) as ei:
+        fg.solve_cg_operator(20, 20, A, np.ones(20), np.zeros(20), fg.SolverConfig(max_iter=2))
+    assert ei.value.max_iter == 2
+    # zero operator: p^T A p = 0 -> breakdown (lib.rs:245-249)
+    Z = fg.CsrMatrix(np.arange(21, dtype=np.uint64), np.arange(20, dtype=np.uint32), np.zeros(20), ctx=ctx)
+    with pytest.raises(fg.Linger):
+        fg.solve_cg_operator(20, 20, Z, np.ones(20), np.zeros(20))
+    # PCG returns converged = false on max_iter (par_solver.rs:340-345)
+    res = fg.solve_pcg_jacobi(A, np.ones(20), np.zeros(20), fg.SolverConfig(max_iter=1))
+    assert not res.converged and res.iterations == 1
+    # zero right-hand side: converged in 0 iterations
+    res = fg.solve_pcg_jacobi(A, np.zeros(20), np.zeros(20))
+    assert res.converged and res.iterations == 0
+
+
+def _hex_problem(fg, fr, ctx, p, n, kappa, dirichlet=True, dtype=None):
+    op = fg.HexPAOperator(p, p + 1, n, form=fg.DIFFUSION, kappa=kappa, dirichlet=dirichlet,
+                          dtype=fg.F64 if dtype is None else dtype, ctx=ctx)
+    return op
+
+
+def test_pcg_chebyshev_hex(fg, fr, ctx):
+    """Config-2 family at 12^3 Q2: PCG + Chebyshev-Jacobi(4) vs the oracle."""
+    n, p = 12, 2
+    op = _hex_problem(fg, fr, ctx, p, n, "sinprod")
+    b = op.rhs_unit()
+    x = np.zeros(op.n_dofs())
+    res, lmax = fg.solve_pcg_chebyshev(op, b, x, fg.SolverConfig(rtol=1e-8), degree=4)
+    assert res.converged
+    d = fr.hex_desc(p, p + 1, n, kind=fr.FR_DIFFUSION, kappa=fr.coef_sinprod(), dirichlet=True)
+    c = fr.hex_vertices(n, n, n, 0.0, 0)
+    Dq, Mq = fr.hex_qdata(d, c)
+    ob = fr._Op(op.n_dofs(), hexop=(d, Dq, Mq, 4))
+    diag = fr.hex_diagonal(d, Dq, Mq)
+    assert np.abs(b - fr.hex_rhs_unit(d, c)).max() < 1e-15
+    lmax_o = 1.1 * fr.estimate_lmax_jacobi(ob, diag, 20)
+    assert abs(lmax - lmax_o) < 1e-10 * lmax_o
+    xo, ro = fr.pcg_chebyshev(ob, diag, 4, lmax / 30, lmax, b, np.zeros(op.n_dofs()), rtol=1e-8)
+    assert abs(res.iterations - ro.iterations) <= 1
+    # exact solution of the discrete system (direct solve) within 1e-10 relative
+    rp, ci, v = fr.hex_assemble_csr(d, c)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    A = sp.csr_matrix((v, ci.astype(np.int64), rp.astype(np.int64)), shape=(len(b), len(b)))
+    xe = spl.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-7
+    x2 = np.zeros_like(x)
+    res2, _ = fg.solve_pcg_chebyshev(op, b, x2, fg.SolverConfig(rtol=1e-12), degree=4)
+    assert np.linalg.norm(x2 - xe) / np.linalg.norm(xe) < 1e-10
+
+
+def test_pcg_jacobi_on_pa(fg, fr, ctx):
+    op = _hex_problem(fg, fr, ctx, 3, 6, dict(kind="grf", seed=4))
+    b = op.rhs_random(9)
+    x = np.zeros(op.n_dofs())
+    res = fg.solve_pcg_jacobi(op, b, x, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    assert res.converged
+    r = b - op.matvec(x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+
+
+def test_ir_mixed(fg, fr, ctx):
+    """FP32 inner PCG + FP64 refinement reaches 1e-10 and matches the FP64 solve."""
+    n, p = 10, 3
+    op64 = _hex_problem(fg, fr, ctx, p, n, dict(kind="grf", seed=1))
+    op32 = op64.convert(fg.F32)
+    b = op64.rhs_random(5)
+    x = np.zeros(op64.n_dofs())
+    res, inner = fg.solve_ir_mixed(op64, op32, b, x, fg.SolverConfig(rtol=1e-10), inner_rtol=1e-3)
+    assert res.converged and res.final_residual < 1e-10 and inner > 0
+    x64 = np.zeros_like(x)
+    r64, _ = fg.solve_pcg_chebyshev(op64, b, x64, fg.SolverConfig(rtol=1e-13, max_iter=5000))
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
