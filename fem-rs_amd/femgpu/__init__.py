@@ -361,10 +361,12 @@ def _coef_struct(kappa, qp_keep):
         return kappa
     if kappa is None:
         return _Coef(COEF_CONST, 1.0, 0, 0.0, 0.0, 0.0, 0.0, 0, None)
+    if isinstance(kappa, str):
+        if kappa != "sinprod":
+            raise ValueError(f"unknown coefficient {kappa!r}")
+        return _Coef(COEF_SINPROD, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0, None)
     if np.isscalar(kappa):
         return _Coef(COEF_CONST, float(kappa), 0, 0.0, 0.0, 0.0, 0.0, 0, None)
-    if isinstance(kappa, str) and kappa == "sinprod":
-        return _Coef(COEF_SINPROD, 0.0, 0, 0.0, 0.0, 0.0, 0.0, 0, None)
     if isinstance(kappa, dict) and kappa.get("kind") == "grf":
         return _Coef(COEF_GRF, 0.0, int(kappa.get("n_features", 256)), float(kappa.get("sigma", 1.5)),
                      float(kappa.get("length", 0.1)), float(kappa.get("lo", 1e-2)),

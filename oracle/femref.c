@@ -82,6 +82,27 @@ int fr_gauss_legendre(int n, double *x, double *w) {
         x[0] = -b; x[1] = -a; x[2] = 0.0; x[3] = a; x[4] = b;
         w[0] = wb; w[1] = wa; w[2] = 128.0 / 225.0; w[3] = wa; w[4] = wb; return 0;
     }
+    case 6: {
+        /* no closed form: Newton on P6 (three-term recurrence) from the
+         * Chebyshev-like guesses cos(pi (i + 3/4) / (n + 1/2)); w = 2 / ((1 - x^2) P6'(x)^2) */
+        for (int i = 0; i < 6; ++i) {
+            double z = cos(FR_PI * (i + 0.75) / 6.5), dp = 1.0;
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1.0, p1 = z;
+                for (int k = 2; k <= 6; ++k) {
+                    double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1; p1 = p2;
+                }
+                dp = 6.0 * (z * p1 - p0) / (z * z - 1.0);
+                double dz = p1 / dp;
+                z -= dz;
+                if (fabs(dz) < 1e-16) break;
+            }
+            x[5 - i] = z;
+            w[5 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+        }
+        return 0;
+    }
     default: return -1;
     }
 }

@@ -33,7 +33,7 @@ void fr_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 double fr_uniform(uint64_t seed, uint32_t stream, uint64_t index);
 
 /* --------------------------------------------------------- quadrature ---- */
-/* Gauss-Legendre on [-1,1], n = 1..5.  quadrature.rs:12-31 (n <= 4); n = 5 from
+/* Gauss-Legendre on [-1,1], n = 1..6.  quadrature.rs:12-31 (n <= 4); n = 5 from
  * the closed form.  Returns 0 on success, -1 for unsupported n. */
 int fr_gauss_legendre(int n, double *x, double *w);
 /* Triangle rule, reference triangle (0,0),(1,0),(0,1): quadrature.rs:169-246.

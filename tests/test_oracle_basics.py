@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6])
 def test_gauss_legendre_exact(fr, n):
     x, w = fr.gauss_legendre(n)
     assert abs(w.sum() - 2.0) < 1e-14
