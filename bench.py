@@ -162,7 +162,8 @@ def main():
             "unit": "GB/s",
             "frac": round(bytes_diff / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": f"k_pa_hex<{p + 1},{p + 1},double,2> (PA diffusion apply)",
+            "kernel": f"PA diffusion apply = k_pa_hex<{p + 1},{p + 1},double,2> element kernel + "
+                      f"k_hex_gather<double,{p}> (HIP events around both)",
             "algorithmic_bytes": bytes_diff,
         },
         "cpu_baseline": cpu,

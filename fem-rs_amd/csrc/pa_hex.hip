@@ -90,6 +90,13 @@ __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long
     return gx == 0 || gx == g.Nx - 1 || gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1;
 }
 
+// Element-vector layout: for each (ez, ey, k, j) an x-row holding every
+// element's N1 local values: [ez][ey][k][j][ex][i].
+__host__ __device__ __forceinline__ size_t evec_index(const HexGrid &g, int N1, int ex, int ey,
+                                                      int ez, int j, int k) {
+    return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.nx * N1 + (size_t)ex * N1;
+}
+
 template <int N1, int Q, class T, int FORM>
 __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm,
@@ -269,9 +276,9 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             FB[a] = sA[le][k][j][a];
             if constexpr (DIFF) FG[a] = sAg[le][k][j][a];
         }
-        const long long gy = (long long)ey * P + j, gz = (long long)ez * P + k;
-        const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
-        const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+        // element-vector row (j,k), stored x-row-major: consecutive elements of
+        // an x-row write one contiguous run
+        T *dst = y + evec_index(g, N1, ex, ey, ez, j, k);
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             T s = 0;
@@ -280,11 +287,67 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 s += Bt[a * N1 + i] * FB[a];
                 if constexpr (DIFF) s += Gt[a * N1 + i] * FG[a];
             }
-            const long long gx = (long long)ex * P + i;
-            const bool bd = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
-            if (!bd) unsafeAtomicAdd(&y[base + i], s);
+            dst[i] = s;
         }
     }
+}
+
+// Element-vector -> DOF vector: every DOF sums the (1, 2, 4 or 8) element
+// contributions that share it, in a fixed order (bitwise reproducible, no
+// atomics).  One thread per DOF of an x-row; rows are (gy, gz).
+// mode 0: y = s, 1: y += s.  Essential DOFs: y (+)= x.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    int mode, int dirichlet, const int *skip) {
+    constexpr int N1 = P + 1;
+    if (skip && *skip) return;
+    const int nxb = (int)((g.Nx + 255) / 256);
+    const int row = blockIdx.x / nxb;
+    const int gx = (blockIdx.x - row * nxb) * 256 + threadIdx.x;
+    const int Ny = (int)g.Ny, Nx = (int)g.Nx;
+    const int gy = row % Ny, gz = row / Ny;
+    if (gx >= Nx || gz >= (int)g.Nz) return;
+    const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
+    if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1)) {
+        if (mode == 0) y[gi] = x[gi];
+        else y[gi] += x[gi];
+        return;
+    }
+    int ex[2], ix[2], ey[2], iy[2], ez[2], iz[2];
+    int nxc = 0, nyc = 0, nzc = 0;
+    {
+        const int e0 = gx / P, i0 = gx - e0 * P;
+        if (i0) { ex[0] = e0; ix[0] = i0; nxc = 1; }
+        else {
+            if (e0 > 0) { ex[nxc] = e0 - 1; ix[nxc++] = P; }
+            if (e0 < g.nx) { ex[nxc] = e0; ix[nxc++] = 0; }
+        }
+    }
+    {
+        const int e0 = gy / P, i0 = gy - e0 * P;
+        if (i0) { ey[0] = e0; iy[0] = i0; nyc = 1; }
+        else {
+            if (e0 > 0) { ey[nyc] = e0 - 1; iy[nyc++] = P; }
+            if (e0 < g.ny) { ey[nyc] = e0; iy[nyc++] = 0; }
+        }
+    }
+    {
+        const int e0 = gz / P, i0 = gz - e0 * P;
+        if (i0) { ez[0] = e0; iz[0] = i0; nzc = 1; }
+        else {
+            if (e0 > 0) { ez[nzc] = e0 - 1; iz[nzc++] = P; }
+            if (e0 < g.nz) { ez[nzc] = e0; iz[nzc++] = 0; }
+        }
+    }
+    T s = 0;
+    for (int c = 0; c < nzc; ++c)
+        for (int b = 0; b < nyc; ++b) {
+            const T *rowp = ye + evec_index(g, N1, 0, ey[b], ez[c], iy[b], iz[c]);
+            for (int a = 0; a < nxc; ++a) s += rowp[(size_t)ex[a] * N1 + ix[a]];
+        }
+    if (mode == 0) y[gi] = s;
+    else y[gi] += s;
 }
 
 // mode 0: y = bdr ? x : 0   (apply_zero)
@@ -477,6 +540,20 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
     return false;
 }
 
+template <class T>
+static void launch_gather(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
+                          const int *skip, hipStream_t s) {
+    const unsigned nxb = (unsigned)((g.Nx + 255) / 256);
+    const dim3 grid(nxb * (unsigned)(g.Ny * g.Nz));
+    switch (p) {
+    case 1: hipLaunchKernelGGL((k_hex_gather<T, 1>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 2: hipLaunchKernelGGL((k_hex_gather<T, 2>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 3: hipLaunchKernelGGL((k_hex_gather<T, 3>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 4: hipLaunchKernelGGL((k_hex_gather<T, 4>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    default: hipLaunchKernelGGL((k_hex_gather<T, 5>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    }
+}
+
 static bool supported(int n1, int q) {
     return (n1 >= 2 && n1 <= 6) && (q == n1 || (q == n1 + 1 && n1 <= 5));
 }
@@ -485,16 +562,18 @@ struct HexOp final : femgpu_op {
     femgpu_hex_desc desc{};
     HexGrid g{};
     DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
+    DevBuf evec;  // element vector [E][(p+1)^3] between the two apply kernels
 
     const char *name() const override { return "pa_hex"; }
 
+    // y (+)= A x: element kernel into the element vector, then the DOF gather
+    // (which also applies the essential rows y = x).
     template <class T> void apply_t(const T *x, T *y, bool acc, const int *skip) {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
-        if (!acc) hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, x, y, 0, dir, skip);
-        else if (dir) hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, x, y, 1, dir, skip);
         if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
-        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, y, dir, skip, s);
+        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, evec.as<T>(), dir, skip, s);
+        launch_gather<T>(desc.p, g, evec.as<T>(), x, y, acc ? 1 : 0, dir, skip, s);
         if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
     }
 
@@ -547,6 +626,7 @@ struct HexOp final : femgpu_op {
         const size_t es = esize();
         if (desc.form & 2) qd.alloc(6 * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
+        evec.alloc((size_t)g.E * n1 * n1 * n1 * es);
         if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
         else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
         FG_HIP(hipStreamSynchronize(s));
@@ -678,18 +758,19 @@ int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
         void *dst = mem == FEMGPU_DEVICE ? b : out.ptr;
         hipStream_t s = c->stream;
         const int n1 = op->desc.p + 1, q = op->desc.q;
+        // b = M_1 1 (mass with weights w|J|), then zero on essential DOFs
         if (op->dtype == FEMGPU_F64) {
             op->setup_t<double>(1, 1.0, true, nullptr, tq.as<double>());
             fill<double>(s, ones.as<double>(), 1.0, op->n);
-            fill<double>(s, (double *)dst, 0.0, op->n);
-            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), ones.as<double>(), (double *)dst, 0, nullptr, s);
+            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), ones.as<double>(), op->evec.as<double>(), 0, nullptr, s);
+            launch_gather<double>(op->desc.p, op->g, op->evec.as<double>(), ones.as<double>(), (double *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<double>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const double *)dst, (double *)dst, 3, op->desc.dirichlet, nullptr);
         } else {
             op->setup_t<float>(1, 1.0, true, nullptr, tq.as<float>());
             fill<float>(s, ones.as<float>(), 1.f, op->n);
-            fill<float>(s, (float *)dst, 0.f, op->n);
-            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), ones.as<float>(), (float *)dst, 0, nullptr, s);
+            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), ones.as<float>(), op->evec.as<float>(), 0, nullptr, s);
+            launch_gather<float>(op->desc.p, op->g, op->evec.as<float>(), ones.as<float>(), (float *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<float>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const float *)dst, (float *)dst, 3, op->desc.dirichlet, nullptr);
         }
