@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (counter calibration)")
     return ap.parse_args()
 
 
@@ -87,6 +89,13 @@ def main():
     x = dev()
     zero = np.zeros(N)
     cfg = fg.SolverConfig(rtol=1e-8, max_iter=2000)
+
+    if a.copy_calib_mb:
+        nc = a.copy_calib_mb * (1 << 20) // 8
+        src = fg.DeviceArray(ctx, nc, np.float64, np.ones(nc))
+        dst = fg.DeviceArray(ctx, nc, np.float64)
+        dst.copy_from(src)  # one copy kernel of known size: reads and writes nc*8 bytes
+        del src, dst
 
     # PA apply rates (device vectors, HIP events)
     xv = dev().upload(np.random.default_rng(0).standard_normal(N))

@@ -294,7 +294,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
 
 // Element-vector -> DOF vector: every DOF sums the (1, 2, 4 or 8) element
 // contributions that share it, in a fixed order (bitwise reproducible, no
-// atomics).  One thread per DOF of an x-row; rows are (gy, gz).
+// atomics).  One thread per DOF of a z-plane (blockIdx.y = gz), consecutive
+// threads along x so the element-vector rows are read contiguously.
 // mode 0: y = s, 1: y += s.  Essential DOFs: y (+)= x.
 template <class T, int P>
 __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
@@ -302,12 +303,10 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
                                                     int mode, int dirichlet, const int *skip) {
     constexpr int N1 = P + 1;
     if (skip && *skip) return;
-    const int nxb = (int)((g.Nx + 255) / 256);
-    const int row = blockIdx.x / nxb;
-    const int gx = (blockIdx.x - row * nxb) * 256 + threadIdx.x;
     const int Ny = (int)g.Ny, Nx = (int)g.Nx;
-    const int gy = row % Ny, gz = row / Ny;
-    if (gx >= Nx || gz >= (int)g.Nz) return;
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= Nx * Ny) return;
+    const int gy = id / Nx, gx = id - gy * Nx, gz = blockIdx.y;
     const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
     if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1)) {
         if (mode == 0) y[gi] = x[gi];
@@ -543,8 +542,7 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
 template <class T>
 static void launch_gather(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
                           const int *skip, hipStream_t s) {
-    const unsigned nxb = (unsigned)((g.Nx + 255) / 256);
-    const dim3 grid(nxb * (unsigned)(g.Ny * g.Nz));
+    const dim3 grid((unsigned)((g.Nx * g.Ny + 255) / 256), (unsigned)g.Nz);
     switch (p) {
     case 1: hipLaunchKernelGGL((k_hex_gather<T, 1>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
     case 2: hipLaunchKernelGGL((k_hex_gather<T, 2>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
@@ -678,7 +676,8 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         g.E = (long long)d.nx * d.ny * d.nz;
         g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
         op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
-        FG_REQUIRE(op->n < (1ull << 40), FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
+        FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536,
+                   FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
         const size_t nv = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
         hipStream_t s = c->stream;
         op->coords.alloc(nv * 3 * 8);
