@@ -5,6 +5,8 @@
 // parallel/src/par_solver.rs:15-78).  Dots accumulate in double for both f64
 // and f32 vectors.
 #include "internal.hpp"
+
+#include <cstring>
 #include "reduce.hpp"
 
 namespace femgpu {
@@ -109,6 +111,20 @@ double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dt
     FG_HIP(hipMemcpyAsync(c->pinned + 63, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FG_HIP(hipStreamSynchronize(c->stream));
     return c->pinned[63];
+}
+
+void dots_host(femgpu_ctx *c, int m, const void *const *a, const void *const *b, size_t n,
+               femgpu_dtype dt, double *out) {
+    FG_REQUIRE(m >= 0 && m <= kMultiDotMax, FEMGPU_ERR_INVALID_ARG, "dots_host: too many dot products");
+    if (m == 0) return;
+    double *o = c->scalars.as<double>() + kMultiDotBase;
+    for (int j = 0; j < m; ++j) {
+        if (dt == FEMGPU_F64) dot<double>(c, (const double *)a[j], (const double *)b[j], n, o + j);
+        else dot<float>(c, (const float *)a[j], (const float *)b[j], n, o + j);
+    }
+    FG_HIP(hipMemcpyAsync(c->pinned + kMultiDotBase, o, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->pinned + kMultiDotBase, sizeof(double) * m);
 }
 
 void convert_f64_to_f32(hipStream_t s, float *y, const double *x, size_t n, double scale) {

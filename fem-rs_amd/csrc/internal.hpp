@@ -87,6 +87,13 @@ struct DevBuf {
 
 // Reduction partials: a fixed grid gives a deterministic summation order.
 constexpr int kRedBlocks = 1024;
+// Device scalar slots per context (and the pinned host mirror): 0..63 solver
+// scalars, 64..255 results of batched dot products.
+constexpr int kScalarSlots = 256;
+constexpr int kMultiDotBase = 64, kMultiDotMax = kScalarSlots - kMultiDotBase;
+// Operator workspace slots 0..199 belong to the solvers; the network's
+// energy-loss training uses two from this one on.
+constexpr int kUnetOpSlot = 200;
 constexpr int kRedThreads = 256;
 
 }  // namespace femgpu
@@ -144,6 +151,10 @@ template <class T> void inv_diag(hipStream_t s, T *inv, const T *d, size_t n, do
 // Deterministic dot products in double: writes partials then finalises into out[0].
 template <class T> void dot(femgpu_ctx *c, const T *a, const T *b, size_t n, double *out_dev);
 double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dtype dt);
+// m dot products a[j].b[j] (m <= kMultiDotMax) enqueued back to back, read
+// back with one copy and one synchronisation.
+void dots_host(femgpu_ctx *c, int m, const void *const *a, const void *const *b, size_t n,
+               femgpu_dtype dt, double *out);
 void convert_f64_to_f32(hipStream_t s, float *y, const double *x, size_t n, double scale);
 void convert_f32_to_f64_axpy(hipStream_t s, double *y, const float *x, size_t n, double scale);
 
@@ -153,6 +164,15 @@ struct VecArg {
     void *host = nullptr;  // host pointer (if staged)
     size_t bytes = 0;
 };
+
+// hex operator helpers used by the learned preconditioner
+void hex_log_kappa_nodes(femgpu_op *op, float *out_dev);
+bool hex_grid_dims(femgpu_op *op, int *Nx, int *Ny, int *Nz);
+
+// learned preconditioner (unet.hip): z = |r| N(r/|r|) on device vectors
+bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm);
+long long unet_voxels(const femgpu_unet *u);
+femgpu_ctx *unet_ctx(const femgpu_unet *u);
 
 // hex mesh generator (host) shared by the C ABI and set-up code
 void hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, double *coords);

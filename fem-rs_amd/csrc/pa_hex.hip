@@ -716,6 +716,56 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
     return op;
 }
 
+// log(kappa)/log(100) at every DOF node (trilinear position of the equispaced
+// node), the auxiliary input channel of the learned preconditioner.
+__global__ void k_hex_log_kappa_nodes(const HexGrid g, int P, const double *__restrict__ coords,
+                                      CoefDev coef, float *__restrict__ out) {
+    const size_t N = (size_t)g.Nx * g.Ny * g.Nz;
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < N;
+         id += (size_t)gridDim.x * blockDim.x) {
+        const long long gx = (long long)(id % g.Nx), r = (long long)(id / g.Nx);
+        const long long gy = r % g.Ny, gz = r / g.Ny;
+        const long long e3[3] = {min(gx / P, (long long)g.nx - 1), min(gy / P, (long long)g.ny - 1),
+                                 min(gz / P, (long long)g.nz - 1)};
+        const long long gg[3] = {gx, gy, gz};
+        double xi[3];
+        for (int d = 0; d < 3; ++d) xi[d] = -1.0 + 2.0 * (double)(gg[d] - e3[d] * P) / P;
+        const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                  sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+        double xp[3] = {0, 0, 0};
+        for (int v = 0; v < 8; ++v) {
+            const size_t gv = (size_t)(e3[0] + (sx[v] > 0)) +
+                              (size_t)(g.nx + 1) * ((size_t)(e3[1] + (sy[v] > 0)) + (size_t)(g.ny + 1) * (e3[2] + (sz[v] > 0)));
+            const double phi = 0.125 * (1 + sx[v] * xi[0]) * (1 + sy[v] * xi[1]) * (1 + sz[v] * xi[2]);
+            for (int d = 0; d < 3; ++d) xp[d] += phi * coords[3 * gv + d];
+        }
+        const double k = coef.kind == FEMGPU_COEF_QP_ARRAY ? 1.0 : coef_eval(coef, xp, 0, 0, 1);
+        out[id] = (float)(log(k) / log(100.0));
+    }
+}
+
+void hex_log_kappa_nodes(femgpu_op *op_, float *out) {
+    auto *op = dynamic_cast<HexOp *>(op_);
+    FG_REQUIRE(op, FEMGPU_ERR_INVALID_ARG, "log_kappa_nodes: not a hex operator");
+    CoefDev c{};
+    const femgpu_coef &k = op->desc.kappa;
+    c.kind = (op->desc.form & 2) ? k.kind : FEMGPU_COEF_CONST;
+    c.value = (op->desc.form & 2) ? k.value : 1.0;
+    c.nf = k.n_features; c.lo = k.lo; c.hi = k.hi;
+    c.scale = k.kind == FEMGPU_COEF_GRF ? k.sigma * std::sqrt(2.0 / k.n_features) : 0.0;
+    c.wf = op->grf_w.as<double>(); c.bf = op->grf_b.as<double>(); c.qp = op->kqp.as<double>();
+    hipLaunchKernelGGL(k_hex_log_kappa_nodes, dim3(grid1(op->n)), dim3(256), 0, op->ctx->stream, op->g,
+                       op->desc.p, op->coords.as<double>(), c, out);
+    FG_HIP(hipGetLastError());
+}
+
+bool hex_grid_dims(femgpu_op *op_, int *Nx, int *Ny, int *Nz) {
+    auto *op = dynamic_cast<HexOp *>(op_);
+    if (!op) return false;
+    *Nx = (int)op->g.Nx; *Ny = (int)op->g.Ny; *Nz = (int)op->g.Nz;
+    return true;
+}
+
 }  // namespace femgpu
 
 using namespace femgpu;

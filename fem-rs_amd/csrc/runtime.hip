@@ -63,11 +63,11 @@ int femgpu_ctx_create(int device, femgpu_ctx **out) {
         try {
             FG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             c->partials.alloc(sizeof(double) * kRedBlocks * 8);
-            c->scalars.alloc(sizeof(double) * 64);
+            c->scalars.alloc(sizeof(double) * kScalarSlots);
             FG_HIP(hipMemset(c->scalars.ptr, 0, c->scalars.bytes));
             c->flags.alloc(sizeof(int) * 16);
             FG_HIP(hipMemset(c->flags.ptr, 0, c->flags.bytes));
-            FG_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * 64, hipHostMallocDefault));
+            FG_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * kScalarSlots, hipHostMallocDefault));
             FG_HIP(hipEventCreate(&c->t0));
             FG_HIP(hipEventCreate(&c->t1));
         } catch (...) {

@@ -733,3 +733,203 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- learned preconditioner --
+namespace femgpu {
+
+// Flexible CG (Notay, SIAM J. Sci. Comput. 22 (2000)) truncated to the last
+// `trunc` directions (A-orthogonalisation window), with the two-stage preconditioner
+//   n = N(r), gamma = n.r / n.An (Galerkin step: the network's output scale
+//   does not matter), r1 = r - gamma A n, z = gamma n + cheb(r1)
+// i.e. a learned coarse correction followed by Chebyshev-Jacobi smoothing.
+constexpr int kFcgTruncMax = 64;
+
+template <class T>
+static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, const T *b, T *x,
+                       double rtol, double atol, uint64_t max_iter, int patience, int trunc,
+                       femgpu_solve_result *res, femgpu_fcg_info *info) {
+    femgpu_ctx *c = K.c;
+    hipStream_t s = K.s;
+    const size_t n = K.n;
+    const femgpu_dtype dt = sizeof(T) == 8 ? FEMGPU_F64 : FEMGPU_F32;
+    T *r = K.vec(K.W_R), *z = K.vec(K.W_Z), *az = K.vec(K.W_AP);
+    T *nn = K.vec(12), *An = K.vec(13), *r1 = K.vec(14), *z1 = K.vec(15), *az1 = K.vec(16);
+    std::vector<T *> P(trunc), Q(trunc);
+    std::vector<double> pq_hist(trunc, 1.0), beta(trunc);
+    for (int j = 0; j < trunc; ++j) { P[j] = K.vec(20 + 2 * j); Q[j] = K.vec(21 + 2 * j); }
+    const T *inv = K.vec(K.W_INV);
+    auto dot = [&](const T *a, const T *bb) { return dot_host(c, a, bb, n, dt); };
+    FG_HIP(hipMemsetAsync(K.flags, 0, sizeof(int), s));
+    // Two-level preconditioner, z = B r, together with A z:
+    //   z1 = C r              (Chebyshev-Jacobi smoothing)
+    //   r1 = r - A z1         (the smooth residual the smoother leaves)
+    //   n  = N(r1), gamma = n.r1 / n.An   (network correction, Galerkin-scaled)
+    //   z  = z1 + gamma n,    A z = A z1 + gamma A n
+    // Without a usable network output z = C r.
+    auto precond = [&](const T *rin, T *zout, T *azout) {
+        K.cheb_apply(cp, rin, inv, z1);
+        K.apply(z1, az1, false);
+        copy<T>(s, r1, rin, n);
+        axpby<T>(s, r1, -1.0, az1, 1.0, n);          // r1 = r - A z1
+        copy<T>(s, zout, z1, n);
+        copy<T>(s, azout, az1, n);
+        const double rn = std::sqrt(dot(r1, r1));
+        if (!unet_apply_dev(net, r1, nn, dt, rn)) return;
+        K.apply(nn, An, false);
+        const void *da[2] = {nn, nn}, *db[2] = {An, r1};
+        double d[2];
+        dots_host(c, 2, da, db, n, dt, d);
+        const double nAn = d[0], nr = d[1];
+        if (!(nAn > 0.0) || !std::isfinite(nAn) || !std::isfinite(nr)) return;
+        const double gamma = nr / nAn;
+        axpby<T>(s, zout, gamma, nn, 1.0, n);
+        axpby<T>(s, azout, gamma, An, 1.0, n);
+        ++info->net_corrections;
+    };
+    info->fcg_iterations = 0; info->fallback_iterations = 0; info->fell_back = 0; info->fallback_reason = 0;
+    info->net_corrections = 0;
+    auto true_residual = [&]() {
+        K.apply(x, az, false);
+        copy<T>(s, r, b, n);
+        axpby<T>(s, r, -1.0, az, 1.0, n);              // r = b - A x
+        return std::sqrt(dot(r, r));
+    };
+    double rr = true_residual();
+    const double bnorm = std::sqrt(dot(b, b));
+    res->final_residual_is_relative = 1;
+    res->error_estimate = 0.0;
+    if (bnorm < 1e-30) {
+        res->converged = 1; res->iterations = 0; res->final_residual = 0.0;
+        return FEMGPU_OK;
+    }
+    double best = rr;
+    int since_best = 0, reason = 0, replacements = 0;
+    precond(r, z, az);
+    uint64_t it = 0, stored = 0;  // stored: directions kept so far (<= trunc)
+    for (; it < max_iter; ++it) {
+        // p = z - sum_j (z.q_j / p_j.q_j) p_j and q = A p = A z - sum_j (...) q_j
+        // over the kept directions (A-orthogonalisation, Notay's FCG).
+        const int slot = (int)(it % trunc);
+        T *p = P[slot], *qk = Q[slot];
+        const int m = (int)std::min<uint64_t>(stored, (uint64_t)trunc - 1);
+        std::vector<const void *> da(m), db(m);
+        std::vector<int> sj(m);
+        for (int j = 0; j < m; ++j) {
+            sj[j] = (int)((it - 1 - j) % trunc);
+            da[j] = z; db[j] = Q[sj[j]];
+        }
+        dots_host(c, m, da.data(), db.data(), n, dt, beta.data());
+        copy<T>(s, p, z, n);
+        copy<T>(s, qk, az, n);
+        for (int j = 0; j < m; ++j) {
+            const double bj = -beta[j] / pq_hist[sj[j]];
+            axpby<T>(s, p, bj, P[sj[j]], 1.0, n);
+            axpby<T>(s, qk, bj, Q[sj[j]], 1.0, n);
+        }
+        const void *ea[2] = {p, p}, *eb[2] = {qk, r};
+        double e2[2];
+        dots_host(c, 2, ea, eb, n, dt, e2);
+        const double pq = e2[0];
+        if (!std::isfinite(pq)) { reason = 2; break; }
+        if (!(pq > 0.0)) { reason = 3; break; }
+        pq_hist[slot] = pq;
+        if (stored < (uint64_t)trunc) ++stored;
+        const double alpha = e2[1] / pq;
+        axpby<T>(s, x, alpha, p, 1.0, n);
+        axpby<T>(s, r, -alpha, qk, 1.0, n);
+        rr = std::sqrt(dot(r, r));
+        if (!std::isfinite(rr)) { reason = 2; break; }
+        if (rr / bnorm < rtol || rr < atol) {
+            // The recurrence residual says converged: confirm with b - A x and,
+            // if that is not yet below the tolerance, continue from it.
+            rr = true_residual();
+            if (rr / bnorm < rtol || rr < atol) {
+                info->fcg_iterations = it + 1;
+                res->converged = 1; res->iterations = it + 1; res->final_residual = rr / bnorm;
+                return FEMGPU_OK;
+            }
+            if (++replacements > 8) { reason = 1; ++it; break; }
+        }
+        if (rr < best) { best = rr; since_best = 0; }
+        else if (++since_best >= patience) { reason = 1; ++it; break; }
+        precond(r, z, az);
+    }
+    info->fcg_iterations = it;
+    if (!reason) {
+        res->converged = 0; res->iterations = it; res->final_residual = rr / bnorm;
+        return FEMGPU_ERR_NOT_CONVERGED;
+    }
+    // Fallback: plain PCG + Chebyshev-Jacobi from the current iterate.  A
+    // non-finite iterate is reset to zero first.
+    info->fell_back = 1; info->fallback_reason = reason;
+    if (reason == 2) fill<T>(s, x, T(0), n);
+    const uint64_t left = max_iter > it ? max_iter - it : 1;
+    femgpu_solve_result r2{};
+    const int rc = K.solve(MODE_PCG, PREC_CHEB, cp, b, x, rtol, atol, left, &r2);
+    info->fallback_iterations = r2.iterations;
+    *res = r2;
+    res->iterations = it + r2.iterations;
+    return rc;
+}
+
+}  // namespace femgpu
+
+extern "C" {
+
+int femgpu_chebyshev_jacobi_apply(femgpu_op *op, const femgpu_cheb_config *cheb, const void *r,
+                                  void *z, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(op && r && z, FEMGPU_ERR_INVALID_ARG, "chebyshev_jacobi_apply: NULL argument");
+        std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
+        FG_HIP(hipSetDevice(op->ctx->device));
+        const size_t bytes = op->n * op->esize();
+        const void *rd = r;
+        void *zd = z;
+        if (mem == FEMGPU_HOST) {
+            rd = op->wsvec(15, bytes); zd = op->wsvec(16, bytes);
+            FG_HIP(hipMemcpyAsync((void *)rd, r, bytes, hipMemcpyHostToDevice, op->ctx->stream));
+        }
+        dispatch(op, [&](auto &k) {
+            using T = std::remove_pointer_t<decltype(k.vec(0))>;
+            ChebParams cp = k.prepare_cheb(cheb);
+            FG_HIP(hipMemsetAsync(k.flags, 0, sizeof(int), k.s));
+            k.cheb_apply(cp, (const T *)rd, (const T *)k.vec(k.W_INV), (T *)zd);
+            return 0;
+        });
+        FG_HIP(hipGetLastError());
+        if (mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(z, zd, bytes, hipMemcpyDeviceToHost, op->ctx->stream));
+        FG_HIP(hipStreamSynchronize(op->ctx->stream));
+        return 0;
+    });
+}
+
+int femgpu_solve_fcg_learned(femgpu_op *op, femgpu_unet *net, const femgpu_cheb_config *cheb,
+                             const void *b, void *x, femgpu_mem mem,
+                             const femgpu_solver_config *cfg, femgpu_solve_result *res,
+                             femgpu_fcg_info *info) {
+    return guard([&] {
+        FG_REQUIRE(op && net && b && x && res && info, FEMGPU_ERR_INVALID_ARG, "solve_fcg_learned: NULL argument");
+        check_cfg(cfg);
+        FG_REQUIRE(unet_voxels(net) == (long long)op->n, FEMGPU_ERR_DIM_MISMATCH,
+                   "solve_fcg_learned: network grid and operator size differ");
+        FG_REQUIRE(unet_ctx(net) == op->ctx, FEMGPU_ERR_INVALID_ARG, "solve_fcg_learned: different contexts");
+        std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
+        FG_HIP(hipSetDevice(op->ctx->device));
+        Staged st(op, b, x, mem, 17, 18);
+        const int patience = info->patience > 0 ? info->patience : 20;
+        const int trunc = info->truncation > 0 ? (info->truncation < kFcgTruncMax ? info->truncation : kFcgTruncMax) : 16;
+        const int rc = dispatch(op, [&](auto &k) {
+            using T = std::remove_pointer_t<decltype(k.vec(0))>;
+            ChebParams cp = k.prepare_cheb(cheb);
+            return fcg_learned<T>(k, net, cp, (const T *)st.b, (T *)st.x, cfg->rtol, cfg->atol,
+                                  cfg->max_iter, patience, trunc, res, info);
+        });
+        st.finish();
+        if (rc == FEMGPU_ERR_NOT_CONVERGED)
+            return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +
+                                " iterations (residual = " + std::to_string(res->final_residual) + ")");
+        return rc;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1142 @@
+// Learned preconditioner: a 3-level 3-D convolutional U-Net on the structured
+// DOF grid, BF16 activations and weights on MFMA with FP32 accumulation, with
+// hand-written forward, backward and Adam kernels.  No reference counterpart.
+//
+// Network (grid levels D0 = DOF grid, D1 = ceil(D0/2), D2 = ceil(D1/2);
+// channels 16 / 32 / 64; every conv is 3x3x3 with zero padding):
+//   e0 = relu(conv_{4->16}(in))            @D0   in = (r/|r|, aux, 0, 0)
+//   e1 = relu(conv_{16->32}(pool(e0)))     @D1
+//   e2 = relu(conv_{32->64}(pool(e1)))     @D2
+//   d1 = relu(up(conv_{64->32}(e2)) + e1)  @D1
+//   d0 = relu(up(conv_{32->16}(d1)) + e0)  @D0
+//   out = conv_{16->1}(d0)                 @D0
+// pool = 2x2x2 average (ceil), up = nearest.  The decoder convolves at the
+// coarse level before upsampling, so the full-resolution work is two convs.
+//
+// Activations are channels-last, v = x + nx (y + ny z).  A conv is an implicit
+// GEMM: M = voxels, N = output channels, K = 27 taps x input channels with
+// k = tap * Cin + c, tap = (dz+1)*9 + (dy+1)*3 + (dx+1).  Weights are stored
+// [Cout][Kpad] (Kpad = 27*Cin rounded up to 16) so one lane's four K values are
+// one 8-byte load.  MFMA v_mfma_f32_16x16x16_bf16: lane l holds A[m = l%16]
+// [k = 4(l/16)..+3], B[k = 4(l/16)..+3][n = l%16] and D[m = 4(l/16)+i][n = l%16].
+#include "internal.hpp"
+#include "philox.hpp"
+#include "reduce.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+
+namespace femgpu {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct Vol {
+    int nx, ny, nz;
+    __host__ __device__ long long n() const { return (long long)nx * ny * nz; }
+};
+
+static inline Vol half(Vol d) { return Vol{(d.nx + 1) / 2, (d.ny + 1) / 2, (d.nz + 1) / 2}; }
+
+static inline int kpad_of(int cin) { return (27 * cin + 15) / 16 * 16; }
+
+static unsigned g1(long long n, int b = 256) {
+    long long g = (n + b - 1) / b;
+    if (g > (1ll << 30)) g = 1ll << 30;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ------------------------------------------------------------ conv fwd --
+// out[v][n] = act(sum_k A[v][k] W[n][k] + bias[n] (+ skip[v][n]))
+// OUTF: 0 -> bf16 output with bias/skip/relu; 1 -> f32 output, plain sum.
+template <int COUT, int OUTF>
+__global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int cin, int kpad,
+                                                  const uint16_t *__restrict__ in,
+                                                  const uint16_t *__restrict__ W,
+                                                  const float *__restrict__ bias,
+                                                  const uint16_t *__restrict__ skip,
+                                                  void *__restrict__ outp, int relu) {
+    constexpr int NT = COUT / 16, MT = 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long V = d.n();
+    const long long vbase = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
+    if (vbase >= V) return;  // uniform per wave; no block barriers in this kernel
+    int vx[MT], vy[MT], vz[MT];
+    bool vin[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const long long v = vbase + mt * 16 + (lane & 15);
+        vin[mt] = v < V;
+        const long long vv = vin[mt] ? v : 0;
+        vx[mt] = (int)(vv % d.nx);
+        const long long r = vv / d.nx;
+        vy[mt] = (int)(r % d.ny);
+        vz[mt] = (int)(r / d.ny);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int ksub = (lane >> 4) * 4;
+    for (int k0 = 0; k0 < kpad; k0 += 16) {
+        const int k = k0 + ksub;
+        const int tap = k / cin, c = k - tap * cin;
+        bf16x4 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            b[nt] = *reinterpret_cast<const bf16x4 *>(W + (size_t)(nt * 16 + (lane & 15)) * kpad + k);
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            bf16x4 a = bf16x4{0, 0, 0, 0};
+            const int x = vx[mt] + dx, y = vy[mt] + dy, z = vz[mt] + dz;
+            if (tap < 27 && vin[mt] && x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz) {
+                const size_t nb = (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z);
+                a = *reinterpret_cast<const bf16x4 *>(in + nb * cin + c);
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int n = nt * 16 + (lane & 15);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const long long v = vbase + mt * 16 + (lane >> 4) * 4 + i;
+                if (v >= V) continue;
+                float val = acc[mt][nt][i];
+                if (OUTF) {
+                    reinterpret_cast<float *>(outp)[(size_t)v * COUT + n] = val;
+                } else {
+                    if (bias) val += bias[n];
+                    if (skip) val += bf2f(skip[(size_t)v * COUT + n]);
+                    if (relu) val = fmaxf(val, 0.f);
+                    reinterpret_cast<uint16_t *>(outp)[(size_t)v * COUT + n] = f2bf(val);
+                }
+            }
+        }
+}
+
+// ----------------------------------------------------- conv weight grad --
+// part[tap][chunk][n][c] = sum_{v in chunk} dy[v][n] x[v + off(tap)][c],
+// n < cout (padded to 16s), c < cin (padded to 16s).  Grid (chunks, 27).
+template <int NT, int CT>
+__global__ __launch_bounds__(256) void k_conv_wgrad(Vol d, int cin, int cout,
+                                                    const uint16_t *__restrict__ x,
+                                                    const uint16_t *__restrict__ dy,
+                                                    float *__restrict__ part, long long chunk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tap = blockIdx.y;
+    const int dx = tap % 3 - 1, dyo = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const long long V = d.n();
+    const long long v0 = (long long)blockIdx.x * chunk, v1 = v0 + chunk < V ? v0 + chunk : V;
+    f32x4 acc[NT][CT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[nt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long long vs = v0 + wave * 16; vs < v1; vs += 64) {
+        bf16x4 a[NT], b[CT];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long v = vs + (lane >> 4) * 4 + i;
+            const bool ok = v < v1;
+            // A[m = n][k = v]
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int n = nt * 16 + (lane & 15);
+                a[nt][i] = (ok && n < cout) ? (short)dy[(size_t)v * cout + n] : (short)0;
+            }
+            // B[k = v][n = c]
+            size_t nb = 0;
+            bool in = false;
+            if (ok) {
+                const int xx = (int)(v % d.nx);
+                const long long r = v / d.nx;
+                const int yy = (int)(r % d.ny), zz = (int)(r / d.ny);
+                const int X = xx + dx, Y = yy + dyo, Z = zz + dz;
+                in = X >= 0 && X < d.nx && Y >= 0 && Y < d.ny && Z >= 0 && Z < d.nz;
+                nb = (size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z);
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const int c = ct * 16 + (lane & 15);
+                b[ct][i] = (in && c < cin) ? (short)x[nb * cin + c] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc[nt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[nt], b[ct], acc[nt][ct], 0, 0, 0);
+    }
+    // block reduction of the 4 waves through LDS, then one partial per block
+    __shared__ float red[4][NT * CT * 256];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = (lane >> 4) * 4 + i, c = lane & 15;
+                red[wave][(nt * CT + ct) * 256 + m * 16 + c] = acc[nt][ct][i];
+            }
+    __syncthreads();
+    const int ncp = CT * 16, nnp = NT * 16;
+    float *dst = part + ((size_t)tap * gridDim.x + blockIdx.x) * (size_t)(nnp * ncp);
+    for (int t = threadIdx.x; t < NT * CT * 256; t += 256) {
+        const int tile = t / 256, w = t % 256, nt = tile / CT, ct = tile % CT;
+        const int m = w / 16, c = w % 16;
+        const float s = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        dst[(size_t)(nt * 16 + m) * ncp + ct * 16 + c] = s;
+    }
+}
+
+// dW[n][tap*cin + c] = sum over chunks (fixed order) of part
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nchunk, int cin, int cout,
+                               int nnp, int ncp, int kpad, float *__restrict__ dW) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= 27 * cout * cin) return;
+    const int c = id % cin, n = (id / cin) % cout, tap = id / (cin * cout);
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch)
+        s += part[((size_t)tap * nchunk + ch) * (size_t)(nnp * ncp) + (size_t)n * ncp + c];
+    dW[(size_t)n * kpad + tap * cin + c] = (float)s;
+}
+
+// ----------------------------------------------- 1-channel output layer --
+// out[v] = b + sum_tap sum_c W[tap*16 + c] d0[v+off][c]     (VALU, f32 out)
+__global__ __launch_bounds__(256) void k_out_fwd(Vol d, const uint16_t *__restrict__ in,
+                                                 const float *__restrict__ W,
+                                                 const float *__restrict__ bptr,
+                                                 float *__restrict__ out) {
+    __shared__ float sW[432];
+    for (int i = threadIdx.x; i < 432; i += 256) sW[i] = W[i];
+    __syncthreads();
+    const long long V = d.n();
+    const long long v = blockIdx.x * 256ll + threadIdx.x;
+    if (v >= V) return;
+    const int x = (int)(v % d.nx);
+    const long long r = v / d.nx;
+    const int y = (int)(r % d.ny), z = (int)(r / d.ny);
+    float s = bptr[0];
+    for (int tap = 0; tap < 27; ++tap) {
+        const int X = x + tap % 3 - 1, Y = y + (tap / 3) % 3 - 1, Z = z + tap / 9 - 1;
+        if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
+        const uint16_t *p = in + ((size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z)) * 16;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += sW[tap * 16 + c] * bf2f(p[c]);
+    }
+    out[v] = s;
+}
+
+// dIn[u][c] = sum_tap W[tap*16+c] dout[u - off(tap)]   (f32 out)
+__global__ __launch_bounds__(256) void k_out_bwd_data(Vol d, const float *__restrict__ dout,
+                                                      const float *__restrict__ W,
+                                                      float *__restrict__ din) {
+    __shared__ float sW[432];
+    for (int i = threadIdx.x; i < 432; i += 256) sW[i] = W[i];
+    __syncthreads();
+    const long long V = d.n();
+    const long long u = blockIdx.x * 256ll + threadIdx.x;
+    if (u >= V) return;
+    const int x = (int)(u % d.nx);
+    const long long r = u / d.nx;
+    const int y = (int)(r % d.ny), z = (int)(r / d.ny);
+    float acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = 0.f;
+    for (int tap = 0; tap < 27; ++tap) {
+        const int X = x - (tap % 3 - 1), Y = y - ((tap / 3) % 3 - 1), Z = z - (tap / 9 - 1);
+        if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
+        const float g = dout[(size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z)];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += sW[tap * 16 + c] * g;
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) din[(size_t)u * 16 + c] = acc[c];
+}
+
+// ----------------------------------------------------- element-wise ops --
+// avg pool 2x2x2 (ceil): out[V1][C] from in[V0][C]  (bf16)
+__global__ void k_pool(Vol f, Vol c, int C, const uint16_t *__restrict__ in, uint16_t *__restrict__ out) {
+    const long long total = c.n() * C;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(id % C);
+        const long long v = id / C;
+        const int x = (int)(v % c.nx);
+        const long long r = v / c.nx;
+        const int y = (int)(r % c.ny), z = (int)(r / c.ny);
+        float s = 0.f;
+        int cnt = 0;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int X = 2 * x + dx, Y = 2 * y + dy, Z = 2 * z + dz;
+                    if (X >= f.nx || Y >= f.ny || Z >= f.nz) continue;
+                    s += bf2f(in[((size_t)X + (size_t)f.nx * ((size_t)Y + (size_t)f.ny * Z)) * C + ch]);
+                    ++cnt;
+                }
+        out[id] = f2bf(s / cnt);
+    }
+}
+
+// unpool (pool backward): dfine[v][ch] += dcoarse[parent][ch] / count(parent)   (f32)
+__global__ void k_unpool_add(Vol f, Vol c, int C, const float *__restrict__ dc, float *__restrict__ df) {
+    const long long total = f.n() * C;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(id % C);
+        const long long v = id / C;
+        const int x = (int)(v % f.nx);
+        const long long r = v / f.nx;
+        const int y = (int)(r % f.ny), z = (int)(r / f.ny);
+        const int X = x / 2, Y = y / 2, Z = z / 2;
+        const int cnt = (2 * X + 1 < f.nx ? 2 : 1) * (2 * Y + 1 < f.ny ? 2 : 1) * (2 * Z + 1 < f.nz ? 2 : 1);
+        df[id] += dc[((size_t)X + (size_t)c.nx * ((size_t)Y + (size_t)c.ny * Z)) * C + ch] / cnt;
+    }
+}
+
+// d = relu(up(t) + e): t coarse f32 [Vc][C], e fine bf16, d fine bf16
+__global__ void k_up_add_relu(Vol f, Vol c, int C, const float *__restrict__ t,
+                              const uint16_t *__restrict__ e, uint16_t *__restrict__ d) {
+    const long long total = f.n() * C;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(id % C);
+        const long long v = id / C;
+        const int x = (int)(v % f.nx);
+        const long long r = v / f.nx;
+        const int y = (int)(r % f.ny), z = (int)(r / f.ny);
+        const float tv = t[((size_t)(x / 2) + (size_t)c.nx * ((size_t)(y / 2) + (size_t)c.ny * (z / 2))) * C + ch];
+        d[id] = f2bf(fmaxf(tv + bf2f(e[id]), 0.f));
+    }
+}
+
+// backward of d = relu(up(t) + e): g = dd * (d > 0); de = g (or +=); dt[coarse] = sum children g
+__global__ void k_up_add_relu_bwd_fine(long long n, const float *__restrict__ dd,
+                                       const uint16_t *__restrict__ d, float *__restrict__ g) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        g[i] = bf2f(d[i]) > 0.f ? dd[i] : 0.f;
+}
+__global__ void k_sum_children(Vol f, Vol c, int C, const float *__restrict__ g, float *__restrict__ dt) {
+    const long long total = c.n() * C;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int ch = (int)(id % C);
+        const long long v = id / C;
+        const int x = (int)(v % c.nx);
+        const long long r = v / c.nx;
+        const int y = (int)(r % c.ny), z = (int)(r / c.ny);
+        float s = 0.f;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const int X = 2 * x + dx, Y = 2 * y + dy, Z = 2 * z + dz;
+                    if (X >= f.nx || Y >= f.ny || Z >= f.nz) continue;
+                    s += g[((size_t)X + (size_t)f.nx * ((size_t)Y + (size_t)f.ny * Z)) * C + ch];
+                }
+        dt[id] = s;
+    }
+}
+
+// y_bf16 = x_f32 * (mask ? (m > 0) : 1)
+__global__ void k_to_bf16_mask(long long n, const float *__restrict__ x, const uint16_t *__restrict__ m,
+                               uint16_t *__restrict__ y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = f2bf((m && !(bf2f(m[i]) > 0.f)) ? 0.f : x[i]);
+}
+
+__global__ void k_add_bias_f32(long long V, int C, const float *__restrict__ b, float *__restrict__ t) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < V * C;
+         i += (long long)gridDim.x * blockDim.x)
+        t[i] += b[i % C];
+}
+
+__global__ void k_add_f32(long long n, const float *__restrict__ a, float *__restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] += a[i];
+}
+
+// per-channel sums of dy (bias gradient): partials per block then host-side-free final kernel
+__global__ __launch_bounds__(256) void k_chan_sum_partial(long long V, int C, const uint16_t *__restrict__ dy,
+                                                          const float *__restrict__ dyf,
+                                                          double *__restrict__ part) {
+    // thread handles channel (threadIdx.x % C) over voxels strided by (256 / C) * gridDim
+    const int per = 256 / C;
+    const int ch = threadIdx.x % C, lane_v = threadIdx.x / C;
+    double s = 0.0;
+    if (lane_v < per)
+        for (long long v = (long long)blockIdx.x * per + lane_v; v < V; v += (long long)gridDim.x * per)
+            s += dyf ? (double)dyf[v * C + ch] : (double)bf2f(dy[v * C + ch]);
+    __shared__ double red[256];
+    red[threadIdx.x] = (lane_v < per) ? s : 0.0;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        double t = 0.0;
+        for (int k = 0; k < per; ++k) t += red[k * C + threadIdx.x];
+        part[(size_t)blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+__global__ void k_chan_sum_final(int nb, int C, const double *__restrict__ part, float *__restrict__ out) {
+    const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ch >= C) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(size_t)b * C + ch];
+    out[ch] = (float)s;
+}
+
+// Wt[c][tap' * cout + n] = W[n][tap * cin + c], tap' = 26 - tap (data-gradient weights)
+__global__ void k_flip_weights(int cin, int cout, int kpad, int kpadT, const uint16_t *__restrict__ W,
+                               uint16_t *__restrict__ Wt) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= cin * kpadT) return;
+    const int c = id / kpadT, kk = id % kpadT;
+    uint16_t v = 0;
+    if (kk < 27 * cout) {
+        const int tapT = kk / cout, n = kk % cout, tap = 26 - tapT;
+        v = W[(size_t)n * kpad + tap * cin + c];
+    }
+    Wt[id] = v;
+}
+
+// network input: in[v] = (r[v] * scale, aux[v], 0, 0)
+template <class T>
+__global__ void k_make_input(long long V, const T *__restrict__ r, double scale,
+                             const float *__restrict__ aux, uint16_t *__restrict__ in) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < V;
+         v += (long long)gridDim.x * blockDim.x) {
+        in[v * 4 + 0] = f2bf((float)((double)r[v] * scale));
+        in[v * 4 + 1] = f2bf(aux ? aux[v] : 0.f);
+        in[v * 4 + 2] = 0;
+        in[v * 4 + 3] = 0;
+    }
+}
+
+// loss 0.5 |o - t|^2 / tt and dout = (o - t) / tt; partials of the loss
+__global__ __launch_bounds__(256) void k_loss(long long V, const float *__restrict__ o,
+                                              const float *__restrict__ t, double inv_tt,
+                                              float *__restrict__ dout, double *__restrict__ part) {
+    double v[1] = {0.0};
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < V; i += (long long)gridDim.x * 256) {
+        const double e = (double)o[i] - (double)t[i];
+        dout[i] = (float)(e * inv_tt);
+        v[0] += 0.5 * e * e * inv_tt;
+    }
+    block_sum<1, 256>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Adam on FP32 master weights; padding entries (mask == 0) stay zero.
+__global__ void k_adam(long long n, float *__restrict__ w, const float *__restrict__ g,
+                       float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                       float eps, float c1, float c2, const uint8_t *__restrict__ mask) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (mask && !mask[i]) continue;
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+__global__ void k_f32_to_bf16(long long n, const float *__restrict__ a, uint16_t *__restrict__ b) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] = f2bf(a[i]);
+}
+
+template <class T>
+__global__ void k_out_scale(long long V, const float *__restrict__ o, double s, T *__restrict__ z,
+                            int accumulate) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < V;
+         v += (long long)gridDim.x * blockDim.x) {
+        const T val = (T)((double)o[v] * s);
+        if (accumulate) z[v] += val;
+        else z[v] = val;
+    }
+}
+
+// Energy-norm loss: partials of a = n.r, c = n.(A n), E = e.r with n the
+// network output, r and e scaled by sc (A e = r).
+template <class T>
+__global__ __launch_bounds__(256) void k_energy_dots(long long V, const float *__restrict__ o,
+                                                     const float *__restrict__ r,
+                                                     const float *__restrict__ e,
+                                                     const T *__restrict__ An, double sc,
+                                                     double *__restrict__ part) {
+    double v[3] = {0, 0, 0};
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < V; i += (long long)gridDim.x * 256) {
+        const double ri = (double)r[i] * sc;
+        v[0] += (double)o[i] * ri;
+        v[1] += (double)o[i] * (double)An[i];
+        v[2] += (double)e[i] * sc * ri;
+    }
+    block_sum<3, 256>(v);
+    if (threadIdx.x == 0) {
+        part[0 * kRedBlocks + blockIdx.x] = v[0];
+        part[1 * kRedBlocks + blockIdx.x] = v[1];
+        part[2 * kRedBlocks + blockIdx.x] = v[2];
+    }
+}
+
+// d loss / d n = coef (r - gamma A n) with coef = -2 a / (c E), gamma = a / c.
+template <class T>
+__global__ void k_energy_grad(long long V, const float *__restrict__ r, double sc,
+                              const T *__restrict__ An, double coef, double gamma,
+                              float *__restrict__ dout) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < V;
+         i += (long long)gridDim.x * blockDim.x)
+        dout[i] = (float)(coef * ((double)r[i] * sc - gamma * (double)An[i]));
+}
+
+// ------------------------------------------------------------ network --
+struct Layer {
+    int cin, cout, kpad;
+    Vol d;  // grid the conv runs on
+    size_t woff, boff, wn;  // offsets of W [cout][kpad] and bias [cout] in the parameter array
+};
+
+}  // namespace femgpu
+
+// Opaque C-ABI handle.
+struct femgpu_unet {
+    femgpu_ctx *ctx = nullptr;
+    femgpu::Vol d0{}, d1{}, d2{};
+    femgpu::Layer L[6];
+    size_t nparam = 0;
+    femgpu::DevBuf params, grads, adam_m, adam_v, mask, wbf, wflip, aux;
+    // activations (bf16) and gradients (f32)
+    femgpu::DevBuf in, e0, p0, e1, p1, e2, t1, d1b, t0, d0b, out;
+    femgpu::DevBuf gout, gd0, ge0, gt0, gd1, ge1, gt1, ge2, gp1, gp0, gtmp, gbf, part, dpart;
+    long long step = 0;
+    uint64_t seed = 0;
+};
+
+namespace femgpu {
+
+static void unet_layout(femgpu_unet *u) {
+    const int cins[6] = {4, 16, 32, 64, 32, 16}, couts[6] = {16, 32, 64, 32, 16, 1};
+    const Vol ds[6] = {u->d0, u->d1, u->d2, u->d2, u->d1, u->d0};
+    size_t off = 0;
+    for (int l = 0; l < 6; ++l) {
+        Layer &L = u->L[l];
+        L.cin = cins[l]; L.cout = couts[l]; L.d = ds[l];
+        L.kpad = l == 5 ? 432 : kpad_of(L.cin);
+        L.wn = (size_t)L.cout * L.kpad;
+        L.woff = off; off += L.wn;
+        L.boff = off; off += L.cout;
+    }
+    u->nparam = off;
+}
+
+static void unet_alloc(femgpu_unet *u) {
+    const long long V0 = u->d0.n(), V1 = u->d1.n(), V2 = u->d2.n();
+    auto bf = [](DevBuf &b, long long n) { b.alloc((size_t)n * 2); };
+    auto f = [](DevBuf &b, long long n) { b.alloc((size_t)n * 4); };
+    f(u->params, u->nparam); f(u->grads, u->nparam); f(u->adam_m, u->nparam); f(u->adam_v, u->nparam);
+    u->mask.alloc(u->nparam);
+    bf(u->wbf, u->nparam);
+    size_t maxflip = 0;
+    for (int l = 1; l < 5; ++l) maxflip = std::max(maxflip, (size_t)u->L[l].cin * kpad_of(u->L[l].cout));
+    bf(u->wflip, maxflip);
+    f(u->aux, V0);
+    bf(u->in, V0 * 4); bf(u->e0, V0 * 16); bf(u->p0, V1 * 16); bf(u->e1, V1 * 32); bf(u->p1, V2 * 32);
+    bf(u->e2, V2 * 64); bf(u->d1b, V1 * 32); bf(u->d0b, V0 * 16);
+    f(u->t1, V2 * 32); f(u->t0, V1 * 16); f(u->out, V0);
+    f(u->gout, V0); f(u->gd0, V0 * 16); f(u->ge0, V0 * 16); f(u->gt0, V1 * 16); f(u->gd1, V1 * 32);
+    f(u->ge1, V1 * 32); f(u->gt1, V2 * 32); f(u->ge2, V2 * 64); f(u->gp1, V2 * 32); f(u->gp0, V1 * 16);
+    f(u->gtmp, std::max(V0 * 16, V1 * 32));
+    bf(u->gbf, std::max({V0 * 16, V1 * 32, V2 * 64}));
+    u->dpart.alloc(sizeof(double) * 1024 * 64);
+    u->part.alloc(sizeof(float) * 27 * 256 * 64 * 64);
+}
+
+static int wgrad_chunks(long long V) {
+    long long c = (V + 4095) / 4096;
+    if (c > 256) c = 256;
+    return (int)(c < 1 ? 1 : c);
+}
+
+static void conv_fwd(femgpu_unet *u, int l, const uint16_t *in, const uint16_t *W, const float *bias,
+                     const uint16_t *skip, void *out, bool relu, bool outf32, int cout_override = 0) {
+    const Layer &L = u->L[l];
+    hipStream_t s = u->ctx->stream;
+    const int cout = cout_override ? cout_override : L.cout;
+    const int cin = cout_override ? L.cout : L.cin;  // data-gradient: channels swap
+    const int kpad = cout_override ? kpad_of(L.cout) : L.kpad;
+    const long long V = L.d.n();
+    const dim3 grid((unsigned)((V + 255) / 256)), block(256);
+#define CF(CO, OF)                                                                              \
+    hipLaunchKernelGGL((k_conv_fwd<CO, OF>), grid, block, 0, s, L.d, cin, kpad, in, W, bias, skip, \
+                       out, relu ? 1 : 0)
+    if (outf32) {
+        if (cout == 16) CF(16, 1); else if (cout == 32) CF(32, 1); else CF(64, 1);
+    } else {
+        if (cout == 16) CF(16, 0); else if (cout == 32) CF(32, 0); else CF(64, 0);
+    }
+#undef CF
+}
+
+// dW (f32, into grads at layer offset) from input x (bf16 [V][cin]) and dy (bf16 [V][cout])
+static void conv_wgrad(femgpu_unet *u, int l, const uint16_t *x, const uint16_t *dy) {
+    const Layer &L = u->L[l];
+    hipStream_t s = u->ctx->stream;
+    const long long V = L.d.n();
+    const int nch = wgrad_chunks(V);
+    const long long chunk = (V + nch - 1) / nch;
+    const int NT = (L.cout + 15) / 16, CT = (L.cin + 15) / 16;
+    float *part = u->part.as<float>();
+    const dim3 grid((unsigned)nch, 27), block(256);
+#define WG(A, B) hipLaunchKernelGGL((k_conv_wgrad<A, B>), grid, block, 0, s, L.d, L.cin, L.cout, x, dy, part, chunk)
+    if (NT == 1 && CT == 1) WG(1, 1);
+    else if (NT == 2 && CT == 1) WG(2, 1);
+    else if (NT == 4 && CT == 2) WG(4, 2);
+    else if (NT == 2 && CT == 4) WG(2, 4);
+    else if (NT == 1 && CT == 2) WG(1, 2);
+    else throw Error(FEMGPU_ERR_UNSUPPORTED, "conv_wgrad: unsupported channel tiling");
+#undef WG
+    float *dW = u->grads.as<float>() + L.woff;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(g1(27ll * L.cout * L.cin)), dim3(256), 0, s, part, nch, L.cin,
+                       L.cout, NT * 16, CT * 16, L.kpad, dW);
+}
+
+static void chan_sum(femgpu_unet *u, long long V, int C, const uint16_t *dy, const float *dyf, float *out) {
+    hipStream_t s = u->ctx->stream;
+    const int nb = 512;
+    hipLaunchKernelGGL(k_chan_sum_partial, dim3(nb), dim3(256), 0, s, V, C, dy, dyf, u->dpart.as<double>());
+    hipLaunchKernelGGL(k_chan_sum_final, dim3(1), dim3(64), 0, s, nb, C, (const double *)u->dpart.as<double>(), out);
+}
+
+static void sync_bf16(femgpu_unet *u) {
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3(g1((long long)u->nparam)), dim3(256), 0, u->ctx->stream,
+                       (long long)u->nparam, (const float *)u->params.as<float>(), u->wbf.as<uint16_t>());
+}
+
+static const uint16_t *Wb(femgpu_unet *u, int l) { return u->wbf.as<uint16_t>() + u->L[l].woff; }
+static const float *Bf(femgpu_unet *u, int l) { return u->params.as<float>() + u->L[l].boff; }
+
+// forward from u->in; leaves every activation for the backward pass; out f32 [V0]
+static void forward(femgpu_unet *u) {
+    hipStream_t s = u->ctx->stream;
+    const Vol D0 = u->d0, D1 = u->d1, D2 = u->d2;
+    conv_fwd(u, 0, u->in.as<uint16_t>(), Wb(u, 0), Bf(u, 0), nullptr, u->e0.ptr, true, false);
+    hipLaunchKernelGGL(k_pool, dim3(g1(D1.n() * 16)), dim3(256), 0, s, D0, D1, 16, (const uint16_t *)u->e0.as<uint16_t>(), u->p0.as<uint16_t>());
+    conv_fwd(u, 1, u->p0.as<uint16_t>(), Wb(u, 1), Bf(u, 1), nullptr, u->e1.ptr, true, false);
+    hipLaunchKernelGGL(k_pool, dim3(g1(D2.n() * 32)), dim3(256), 0, s, D1, D2, 32, (const uint16_t *)u->e1.as<uint16_t>(), u->p1.as<uint16_t>());
+    conv_fwd(u, 2, u->p1.as<uint16_t>(), Wb(u, 2), Bf(u, 2), nullptr, u->e2.ptr, true, false);
+    // decoder: t1 = conv(e2) + b (f32), d1 = relu(up(t1) + e1)
+    conv_fwd(u, 3, u->e2.as<uint16_t>(), Wb(u, 3), nullptr, nullptr, u->t1.ptr, false, true);
+    hipLaunchKernelGGL(k_add_bias_f32, dim3(g1(D2.n() * 32)), dim3(256), 0, s, D2.n(), 32, Bf(u, 3), u->t1.as<float>());
+    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D1.n() * 32)), dim3(256), 0, s, D1, D2, 32, (const float *)u->t1.as<float>(),
+                       (const uint16_t *)u->e1.as<uint16_t>(), u->d1b.as<uint16_t>());
+    conv_fwd(u, 4, u->d1b.as<uint16_t>(), Wb(u, 4), nullptr, nullptr, u->t0.ptr, false, true);
+    hipLaunchKernelGGL(k_add_bias_f32, dim3(g1(D1.n() * 16)), dim3(256), 0, s, D1.n(), 16, Bf(u, 4), u->t0.as<float>());
+    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D0.n() * 16)), dim3(256), 0, s, D0, D1, 16, (const float *)u->t0.as<float>(),
+                       (const uint16_t *)u->e0.as<uint16_t>(), u->d0b.as<uint16_t>());
+    hipLaunchKernelGGL(k_out_fwd, dim3(g1(D0.n())), dim3(256), 0, s, D0, (const uint16_t *)u->d0b.as<uint16_t>(),
+                       (const float *)(u->params.as<float>() + u->L[5].woff), Bf(u, 5), u->out.as<float>());
+}
+
+// backward from u->gout (dL/dout, f32 [V0]); fills u->grads
+static void backward(femgpu_unet *u) {
+    hipStream_t s = u->ctx->stream;
+    const Vol D0 = u->d0, D1 = u->d1, D2 = u->d2;
+    const long long V0 = D0.n(), V1 = D1.n(), V2 = D2.n();
+    float *G = u->grads.as<float>();
+    uint16_t *gbf = u->gbf.as<uint16_t>();
+    FG_HIP(hipMemsetAsync(G, 0, u->nparam * 4, s));
+    // L5 (16 -> 1, VALU): dW5 via the MFMA weight-gradient kernel on bf16 dout
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V0)), dim3(256), 0, s, V0, (const float *)u->gout.as<float>(), nullptr, gbf);
+    conv_wgrad(u, 5, u->d0b.as<uint16_t>(), gbf);
+    chan_sum(u, V0, 1, nullptr, u->gout.as<float>(), G + u->L[5].boff);
+    hipLaunchKernelGGL(k_out_bwd_data, dim3(g1(V0)), dim3(256), 0, s, D0, (const float *)u->gout.as<float>(),
+                       (const float *)(u->params.as<float>() + u->L[5].woff), u->gd0.as<float>());
+    // d0 = relu(up(t0) + e0): g = gd0 * (d0 > 0) -> ge0 (skip) and gt0 (sum of children)
+    hipLaunchKernelGGL(k_up_add_relu_bwd_fine, dim3(g1(V0 * 16)), dim3(256), 0, s, V0 * 16, (const float *)u->gd0.as<float>(),
+                       (const uint16_t *)u->d0b.as<uint16_t>(), u->ge0.as<float>());
+    hipLaunchKernelGGL(k_sum_children, dim3(g1(V1 * 16)), dim3(256), 0, s, D0, D1, 16, (const float *)u->ge0.as<float>(), u->gt0.as<float>());
+    // L4: t0 = conv_{32->16}(d1) + b4
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V1 * 16)), dim3(256), 0, s, V1 * 16, (const float *)u->gt0.as<float>(), nullptr, gbf);
+    conv_wgrad(u, 4, u->d1b.as<uint16_t>(), gbf);
+    chan_sum(u, V1, 16, gbf, nullptr, G + u->L[4].boff);
+    hipLaunchKernelGGL(k_flip_weights, dim3(g1(32ll * kpad_of(16))), dim3(256), 0, s, 32, 16, u->L[4].kpad, kpad_of(16), Wb(u, 4), u->wflip.as<uint16_t>());
+    conv_fwd(u, 4, gbf, u->wflip.as<uint16_t>(), nullptr, nullptr, u->gd1.ptr, false, true, 32);
+    // d1 = relu(up(t1) + e1)
+    hipLaunchKernelGGL(k_up_add_relu_bwd_fine, dim3(g1(V1 * 32)), dim3(256), 0, s, V1 * 32, (const float *)u->gd1.as<float>(),
+                       (const uint16_t *)u->d1b.as<uint16_t>(), u->ge1.as<float>());
+    hipLaunchKernelGGL(k_sum_children, dim3(g1(V2 * 32)), dim3(256), 0, s, D1, D2, 32, (const float *)u->ge1.as<float>(), u->gt1.as<float>());
+    // L3: t1 = conv_{64->32}(e2) + b3
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V2 * 32)), dim3(256), 0, s, V2 * 32, (const float *)u->gt1.as<float>(), nullptr, gbf);
+    conv_wgrad(u, 3, u->e2.as<uint16_t>(), gbf);
+    chan_sum(u, V2, 32, gbf, nullptr, G + u->L[3].boff);
+    hipLaunchKernelGGL(k_flip_weights, dim3(g1(64ll * kpad_of(32))), dim3(256), 0, s, 64, 32, u->L[3].kpad, kpad_of(32), Wb(u, 3), u->wflip.as<uint16_t>());
+    conv_fwd(u, 3, gbf, u->wflip.as<uint16_t>(), nullptr, nullptr, u->ge2.ptr, false, true, 64);
+    // L2: e2 = relu(conv_{32->64}(p1) + b2)
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V2 * 64)), dim3(256), 0, s, V2 * 64, (const float *)u->ge2.as<float>(),
+                       (const uint16_t *)u->e2.as<uint16_t>(), gbf);
+    conv_wgrad(u, 2, u->p1.as<uint16_t>(), gbf);
+    chan_sum(u, V2, 64, gbf, nullptr, G + u->L[2].boff);
+    hipLaunchKernelGGL(k_flip_weights, dim3(g1(32ll * kpad_of(64))), dim3(256), 0, s, 32, 64, u->L[2].kpad, kpad_of(64), Wb(u, 2), u->wflip.as<uint16_t>());
+    conv_fwd(u, 2, gbf, u->wflip.as<uint16_t>(), nullptr, nullptr, u->gp1.ptr, false, true, 32);
+    hipLaunchKernelGGL(k_unpool_add, dim3(g1(V1 * 32)), dim3(256), 0, s, D1, D2, 32, (const float *)u->gp1.as<float>(), u->ge1.as<float>());
+    // L1: e1 = relu(conv_{16->32}(p0) + b1)
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V1 * 32)), dim3(256), 0, s, V1 * 32, (const float *)u->ge1.as<float>(),
+                       (const uint16_t *)u->e1.as<uint16_t>(), gbf);
+    conv_wgrad(u, 1, u->p0.as<uint16_t>(), gbf);
+    chan_sum(u, V1, 32, gbf, nullptr, G + u->L[1].boff);
+    hipLaunchKernelGGL(k_flip_weights, dim3(g1(16ll * kpad_of(32))), dim3(256), 0, s, 16, 32, u->L[1].kpad, kpad_of(32), Wb(u, 1), u->wflip.as<uint16_t>());
+    conv_fwd(u, 1, gbf, u->wflip.as<uint16_t>(), nullptr, nullptr, u->gp0.ptr, false, true, 16);
+    hipLaunchKernelGGL(k_unpool_add, dim3(g1(V0 * 16)), dim3(256), 0, s, D0, D1, 16, (const float *)u->gp0.as<float>(), u->ge0.as<float>());
+    // L0: e0 = relu(conv_{4->16}(in) + b0)
+    hipLaunchKernelGGL(k_to_bf16_mask, dim3(g1(V0 * 16)), dim3(256), 0, s, V0 * 16, (const float *)u->ge0.as<float>(),
+                       (const uint16_t *)u->e0.as<uint16_t>(), gbf);
+    conv_wgrad(u, 0, u->in.as<uint16_t>(), gbf);
+    chan_sum(u, V0, 16, gbf, nullptr, G + u->L[0].boff);
+}
+
+}  // namespace femgpu
+
+namespace femgpu {
+
+// z = |r|/sqrt(V) N(r sqrt(V)/|r|) written as type T (accumulate: z +=).  Used by
+// the flexible-CG preconditioner; returns false when |r| == 0.
+template <class T>
+static bool unet_apply_t(femgpu_unet *u, const T *r, T *z, double rnorm) {
+    hipStream_t s = u->ctx->stream;
+    const long long V = u->d0.n();
+    if (!(rnorm > 0.0) || !std::isfinite(rnorm)) return false;
+    const double sc = std::sqrt((double)V) / rnorm;
+    hipLaunchKernelGGL(k_make_input<T>, dim3(g1(V)), dim3(256), 0, s, V, r, sc, (const float *)u->aux.as<float>(),
+                       u->in.as<uint16_t>());
+    forward(u);
+    hipLaunchKernelGGL(k_out_scale<T>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->out.as<float>(), 1.0 / sc, z, 0);
+    return true;
+}
+
+bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm) {
+    if (dt == FEMGPU_F64) return unet_apply_t<double>(u, (const double *)r, (double *)z, rnorm);
+    return unet_apply_t<float>(u, (const float *)r, (float *)z, rnorm);
+}
+
+long long unet_voxels(const femgpu_unet *u) { return u->d0.n(); }
+femgpu_ctx *unet_ctx(const femgpu_unet *u) { return u->ctx; }
+
+}  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+int femgpu_unet_create(femgpu_ctx *c, int nx, int ny, int nz, uint64_t seed, femgpu_unet **out) {
+    return guard([&] {
+        FG_REQUIRE(c && out && nx > 0 && ny > 0 && nz > 0, FEMGPU_ERR_INVALID_ARG, "unet_create: bad arguments");
+        FG_REQUIRE((long long)nx * ny * nz < (1ll << 31), FEMGPU_ERR_INVALID_ARG, "unet_create: grid too large");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        auto *u = new femgpu_unet();
+        try {
+            u->ctx = c;
+            u->seed = seed;
+            u->d0 = Vol{nx, ny, nz};
+            u->d1 = half(u->d0);
+            u->d2 = half(u->d1);
+            unet_layout(u);
+            unet_alloc(u);
+            // He initialisation (normal, variance 2 / fan_in) from Philox; biases 0;
+            // the output layer starts small.
+            std::vector<float> p(u->nparam, 0.f);
+            std::vector<uint8_t> mk(u->nparam, 0);
+            uint64_t idx = 0;
+            for (int l = 0; l < 6; ++l) {
+                const Layer &L = u->L[l];
+                const int real_cin = l == 0 ? 2 : L.cin;
+                const double sd = std::sqrt(2.0 / (27.0 * real_cin)) * (l == 5 ? 0.1 : 1.0);
+                for (int n = 0; n < L.cout; ++n)
+                    for (int k = 0; k < 27 * L.cin; ++k) {
+                        const size_t at = L.woff + (size_t)n * L.kpad + k;
+                        mk[at] = 1;
+                        const double u1 = philox_u01(seed, 30, idx), u2 = philox_u01(seed, 31, idx);
+                        ++idx;
+                        p[at] = (float)(sd * std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(2.0 * M_PI * u2));
+                    }
+                for (int n = 0; n < L.cout; ++n) mk[L.boff + n] = 1;
+            }
+            hipStream_t s = c->stream;
+            FG_HIP(hipMemcpyAsync(u->params.ptr, p.data(), u->nparam * 4, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemcpyAsync(u->mask.ptr, mk.data(), u->nparam, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemsetAsync(u->adam_m.ptr, 0, u->nparam * 4, s));
+            FG_HIP(hipMemsetAsync(u->adam_v.ptr, 0, u->nparam * 4, s));
+            FG_HIP(hipMemsetAsync(u->aux.ptr, 0, (size_t)u->d0.n() * 4, s));
+            sync_bf16(u);
+            FG_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            delete u;
+            throw;
+        }
+        *out = u;
+        return 0;
+    });
+}
+
+int femgpu_unet_destroy(femgpu_unet *u) {
+    if (!u) return 0;
+    (void)hipSetDevice(u->ctx->device);
+    (void)hipStreamSynchronize(u->ctx->stream);
+    delete u;
+    return 0;
+}
+
+int femgpu_unet_n_params(const femgpu_unet *u, uint64_t *n) {
+    return guard([&] {
+        FG_REQUIRE(u && n, FEMGPU_ERR_INVALID_ARG, "unet_n_params: NULL");
+        *n = u->nparam;
+        return 0;
+    });
+}
+
+int femgpu_unet_layer_info(const femgpu_unet *u, int layer, int *cin, int *cout, int *kpad,
+                           uint64_t *woff, uint64_t *boff) {
+    return guard([&] {
+        FG_REQUIRE(u && layer >= 0 && layer < 6, FEMGPU_ERR_INVALID_ARG, "unet_layer_info: bad layer");
+        const Layer &L = u->L[layer];
+        if (cin) *cin = L.cin;
+        if (cout) *cout = L.cout;
+        if (kpad) *kpad = L.kpad;
+        if (woff) *woff = L.woff;
+        if (boff) *boff = L.boff;
+        return 0;
+    });
+}
+
+int femgpu_unet_get_params(femgpu_unet *u, float *host) {
+    return guard([&] {
+        FG_REQUIRE(u && host, FEMGPU_ERR_INVALID_ARG, "unet_get_params: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipMemcpyAsync(host, u->params.ptr, u->nparam * 4, hipMemcpyDeviceToHost, u->ctx->stream));
+        FG_HIP(hipStreamSynchronize(u->ctx->stream));
+        return 0;
+    });
+}
+
+int femgpu_unet_set_params(femgpu_unet *u, const float *host) {
+    return guard([&] {
+        FG_REQUIRE(u && host, FEMGPU_ERR_INVALID_ARG, "unet_set_params: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipMemcpyAsync(u->params.ptr, host, u->nparam * 4, hipMemcpyHostToDevice, u->ctx->stream));
+        sync_bf16(u);
+        FG_HIP(hipStreamSynchronize(u->ctx->stream));
+        return 0;
+    });
+}
+
+int femgpu_unet_set_aux(femgpu_unet *u, const float *aux, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(u && aux, FEMGPU_ERR_INVALID_ARG, "unet_set_aux: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipMemcpyAsync(u->aux.ptr, aux, (size_t)u->d0.n() * 4,
+                              mem == FEMGPU_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, u->ctx->stream));
+        FG_HIP(hipStreamSynchronize(u->ctx->stream));
+        return 0;
+    });
+}
+
+// Raw network: out = N(in) with in channel 0 = x0 (as given, no scaling) and
+// channel 1 = the aux field.  Keeps the activations for femgpu_unet_backward.
+int femgpu_unet_forward(femgpu_unet *u, const float *x0, float *out, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(u && x0 && out, FEMGPU_ERR_INVALID_ARG, "unet_forward: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipSetDevice(u->ctx->device));
+        hipStream_t s = u->ctx->stream;
+        const long long V = u->d0.n();
+        const float *xd = x0;
+        DevBuf st;
+        if (mem == FEMGPU_HOST) {
+            st.alloc(V * 4);
+            FG_HIP(hipMemcpyAsync(st.ptr, x0, V * 4, hipMemcpyHostToDevice, s));
+            xd = st.as<float>();
+        }
+        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, xd, 1.0,
+                           (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
+        forward(u);
+        FG_HIP(hipMemcpyAsync(out, u->out.ptr, V * 4, mem == FEMGPU_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+// Parameter gradient of sum(dout * out) at the last forward input.
+int femgpu_unet_backward(femgpu_unet *u, const float *dout, float *grads, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(u && dout && grads, FEMGPU_ERR_INVALID_ARG, "unet_backward: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipSetDevice(u->ctx->device));
+        hipStream_t s = u->ctx->stream;
+        const long long V = u->d0.n();
+        FG_HIP(hipMemcpyAsync(u->gout.ptr, dout, V * 4, mem == FEMGPU_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
+        backward(u);
+        FG_HIP(hipMemcpyAsync(grads, u->grads.ptr, u->nparam * 4,
+                              mem == FEMGPU_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+// One Adam step on the pair (r, e): input r sqrt(V)/|r|, target e sqrt(V)/|e|,
+// loss 0.5 |N - t|^2 / |t|^2.
+int femgpu_unet_train_step(femgpu_unet *u, const float *r, const float *e, femgpu_mem mem, float lr,
+                           double *loss) {
+    return guard([&] {
+        FG_REQUIRE(u && r && e, FEMGPU_ERR_INVALID_ARG, "unet_train_step: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipSetDevice(u->ctx->device));
+        hipStream_t s = u->ctx->stream;
+        femgpu_ctx *c = u->ctx;
+        const long long V = u->d0.n();
+        const float *rd = r, *ed = e;
+        DevBuf sr, se;
+        if (mem == FEMGPU_HOST) {
+            sr.alloc(V * 4); se.alloc(V * 4);
+            FG_HIP(hipMemcpyAsync(sr.ptr, r, V * 4, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemcpyAsync(se.ptr, e, V * 4, hipMemcpyHostToDevice, s));
+            rd = sr.as<float>(); ed = se.as<float>();
+        }
+        const double rn = std::sqrt(dot_host(c, rd, rd, V, FEMGPU_F32));
+        const double en = std::sqrt(dot_host(c, ed, ed, V, FEMGPU_F32));
+        FG_REQUIRE(rn > 0 && en > 0 && std::isfinite(rn) && std::isfinite(en), FEMGPU_ERR_INVALID_ARG,
+                   "unet_train_step: zero or non-finite pair");
+        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, rd, std::sqrt((double)V) / rn,
+                           (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
+        forward(u);
+        // target t = e sqrt(V)/|e| into gtmp (f32), |t|^2 = V
+        float *t = u->gtmp.as<float>();
+        hipLaunchKernelGGL(k_out_scale<float>, dim3(g1(V)), dim3(256), 0, s, V, ed, std::sqrt((double)V) / en, t, 0);
+        const int nb = red_blocks(V);
+        hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, V, (const float *)u->out.as<float>(), (const float *)t,
+                           1.0 / (double)V, u->gout.as<float>(), c->partials.as<double>());
+        std::vector<double> hp(nb);
+        FG_HIP(hipMemcpyAsync(hp.data(), c->partials.ptr, nb * 8, hipMemcpyDeviceToHost, s));
+        backward(u);
+        ++u->step;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)u->step), c2 = 1.f - std::pow(b2, (float)u->step);
+        hipLaunchKernelGGL(k_adam, dim3(g1((long long)u->nparam)), dim3(256), 0, s, (long long)u->nparam,
+                           u->params.as<float>(), (const float *)u->grads.as<float>(), u->adam_m.as<float>(),
+                           u->adam_v.as<float>(), lr, b1, b2, 1e-8f, c1, c2, (const uint8_t *)u->mask.as<uint8_t>());
+        sync_bf16(u);
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(s));
+        double L = 0.0;
+        for (double v : hp) L += v;
+        if (loss) *loss = L;
+        return 0;
+    });
+}
+
+// One Adam step on the energy-norm loss 1 - (n.r)^2 / ((n.A n)(e.r)) of the
+// Galerkin-scaled network correction n = N(r sqrt(V)/|r|).
+int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r, const float *e,
+                                  femgpu_mem mem, float lr, double *loss) {
+    return guard([&] {
+        FG_REQUIRE(u && op && r && e, FEMGPU_ERR_INVALID_ARG, "unet_train_step_energy: NULL");
+        FG_REQUIRE(op->ctx == u->ctx, FEMGPU_ERR_INVALID_ARG, "unet_train_step_energy: different contexts");
+        FG_REQUIRE((long long)op->n == u->d0.n(), FEMGPU_ERR_DIM_MISMATCH,
+                   "unet_train_step_energy: network grid and operator size differ");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipSetDevice(u->ctx->device));
+        hipStream_t s = u->ctx->stream;
+        femgpu_ctx *c = u->ctx;
+        const long long V = u->d0.n();
+        const float *rd = r, *ed = e;
+        DevBuf sr, se;
+        if (mem == FEMGPU_HOST) {
+            sr.alloc(V * 4); se.alloc(V * 4);
+            FG_HIP(hipMemcpyAsync(sr.ptr, r, V * 4, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemcpyAsync(se.ptr, e, V * 4, hipMemcpyHostToDevice, s));
+            rd = sr.as<float>(); ed = se.as<float>();
+        }
+        const double rn = std::sqrt(dot_host(c, rd, rd, V, FEMGPU_F32));
+        FG_REQUIRE(rn > 0 && std::isfinite(rn), FEMGPU_ERR_INVALID_ARG,
+                   "unet_train_step_energy: zero or non-finite residual");
+        const double sc = std::sqrt((double)V) / rn;
+        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, rd, sc,
+                           (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
+        forward(u);
+        const int nb = red_blocks(V);
+        double a = 0, cc = 0, E = 0;
+        auto energy = [&](auto tag) {
+            using T = decltype(tag);
+            T *nT = (T *)op->wsvec(kUnetOpSlot, V * sizeof(T));
+            T *AnT = (T *)op->wsvec(kUnetOpSlot + 1, V * sizeof(T));
+            hipLaunchKernelGGL(k_out_scale<T>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->out.as<float>(),
+                               1.0, nT, 0);
+            op->apply_dev(nT, AnT, false, nullptr);
+            hipLaunchKernelGGL(k_energy_dots<T>, dim3(nb), dim3(256), 0, s, V, (const float *)u->out.as<float>(),
+                               rd, ed, (const T *)AnT, sc, c->partials.as<double>());
+            std::vector<double> hp(3 * (size_t)kRedBlocks);
+            FG_HIP(hipMemcpyAsync(hp.data(), c->partials.ptr, hp.size() * 8, hipMemcpyDeviceToHost, s));
+            FG_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < nb; ++i) { a += hp[i]; cc += hp[kRedBlocks + i]; E += hp[2 * kRedBlocks + i]; }
+            FG_REQUIRE(cc > 0 && E > 0 && std::isfinite(a) && std::isfinite(cc) && std::isfinite(E),
+                       FEMGPU_ERR_INVALID_ARG,
+                       "unet_train_step_energy: network output or pair gives no positive energy");
+            const double gamma = a / cc, coef = -2.0 * a / (cc * E);
+            hipLaunchKernelGGL(k_energy_grad<T>, dim3(g1(V)), dim3(256), 0, s, V, rd, sc, (const T *)AnT, coef,
+                               gamma, u->gout.as<float>());
+        };
+        if (op->dtype == FEMGPU_F64) energy(double{});
+        else energy(float{});
+        backward(u);
+        ++u->step;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)u->step), c2 = 1.f - std::pow(b2, (float)u->step);
+        hipLaunchKernelGGL(k_adam, dim3(g1((long long)u->nparam)), dim3(256), 0, s, (long long)u->nparam,
+                           u->params.as<float>(), (const float *)u->grads.as<float>(), u->adam_m.as<float>(),
+                           u->adam_v.as<float>(), lr, b1, b2, 1e-8f, c1, c2, (const uint8_t *)u->mask.as<uint8_t>());
+        sync_bf16(u);
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(s));
+        if (loss) *loss = 1.0 - a * a / (cc * E);
+        return 0;
+    });
+}
+
+// z = preconditioner correction of r (see unet_apply_t); r, z of type dtype.
+int femgpu_unet_apply(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, femgpu_mem mem) {
+    return guard([&] {
+        FG_REQUIRE(u && r && z, FEMGPU_ERR_INVALID_ARG, "unet_apply: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        FG_HIP(hipSetDevice(u->ctx->device));
+        femgpu_ctx *c = u->ctx;
+        hipStream_t s = c->stream;
+        const long long V = u->d0.n();
+        const size_t es = dt == FEMGPU_F64 ? 8 : 4;
+        const void *rd = r;
+        void *zd = z;
+        DevBuf sr, sz;
+        if (mem == FEMGPU_HOST) {
+            sr.alloc(V * es); sz.alloc(V * es);
+            FG_HIP(hipMemcpyAsync(sr.ptr, r, V * es, hipMemcpyHostToDevice, s));
+            rd = sr.ptr; zd = sz.ptr;
+        }
+        const double rn = std::sqrt(dot_host(c, rd, rd, V, dt));
+        if (!unet_apply_dev(u, rd, zd, dt, rn)) FG_HIP(hipMemsetAsync(zd, 0, V * es, s));
+        if (mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(z, zd, V * es, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+// Checkpoint: "FGUNET01", grid (3 x int32), nparam (u64), adam step (i64),
+// params, adam m, adam v (f32), FNV-1a 64 checksum of everything before it.
+static uint64_t fnv1a(const void *p, size_t n, uint64_t h) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+int femgpu_unet_save(femgpu_unet *u, const char *path) {
+    return guard([&] {
+        FG_REQUIRE(u && path, FEMGPU_ERR_INVALID_ARG, "unet_save: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        std::vector<float> p(u->nparam), m(u->nparam), v(u->nparam);
+        hipStream_t s = u->ctx->stream;
+        FG_HIP(hipMemcpyAsync(p.data(), u->params.ptr, u->nparam * 4, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipMemcpyAsync(m.data(), u->adam_m.ptr, u->nparam * 4, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipMemcpyAsync(v.data(), u->adam_v.ptr, u->nparam * 4, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipStreamSynchronize(s));
+        std::ofstream f(path, std::ios::binary);
+        FG_REQUIRE(f.good(), FEMGPU_ERR_INVALID_ARG, std::string("unet_save: cannot open ") + path);
+        const char magic[8] = {'F', 'G', 'U', 'N', 'E', 'T', '0', '1'};
+        const int32_t dims[3] = {u->d0.nx, u->d0.ny, u->d0.nz};
+        const uint64_t np = u->nparam;
+        const int64_t step = u->step;
+        uint64_t h = 1469598103934665603ull;
+        auto put = [&](const void *d, size_t n) { f.write((const char *)d, n); h = fnv1a(d, n, h); };
+        put(magic, 8); put(dims, 12); put(&np, 8); put(&step, 8);
+        put(p.data(), np * 4); put(m.data(), np * 4); put(v.data(), np * 4);
+        f.write((const char *)&h, 8);
+        FG_REQUIRE(f.good(), FEMGPU_ERR_INVALID_ARG, "unet_save: write failed");
+        return 0;
+    });
+}
+
+int femgpu_unet_load(femgpu_unet *u, const char *path) {
+    return guard([&] {
+        FG_REQUIRE(u && path, FEMGPU_ERR_INVALID_ARG, "unet_load: NULL");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        std::ifstream f(path, std::ios::binary);
+        FG_REQUIRE(f.good(), FEMGPU_ERR_INVALID_ARG, std::string("unet_load: cannot open ") + path);
+        char magic[8];
+        int32_t dims[3];
+        uint64_t np = 0, hs = 0;
+        int64_t step = 0;
+        uint64_t h = 1469598103934665603ull;
+        auto get = [&](void *d, size_t n) { f.read((char *)d, n); h = fnv1a(d, n, h); };
+        get(magic, 8);
+        FG_REQUIRE(f.good() && std::memcmp(magic, "FGUNET01", 8) == 0, FEMGPU_ERR_INVALID_ARG, "unet_load: not a checkpoint");
+        get(dims, 12); get(&np, 8); get(&step, 8);
+        FG_REQUIRE(np == u->nparam, FEMGPU_ERR_DIM_MISMATCH, "unet_load: parameter count differs");
+        std::vector<float> p(np), m(np), v(np);
+        get(p.data(), np * 4); get(m.data(), np * 4); get(v.data(), np * 4);
+        f.read((char *)&hs, 8);
+        FG_REQUIRE(f.good() && hs == h, FEMGPU_ERR_INVALID_ARG, "unet_load: checksum mismatch");
+        hipStream_t s = u->ctx->stream;
+        FG_HIP(hipMemcpyAsync(u->params.ptr, p.data(), np * 4, hipMemcpyHostToDevice, s));
+        FG_HIP(hipMemcpyAsync(u->adam_m.ptr, m.data(), np * 4, hipMemcpyHostToDevice, s));
+        FG_HIP(hipMemcpyAsync(u->adam_v.ptr, v.data(), np * 4, hipMemcpyHostToDevice, s));
+        u->step = step;
+        sync_bf16(u);
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+}  // extern "C"
+
+extern "C" int femgpu_unet_aux_from_hex(femgpu_unet *u, femgpu_op *hex) {
+    return guard([&] {
+        FG_REQUIRE(u && hex, FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: NULL");
+        int Nx = 0, Ny = 0, Nz = 0;
+        FG_REQUIRE(hex_grid_dims(hex, &Nx, &Ny, &Nz), FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: not a hex operator");
+        FG_REQUIRE(Nx == u->d0.nx && Ny == u->d0.ny && Nz == u->d0.nz, FEMGPU_ERR_DIM_MISMATCH,
+                   "unet_aux_from_hex: grid of the network and the operator differ");
+        FG_REQUIRE(hex->ctx == u->ctx, FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: different contexts");
+        std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
+        hex_log_kappa_nodes(hex, u->aux.as<float>());
+        FG_HIP(hipStreamSynchronize(u->ctx->stream));
+        return 0;
+    });
+}

@@ -39,6 +39,20 @@ if not os.path.exists(LIB_PATH):
 
 lib = C.CDLL(LIB_PATH)
 
+# GPU objects still alive at interpreter exit are left to the process teardown:
+# calling into the HIP runtime while it shuts down is not safe.
+import atexit  # noqa: E402
+
+_EXITING = False
+
+
+def _mark_exit():
+    global _EXITING
+    _EXITING = True
+
+
+atexit.register(_mark_exit)
+
 P = C.c_void_p
 I = C.c_int
 D = C.c_double
@@ -117,6 +131,32 @@ _sig("femgpu_solve_ir_mixed", [P, P, C.POINTER(_IrConfig), P, P, I, C.POINTER(_S
                                C.POINTER(_SolveResult), C.POINTER(U64)])
 
 
+class _FcgInfo(C.Structure):
+    _fields_ = [("patience", I), ("truncation", I), ("fcg_iterations", U64), ("fallback_iterations", U64),
+                ("fell_back", I), ("fallback_reason", I), ("net_corrections", U64)]
+
+
+_sig("femgpu_chebyshev_jacobi_apply", [P, C.POINTER(_ChebConfig), P, P, I])
+_sig("femgpu_unet_create", [P, I, I, I, U64, C.POINTER(P)])
+_sig("femgpu_unet_destroy", [P])
+_sig("femgpu_unet_n_params", [P, C.POINTER(U64)])
+_sig("femgpu_unet_layer_info", [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(U64),
+                                C.POINTER(U64)])
+_sig("femgpu_unet_get_params", [P, P])
+_sig("femgpu_unet_set_params", [P, P])
+_sig("femgpu_unet_set_aux", [P, P, I])
+_sig("femgpu_unet_aux_from_hex", [P, P])
+_sig("femgpu_unet_forward", [P, P, P, I])
+_sig("femgpu_unet_backward", [P, P, P, I])
+_sig("femgpu_unet_train_step", [P, P, P, I, C.c_float, C.POINTER(D)])
+_sig("femgpu_unet_train_step_energy", [P, P, P, P, I, C.c_float, C.POINTER(D)])
+_sig("femgpu_unet_apply", [P, P, P, I, I])
+_sig("femgpu_unet_save", [P, C.c_char_p])
+_sig("femgpu_unet_load", [P, C.c_char_p])
+_sig("femgpu_solve_fcg_learned", [P, P, C.POINTER(_ChebConfig), P, P, I, C.POINTER(_SolverConfig),
+                                  C.POINTER(_SolveResult), C.POINTER(_FcgInfo)])
+
+
 # ---------------------------------------------------------------- errors --
 class FemGpuError(RuntimeError):
     def __init__(self, code, msg):
@@ -182,7 +222,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
-            lib.femgpu_ctx_destroy(self.h)
+            if not _EXITING:
+                lib.femgpu_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -230,7 +271,8 @@ class DeviceArray:
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib.femgpu_free(self.ctx.h, self.ptr)
+            if not _EXITING:
+                lib.femgpu_free(self.ctx.h, self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -314,7 +356,8 @@ class Operator:
 
     def close(self):
         if getattr(self, "h", None):
-            lib.femgpu_op_destroy(self.h)
+            if not _EXITING:
+                lib.femgpu_op_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -605,3 +648,146 @@ def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0
     if rc not in (OK, ERR_NOT_CONVERGED):
         _check(rc)
     return _result(r), inner.value
+
+
+def chebyshev_jacobi_apply(op, r, z, degree=4, ratio=30.0, safety=1.1, power_iters=20, lmax=0.0):
+    """z = p(D^-1 A) D^-1 r, the textbook Chebyshev-Jacobi polynomial."""
+    rp, m = _vec(r, op.dtype, op.n_dofs(), "r")
+    zp, _ = _vec(z, op.dtype, op.n_dofs(), "z")
+    cc = _ChebConfig(degree, ratio, safety, power_iters, lmax)
+    _check(lib.femgpu_chebyshev_jacobi_apply(op.h, C.byref(cc), rp, zp, m))
+
+
+# ------------------------------------------------- learned preconditioner --
+class UNet:
+    """3-level 3-D conv U-Net (16/32/64 channels, BF16 on MFMA) on an
+    nx*ny*nz grid; see include/femgpu.h for the parameter layout."""
+
+    def __init__(self, shape, seed=0, ctx=None):
+        self.ctx = ctx or default_context()
+        nx, ny, nz = shape
+        self.shape = (nx, ny, nz)
+        self.V = nx * ny * nz
+        h = P()
+        _check(lib.femgpu_unet_create(self.ctx.h, nx, ny, nz, seed, C.byref(h)))
+        self.h = h
+        n = U64()
+        _check(lib.femgpu_unet_n_params(h, C.byref(n)))
+        self.n_params = n.value
+
+    @classmethod
+    def for_operator(cls, op, seed=0):
+        n1 = op.p
+        shape = tuple(s * n1 + 1 for s in op.shape)
+        net = cls(shape, seed, op.ctx)
+        _check(lib.femgpu_unet_aux_from_hex(net.h, op.h))
+        return net
+
+    def layer_info(self, layer):
+        ci, co, kp = I(), I(), I()
+        wo, bo = U64(), U64()
+        _check(lib.femgpu_unet_layer_info(self.h, layer, C.byref(ci), C.byref(co), C.byref(kp),
+                                          C.byref(wo), C.byref(bo)))
+        return dict(cin=ci.value, cout=co.value, kpad=kp.value, woff=wo.value, boff=bo.value)
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib.femgpu_unet_get_params(self.h, p.ctypes.data))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(lib.femgpu_unet_set_params(self.h, p.ctypes.data))
+
+    def set_aux(self, aux):
+        a = np.ascontiguousarray(aux, np.float32).ravel()
+        assert a.size == self.V
+        _check(lib.femgpu_unet_set_aux(self.h, a.ctypes.data, HOST))
+
+    def forward(self, x0):
+        x0 = np.ascontiguousarray(x0, np.float32).ravel()
+        out = np.zeros(self.V, np.float32)
+        _check(lib.femgpu_unet_forward(self.h, x0.ctypes.data, out.ctypes.data, HOST))
+        return out
+
+    def backward(self, dout):
+        dout = np.ascontiguousarray(dout, np.float32).ravel()
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib.femgpu_unet_backward(self.h, dout.ctypes.data, g.ctypes.data, HOST))
+        return g
+
+    def train_step(self, r, e, lr=1e-3):
+        loss = D()
+        if isinstance(r, DeviceArray):
+            _check(lib.femgpu_unet_train_step(self.h, r.ptr, e.ptr, DEVICE, lr, C.byref(loss)))
+        else:
+            r = np.ascontiguousarray(r, np.float32); e = np.ascontiguousarray(e, np.float32)
+            _check(lib.femgpu_unet_train_step(self.h, r.ctypes.data, e.ctypes.data, HOST, lr,
+                                              C.byref(loss)))
+        return loss.value
+
+    def train_step_energy(self, op, r, e, lr=1e-3):
+        """Adam step on |e - g N(r)|_A^2 / |e|_A^2 (g the Galerkin step); returns
+        that relative energy-norm error before the step."""
+        loss = D()
+        r = np.ascontiguousarray(r, np.float32); e = np.ascontiguousarray(e, np.float32)
+        _check(lib.femgpu_unet_train_step_energy(self.h, op.h, r.ctypes.data, e.ctypes.data, HOST, lr,
+                                                 C.byref(loss)))
+        return loss.value
+
+    def aux_from(self, op):
+        """aux channel = log(kappa)/log(100) at the DOF nodes of hex operator op."""
+        _check(lib.femgpu_unet_aux_from_hex(self.h, op.h))
+
+    def apply(self, r, dtype=F64):
+        r = np.ascontiguousarray(r, _npdt(dtype))
+        z = np.zeros_like(r)
+        _check(lib.femgpu_unet_apply(self.h, r.ctypes.data, z.ctypes.data, dtype, HOST))
+        return z
+
+    def save(self, path):
+        _check(lib.femgpu_unet_save(self.h, str(path).encode()))
+
+    def load(self, path):
+        _check(lib.femgpu_unet_load(self.h, str(path).encode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            if not _EXITING:
+                lib.femgpu_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class FcgInfo:
+    fcg_iterations: int
+    fallback_iterations: int
+    fell_back: bool
+    fallback_reason: int  # 1 stagnation, 2 non-finite, 3 p^T A p <= 0
+    net_corrections: int  # preconditioner applications that used the network
+
+
+def solve_fcg_learned(a, net, b, x, cfg=None, degree=4, ratio=30.0, safety=1.1, power_iters=20,
+                      lmax=0.0, patience=20, truncation=16):
+    """Flexible CG whose preconditioner is Chebyshev-Jacobi smoothing followed by
+    the network's Galerkin-scaled correction of the remaining residual; falls
+    back to plain PCG-Chebyshev on stagnation.  Returns (SolveResult, FcgInfo)."""
+    cfg = cfg or SolverConfig()
+    _check_dims(a, b, x)
+    bp, xp, m = _solve_vectors(a, b, x)
+    r = _SolveResult()
+    info = _FcgInfo(patience, truncation, 0, 0, 0, 0, 0)
+    cc = _ChebConfig(degree, ratio, safety, power_iters, lmax)
+    rc = lib.femgpu_solve_fcg_learned(a.h, net.h, C.byref(cc), bp, xp, m, C.byref(cfg._c()),
+                                      C.byref(r), C.byref(info))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        _check(rc)
+    return _result(r), FcgInfo(int(info.fcg_iterations), int(info.fallback_iterations),
+                               bool(info.fell_back), int(info.fallback_reason), int(info.net_corrections))

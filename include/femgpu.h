@@ -249,6 +249,75 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
                           const void *b, void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
                           femgpu_solve_result *res, uint64_t *inner_iterations);
 
+/* Textbook Chebyshev-Jacobi preconditioner application z = p(D^-1 A) D^-1 r
+ * (the polynomial used inside femgpu_solve_pcg_chebyshev). */
+int femgpu_chebyshev_jacobi_apply(femgpu_op *op, const femgpu_cheb_config *cheb, const void *r,
+                                  void *z, femgpu_mem mem);
+
+/* ------------------------------------------- learned preconditioner ---- */
+/* No reference counterpart.  A 3-level 3-D convolutional U-Net (16/32/64
+ * channels, BF16 weights and activations on MFMA with FP32 accumulation) on
+ * a structured grid of nx*ny*nz points (the hex DOF grid).  Input channels:
+ * the residual (scaled to unit RMS) and an auxiliary field (log kappa).
+ * Parameters are FP32 masters laid out per layer as W[cout][kpad] (k = tap *
+ * cin + c, tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by bias[cout]. */
+typedef struct femgpu_unet femgpu_unet;
+int femgpu_unet_create(femgpu_ctx *ctx, int nx, int ny, int nz, uint64_t seed, femgpu_unet **out);
+int femgpu_unet_destroy(femgpu_unet *net);
+int femgpu_unet_n_params(const femgpu_unet *net, uint64_t *n);
+int femgpu_unet_layer_info(const femgpu_unet *net, int layer, int *cin, int *cout, int *kpad,
+                           uint64_t *woff, uint64_t *boff);
+int femgpu_unet_get_params(femgpu_unet *net, float *host);
+int femgpu_unet_set_params(femgpu_unet *net, const float *host);
+/* Auxiliary input channel (f32 per grid point). */
+int femgpu_unet_set_aux(femgpu_unet *net, const float *aux, femgpu_mem mem);
+/* aux = log(kappa)/log(100) at the DOF nodes of a hex operator of the same grid. */
+int femgpu_unet_aux_from_hex(femgpu_unet *net, femgpu_op *hex);
+/* out = N(x0, aux) (raw, no scaling); keeps activations for the backward. */
+int femgpu_unet_forward(femgpu_unet *net, const float *x0, float *out, femgpu_mem mem);
+/* grads = d(sum dout . out)/d params at the last forward input. */
+int femgpu_unet_backward(femgpu_unet *net, const float *dout, float *grads, femgpu_mem mem);
+/* One Adam step on a (residual, error) pair, both normalised to unit RMS;
+ * loss = 0.5 |N - t|^2 / |t|^2. */
+int femgpu_unet_train_step(femgpu_unet *net, const float *r, const float *e, femgpu_mem mem,
+                           float lr, double *loss);
+/* One Adam step on the energy-norm loss of the Galerkin-scaled correction:
+ * with n = N(r sqrt(V)/|r|) and g = n.r / n.A n,
+ *   loss = |e - g n|_A^2 / |e|_A^2 = 1 - (n.r)^2 / ((n.A n)(e.r)),
+ * the relative A-norm error the preconditioner leaves (r = A e; e only sets
+ * the normalisation).  r, e are f32; A is applied in op's precision. */
+int femgpu_unet_train_step_energy(femgpu_unet *net, femgpu_op *op, const float *r, const float *e,
+                                  femgpu_mem mem, float lr, double *loss);
+/* z = |r|/sqrt(V) N(r sqrt(V)/|r|), vectors of type dtype. */
+int femgpu_unet_apply(femgpu_unet *net, const void *r, void *z, femgpu_dtype dtype, femgpu_mem mem);
+/* Weights + Adam state, with a 64-bit FNV-1a checksum. */
+int femgpu_unet_save(femgpu_unet *net, const char *path);
+int femgpu_unet_load(femgpu_unet *net, const char *path);
+
+/* Flexible CG (Notay's FCG, truncated to the last `truncation` directions)
+ * with a two-level preconditioner: Chebyshev-Jacobi smoothing, then the
+ * network's correction of the smooth residual the smoother leaves, scaled by
+ * its optimal (Galerkin) step:
+ *   z1 = C r, r1 = r - A z1, n = N(r1), g = n.r1 / n.A n, z = z1 + g n.
+ * A p comes from the same recurrence as p (A z = A z1 + g A n).  Falls back to
+ * plain PCG + Chebyshev-Jacobi from the current iterate on stagnation (no new
+ * best residual for `patience` iterations), a non-finite value or loss of
+ * p^T A p > 0.  Convergence is judged on |b - A x|/|b| < rtol with the
+ * residual recomputed in the operator's precision. */
+typedef struct {
+    int patience;              /* iterations without a new best residual; default 20 */
+    int truncation;            /* directions kept for A-orthogonalisation; default 16, max 64 */
+    uint64_t fcg_iterations;   /* out: iterations done in FCG */
+    uint64_t fallback_iterations; /* out: PCG iterations after a fallback */
+    int fell_back;             /* out: 1 if the fallback ran */
+    int fallback_reason;       /* out: 1 stagnation, 2 non-finite, 3 p^T A p <= 0 */
+    uint64_t net_corrections;  /* out: preconditioner applications that used the network */
+} femgpu_fcg_info;
+int femgpu_solve_fcg_learned(femgpu_op *op, femgpu_unet *net, const femgpu_cheb_config *cheb,
+                             const void *b, void *x, femgpu_mem mem,
+                             const femgpu_solver_config *cfg, femgpu_solve_result *res,
+                             femgpu_fcg_info *info);
+
 #ifdef __cplusplus
 }
 #endif
