@@ -14,6 +14,15 @@ apply), the roofline fraction of the PA diffusion kernel (algorithmic bytes
 8 * (2N + 6 E 27) per launch over its HIP-event-timed mean duration) and a CPU
 baseline (the oracle's sum-factorised PCG-Chebyshev on the host cores, bounded
 sample).
+
+"legs" compares, on the same operator and right-hand side, the plain FP64
+solve with (a) FP32-inner PCG-Chebyshev + FP64 iterative refinement and (b)
+the same refinement around the FP32 flexible CG with the learned U-Net
+preconditioner, trained here first (energy-norm loss, pairs harvested from
+PCG-Chebyshev iterates of two other right-hand sides; training time is
+reported and is not part of the solve time).  Each leg reports its solve
+time, iterations, FP64 relative residual and relative error against an FP64
+PCG-Chebyshev solution to 1e-12.
 """
 import argparse
 import json
@@ -41,6 +50,8 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--no-legs", action="store_true", help="skip the mixed / learned comparison")
+    ap.add_argument("--train-steps", type=int, default=300)
     return ap.parse_args()
 
 
@@ -66,6 +77,73 @@ def cpu_baseline(n, p, iters, lmax):
             "kind": "port",
             "sample": f"oracle PCG + Chebyshev-Jacobi(4), sum-factorised FP64 PA, same Q{p} {n}^3 "
                       f"problem, {res.iterations} iterations ({dt:.2f} s)"}
+
+
+def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, a):
+    from femgpu import learned
+    N = K.n_dofs()
+    dev = lambda dt=np.float64: fg.DeviceArray(ctx, N, dt)
+    x0 = dev()
+    # FP64 reference solution (not timed)
+    x.copy_from(x0)
+    fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-12, max_iter=20000), lmax=lmax)
+    xref = x.download()
+    err = lambda v: float(np.linalg.norm(v - xref) / np.linalg.norm(xref))
+    bh = b.download()
+    relres = lambda v: float(np.linalg.norm(bh - K.matvec(v)) / np.linalg.norm(bh))
+    K32 = K.convert(fg.F32)
+    cfg = fg.SolverConfig(rtol=1e-8, max_iter=20000)
+
+    def timed(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            x.copy_from(x0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return out, float(np.median(ts))
+
+    (res_m, inner_m), t_m = timed(lambda: fg.solve_ir_mixed(K, K32, b, x, cfg, inner_rtol=1e-4, lmax=lmax))
+    xm = x.download()
+    mixed = {"method": "FP32 PCG + Chebyshev-Jacobi(4) inner (FP64 dots), FP64 refinement",
+             "solve_time_ms": round(1e3 * t_m, 3), "outer_iterations": res_m.iterations,
+             "inner_iterations": int(inner_m), "rel_residual_fp64": relres(xm), "rel_error": err(xm),
+             "speedup_vs_plain_fp64": round(t_plain / t_m, 3)}
+    # learned preconditioner: pairs from two other right-hand sides, energy-norm training
+    t0 = time.perf_counter()
+    pairs = []
+    for sd in (101, 102):
+        pairs += learned.harvest_pairs(K, K.rhs_random(sd), iters=(1, 3, 6, 10, 20, 35, 60), lmax=lmax)
+    net = fg.UNet.for_operator(K, seed=11)
+    rng = np.random.default_rng(0)
+    losses = []
+    for st in range(a.train_steps):
+        r_, e_ = pairs[rng.integers(len(pairs))]
+        losses.append(net.train_step_energy(K, r_, e_, 2e-3 * 0.1 ** (st / max(a.train_steps, 1))))
+    t_train = time.perf_counter() - t0
+    (res_l, inner_l, info_l), t_l = timed(lambda: fg.solve_ir_learned(K, K32, net, b, x, cfg, inner_rtol=1e-4, lmax=lmax))
+    xl = x.download()
+    (res_f, info_f), t_f = timed(lambda: fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax))
+    xf = x.download()
+    ml = {"method": "FP32 flexible CG (Chebyshev-Jacobi(4) + U-Net correction, 16 directions) inner, "
+                    "FP64 refinement",
+          "solve_time_ms": round(1e3 * t_l, 3), "outer_iterations": res_l.iterations,
+          "inner_iterations": int(inner_l), "fell_back": info_l.fell_back,
+          "rel_residual_fp64": relres(xl), "rel_error": err(xl),
+          "speedup_vs_plain_fp64": round(t_plain / t_l, 3),
+          "train": {"pairs": len(pairs), "steps": a.train_steps, "time_s": round(t_train, 2),
+                    "energy_loss_first_last": [round(float(np.mean(losses[:20])), 4),
+                                               round(float(np.mean(losses[-20:])), 4)]}}
+    fcg64 = {"method": "FP64 flexible CG with the U-Net (no refinement)",
+             "solve_time_ms": round(1e3 * t_f, 3), "iterations": res_f.iterations,
+             "rel_residual_fp64": relres(xf), "rel_error": err(xf),
+             "speedup_vs_plain_fp64": round(t_plain / t_f, 3)}
+    return {"plain_fp64": {"method": "FP64 PCG + Chebyshev-Jacobi(4)", "solve_time_ms": round(1e3 * t_plain, 3),
+                           "iterations": int(it_plain), "rel_error": err(x_plain)},
+            "mixed_fp32_ir": mixed, "mixed_learned": ml, "learned_fcg_fp64": fcg64}
 
 
 def main():
@@ -110,8 +188,10 @@ def main():
     x.upload(zero)
     _, lmax = fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=1))
 
+    x0 = dev()  # zero initial guess, copied on the device inside the timed step
+
     def step():
-        x.upload(zero)
+        x.copy_from(x0)
         return fg.solve_pcg_chebyshev(K, b, x, cfg, degree=4, lmax=lmax)[0]
 
     for _ in range(a.warmup):
@@ -138,6 +218,8 @@ def main():
 
     ms_step = 1e3 * total / a.steps
     value = N * sum(iters) / total / 1e9
+    t_plain = float(np.median(times))
+    leg = None if a.no_legs else legs(fg, ctx, K, b, x, xs, lmax, t_plain, iters[-1], a)
     cpu = None if a.no_cpu else cpu_baseline(n, p, a.cpu_iters, lmax)
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
@@ -175,6 +257,7 @@ def main():
                       f"k_hex_gather<double,{p}> (HIP events around both)",
             "algorithmic_bytes": bytes_diff,
         },
+        "legs": leg,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

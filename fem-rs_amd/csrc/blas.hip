@@ -6,6 +6,7 @@
 // and f32 vectors.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include "reduce.hpp"
 
@@ -113,6 +114,81 @@ double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dt
     return c->pinned[63];
 }
 
+// ---- multi-vector kernels (flexible CG orthogonalisation) ----
+// partials[j][block] of a . v[j], j < m <= 8, in one pass over a
+template <class T>
+__global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ a, VecPack<T> vs, size_t n,
+                                                  double *__restrict__ partials) {
+    double v[kPack];
+#pragma unroll
+    for (int j = 0; j < kPack; ++j) v[j] = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double ai = (double)a[i];
+#pragma unroll
+        for (int j = 0; j < kPack; ++j)
+            if (j < vs.m) v[j] += ai * (double)vs.v[j][i];
+    }
+    block_sum<kPack, 256>(v);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < vs.m; ++j) partials[(size_t)j * kRedBlocks + blockIdx.x] = v[j];
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_m(const double *__restrict__ partials, int nb,
+                                                     int m, double *__restrict__ out) {
+    for (int j = 0; j < m; ++j) {
+        double v[1] = {threadIdx.x < (unsigned)nb ? partials[(size_t)j * kRedBlocks + threadIdx.x] : 0.0};
+        block_sum<1, 1024>(v);
+        if (threadIdx.x == 0) out[j] = v[0];
+        __syncthreads();
+    }
+}
+
+// y = x0 + sum_{j<m} c[j] v[j] (m <= 8); y may be x0
+template <class T>
+__global__ void k_multiaxpy(T *y, const T *x0, VecPack<T> vs, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double s = (double)x0[i];
+#pragma unroll
+        for (int j = 0; j < kPack; ++j)
+            if (j < vs.m) s += vs.c[j] * (double)vs.v[j][i];
+        y[i] = (T)s;
+    }
+}
+
+template <class T>
+void multidot_host(femgpu_ctx *c, const T *a, const T *const *v, int m, size_t n, double *out) {
+    FG_REQUIRE(m >= 0 && m <= kMultiDotMax, FEMGPU_ERR_INVALID_ARG, "multidot: too many vectors");
+    if (m == 0) return;
+    const int nb = red_blocks(n);
+    double *o = c->scalars.as<double>() + kMultiDotBase;
+    for (int j0 = 0; j0 < m; j0 += kPack) {
+        VecPack<T> p{};
+        p.m = std::min(kPack, m - j0);
+        for (int j = 0; j < p.m; ++j) p.v[j] = v[j0 + j];
+        hipLaunchKernelGGL(k_multidot<T>, dim3(nb), dim3(256), 0, c->stream, a, p, n, c->partials.as<double>());
+        hipLaunchKernelGGL(k_finalize_m, dim3(1), dim3(1024), 0, c->stream,
+                           (const double *)c->partials.as<double>(), nb, p.m, o + j0);
+    }
+    FG_HIP(hipMemcpyAsync(c->pinned + kMultiDotBase, o, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->pinned + kMultiDotBase, sizeof(double) * m);
+}
+
+template <class T>
+void multiaxpy(hipStream_t s, T *y, const T *x0, const T *const *v, const double *coef, int m, size_t n) {
+    if (m == 0) {
+        if (y != x0) copy<T>(s, y, x0, n);
+        return;
+    }
+    for (int j0 = 0; j0 < m; j0 += kPack) {
+        VecPack<T> p{};
+        p.m = std::min(kPack, m - j0);
+        for (int j = 0; j < p.m; ++j) { p.v[j] = v[j0 + j]; p.c[j] = coef[j0 + j]; }
+        hipLaunchKernelGGL(k_multiaxpy<T>, dim3(grid_for(n)), dim3(256), 0, s, y, j0 == 0 ? x0 : (const T *)y, p, n);
+    }
+}
+
 void dots_host(femgpu_ctx *c, int m, const void *const *a, const void *const *b, size_t n,
                femgpu_dtype dt, double *out) {
     FG_REQUIRE(m >= 0 && m <= kMultiDotMax, FEMGPU_ERR_INVALID_ARG, "dots_host: too many dot products");
@@ -139,7 +215,9 @@ void convert_f32_to_f64_axpy(hipStream_t s, double *y, const float *x, size_t n,
     template void copy<T>(hipStream_t, T *, const T *, size_t);                          \
     template void axpby<T>(hipStream_t, T *, double, const T *, double, size_t);         \
     template void inv_diag<T>(hipStream_t, T *, const T *, size_t, double);              \
-    template void dot<T>(femgpu_ctx *, const T *, const T *, size_t, double *);
+    template void dot<T>(femgpu_ctx *, const T *, const T *, size_t, double *);          \
+    template void multidot_host<T>(femgpu_ctx *, const T *, const T *const *, int, size_t, double *); \
+    template void multiaxpy<T>(hipStream_t, T *, const T *, const T *const *, const double *, int, size_t);
 INST(double)
 INST(float)
 #undef INST

@@ -151,6 +151,20 @@ template <class T> void inv_diag(hipStream_t s, T *inv, const T *d, size_t n, do
 // Deterministic dot products in double: writes partials then finalises into out[0].
 template <class T> void dot(femgpu_ctx *c, const T *a, const T *b, size_t n, double *out_dev);
 double dot_host(femgpu_ctx *c, const void *a, const void *b, size_t n, femgpu_dtype dt);
+// Up to kPack vectors (and coefficients) passed by value to one kernel.
+constexpr int kPack = 8;
+template <class T> struct VecPack {
+    const T *v[kPack];
+    double c[kPack];
+    int m;
+};
+// out[j] = a . v[j], j < m (m <= kMultiDotMax), one pass over a per 8 vectors,
+// one synchronisation.
+template <class T>
+void multidot_host(femgpu_ctx *c, const T *a, const T *const *v, int m, size_t n, double *out);
+// y = x0 + sum_j coef[j] v[j] (y may be x0).
+template <class T>
+void multiaxpy(hipStream_t s, T *y, const T *x0, const T *const *v, const double *coef, int m, size_t n);
 // m dot products a[j].b[j] (m <= kMultiDotMax) enqueued back to back, read
 // back with one copy and one synchronisation.
 void dots_host(femgpu_ctx *c, int m, const void *const *a, const void *const *b, size_t n,

@@ -86,6 +86,18 @@ struct HexGrid {
     long long Nx, Ny, Nz;
 };
 
+// The hardware hands workgroup b to XCD b % 8.  xcd_block() renumbers the
+// workgroups so that each XCD walks one contiguous range of the nlog logical
+// blocks: DOF rows that are neighbours in space then run on the same XCD and
+// share its L2 (the gather reads 18 us -> 20.5 us without it at Q2 64^3).
+// Launch xcd_grid(nlog) = 8 ceil(nlog/8) workgroups; those mapped past nlog
+// exit at once.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nlog) {
+    const unsigned per = (nlog + 7u) / 8u;
+    return (b % 8u) * per + b / 8u;
+}
+static inline unsigned xcd_grid(size_t nlog) { return (unsigned)(8 * ((nlog + 7) / 8)); }
+
 __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long gy, long long gz) {
     return gx == 0 || gx == g.Nx - 1 || gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1;
 }
@@ -95,6 +107,25 @@ __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long
 __host__ __device__ __forceinline__ size_t evec_index(const HexGrid &g, int N1, int ex, int ey,
                                                       int ez, int j, int k) {
     return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.nx * N1 + (size_t)ex * N1;
+}
+
+// Quadrature-data layout.  Elements are grouped in blocks of NE = ne_of(Q)
+// consecutive elements (one element-kernel workgroup each); inside a block the
+// data are plane-major over (component, qz), then element-major over
+// (element, qx + Q qy):
+//   index = ((blk * ncomp + comp) * Q + qz) * NE Q^2 + le Q^2 + (qx + Q qy)
+// Each workgroup streams one contiguous range, and every (component, qz) load
+// of a workgroup is one contiguous run of NE Q^2 values.
+__host__ __device__ __forceinline__ size_t qd_index(int Q, int ncomp, long long e, int comp, int qz,
+                                                    int t) {
+    const int NE = ne_of(Q), T2 = Q * Q;
+    const long long blk = e / NE, le = e - blk * NE;
+    return (((size_t)blk * ncomp + comp) * Q + qz) * (size_t)(NE * T2) + (size_t)le * T2 + t;
+}
+// values per component (E rounded up to whole blocks)
+static inline size_t qd_count(long long E, int Q) {
+    const long long NE = ne_of(Q);
+    return (size_t)((E + NE - 1) / NE) * NE * Q * Q * Q;
 }
 
 template <int N1, int Q, class T, int FORM>
@@ -115,11 +146,14 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     __shared__ T sGB[ND][N1][Q][Q];   // Y out GB, later C_eta
 
     if (skip && *skip) return;
+    // Hardware order (consecutive workgroups on different XCDs): measured
+    // faster here than the XCD-contiguous order, which helps only the gather.
+    const unsigned lb = blockIdx.x;
     const T *Bt = Tab<T>::B(C);
     const T *Gt = Tab<T>::G(C);
     const int le = threadIdx.x / T2;
     const int t = threadIdx.x - le * T2;
-    const long long e = (long long)blockIdx.x * NE + le;
+    const long long e = (long long)lb * NE + le;
     const bool valid = e < g.E;
     int ex = 0, ey = 0, ez = 0;
     if (valid) {
@@ -193,8 +227,9 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) { bg[k] = sBG[le][k][b][a]; gb[k] = sGB[le][k][b][a]; }
             c0[k] = c1[k] = c2[k] = 0;
         }
-        const size_t plane = (size_t)g.E * T2;
-        const size_t off = (size_t)e * T2 + t;
+        constexpr size_t plane = (size_t)NE * T2;
+        const size_t offd = (size_t)lb * 6 * Q * plane + (size_t)le * T2 + t;
+        const size_t offm = (size_t)lb * Q * plane + (size_t)le * T2 + t;
 #pragma unroll
         for (int c = 0; c < Q; ++c) {
             T v0 = 0, v1 = 0, v2 = 0, vm = 0;
@@ -206,12 +241,12 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     g1 += Bt[c * N1 + k] * gb[k];
                     g2 += Gt[c * N1 + k] * bb[k];
                 }
-                const T d0 = qd[(size_t)(0 * Q + c) * plane + off];
-                const T d1 = qd[(size_t)(1 * Q + c) * plane + off];
-                const T d2 = qd[(size_t)(2 * Q + c) * plane + off];
-                const T d3 = qd[(size_t)(3 * Q + c) * plane + off];
-                const T d4 = qd[(size_t)(4 * Q + c) * plane + off];
-                const T d5 = qd[(size_t)(5 * Q + c) * plane + off];
+                const T d0 = qd[(size_t)(0 * Q + c) * plane + offd];
+                const T d1 = qd[(size_t)(1 * Q + c) * plane + offd];
+                const T d2 = qd[(size_t)(2 * Q + c) * plane + offd];
+                const T d3 = qd[(size_t)(3 * Q + c) * plane + offd];
+                const T d4 = qd[(size_t)(4 * Q + c) * plane + offd];
+                const T d5 = qd[(size_t)(5 * Q + c) * plane + offd];
                 v0 = d0 * g0 + d1 * g1 + d2 * g2;
                 v1 = d1 * g0 + d3 * g1 + d4 * g2;
                 v2 = d2 * g0 + d4 * g1 + d5 * g2;
@@ -220,7 +255,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 T uu = 0;
 #pragma unroll
                 for (int k = 0; k < N1; ++k) uu += Bt[c * N1 + k] * bb[k];
-                vm = qm[(size_t)c * plane + off] * uu;
+                vm = qm[(size_t)c * plane + offm] * uu;
             }
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
@@ -294,57 +329,59 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
 
 // Element-vector -> DOF vector: every DOF sums the (1, 2, 4 or 8) element
 // contributions that share it, in a fixed order (bitwise reproducible, no
-// atomics).  One thread per DOF of a z-plane (blockIdx.y = gz), consecutive
-// threads along x so the element-vector rows are read contiguously.
-// mode 0: y = s, 1: y += s.  Essential DOFs: y (+)= x.
+// atomics).  One thread per DOF; a logical block covers 256 consecutive DOFs
+// of one z-plane (bpp blocks per plane), consecutive threads along x so the
+// element-vector rows are read contiguously.  All eight candidate loads are
+// issued before the sum.  mode 0: y = s, 1: y += s.  Essential DOFs: y (+)= x.
 template <class T, int P>
 __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
                                                     const T *__restrict__ x, T *__restrict__ y,
-                                                    int mode, int dirichlet, const int *skip) {
+                                                    int mode, int dirichlet, unsigned bpp,
+                                                    const int *skip) {
     constexpr int N1 = P + 1;
     if (skip && *skip) return;
+    const unsigned nlog = bpp * (unsigned)g.Nz;
+    const unsigned lb = xcd_block(blockIdx.x, nlog);
+    if (lb >= nlog) return;
     const int Ny = (int)g.Ny, Nx = (int)g.Nx;
-    const int id = blockIdx.x * 256 + threadIdx.x;
+    const int gz = (int)(lb / bpp);
+    const int id = (int)(lb - (unsigned)gz * bpp) * 256 + threadIdx.x;
     if (id >= Nx * Ny) return;
-    const int gy = id / Nx, gx = id - gy * Nx, gz = blockIdx.y;
+    const int gy = id / Nx, gx = id - gy * Nx;
     const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
     if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1)) {
         if (mode == 0) y[gi] = x[gi];
         else y[gi] += x[gi];
         return;
     }
-    int ex[2], ix[2], ey[2], iy[2], ez[2], iz[2];
-    int nxc = 0, nyc = 0, nzc = 0;
-    {
-        const int e0 = gx / P, i0 = gx - e0 * P;
-        if (i0) { ex[0] = e0; ix[0] = i0; nxc = 1; }
-        else {
-            if (e0 > 0) { ex[nxc] = e0 - 1; ix[nxc++] = P; }
-            if (e0 < g.nx) { ex[nxc] = e0; ix[nxc++] = 0; }
-        }
+    // per direction two candidates: [0] = (e0 - 1, local P) when the node is
+    // the lower-face node of element e0, [1] = (e0, local i0)
+    int ce[3][2], ci[3][2];
+    bool cv[3][2];
+    const int gg[3] = {gx, gy, gz}, ne[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int e0 = gg[d] / P, i0 = gg[d] - e0 * P;
+        ce[d][0] = e0 - 1; ci[d][0] = P; cv[d][0] = i0 == 0 && e0 > 0;
+        ce[d][1] = e0;     ci[d][1] = i0; cv[d][1] = e0 < ne[d];
     }
-    {
-        const int e0 = gy / P, i0 = gy - e0 * P;
-        if (i0) { ey[0] = e0; iy[0] = i0; nyc = 1; }
-        else {
-            if (e0 > 0) { ey[nyc] = e0 - 1; iy[nyc++] = P; }
-            if (e0 < g.ny) { ey[nyc] = e0; iy[nyc++] = 0; }
-        }
-    }
-    {
-        const int e0 = gz / P, i0 = gz - e0 * P;
-        if (i0) { ez[0] = e0; iz[0] = i0; nzc = 1; }
-        else {
-            if (e0 > 0) { ez[nzc] = e0 - 1; iz[nzc++] = P; }
-            if (e0 < g.nz) { ez[nzc] = e0; iz[nzc++] = 0; }
-        }
-    }
+    T v[8];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                // invalid candidates read ye[0] (always in bounds) and add 0
+                const bool ok = cv[2][c] && cv[1][b] && cv[0][a];
+                const size_t idx =
+                    ok ? evec_index(g, N1, ce[0][a], ce[1][b], ce[2][c], ci[1][b], ci[2][c]) + ci[0][a] : 0;
+                const T val = ye[idx];
+                v[(c * 2 + b) * 2 + a] = ok ? val : T(0);
+            }
     T s = 0;
-    for (int c = 0; c < nzc; ++c)
-        for (int b = 0; b < nyc; ++b) {
-            const T *rowp = ye + evec_index(g, N1, 0, ey[b], ez[c], iy[b], iz[c]);
-            for (int a = 0; a < nxc; ++a) s += rowp[(size_t)ex[a] * N1 + ix[a]];
-        }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) s += v[m];
     if (mode == 0) y[gi] = s;
     else y[gi] += s;
 }
@@ -436,9 +473,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
         const double wd = wq[qx] * wq[qy] * wq[qz] * fabs(det);
-        const size_t plane = (size_t)g.E * T2;
-        const size_t off = (size_t)e * T2 + (qx + Q * qy);
-        if ((form & 1) && qm) qm[(size_t)qz * plane + off] = (T)(mass_only_weight ? wd : wd * rho);
+        const int t2 = qx + Q * qy;
+        if ((form & 1) && qm) qm[qd_index(Q, 1, e, 0, qz, t2)] = (T)(mass_only_weight ? wd : wd * rho);
         if ((form & 2) && qd) {
             const double id = 1.0 / det;
             double I[3][3];
@@ -460,7 +496,7 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
                 s * (I[1][0] * I[2][0] + I[1][1] * I[2][1] + I[1][2] * I[2][2]),
                 s * (I[2][0] * I[2][0] + I[2][1] * I[2][1] + I[2][2] * I[2][2])};
 #pragma unroll
-            for (int comp = 0; comp < 6; ++comp) qd[(size_t)(comp * Q + qz) * plane + off] = (T)D[comp];
+            for (int comp = 0; comp < 6; ++comp) qd[qd_index(Q, 6, e, comp, qz, t2)] = (T)D[comp];
         }
     }
 }
@@ -469,10 +505,11 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
 template <class T>
 __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restrict__ B,
                            const double *__restrict__ G, const T *__restrict__ qd,
-                           const T *__restrict__ qm, int form, int dirichlet, T *__restrict__ diag) {
-    const int L = N1 * N1 * N1, T2 = Q * Q, P = N1 - 1;
+                           const T *__restrict__ qm, int form, T *__restrict__ ye) {
+    // Element diagonals into the element vector; k_hex_gather then sums them
+    // per DOF in a fixed order.
+    const int L = N1 * N1 * N1;
     const size_t total = (size_t)g.E * L;
-    const size_t plane = (size_t)g.E * T2;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
          id += (size_t)gridDim.x * blockDim.x) {
         const long long e = (long long)(id / L);
@@ -480,28 +517,26 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
         const int i = l % N1, j = (l / N1) % N1, k = l / (N1 * N1);
         const int ex = (int)(e % g.nx);
         const int ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
-        const long long gx = (long long)ex * P + i, gy = (long long)ey * P + j, gz = (long long)ez * P + k;
-        if (dirichlet && on_bdr(g, gx, gy, gz)) continue;
         double s = 0.0;
         for (int c = 0; c < Q; ++c)
             for (int b = 0; b < Q; ++b)
                 for (int a = 0; a < Q; ++a) {
-                    const size_t off = (size_t)e * T2 + a + Q * b;
+                    const int t2 = a + Q * b;
                     const double Bi = B[a * N1 + i], Bj = B[b * N1 + j], Bk = B[c * N1 + k];
                     if (form & 2) {
                         const double g0 = G[a * N1 + i] * Bj * Bk, g1 = Bi * G[b * N1 + j] * Bk,
                                      g2 = Bi * Bj * G[c * N1 + k];
                         double d[6];
-                        for (int m = 0; m < 6; ++m) d[m] = (double)qd[(size_t)(m * Q + c) * plane + off];
+                        for (int m = 0; m < 6; ++m) d[m] = (double)qd[qd_index(Q, 6, e, m, c, t2)];
                         s += d[0] * g0 * g0 + d[3] * g1 * g1 + d[5] * g2 * g2 +
                              2.0 * (d[1] * g0 * g1 + d[2] * g0 * g2 + d[4] * g1 * g2);
                     }
                     if (form & 1) {
                         const double ph = Bi * Bj * Bk;
-                        s += (double)qm[(size_t)c * plane + off] * ph * ph;
+                        s += (double)qm[qd_index(Q, 1, e, 0, c, t2)] * ph * ph;
                     }
                 }
-        unsafeAtomicAdd(&diag[gx + g.Nx * (gy + g.Ny * gz)], (T)s);
+        ye[evec_index(g, N1, ex, ey, ez, j, k) + i] = (T)s;
     }
 }
 
@@ -542,13 +577,14 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
 template <class T>
 static void launch_gather(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
                           const int *skip, hipStream_t s) {
-    const dim3 grid((unsigned)((g.Nx * g.Ny + 255) / 256), (unsigned)g.Nz);
+    const unsigned bpp = (unsigned)((g.Nx * g.Ny + 255) / 256);
+    const dim3 grid(xcd_grid((size_t)bpp * (size_t)g.Nz));
     switch (p) {
-    case 1: hipLaunchKernelGGL((k_hex_gather<T, 1>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
-    case 2: hipLaunchKernelGGL((k_hex_gather<T, 2>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
-    case 3: hipLaunchKernelGGL((k_hex_gather<T, 3>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
-    case 4: hipLaunchKernelGGL((k_hex_gather<T, 4>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
-    default: hipLaunchKernelGGL((k_hex_gather<T, 5>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 1: hipLaunchKernelGGL((k_hex_gather<T, 1>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
+    case 2: hipLaunchKernelGGL((k_hex_gather<T, 2>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
+    case 3: hipLaunchKernelGGL((k_hex_gather<T, 3>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
+    case 4: hipLaunchKernelGGL((k_hex_gather<T, 4>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
+    default: hipLaunchKernelGGL((k_hex_gather<T, 5>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
     }
 }
 
@@ -582,10 +618,10 @@ struct HexOp final : femgpu_op {
 
     template <class T> void diag_t(T *d) {
         hipStream_t s = ctx->stream;
-        fill<T>(s, d, T(0), n);
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1((size_t)g.E * (desc.p + 1) * (desc.p + 1) * (desc.p + 1))),
                            dim3(256), 0, s, g, desc.p + 1, desc.q, Bd.as<double>(), Gd.as<double>(),
-                           qd.as<T>(), qm.as<T>(), desc.form, desc.dirichlet, d);
+                           qd.as<T>(), qm.as<T>(), desc.form, evec.as<T>());
+        launch_gather<T>(desc.p, g, evec.as<T>(), d, d, 0, 0, nullptr, s);
         hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, (const T *)d, d, 2,
                            desc.dirichlet, nullptr);
     }
@@ -620,7 +656,7 @@ struct HexOp final : femgpu_op {
         FG_HIP(hipMemcpyAsync(wq.ptr, w, 8 * q, hipMemcpyHostToDevice, s));
         FG_HIP(hipMemcpyAsync(Bd.ptr, B, 8 * q * n1, hipMemcpyHostToDevice, s));
         FG_HIP(hipMemcpyAsync(Gd.ptr, G, 8 * q * n1, hipMemcpyHostToDevice, s));
-        const size_t nq = (size_t)g.E * q * q * q;
+        const size_t nq = qd_count(g.E, q);
         const size_t es = esize();
         if (desc.form & 2) qd.alloc(6 * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
@@ -676,7 +712,7 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         g.E = (long long)d.nx * d.ny * d.nz;
         g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
         op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
-        FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536,
+        FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536 && g.E < (1ll << 31),
                    FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
         const size_t nv = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
         hipStream_t s = c->stream;
@@ -801,7 +837,7 @@ int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
         femgpu_ctx *c = op->ctx;
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         FG_HIP(hipSetDevice(c->device));
-        const size_t nq = (size_t)op->g.E * op->desc.q * op->desc.q * op->desc.q;
+        const size_t nq = qd_count(op->g.E, op->desc.q);
         const size_t bytes = op->n * op->esize();
         DevBuf tq(nq * op->esize()), ones(bytes), out(mem == FEMGPU_DEVICE ? 0 : bytes);
         void *dst = mem == FEMGPU_DEVICE ? b : out.ptr;

@@ -18,6 +18,7 @@
 #include "internal.hpp"
 #include "reduce.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -532,6 +533,77 @@ static void check_cfg(const femgpu_solver_config *cfg) {
 
 using namespace femgpu;
 
+// FP64 iterative refinement around an FP32 inner solver:
+// inner(K32, cp, r32, e32, rtol, max_iter, &result) solves A32 e = r32.
+template <class Inner>
+static int ir_refine(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *ir, const void *b,
+                     void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
+                     femgpu_solve_result *res, uint64_t *inner_iterations, Inner &&inner) {
+    FG_REQUIRE(op64 && op32 && ir && b && x && res, FEMGPU_ERR_INVALID_ARG,
+               "solve_ir_mixed: NULL argument");
+    FG_REQUIRE(op64->dtype == FEMGPU_F64 && op32->dtype == FEMGPU_F32, FEMGPU_ERR_INVALID_ARG,
+               "solve_ir_mixed: need an f64 and an f32 operator");
+    FG_REQUIRE(op64->n == op32->n && op64->ctx == op32->ctx, FEMGPU_ERR_DIM_MISMATCH,
+               "solve_ir_mixed: operators differ in size or context");
+    check_cfg(cfg);
+    femgpu_ctx *c = op64->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    FG_HIP(hipSetDevice(c->device));
+    Staged st(op64, b, x, mem, 8, 9);
+    Krylov<double> K64(op64);
+    Krylov<float> K32(op32);
+    const size_t n = op64->n;
+    hipStream_t s = c->stream;
+    double *r64 = K64.vec(K64.W_R);
+    float *r32 = K32.vec(K32.W_B), *e32 = K32.vec(K32.W_X);
+    const double *bd = (const double *)st.b;
+    double *xd = (double *)st.x;
+    ChebParams cp = K32.prepare_cheb(&ir->cheb);
+    const double bnorm = std::sqrt(dot_host(c, bd, bd, n, FEMGPU_F64));
+    uint64_t inner_total = 0, outer = 0;
+    double rel = 0.0;
+    auto residual = [&]() {
+        op64->apply_dev(xd, r64, false, nullptr);
+        hipLaunchKernelGGL(k_sub_from<double>, dim3(vgrid(n)), dim3(256), 0, s, n, bd, r64);
+        return std::sqrt(dot_host(c, r64, r64, n, FEMGPU_F64));
+    };
+    double rnorm = residual();
+    rel = bnorm > 0 ? rnorm / bnorm : 0.0;
+    const uint64_t max_outer = ir->max_outer ? ir->max_outer : 50;
+    const double inner_rtol = ir->inner_rtol > 0 ? ir->inner_rtol : 1e-4;
+    const uint64_t inner_max = ir->inner_max_iter ? ir->inner_max_iter : cfg->max_iter;
+    while (bnorm > 0 && !(rel < cfg->rtol) && outer < max_outer && inner_total < cfg->max_iter) {
+        convert_f64_to_f32(s, r32, r64, n, 1.0 / rnorm);
+        fill<float>(s, e32, 0.f, n);
+        femgpu_solve_result ir_res{};
+        // The inner problem is scaled to |r| = 1, so an inner relative
+        // residual t leaves about t |r| outside: solve no further than the
+        // outer target needs (half of it, for the FP32 / FP64 residual gap).
+        const double t = std::max(inner_rtol, 0.5 * cfg->rtol * bnorm / rnorm);
+        const uint64_t cap = std::min<uint64_t>(inner_max, cfg->max_iter - inner_total);
+        const int rc = inner(K32, cp, r32, e32, t, cap, &ir_res);
+        inner_total += ir_res.iterations;
+        (void)rc;  // an inner solve that stops early still yields a correction
+        convert_f32_to_f64_axpy(s, xd, e32, n, rnorm);
+        const double prev = rel;
+        rnorm = residual();
+        rel = rnorm / bnorm;
+        ++outer;
+        if (!(rel < prev)) break;  // stagnation: stop refining
+    }
+    st.finish();
+    if (inner_iterations) *inner_iterations = inner_total;
+    res->iterations = outer;
+    res->final_residual = rel;
+    res->final_residual_is_relative = 1;
+    res->converged = bnorm == 0 || rel < cfg->rtol;
+    res->error_estimate = 0.0;
+    if (!res->converged)
+        return fail(FEMGPU_ERR_NOT_CONVERGED,
+                    "mixed-precision refinement stopped at relative residual " + std::to_string(rel));
+    return 0;
+}
+
 extern "C" {
 
 int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
@@ -671,64 +743,11 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
                           const void *b, void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
                           femgpu_solve_result *res, uint64_t *inner_iterations) {
     return guard([&] {
-        FG_REQUIRE(op64 && op32 && ir && b && x && res, FEMGPU_ERR_INVALID_ARG,
-                   "solve_ir_mixed: NULL argument");
-        FG_REQUIRE(op64->dtype == FEMGPU_F64 && op32->dtype == FEMGPU_F32, FEMGPU_ERR_INVALID_ARG,
-                   "solve_ir_mixed: need an f64 and an f32 operator");
-        FG_REQUIRE(op64->n == op32->n && op64->ctx == op32->ctx, FEMGPU_ERR_DIM_MISMATCH,
-                   "solve_ir_mixed: operators differ in size or context");
-        check_cfg(cfg);
-        femgpu_ctx *c = op64->ctx;
-        std::lock_guard<std::recursive_mutex> lk(c->mu);
-        FG_HIP(hipSetDevice(c->device));
-        Staged st(op64, b, x, mem, 8, 9);
-        Krylov<double> K64(op64);
-        Krylov<float> K32(op32);
-        const size_t n = op64->n;
-        hipStream_t s = c->stream;
-        double *r64 = K64.vec(K64.W_R);
-        float *r32 = K32.vec(K32.W_B), *e32 = K32.vec(K32.W_X);
-        const double *bd = (const double *)st.b;
-        double *xd = (double *)st.x;
-        ChebParams cp = K32.prepare_cheb(&ir->cheb);
-        const double bnorm = std::sqrt(dot_host(c, bd, bd, n, FEMGPU_F64));
-        uint64_t inner_total = 0, outer = 0;
-        double rel = 0.0;
-        auto residual = [&]() {
-            op64->apply_dev(xd, r64, false, nullptr);
-            hipLaunchKernelGGL(k_sub_from<double>, dim3(vgrid(n)), dim3(256), 0, s, n, bd, r64);
-            return std::sqrt(dot_host(c, r64, r64, n, FEMGPU_F64));
-        };
-        double rnorm = residual();
-        rel = bnorm > 0 ? rnorm / bnorm : 0.0;
-        const uint64_t max_outer = ir->max_outer ? ir->max_outer : 50;
-        const double inner_rtol = ir->inner_rtol > 0 ? ir->inner_rtol : 1e-4;
-        const uint64_t inner_max = ir->inner_max_iter ? ir->inner_max_iter : cfg->max_iter;
-        while (bnorm > 0 && !(rel < cfg->rtol) && outer < max_outer) {
-            convert_f64_to_f32(s, r32, r64, n, 1.0 / rnorm);
-            fill<float>(s, e32, 0.f, n);
-            femgpu_solve_result ir_res{};
-            const int rc = K32.solve(MODE_PCG, PREC_CHEB, cp, r32, e32, inner_rtol, 0.0, inner_max, &ir_res);
-            inner_total += ir_res.iterations;
-            (void)rc;  // an inner solve that stops early still yields a correction
-            convert_f32_to_f64_axpy(s, xd, e32, n, rnorm);
-            const double prev = rel;
-            rnorm = residual();
-            rel = rnorm / bnorm;
-            ++outer;
-            if (!(rel < prev)) break;  // stagnation: stop refining
-        }
-        st.finish();
-        if (inner_iterations) *inner_iterations = inner_total;
-        res->iterations = outer;
-        res->final_residual = rel;
-        res->final_residual_is_relative = 1;
-        res->converged = bnorm == 0 || rel < cfg->rtol;
-        res->error_estimate = 0.0;
-        if (!res->converged)
-            return fail(FEMGPU_ERR_NOT_CONVERGED,
-                        "mixed-precision refinement stopped at relative residual " + std::to_string(rel));
-        return 0;
+        return ir_refine(op64, op32, ir, b, x, mem, cfg, res, inner_iterations,
+                         [&](Krylov<float> &K32, const ChebParams &cp, const float *r32, float *e32,
+                             double t, uint64_t cap, femgpu_solve_result *ir_res) {
+                             return K32.solve(MODE_PCG, PREC_CHEB, cp, r32, e32, t, 0.0, cap, ir_res);
+                         });
     });
 }
 
@@ -738,10 +757,10 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
 namespace femgpu {
 
 // Flexible CG (Notay, SIAM J. Sci. Comput. 22 (2000)) truncated to the last
-// `trunc` directions (A-orthogonalisation window), with the two-stage preconditioner
-//   n = N(r), gamma = n.r / n.An (Galerkin step: the network's output scale
-//   does not matter), r1 = r - gamma A n, z = gamma n + cheb(r1)
-// i.e. a learned coarse correction followed by Chebyshev-Jacobi smoothing.
+// `trunc` directions (A-orthogonalisation window), with a two-level
+// preconditioner: Chebyshev-Jacobi smoothing z1 = C r, then the network's
+// correction of the smooth residual r1 = r - A z1 that the smoother leaves,
+// scaled by its Galerkin step (the network's output scale does not matter).
 constexpr int kFcgTruncMax = 64;
 
 template <class T>
@@ -812,20 +831,15 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
         const int slot = (int)(it % trunc);
         T *p = P[slot], *qk = Q[slot];
         const int m = (int)std::min<uint64_t>(stored, (uint64_t)trunc - 1);
-        std::vector<const void *> da(m), db(m);
-        std::vector<int> sj(m);
+        std::vector<const T *> vp(m), vq(m);
         for (int j = 0; j < m; ++j) {
-            sj[j] = (int)((it - 1 - j) % trunc);
-            da[j] = z; db[j] = Q[sj[j]];
+            const int sj = (int)((it - 1 - j) % trunc);
+            vp[j] = P[sj]; vq[j] = Q[sj];
         }
-        dots_host(c, m, da.data(), db.data(), n, dt, beta.data());
-        copy<T>(s, p, z, n);
-        copy<T>(s, qk, az, n);
-        for (int j = 0; j < m; ++j) {
-            const double bj = -beta[j] / pq_hist[sj[j]];
-            axpby<T>(s, p, bj, P[sj[j]], 1.0, n);
-            axpby<T>(s, qk, bj, Q[sj[j]], 1.0, n);
-        }
+        multidot_host<T>(c, z, vq.data(), m, n, beta.data());
+        for (int j = 0; j < m; ++j) beta[j] = -beta[j] / pq_hist[(int)((it - 1 - j) % trunc)];
+        multiaxpy<T>(s, p, z, vp.data(), beta.data(), m, n);
+        multiaxpy<T>(s, qk, az, vq.data(), beta.data(), m, n);
         const void *ea[2] = {p, p}, *eb[2] = {qk, r};
         double e2[2];
         dots_host(c, 2, ea, eb, n, dt, e2);
@@ -928,6 +942,37 @@ int femgpu_solve_fcg_learned(femgpu_op *op, femgpu_unet *net, const femgpu_cheb_
         if (rc == FEMGPU_ERR_NOT_CONVERGED)
             return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +
                                 " iterations (residual = " + std::to_string(res->final_residual) + ")");
+        return rc;
+    });
+}
+
+int femgpu_solve_ir_learned(femgpu_op *op64, femgpu_op *op32, femgpu_unet *net,
+                            const femgpu_ir_config *ir, const void *b, void *x, femgpu_mem mem,
+                            const femgpu_solver_config *cfg, femgpu_solve_result *res,
+                            uint64_t *inner_iterations, femgpu_fcg_info *info) {
+    return guard([&] {
+        FG_REQUIRE(net && info, FEMGPU_ERR_INVALID_ARG, "solve_ir_learned: NULL argument");
+        FG_REQUIRE(op32 && unet_voxels(net) == (long long)op32->n, FEMGPU_ERR_DIM_MISMATCH,
+                   "solve_ir_learned: network grid and operator size differ");
+        FG_REQUIRE(unet_ctx(net) == op32->ctx, FEMGPU_ERR_INVALID_ARG, "solve_ir_learned: different contexts");
+        const int patience = info->patience > 0 ? info->patience : 20;
+        const int trunc = info->truncation > 0 ? std::min(info->truncation, kFcgTruncMax) : 16;
+        femgpu_fcg_info tot = *info;
+        tot.fcg_iterations = tot.fallback_iterations = tot.net_corrections = 0;
+        tot.fell_back = 0; tot.fallback_reason = 0;
+        const int rc = ir_refine(op64, op32, ir, b, x, mem, cfg, res, inner_iterations,
+                                 [&](Krylov<float> &K32, const ChebParams &cp, const float *r32, float *e32,
+                                     double t, uint64_t cap, femgpu_solve_result *ir_res) {
+                                     femgpu_fcg_info one = *info;
+                                     const int r = fcg_learned<float>(K32, net, cp, r32, e32, t, 0.0, cap,
+                                                                      patience, trunc, ir_res, &one);
+                                     tot.fcg_iterations += one.fcg_iterations;
+                                     tot.fallback_iterations += one.fallback_iterations;
+                                     tot.net_corrections += one.net_corrections;
+                                     if (one.fell_back) { tot.fell_back = 1; tot.fallback_reason = one.fallback_reason; }
+                                     return r;
+                                 });
+        *info = tot;
         return rc;
     });
 }

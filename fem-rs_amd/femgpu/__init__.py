@@ -155,6 +155,8 @@ _sig("femgpu_unet_save", [P, C.c_char_p])
 _sig("femgpu_unet_load", [P, C.c_char_p])
 _sig("femgpu_solve_fcg_learned", [P, P, C.POINTER(_ChebConfig), P, P, I, C.POINTER(_SolverConfig),
                                   C.POINTER(_SolveResult), C.POINTER(_FcgInfo)])
+_sig("femgpu_solve_ir_learned", [P, P, P, C.POINTER(_IrConfig), P, P, I, C.POINTER(_SolverConfig),
+                                   C.POINTER(_SolveResult), C.POINTER(U64), C.POINTER(_FcgInfo)])
 
 
 # ---------------------------------------------------------------- errors --
@@ -633,7 +635,7 @@ def estimate_spectral_radius(a, n_iter):
 
 
 def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0, max_outer=50,
-                   degree=4, ratio=30.0, safety=1.1, power_iters=20):
+                   degree=4, ratio=30.0, safety=1.1, power_iters=20, lmax=0.0):
     """FP32-inner / FP64-outer iterative refinement; returns (SolveResult with
     outer iterations, total inner iterations)."""
     cfg = cfg or SolverConfig()
@@ -642,12 +644,34 @@ def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0
     r = _SolveResult()
     inner = U64()
     irc = _IrConfig(inner_rtol, inner_max_iter, max_outer,
-                    _ChebConfig(degree, ratio, safety, power_iters, 0.0))
+                    _ChebConfig(degree, ratio, safety, power_iters, lmax))
     rc = lib.femgpu_solve_ir_mixed(op64.h, op32.h, C.byref(irc), bp, xp, m, C.byref(cfg._c()),
                                    C.byref(r), C.byref(inner))
     if rc not in (OK, ERR_NOT_CONVERGED):
         _check(rc)
     return _result(r), inner.value
+
+
+def solve_ir_learned(op64, op32, net, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0, max_outer=50,
+                     degree=4, ratio=30.0, safety=1.1, power_iters=20, patience=20, truncation=16,
+                     lmax=0.0):
+    """FP64 refinement around the FP32 flexible CG with the network; returns
+    (SolveResult with outer iterations, total inner iterations, FcgInfo totals)."""
+    cfg = cfg or SolverConfig()
+    _check_dims(op64, b, x)
+    bp, xp, m = _solve_vectors(op64, b, x)
+    r = _SolveResult()
+    inner = U64()
+    info = _FcgInfo(patience, truncation, 0, 0, 0, 0, 0)
+    irc = _IrConfig(inner_rtol, inner_max_iter, max_outer,
+                    _ChebConfig(degree, ratio, safety, power_iters, lmax))
+    rc = lib.femgpu_solve_ir_learned(op64.h, op32.h, net.h, C.byref(irc), bp, xp, m, C.byref(cfg._c()),
+                                     C.byref(r), C.byref(inner), C.byref(info))
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        _check(rc)
+    return _result(r), inner.value, FcgInfo(int(info.fcg_iterations), int(info.fallback_iterations),
+                                            bool(info.fell_back), int(info.fallback_reason),
+                                            int(info.net_corrections))
 
 
 def chebyshev_jacobi_apply(op, r, z, degree=4, ratio=30.0, safety=1.1, power_iters=20, lmax=0.0):

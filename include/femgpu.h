@@ -238,11 +238,14 @@ int femgpu_estimate_spectral_radius(femgpu_op *op, int n_iter, double *rho);
 /* Mixed-precision iterative refinement (no reference counterpart): outer
  * FP64 residual r = b - A64 u, update u += e and the norms in FP64 on the
  * GPU; inner Chebyshev-Jacobi PCG in FP32 (op32: same operator in FP32) with
- * FP64 dot products, to inner_rtol.  Converges when |r|/|b| < cfg->rtol. */
+ * FP64 dot products on r/|r|, to max(inner_rtol, 0.5 rtol |b|/|r|), i.e. no
+ * further than the outer target needs.  Converges when |r|/|b| < cfg->rtol;
+ * cfg->max_iter bounds the total inner iterations.  Stops early when an outer
+ * step does not reduce |r|. */
 typedef struct {
-    double inner_rtol;
-    uint64_t inner_max_iter;
-    uint64_t max_outer;
+    double inner_rtol;         /* default 1e-4 */
+    uint64_t inner_max_iter;   /* per inner solve; 0: cfg->max_iter */
+    uint64_t max_outer;        /* default 50 */
     femgpu_cheb_config cheb;
 } femgpu_ir_config;
 int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *ir,
@@ -317,6 +320,14 @@ int femgpu_solve_fcg_learned(femgpu_op *op, femgpu_unet *net, const femgpu_cheb_
                              const void *b, void *x, femgpu_mem mem,
                              const femgpu_solver_config *cfg, femgpu_solve_result *res,
                              femgpu_fcg_info *info);
+/* Mixed precision with the learned preconditioner: the refinement of
+ * femgpu_solve_ir_mixed (FP64 residual, update and norms) around an FP32
+ * inner solve by the flexible CG above (network + Chebyshev-Jacobi on op32).
+ * info (patience, truncation in; totals over the inner solves out). */
+int femgpu_solve_ir_learned(femgpu_op *op64, femgpu_op *op32, femgpu_unet *net,
+                            const femgpu_ir_config *ir, const void *b, void *x, femgpu_mem mem,
+                            const femgpu_solver_config *cfg, femgpu_solve_result *res,
+                            uint64_t *inner_iterations, femgpu_fcg_info *info);
 
 #ifdef __cplusplus
 }

@@ -175,3 +175,18 @@ def test_learned_gain_over_plain(fg, ctx):
         print(f"seed {s}: plain PCG-Chebyshev {plain.iterations} it {tp * 1e3:.1f} ms, "
               f"learned FCG {lr.iterations} it {tl * 1e3:.1f} ms")
     assert min(gains) > 1.0, gains
+
+
+def test_ir_learned_matches_fp64(fg, ctx):
+    """FP64 refinement around the FP32 flexible CG with the network reaches
+    1e-10 on the recomputed FP64 residual and matches the FP64 solve."""
+    op, b = _problem(fg, ctx, 8, 4)
+    op32 = op.convert(fg.F32)
+    net = fg.UNet.for_operator(op, seed=3)
+    x = np.zeros(op.n_dofs())
+    res, inner, info = fg.solve_ir_learned(op, op32, net, b, x, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    assert res.converged and inner > 0 and info.net_corrections > 0
+    assert np.linalg.norm(b - op.matvec(x)) / np.linalg.norm(b) < 1e-10
+    xr = np.zeros_like(x)
+    fg.solve_pcg_chebyshev(op, b, xr, fg.SolverConfig(rtol=1e-13, max_iter=5000))
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
