@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the fem-rs solve path on one MI355X.
 
-Workload (one step = one full solve, inputs resident in HBM):
+Default workload (one step = one full solve, inputs resident in HBM):
   3-D diffusion -div(kappa grad u) = 1 on the unit cube, 64^3 hex elements,
   H1 Q2 (129^3 = 2,146,689 DOFs), kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z),
   homogeneous Dirichlet BCs, FP64 PA diffusion apply (3^3 Gauss points),
   PCG + Chebyshev-Jacobi(4) preconditioner to 1e-8 relative residual, x0 = 0.
 
 value = solve throughput N_dofs * PCG iterations / solve time (GDOF/s).  The
-line also carries the PA apply rates (diffusion and mass, FP64), the solve
-time, the FP64 residual of the returned solution (recomputed with a fresh
-apply), the roofline fraction of the PA diffusion kernel (algorithmic bytes
-8 * (2N + 6 E 27) per launch over its HIP-event-timed mean duration) and a CPU
-baseline (the oracle's sum-factorised PCG-Chebyshev on the host cores, bounded
-sample).
+line also carries the PA apply rates (the operator's form and, for the
+default workload, the mass form; FP64), the solve time, the FP64 residual of
+the returned solution (recomputed with a fresh apply), the roofline fraction
+of the PA apply (algorithmic bytes 8 (2N + s E q^3) per apply, s = 6
+diffusion / 7 diffusion + mass, over the HIP-event-timed mean duration of its
+two kernels), the HBM traffic per apply from the committed counter summary
+for that workload (profiles/*_hbm_traffic.md) and a CPU baseline (the
+oracle's sum-factorised PCG-Chebyshev on the host cores, bounded sample).
 
 "legs" compares, on the same operator and right-hand side, the plain FP64
 solve with (a) FP32-inner PCG-Chebyshev + FP64 iterative refinement and (b)
@@ -22,11 +24,20 @@ preconditioner, trained here first (energy-norm loss, pairs harvested from
 PCG-Chebyshev iterates of two other right-hand sides; training time is
 reported and is not part of the solve time).  Each leg reports its solve
 time, iterations, FP64 relative residual and relative error against an FP64
-PCG-Chebyshev solution to 1e-12.
+PCG-Chebyshev solution to rtol / 100.
+
+--workload selects the larger configurations (one solve takes seconds to
+tens of seconds there, so use --steps 1 --warmup 0):
+  q3-96-grf         Q3, 96^3, kappa = exp(GRF) clipped to [1e-2, 1e2], random
+                    nodal RHS, rtol 1e-10 (24,137,569 DOFs)
+  q4-128-perturbed  Q4, 128^3, vertices perturbed by up to 0.2 h, diffusion +
+                    mass (rho = 1), kappa = exp(GRF), random RHS, rtol 1e-8
+                    (135,005,697 DOFs)
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -37,14 +48,33 @@ sys.path.insert(0, os.path.join(ROOT, "fem-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+WORKLOADS = {
+    "q2-64-sinprod": dict(p=2, n=64, form="diffusion", kappa="sinprod", amp=0.0, rhs="unit",
+                          rtol=1e-8, mass_apply=True,
+                          desc="3-D diffusion, {n}^3 hex Q{p} ({N} DOFs), kappa = 1 + 0.5 sin sin sin, "
+                               "Dirichlet, FP64 PA ({q}^3 Gauss points), PCG + Chebyshev-Jacobi(4) to 1e-8"),
+    "q3-96-grf": dict(p=3, n=96, form="diffusion", kappa=("grf", 7), amp=0.0, rhs=("random", 11),
+                      rtol=1e-10, mass_apply=False,
+                      desc="3-D diffusion, {n}^3 hex Q{p} ({N} DOFs), kappa = exp(GRF, l = 0.1, sigma = 1.5) "
+                           "in [1e-2, 1e2] (seed 7), random nodal RHS (seed 11), Dirichlet, FP64 PA "
+                           "({q}^3 Gauss points), PCG + Chebyshev-Jacobi(4) to 1e-10"),
+    "q4-128-perturbed": dict(p=4, n=128, form="diff_mass", kappa=("grf", 7), amp=0.2, rhs=("random", 11),
+                             rtol=1e-8, mass_apply=False,
+                             desc="3-D diffusion + mass (rho = 1), {n}^3 hex Q{p} ({N} DOFs), vertices "
+                                  "perturbed <= 0.2 h (seed 5), kappa = exp(GRF) in [1e-2, 1e2] (seed 7), "
+                                  "random nodal RHS (seed 11), Dirichlet, FP64 PA ({q}^3 Gauss points), "
+                                  "PCG + Chebyshev-Jacobi(4) to 1e-8"),
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64, help="elements per direction")
-    ap.add_argument("--p", type=int, default=2)
+    ap.add_argument("--workload", default="q2-64-sinprod", choices=sorted(WORKLOADS))
+    ap.add_argument("--n", type=int, default=0, help="override elements per direction")
+    ap.add_argument("--p", type=int, default=0, help="override the order")
     ap.add_argument("--apply-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=10)
@@ -55,9 +85,28 @@ def parse():
     return ap.parse_args()
 
 
+def make_operator(fg, ctx, w, n, p):
+    form = fg.DIFFUSION if w["form"] == "diffusion" else fg.DIFF_MASS
+    kappa = w["kappa"] if isinstance(w["kappa"], str) else fg.grf(w["kappa"][1])
+    verts = fg.hex_vertices(n, n, n, amp=w["amp"], seed=5) if w["amp"] > 0 else None
+    return fg.HexPAOperator(p, p + 1, n, form=form, kappa=kappa, rho=1.0, dirichlet=True, vertices=verts,
+                            ctx=ctx)
+
+
+def committed_traffic(name, n, p):
+    """HBM bytes per apply from the committed counter summary of this
+    workload (its line 'traffic_bytes_per_apply: B'), or None."""
+    path = os.path.join(ROOT, "profiles", f"pa_hex_{name}_hbm_traffic.md")
+    w = WORKLOADS[name]
+    if not os.path.exists(path) or (n, p) != (w["n"], w["p"]):
+        return None
+    m = re.search(r"traffic_bytes_per_apply:\s*([0-9]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(n, p, iters, lmax):
     """Oracle PCG-Chebyshev(4) (sum-factorised PA, OpenMP over element colours)
-    on the host cores, capped at `iters` iterations."""
+    on the host cores, capped at `iters` iterations (default workload only)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import femref as fr
     threads = min(os.cpu_count() or 1, 16)
@@ -79,23 +128,26 @@ def cpu_baseline(n, p, iters, lmax):
                       f"problem, {res.iterations} iterations ({dt:.2f} s)"}
 
 
-def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, a):
+def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, rtol, a):
     from femgpu import learned
     N = K.n_dofs()
-    dev = lambda dt=np.float64: fg.DeviceArray(ctx, N, dt)
-    x0 = dev()
+    x0 = fg.DeviceArray(ctx, N, np.float64)
     # FP64 reference solution (not timed)
+    ref_rtol = rtol / 100
     x.copy_from(x0)
-    fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-12, max_iter=20000), lmax=lmax)
+    fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=ref_rtol, max_iter=100000), lmax=lmax)
     xref = x.download()
     err = lambda v: float(np.linalg.norm(v - xref) / np.linalg.norm(xref))
     bh = b.download()
     relres = lambda v: float(np.linalg.norm(bh - K.matvec(v)) / np.linalg.norm(bh))
     K32 = K.convert(fg.F32)
-    cfg = fg.SolverConfig(rtol=1e-8, max_iter=20000)
+    cfg = fg.SolverConfig(rtol=rtol, max_iter=100000)
+    big = N > 10_000_000
 
-    def timed(fn, reps=3):
-        fn()
+    def timed(fn):
+        reps = 1 if big else 3
+        if not big:
+            fn()
         ts = []
         for _ in range(reps):
             x.copy_from(x0)
@@ -108,6 +160,7 @@ def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, a):
 
     (res_m, inner_m), t_m = timed(lambda: fg.solve_ir_mixed(K, K32, b, x, cfg, inner_rtol=1e-4, lmax=lmax))
     xm = x.download()
+    print(f"bench.py: mixed leg {1e3 * t_m:.1f} ms", file=sys.stderr, flush=True)
     mixed = {"method": "FP32 PCG + Chebyshev-Jacobi(4) inner (FP64 dots), FP64 refinement",
              "solve_time_ms": round(1e3 * t_m, 3), "outer_iterations": res_m.iterations,
              "inner_iterations": int(inner_m), "rel_residual_fp64": relres(xm), "rel_error": err(xm),
@@ -124,10 +177,10 @@ def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, a):
         r_, e_ = pairs[rng.integers(len(pairs))]
         losses.append(net.train_step_energy(K, r_, e_, 2e-3 * 0.1 ** (st / max(a.train_steps, 1))))
     t_train = time.perf_counter() - t0
-    (res_l, inner_l, info_l), t_l = timed(lambda: fg.solve_ir_learned(K, K32, net, b, x, cfg, inner_rtol=1e-4, lmax=lmax))
+    print(f"bench.py: trained {a.train_steps} steps in {t_train:.1f} s", file=sys.stderr, flush=True)
+    (res_l, inner_l, info_l), t_l = timed(
+        lambda: fg.solve_ir_learned(K, K32, net, b, x, cfg, inner_rtol=1e-4, lmax=lmax))
     xl = x.download()
-    (res_f, info_f), t_f = timed(lambda: fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax))
-    xf = x.download()
     ml = {"method": "FP32 flexible CG (Chebyshev-Jacobi(4) + U-Net correction, 16 directions) inner, "
                     "FP64 refinement",
           "solve_time_ms": round(1e3 * t_l, 3), "outer_iterations": res_l.iterations,
@@ -137,13 +190,17 @@ def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, a):
           "train": {"pairs": len(pairs), "steps": a.train_steps, "time_s": round(t_train, 2),
                     "energy_loss_first_last": [round(float(np.mean(losses[:20])), 4),
                                                round(float(np.mean(losses[-20:])), 4)]}}
-    fcg64 = {"method": "FP64 flexible CG with the U-Net (no refinement)",
-             "solve_time_ms": round(1e3 * t_f, 3), "iterations": res_f.iterations,
-             "rel_residual_fp64": relres(xf), "rel_error": err(xf),
-             "speedup_vs_plain_fp64": round(t_plain / t_f, 3)}
-    return {"plain_fp64": {"method": "FP64 PCG + Chebyshev-Jacobi(4)", "solve_time_ms": round(1e3 * t_plain, 3),
-                           "iterations": int(it_plain), "rel_error": err(x_plain)},
-            "mixed_fp32_ir": mixed, "mixed_learned": ml, "learned_fcg_fp64": fcg64}
+    out = {"plain_fp64": {"method": "FP64 PCG + Chebyshev-Jacobi(4)", "solve_time_ms": round(1e3 * t_plain, 3),
+                          "iterations": int(it_plain), "rel_error": err(x_plain)},
+           "mixed_fp32_ir": mixed, "mixed_learned": ml, "reference_rtol": ref_rtol}
+    if not big:
+        (res_f, info_f), t_f = timed(lambda: fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax))
+        xf = x.download()
+        out["learned_fcg_fp64"] = {"method": "FP64 flexible CG with the U-Net (no refinement)",
+                                   "solve_time_ms": round(1e3 * t_f, 3), "iterations": res_f.iterations,
+                                   "rel_residual_fp64": relres(xf), "rel_error": err(xf),
+                                   "speedup_vs_plain_fp64": round(t_plain / t_f, 3)}
+    return out
 
 
 def main():
@@ -154,19 +211,24 @@ def main():
         sys.exit(2)
     import femgpu as fg
 
+    w = WORKLOADS[a.workload]
+    n = a.n or w["n"]
+    p = a.p or w["p"]
+    q = p + 1
     ctx = fg.default_context()
-    n, p = a.n, a.p
-    K = fg.HexPAOperator(p, p + 1, n, form=fg.DIFFUSION, kappa="sinprod", dirichlet=True, ctx=ctx)
-    M = fg.HexPAOperator(p, p + 1, n, form=fg.MASS, rho=1.0, dirichlet=False, ctx=ctx)
+    K = make_operator(fg, ctx, w, n, p)
     N = K.n_dofs()
     E = n ** 3
-    q3 = (p + 1) ** 3
+    q3 = q ** 3
+    s_comp = 6 if w["form"] == "diffusion" else 7
     dev = lambda: fg.DeviceArray(ctx, N, np.float64)
     b = dev()
-    K.rhs_unit(b)
+    if w["rhs"] == "unit":
+        K.rhs_unit(b)
+    else:
+        K.rhs_random(w["rhs"][1], b)
     x = dev()
-    zero = np.zeros(N)
-    cfg = fg.SolverConfig(rtol=1e-8, max_iter=2000)
+    cfg = fg.SolverConfig(rtol=w["rtol"], max_iter=100000)
 
     if a.copy_calib_mb:
         nc = a.copy_calib_mb * (1 << 20) // 8
@@ -180,13 +242,19 @@ def main():
     yv = dev()
     K.time_apply(xv, yv, 5)
     ms_apply, ms_kern = K.time_apply(xv, yv, a.apply_reps)
-    msm_apply, msm_kern = M.time_apply(xv, yv, a.apply_reps)
-    bytes_diff = 8.0 * (2 * N + 6 * E * q3)
-    bytes_mass = 8.0 * (2 * N + 1 * E * q3)
+    bytes_apply = 8.0 * (2 * N + s_comp * E * q3)
+    pa = {"operator_gdofs": round(N / (ms_apply * 1e-3) / 1e9, 3), "operator_ms": ms_apply,
+          "operator_kernel_ms": ms_kern}
+    if w["mass_apply"]:
+        M = fg.HexPAOperator(p, q, n, form=fg.MASS, rho=1.0, dirichlet=False, ctx=ctx)
+        msm_apply, msm_kern = M.time_apply(xv, yv, a.apply_reps)
+        pa.update({"mass_gdofs": round(N / (msm_apply * 1e-3) / 1e9, 3), "mass_ms": msm_apply,
+                   "mass_kernel_ms": msm_kern})
+        M.close()
+    del xv, yv
 
     # Chebyshev interval once (part of set-up, as the operator's diagonal)
-    x.upload(zero)
-    _, lmax = fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=1))
+    _, lmax = fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=w["rtol"], max_iter=1))
 
     x0 = dev()  # zero initial guess, copied on the device inside the timed step
 
@@ -206,6 +274,8 @@ def main():
         ctx.synchronize()
         times.append(time.perf_counter() - t0)
         iters.append(res.iterations)
+        print(f"bench.py: step {len(times)}: {res.iterations} iterations, {1e3 * times[-1]:.1f} ms",
+              file=sys.stderr, flush=True)
         if not res.converged:
             print(f"bench.py: solve did not converge ({res})", file=sys.stderr)
             sys.exit(1)
@@ -215,12 +285,16 @@ def main():
     K.apply_zero(x, r)
     xs, rs, bs = x.download(), r.download(), b.download()
     rel_res = float(np.linalg.norm(bs - rs) / np.linalg.norm(bs))
+    del r, rs
 
     ms_step = 1e3 * total / a.steps
     value = N * sum(iters) / total / 1e9
     t_plain = float(np.median(times))
-    leg = None if a.no_legs else legs(fg, ctx, K, b, x, xs, lmax, t_plain, iters[-1], a)
-    cpu = None if a.no_cpu else cpu_baseline(n, p, a.cpu_iters, lmax)
+    leg = None if a.no_legs else legs(fg, ctx, K, b, x, xs, lmax, t_plain, iters[-1], w["rtol"], a)
+    cpu = None
+    if not a.no_cpu and a.workload == "q2-64-sinprod":
+        cpu = cpu_baseline(n, p, a.cpu_iters, lmax)
+    kform = 2 if w["form"] == "diffusion" else 3
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(value, 4),
@@ -234,28 +308,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3-D diffusion, {n}^3 hex Q{p} ({N} DOFs), kappa = 1 + 0.5 sin sin sin, "
-                               "Dirichlet, FP64 PA (3^3 Gauss points), PCG + Chebyshev-Jacobi(4) to 1e-8",
-                   "n_dofs": N, "elements": E, "order": p, "quad_points_1d": p + 1,
+        "config": {"workload": w["desc"].format(n=n, p=p, N=N, q=q), "name": a.workload,
+                   "n_dofs": N, "elements": E, "order": p, "quad_points_1d": q,
                    "global_batch": 1, "parallelism": "single GPU"},
-        "solve": {"iterations": int(iters[-1]), "solve_time_ms": round(1e3 * float(np.median(times)), 4),
+        "solve": {"iterations": int(iters[-1]), "solve_time_ms": round(1e3 * t_plain, 4),
                   "rel_residual_fp64": rel_res, "lmax": lmax},
-        "pa_apply": {
-            "diffusion_gdofs": round(N / (ms_apply * 1e-3) / 1e9, 3),
-            "diffusion_ms": ms_apply, "diffusion_kernel_ms": ms_kern,
-            "mass_gdofs": round(N / (msm_apply * 1e-3) / 1e9, 3),
-            "mass_ms": msm_apply, "mass_kernel_ms": msm_kern,
-        },
+        "pa_apply": pa,
         "roofline": {
             "bound": "hbm",
-            "achieved": round(bytes_diff / (ms_kern * 1e-3) / 1e9, 1),
+            "achieved": round(bytes_apply / (ms_kern * 1e-3) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(bytes_diff / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"PA diffusion apply = k_pa_hex<{p + 1},{p + 1},double,2> element kernel + "
+            "frac": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": committed_traffic(a.workload, n, p),
+            "kernel": f"PA apply = k_pa_hex<{q},{q},double,{kform}> element kernel + "
                       f"k_hex_gather<double,{p}> (HIP events around both)",
-            "algorithmic_bytes": bytes_diff,
+            "algorithmic_bytes": bytes_apply,
         },
         "legs": leg,
         "cpu_baseline": cpu,
