@@ -50,6 +50,29 @@ def test_hex_apply_f64(fg, fr, ctx, p, q, form):
     assert np.abs(y1 - (y0 + ref)).max() < 1e-12 * np.abs(ref).max() + 1e-12 * np.abs(y0).max()
 
 
+# meshes spanning several workgroup tiles in x and y (with partial edge tiles)
+# and several z chunks, so every halo / spill / fix-up path runs
+MULTI_TILE = [(1, 2, (17, 9, 5)), (2, 3, (16, 9, 7)), (2, 3, (15, 5, 9)), (3, 4, (9, 6, 5)),
+              (4, 5, (11, 5, 4)), (5, 6, (8, 3, 3)), (1, 3, (9, 17, 4))]
+
+
+@pytest.mark.parametrize("p,q,n", MULTI_TILE)
+@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("dirichlet", [False, True])
+def test_hex_apply_multi_tile(fg, fr, ctx, p, q, n, form, dirichlet):
+    op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, form, dirichlet=dirichlet)
+    N = op.n_dofs()
+    x = np.random.default_rng(p + 7 * form).standard_normal(N)
+    ref = fr.hex_apply_sumfac(d, Dq, Mq, x)
+    y = np.full(N, np.nan)
+    op.apply_zero(x, y)
+    assert _rel(y, ref) < 1e-12
+    y0 = np.random.default_rng(3).standard_normal(N)
+    y1 = y0.copy()
+    op.apply(x, y1)
+    assert np.abs(y1 - (y0 + ref)).max() < 1e-12 * (np.abs(ref).max() + np.abs(y0).max())
+
+
 @pytest.mark.parametrize("p,q", [(2, 3), (3, 4), (4, 5)])
 def test_hex_dirichlet_diag_rhs(fg, fr, ctx, p, q):
     n = (4, 3, 3)
