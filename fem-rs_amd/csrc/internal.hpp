@@ -136,6 +136,22 @@ struct femgpu_op {
     virtual void apply_dev(const void *x, void *y, bool accumulate, const int *skip) = 0;
     virtual void diagonal_dev(void *d) = 0;
     virtual const char *name() const = 0;
+    // Optional fused forms used by the solvers; false = not provided, and the
+    // caller runs apply_dev followed by its own vector kernel.  Both must give
+    // the results of that pair bit for bit (same per-entry arithmetic, and the
+    // partial sums over the fixed red_blocks(n) x 256 grid-stride grid).
+    //  apply_cheb_dev: t = A x; d = c1 d + c2 inv (r - t); z += d; with last,
+    //                  part[block] = sum r.z  (x may alias z)
+    //  apply_dot_dev:  y = A x; part[block] = sum x.y
+    virtual bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d,
+                                double c1, double c2, int last, double *part, const int *skip) {
+        (void)x; (void)r; (void)inv; (void)z; (void)d; (void)c1; (void)c2; (void)last; (void)part; (void)skip;
+        return false;
+    }
+    virtual bool apply_dot_dev(const void *x, void *y, double *part, const int *skip) {
+        (void)x; (void)y; (void)part; (void)skip;
+        return false;
+    }
     size_t esize() const { return dtype == FEMGPU_F64 ? 8 : 4; }
 };
 

@@ -28,6 +28,7 @@
 #include "basis.hpp"
 #include "internal.hpp"
 #include "philox.hpp"
+#include "reduce.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -327,33 +328,18 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 }
 
-// Element-vector -> DOF vector: every DOF sums the (1, 2, 4 or 8) element
-// contributions that share it, in a fixed order (bitwise reproducible, no
-// atomics).  One thread per DOF; a logical block covers 256 consecutive DOFs
-// of one z-plane (bpp blocks per plane), consecutive threads along x so the
-// element-vector rows are read contiguously.  All eight candidate loads are
-// issued before the sum.  mode 0: y = s, 1: y += s.  Essential DOFs: y (+)= x.
+// Element-vector -> DOF value: sums the (1, 2, 4 or 8) element contributions
+// that share DOF (gx, gy, gz) in a fixed order (bitwise reproducible, no
+// atomics); all eight candidate loads are issued before the sum.  Essential
+// DOFs return x.
 template <class T, int P>
-__global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
-                                                    const T *__restrict__ x, T *__restrict__ y,
-                                                    int mode, int dirichlet, unsigned bpp,
-                                                    const int *skip) {
+__device__ __forceinline__ T gather_dof(const HexGrid &g, const T *__restrict__ ye,
+                                        const T *__restrict__ x, int dirichlet, int gx, int gy,
+                                        int gz, size_t gi) {
     constexpr int N1 = P + 1;
-    if (skip && *skip) return;
-    const unsigned nlog = bpp * (unsigned)g.Nz;
-    const unsigned lb = xcd_block(blockIdx.x, nlog);
-    if (lb >= nlog) return;
     const int Ny = (int)g.Ny, Nx = (int)g.Nx;
-    const int gz = (int)(lb / bpp);
-    const int id = (int)(lb - (unsigned)gz * bpp) * 256 + threadIdx.x;
-    if (id >= Nx * Ny) return;
-    const int gy = id / Nx, gx = id - gy * Nx;
-    const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
-    if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1)) {
-        if (mode == 0) y[gi] = x[gi];
-        else y[gi] += x[gi];
-        return;
-    }
+    if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1))
+        return x[gi];
     // per direction two candidates: [0] = (e0 - 1, local P) when the node is
     // the lower-face node of element e0, [1] = (e0, local i0)
     int ce[3][2], ci[3][2];
@@ -382,8 +368,94 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
     T s = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m) s += v[m];
+    return s;
+}
+
+// One thread per DOF; a logical block covers 256 consecutive DOFs of one
+// z-plane (bpp blocks per plane), consecutive threads along x so the
+// element-vector rows are read contiguously.  mode 0: y = s, 1: y += s.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    int mode, int dirichlet, unsigned bpp,
+                                                    const int *skip) {
+    if (skip && *skip) return;
+    const unsigned nlog = bpp * (unsigned)g.Nz;
+    const unsigned lb = xcd_block(blockIdx.x, nlog);
+    if (lb >= nlog) return;
+    const int Ny = (int)g.Ny, Nx = (int)g.Nx;
+    const int gz = (int)(lb / bpp);
+    const int id = (int)(lb - (unsigned)gz * bpp) * 256 + threadIdx.x;
+    if (id >= Nx * Ny) return;
+    const int gy = id / Nx, gx = id - gy * Nx;
+    const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
+    const T s = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, gi);
     if (mode == 0) y[gi] = s;
     else y[gi] += s;
+}
+
+// Linear DOF index -> (gx, gy, gz).
+__device__ __forceinline__ void dof_coords(const HexGrid &g, size_t i, int &gx, int &gy, int &gz) {
+    if (g.Nx * g.Ny * g.Nz < (1ll << 32)) {
+        const unsigned u = (unsigned)i, nx = (unsigned)g.Nx, ny = (unsigned)g.Ny;
+        const unsigned r = u / nx;
+        gx = (int)(u - r * nx);
+        gz = (int)(r / ny);
+        gy = (int)(r - (unsigned)gz * ny);
+    } else {
+        const size_t r = i / (size_t)g.Nx;
+        gx = (int)(i - r * (size_t)g.Nx);
+        gz = (int)(r / (size_t)g.Ny);
+        gy = (int)(r - (size_t)gz * g.Ny);
+    }
+}
+
+// Gather fused with the Chebyshev step that consumes it (the solver's k_chebk):
+// t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z.
+// Same grid-stride grid as k_chebk, so the partial sums are identical.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_hex_gather_cheb(
+    const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
+    const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2, int last,
+    double *__restrict__ part, int dirichlet, const int *skip) {
+    if (skip && *skip) return;
+    const size_t n = (size_t)g.Nx * g.Ny * g.Nz;
+    double v[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        int gx, gy, gz;
+        dof_coords(g, i, gx, gy, gz);
+        const T t = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, i);  // reads x[i] only
+        const T di = (T)c1 * d[i] + (T)c2 * (inv[i] * (r[i] - t));
+        d[i] = di;
+        const T zi = z[i] + di;
+        z[i] = zi;
+        if (last) v[0] += (double)r[i] * (double)zi;
+    }
+    if (last) {
+        block_sum<1, 256>(v);
+        if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    }
+}
+
+// Gather fused with the dot product that follows it (the solver's k_dot):
+// y = A x, partials of x.y in slot 0, on k_dot's grid.
+template <class T, int P>
+__global__ __launch_bounds__(256) void k_hex_gather_dot(const HexGrid g, const T *__restrict__ ye,
+                                                        const T *__restrict__ x, T *__restrict__ y,
+                                                        double *__restrict__ part, int dirichlet,
+                                                        const int *skip) {
+    if (skip && *skip) return;
+    const size_t n = (size_t)g.Nx * g.Ny * g.Nz;
+    double v[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        int gx, gy, gz;
+        dof_coords(g, i, gx, gy, gz);
+        const T t = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, i);
+        y[i] = t;
+        v[0] += (double)x[i] * (double)t;
+    }
+    block_sum<1, 256>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
 // mode 0: y = bdr ? x : 0   (apply_zero)
@@ -614,6 +686,48 @@ struct HexOp final : femgpu_op {
     void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
         if (dtype == FEMGPU_F64) apply_t<double>((const double *)x, (double *)y, acc, skip);
         else apply_t<float>((const float *)x, (float *)y, acc, skip);
+    }
+
+    template <class T> void fused_t(const T *x, const T *r, const T *inv, T *z, T *d, double c1, double c2,
+                                    int last, T *y, double *part, const int *skip) {
+        hipStream_t s = ctx->stream;
+        const int dir = desc.dirichlet ? 1 : 0;
+        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, evec.as<T>(), dir, skip, s);
+        const dim3 grid((unsigned)red_blocks(n)), block(256);
+        const T *ye = evec.as<T>();
+#define GF(P_)                                                                                              \
+    do {                                                                                                    \
+        if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
+        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
+                                last, part, dir, skip);                                                     \
+    } while (0)
+        switch (desc.p) {
+        case 1: GF(1); break;
+        case 2: GF(2); break;
+        case 3: GF(3); break;
+        case 4: GF(4); break;
+        default: GF(5); break;
+        }
+#undef GF
+    }
+
+    bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d, double c1, double c2,
+                        int last, double *part, const int *skip) override {
+        if (dtype == FEMGPU_F64)
+            fused_t<double>((const double *)x, (const double *)r, (const double *)inv, (double *)z, (double *)d,
+                            c1, c2, last, nullptr, part, skip);
+        else
+            fused_t<float>((const float *)x, (const float *)r, (const float *)inv, (float *)z, (float *)d, c1, c2,
+                           last, nullptr, part, skip);
+        return true;
+    }
+
+    bool apply_dot_dev(const void *x, void *y, double *part, const int *skip) override {
+        if (dtype == FEMGPU_F64)
+            fused_t<double>((const double *)x, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, (double *)y, part, skip);
+        else
+            fused_t<float>((const float *)x, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, (float *)y, part, skip);
+        return true;
     }
 
     template <class T> void diag_t(T *d) {

@@ -375,11 +375,13 @@ template <class T> struct Krylov {
         hipLaunchKernelGGL(k_cheb0<T>, dim3(nb), dim3(256), 0, s, n, r, inv, z, d, 1.0 / theta,
                            cp.degree == 1 ? 1 : 0, part, flags);
         for (int k = 1; k < cp.degree; ++k) {
-            apply(z, t, true);
             const double rho = 1.0 / (2.0 * sigma - rho_prev);
-            hipLaunchKernelGGL(k_chebk<T>, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
-                               rho * rho_prev, 2.0 * rho / delta, k == cp.degree - 1 ? 1 : 0, part,
-                               flags);
+            const int last = k == cp.degree - 1 ? 1 : 0;
+            if (!op->apply_cheb_dev(z, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, part, flags)) {
+                apply(z, t, true);
+                hipLaunchKernelGGL(k_chebk<T>, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
+                                   rho * rho_prev, 2.0 * rho / delta, last, part, flags);
+            }
             rho_prev = rho;
         }
     }
@@ -414,9 +416,11 @@ template <class T> struct Krylov {
         bool stopped = read_flag() != 0;
         const uint64_t check = n < (1u << 20) ? 16 : 4;
         while (!stopped && it < max_iter) {
-            apply(p, ap, true);
-            hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)p, (const T *)ap,
-                               part, 0, flags);
+            if (!op->apply_dot_dev(p, ap, part, flags)) {
+                apply(p, ap, true);
+                hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)p, (const T *)ap,
+                                   part, 0, flags);
+            }
             hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                                flags, brk);
             if (prec == PREC_JACOBI)

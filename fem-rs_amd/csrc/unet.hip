@@ -65,29 +65,37 @@ static unsigned g1(long long n, int b = 256) {
 // ------------------------------------------------------------ conv fwd --
 // out[v][n] = act(sum_k A[v][k] W[n][k] + bias[n] (+ skip[v][n]))
 // OUTF: 0 -> bf16 output with bias/skip/relu; 1 -> f32 output, plain sum.
-template <int COUT, int OUTF>
-__global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int cin, int kpad,
-                                                  const uint16_t *__restrict__ in,
+// One wave computes MT tiles of 16 consecutive voxels x all COUT channels on
+// v_mfma_f32_16x16x16bf16_1k, K = 27 CIN in steps of 16.  For CIN >= 16 a
+// k-step is one tap and 16 channels, so the tap, its offset and the channel
+// base are wave-uniform (scalar registers); per voxel tile the six neighbour
+// bits and the linear index are computed once.  CIN = 4 packs four taps per
+// k-step (lane group g = lane / 16 takes tap 4 s + g).
+template <int CIN, int COUT, int MT, int OUTF>
+__global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_t *__restrict__ in,
                                                   const uint16_t *__restrict__ W,
                                                   const float *__restrict__ bias,
                                                   const uint16_t *__restrict__ skip,
                                                   void *__restrict__ outp, int relu) {
-    constexpr int NT = COUT / 16, MT = 4;
+    static_assert(CIN == 4 || CIN % 16 == 0, "CIN must be 4 or a multiple of 16");
+    constexpr int NT = COUT / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long V = d.n();
     const long long vbase = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
     if (vbase >= V) return;  // uniform per wave; no block barriers in this kernel
-    int vx[MT], vy[MT], vz[MT];
-    bool vin[MT];
+    const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+    long long vv[MT];
+    unsigned nb[MT];  // bit 0: x-1 ok, 1: x+1 ok, 2: y-1, 3: y+1, 4: z-1, 5: z+1, 6: voxel valid
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const long long v = vbase + mt * 16 + (lane & 15);
-        vin[mt] = v < V;
-        const long long vv = vin[mt] ? v : 0;
-        vx[mt] = (int)(vv % d.nx);
-        const long long r = vv / d.nx;
-        vy[mt] = (int)(r % d.ny);
-        vz[mt] = (int)(r / d.ny);
+        const bool ok = v < V;
+        vv[mt] = ok ? v : 0;
+        const int x = (int)(vv[mt] % d.nx);
+        const long long r = vv[mt] / d.nx;
+        const int y = (int)(r % d.ny), z = (int)(r / d.ny);
+        nb[mt] = (x > 0) | ((x < d.nx - 1) << 1) | ((y > 0) << 2) | ((y < d.ny - 1) << 3) | ((z > 0) << 4) |
+                 ((z < d.nz - 1) << 5) | ((unsigned)ok << 6);
     }
     f32x4 acc[MT][NT];
 #pragma unroll
@@ -95,22 +103,32 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int cin, int kpad,
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int ksub = (lane >> 4) * 4;
+    const uint16_t *wrow[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) wrow[nt] = W + (size_t)(nt * 16 + (lane & 15)) * kpad + ksub;
+    auto tap_ok = [](unsigned m, int dx, int dy, int dz) {
+        return ((m >> 6) & 1) && (dx >= 0 || (m & 1)) && (dx <= 0 || (m & 2)) && (dy >= 0 || (m & 4)) &&
+               (dy <= 0 || (m & 8)) && (dz >= 0 || (m & 16)) && (dz <= 0 || (m & 32));
+    };
     for (int k0 = 0; k0 < kpad; k0 += 16) {
-        const int k = k0 + ksub;
-        const int tap = k / cin, c = k - tap * cin;
         bf16x4 b[NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-            b[nt] = *reinterpret_cast<const bf16x4 *>(W + (size_t)(nt * 16 + (lane & 15)) * kpad + k);
+        for (int nt = 0; nt < NT; ++nt) b[nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + k0);
+        int tap, c0;
+        if constexpr (CIN >= 16) {
+            tap = k0 / CIN;            // wave-uniform
+            c0 = k0 - tap * CIN + ksub;
+        } else {
+            tap = k0 / 4 + (lane >> 4);  // four taps per k-step
+            c0 = 0;
+        }
         const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+        const long long off = dz * sz + dy * sy + dx;
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             bf16x4 a = bf16x4{0, 0, 0, 0};
-            const int x = vx[mt] + dx, y = vy[mt] + dy, z = vz[mt] + dz;
-            if (tap < 27 && vin[mt] && x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz) {
-                const size_t nb = (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z);
-                a = *reinterpret_cast<const bf16x4 *>(in + nb * cin + c);
-            }
+            if (tap < 27 && tap_ok(nb[mt], dx, dy, dz))
+                a = *reinterpret_cast<const bf16x4 *>(in + (size_t)(vv[mt] + off) * CIN + c0);
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
@@ -244,9 +262,14 @@ __global__ __launch_bounds__(256) void k_out_fwd(Vol d, const uint16_t *__restri
     for (int tap = 0; tap < 27; ++tap) {
         const int X = x + tap % 3 - 1, Y = y + (tap / 3) % 3 - 1, Z = z + tap / 9 - 1;
         if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
-        const uint16_t *p = in + ((size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z)) * 16;
+        const uint4 *p = reinterpret_cast<const uint4 *>(in + ((size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z)) * 16);
+        const uint4 q0 = p[0], q1 = p[1];  // 16 bf16 channels in two 16-byte loads
+        const unsigned w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-        for (int c = 0; c < 16; ++c) s += sW[tap * 16 + c] * bf2f(p[c]);
+        for (int c = 0; c < 8; ++c) {
+            s += sW[tap * 16 + 2 * c] * bf2f((uint16_t)(w[c] & 0xffffu));
+            s += sW[tap * 16 + 2 * c + 1] * bf2f((uint16_t)(w[c] >> 16));
+        }
     }
     out[v] = s;
 }
@@ -594,15 +617,26 @@ static void conv_fwd(femgpu_unet *u, int l, const uint16_t *in, const uint16_t *
     const int cin = cout_override ? L.cout : L.cin;  // data-gradient: channels swap
     const int kpad = cout_override ? kpad_of(L.cout) : L.kpad;
     const long long V = L.d.n();
-    const dim3 grid((unsigned)((V + 255) / 256)), block(256);
-#define CF(CO, OF)                                                                              \
-    hipLaunchKernelGGL((k_conv_fwd<CO, OF>), grid, block, 0, s, L.d, cin, kpad, in, W, bias, skip, \
-                       out, relu ? 1 : 0)
-    if (outf32) {
-        if (cout == 16) CF(16, 1); else if (cout == 32) CF(32, 1); else CF(64, 1);
-    } else {
-        if (cout == 16) CF(16, 0); else if (cout == 32) CF(32, 0); else CF(64, 0);
-    }
+    // voxel tiles per wave: enough waves to fill the GPU at the coarse levels
+    const int mt = V >= 64ll * 4096 ? 4 : V >= 32ll * 4096 ? 2 : 1;
+    const dim3 grid((unsigned)((V + 64ll * mt - 1) / (64ll * mt))), block(256);
+    const int r = relu ? 1 : 0;
+#define CF(CI, CO, M, OF) hipLaunchKernelGGL((k_conv_fwd<CI, CO, M, OF>), grid, block, 0, s, L.d, kpad, in, W, bias, skip, out, r)
+#define CM(CI, CO, OF)                    \
+    do {                                  \
+        if (mt == 4) CF(CI, CO, 4, OF);   \
+        else if (mt == 2) CF(CI, CO, 2, OF); \
+        else CF(CI, CO, 1, OF);           \
+    } while (0)
+    if (!outf32 && cin == 4 && cout == 16) CM(4, 16, 0);
+    else if (!outf32 && cin == 16 && cout == 32) CM(16, 32, 0);
+    else if (!outf32 && cin == 32 && cout == 64) CM(32, 64, 0);
+    else if (outf32 && cin == 64 && cout == 32) CM(64, 32, 1);
+    else if (outf32 && cin == 32 && cout == 16) CM(32, 16, 1);
+    else if (outf32 && cin == 16 && cout == 32) CM(16, 32, 1);
+    else if (outf32 && cin == 32 && cout == 64) CM(32, 64, 1);
+    else throw Error(FEMGPU_ERR_UNSUPPORTED, "conv_fwd: unsupported channel pair");
+#undef CM
 #undef CF
 }
 

@@ -142,8 +142,8 @@ class FemCeed:
         key = (kind, id(mesh), poly, q)
         if key not in self._cache:
             coords = np.ascontiguousarray(mesh.coords, np.float64).ravel()
-            dofs = np.ascontiguousarray(en[:, :npe])
-            verts = np.ascontiguousarray(en[:, :3])
+            dofs = np.ascontiguousarray(en[:, :npe]).ravel()  # flat, as the C ABI takes them
+            verts = np.ascontiguousarray(en[:, :3]).ravel()
             cls = PAMassOperator if kind == MASS else PADiffusionOperator
             self._cache[key] = (cls(2, poly, coords, verts, dofs, mesh.n_nodes(), 1.0, _Q_TO_ORDER[q],
                                     ctx=self._ctx), mesh)
