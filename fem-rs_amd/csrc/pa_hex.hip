@@ -87,17 +87,6 @@ struct HexGrid {
     long long Nx, Ny, Nz;
 };
 
-// The hardware hands workgroup b to XCD b % 8.  xcd_block() renumbers the
-// workgroups so that each XCD walks one contiguous range of the nlog logical
-// blocks: DOF rows that are neighbours in space then run on the same XCD and
-// share its L2 (the gather reads 18 us -> 20.5 us without it at Q2 64^3).
-// Launch xcd_grid(nlog) = 8 ceil(nlog/8) workgroups; those mapped past nlog
-// exit at once.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nlog) {
-    const unsigned per = (nlog + 7u) / 8u;
-    return (b % 8u) * per + b / 8u;
-}
-static inline unsigned xcd_grid(size_t nlog) { return (unsigned)(8 * ((nlog + 7) / 8)); }
 
 __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long gy, long long gz) {
     return gx == 0 || gx == g.Nx - 1 || gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1;
@@ -328,108 +317,127 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 }
 
-// Element-vector -> DOF value: sums the (1, 2, 4 or 8) element contributions
-// that share DOF (gx, gy, gz) in a fixed order (bitwise reproducible, no
-// atomics); all eight candidate loads are issued before the sum.  Essential
-// DOFs return x.
-template <class T, int P>
-__device__ __forceinline__ T gather_dof(const HexGrid &g, const T *__restrict__ ye,
-                                        const T *__restrict__ x, int dirichlet, int gx, int gy,
-                                        int gz, size_t gi) {
+// Element-vector -> DOF values, one x-segment per thread.  Segment (ex, gy,
+// gz) holds the DOFs gx = ex P + i, i = 0..P-1, of DOF row (gy, gz), and the
+// row's last DOF gx = Nx - 1 too when ex = nx - 1.  Each DOF sums its (1, 2,
+// 4 or 8) element contributions in a fixed order -- z candidate, then y, then
+// x, lower element first -- so applies are bitwise reproducible without
+// atomics; the row candidates, divisions and essential-DOF tests are shared
+// by the segment's DOFs.  Essential DOFs take x.  Returns the DOF count.
+// I: index type of the element vector (uint32_t when it has < 2^31 entries).
+template <class T, int P, class I>
+__device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restrict__ ye,
+                                              const T *__restrict__ x, int dirichlet, int ex, int gy,
+                                              int gz, size_t row0, T (&out)[P + 1]) {
     constexpr int N1 = P + 1;
-    const int Ny = (int)g.Ny, Nx = (int)g.Nx;
-    if (dirichlet && (gx == 0 || gx == Nx - 1 || gy == 0 || gy == Ny - 1 || gz == 0 || gz == g.Nz - 1))
-        return x[gi];
-    // per direction two candidates: [0] = (e0 - 1, local P) when the node is
-    // the lower-face node of element e0, [1] = (e0, local i0)
-    int ce[3][2], ci[3][2];
-    bool cv[3][2];
-    const int gg[3] = {gx, gy, gz}, ne[3] = {g.nx, g.ny, g.nz};
+    const int cnt = ex == g.nx - 1 ? P + 1 : P;
+    // y and z candidates: [0] = (e0 - 1, local P) on an element face, [1] = (e0, i0)
+    int ce[2][2], ci[2][2];
+    bool cv[2][2];
+    const int gg[2] = {gy, gz}, ne[2] = {g.ny, g.nz};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
+    for (int d = 0; d < 2; ++d) {
         const int e0 = gg[d] / P, i0 = gg[d] - e0 * P;
         ce[d][0] = e0 - 1; ci[d][0] = P; cv[d][0] = i0 == 0 && e0 > 0;
         ce[d][1] = e0;     ci[d][1] = i0; cv[d][1] = e0 < ne[d];
     }
-    T v[8];
+    const I rowlen = (I)g.nx * N1;
+    const bool rowbdr = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+#pragma unroll
+    for (int i = 0; i <= P; ++i) out[i] = T(0);
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 2; ++b) {
+            if (!(cv[1][c] && cv[0][b])) continue;
+            const I row = ((((I)ce[1][c] * (I)g.ny + (I)ce[0][b]) * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * rowlen;
+            const T *rp = ye + row + (I)ex * N1;  // this element's N1 values on the row
+            // i = 0: lower neighbour's last value, then this element's first
+            T s0 = out[0];
+            if (ex > 0) s0 += rp[-1];
+            s0 += rp[0];
+            out[0] = s0;
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                // invalid candidates read ye[0] (always in bounds) and add 0
-                const bool ok = cv[2][c] && cv[1][b] && cv[0][a];
-                const size_t idx =
-                    ok ? evec_index(g, N1, ce[0][a], ce[1][b], ce[2][c], ci[1][b], ci[2][c]) + ci[0][a] : 0;
-                const T val = ye[idx];
-                v[(c * 2 + b) * 2 + a] = ok ? val : T(0);
-            }
-    T s = 0;
+            for (int i = 1; i < P; ++i) out[i] += rp[i];
+            if (ex == g.nx - 1) out[P] += rp[P];
+        }
+    const size_t base = row0 + (size_t)ex * P;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) s += v[m];
-    return s;
+    for (int i = 0; i <= P; ++i) {
+        if (i >= cnt) break;
+        const int gx = ex * P + i;
+        if (rowbdr || (dirichlet && (gx == 0 || gx == g.Nx - 1))) out[i] = x[base + i];
+    }
+    return cnt;
 }
 
-// One thread per DOF; a logical block covers 256 consecutive DOFs of one
-// z-plane (bpp blocks per plane), consecutive threads along x so the
-// element-vector rows are read contiguously.  mode 0: y = s, 1: y += s.
-template <class T, int P>
-__global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
-                                                    const T *__restrict__ x, T *__restrict__ y,
-                                                    int mode, int dirichlet, unsigned bpp,
-                                                    const int *skip) {
-    if (skip && *skip) return;
-    const unsigned nlog = bpp * (unsigned)g.Nz;
-    const unsigned lb = xcd_block(blockIdx.x, nlog);
-    if (lb >= nlog) return;
-    const int Ny = (int)g.Ny, Nx = (int)g.Nx;
-    const int gz = (int)(lb / bpp);
-    const int id = (int)(lb - (unsigned)gz * bpp) * 256 + threadIdx.x;
-    if (id >= Nx * Ny) return;
-    const int gy = id / Nx, gx = id - gy * Nx;
-    const size_t gi = (size_t)gx + (size_t)Nx * ((size_t)gy + (size_t)Ny * gz);
-    const T s = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, gi);
-    if (mode == 0) y[gi] = s;
-    else y[gi] += s;
-}
-
-// Linear DOF index -> (gx, gy, gz).
-__device__ __forceinline__ void dof_coords(const HexGrid &g, size_t i, int &gx, int &gy, int &gz) {
-    if (g.Nx * g.Ny * g.Nz < (1ll << 32)) {
-        const unsigned u = (unsigned)i, nx = (unsigned)g.Nx, ny = (unsigned)g.Ny;
+// Segment coordinates of linear segment id sid = ex + nx (gy + Ny gz).
+__device__ __forceinline__ void seg_coords(const HexGrid &g, size_t sid, int &ex, int &gy, int &gz) {
+    if ((double)g.nx * g.Ny * g.Nz < 4294967296.0) {
+        const unsigned u = (unsigned)sid, nx = (unsigned)g.nx, ny = (unsigned)g.Ny;
         const unsigned r = u / nx;
-        gx = (int)(u - r * nx);
+        ex = (int)(u - r * nx);
         gz = (int)(r / ny);
         gy = (int)(r - (unsigned)gz * ny);
     } else {
-        const size_t r = i / (size_t)g.Nx;
-        gx = (int)(i - r * (size_t)g.Nx);
+        const size_t r = sid / (size_t)g.nx;
+        ex = (int)(sid - r * (size_t)g.nx);
         gz = (int)(r / (size_t)g.Ny);
         gy = (int)(r - (size_t)gz * g.Ny);
     }
 }
 
+// y (+)= gathered values; one thread per segment (consecutive threads along x).
+// mode 0: y = s, 1: y += s.
+template <class T, int P, class I>
+__global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__restrict__ ye,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    int mode, int dirichlet, const int *skip) {
+    if (skip && *skip) return;
+    const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
+    const size_t sid = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (sid >= nseg) return;
+    int ex, gy, gz;
+    seg_coords(g, sid, ex, gy, gz);
+    const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
+    T t[P + 1];
+    const int cnt = gather_segment<T, P, I>(g, ye, x, dirichlet, ex, gy, gz, row0, t);
+    T *yp = y + row0 + (size_t)ex * P;
+#pragma unroll
+    for (int i = 0; i <= P; ++i) {
+        if (i >= cnt) break;
+        yp[i] = mode == 0 ? t[i] : yp[i] + t[i];
+    }
+}
+
 // Gather fused with the Chebyshev step that consumes it (the solver's k_chebk):
-// t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z.
-// Same grid-stride grid as k_chebk, so the partial sums are identical.
-template <class T, int P>
+// t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z
+// over the fixed red_blocks(n) x 256 grid-stride grid of segments.
+template <class T, int P, class I>
 __global__ __launch_bounds__(256) void k_hex_gather_cheb(
     const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
     const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2, int last,
     double *__restrict__ part, int dirichlet, const int *skip) {
     if (skip && *skip) return;
-    const size_t n = (size_t)g.Nx * g.Ny * g.Nz;
+    const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
     double v[1] = {0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        int gx, gy, gz;
-        dof_coords(g, i, gx, gy, gz);
-        const T t = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, i);  // reads x[i] only
-        const T di = (T)c1 * d[i] + (T)c2 * (inv[i] * (r[i] - t));
-        d[i] = di;
-        const T zi = z[i] + di;
-        z[i] = zi;
-        if (last) v[0] += (double)r[i] * (double)zi;
+    for (size_t sid = blockIdx.x * (size_t)256 + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * 256) {
+        int ex, gy, gz;
+        seg_coords(g, sid, ex, gy, gz);
+        const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
+        T t[P + 1];
+        const int cnt = gather_segment<T, P, I>(g, ye, x, dirichlet, ex, gy, gz, row0, t);  // reads x of its DOFs
+        const size_t base = row0 + (size_t)ex * P;
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            if (i >= cnt) break;
+            const size_t k = base + i;
+            const T di = (T)c1 * d[k] + (T)c2 * (inv[k] * (r[k] - t[i]));
+            d[k] = di;
+            const T zi = z[k] + di;
+            z[k] = zi;
+            if (last) v[0] += (double)r[k] * (double)zi;
+        }
     }
     if (last) {
         block_sum<1, 256>(v);
@@ -438,21 +446,28 @@ __global__ __launch_bounds__(256) void k_hex_gather_cheb(
 }
 
 // Gather fused with the dot product that follows it (the solver's k_dot):
-// y = A x, partials of x.y in slot 0, on k_dot's grid.
-template <class T, int P>
+// y = A x and partials of x.y in slot 0, on the solver's reduction grid.
+template <class T, int P, class I>
 __global__ __launch_bounds__(256) void k_hex_gather_dot(const HexGrid g, const T *__restrict__ ye,
                                                         const T *__restrict__ x, T *__restrict__ y,
                                                         double *__restrict__ part, int dirichlet,
                                                         const int *skip) {
     if (skip && *skip) return;
-    const size_t n = (size_t)g.Nx * g.Ny * g.Nz;
+    const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
     double v[1] = {0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        int gx, gy, gz;
-        dof_coords(g, i, gx, gy, gz);
-        const T t = gather_dof<T, P>(g, ye, x, dirichlet, gx, gy, gz, i);
-        y[i] = t;
-        v[0] += (double)x[i] * (double)t;
+    for (size_t sid = blockIdx.x * (size_t)256 + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * 256) {
+        int ex, gy, gz;
+        seg_coords(g, sid, ex, gy, gz);
+        const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
+        T t[P + 1];
+        const int cnt = gather_segment<T, P, I>(g, ye, x, dirichlet, ex, gy, gz, row0, t);
+        const size_t base = row0 + (size_t)ex * P;
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            if (i >= cnt) break;
+            y[base + i] = t[i];
+            v[0] += (double)x[base + i] * (double)t[i];
+        }
     }
     block_sum<1, 256>(v);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
@@ -646,18 +661,28 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
     return false;
 }
 
+// element vectors below 2^31 entries use 32-bit index arithmetic in the gathers
+static inline bool evec_small(const HexGrid &g, int n1) { return (double)g.E * n1 * n1 * n1 < 2147483648.0; }
+
+template <class T, class I>
+static void launch_gather_i(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
+                            const int *skip, hipStream_t s) {
+    const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
+    const dim3 grid((unsigned)((nseg + 255) / 256));
+    switch (p) {
+    case 1: hipLaunchKernelGGL((k_hex_gather<T, 1, I>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 2: hipLaunchKernelGGL((k_hex_gather<T, 2, I>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 3: hipLaunchKernelGGL((k_hex_gather<T, 3, I>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    case 4: hipLaunchKernelGGL((k_hex_gather<T, 4, I>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    default: hipLaunchKernelGGL((k_hex_gather<T, 5, I>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, skip); break;
+    }
+}
+
 template <class T>
 static void launch_gather(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
                           const int *skip, hipStream_t s) {
-    const unsigned bpp = (unsigned)((g.Nx * g.Ny + 255) / 256);
-    const dim3 grid(xcd_grid((size_t)bpp * (size_t)g.Nz));
-    switch (p) {
-    case 1: hipLaunchKernelGGL((k_hex_gather<T, 1>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
-    case 2: hipLaunchKernelGGL((k_hex_gather<T, 2>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
-    case 3: hipLaunchKernelGGL((k_hex_gather<T, 3>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
-    case 4: hipLaunchKernelGGL((k_hex_gather<T, 4>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
-    default: hipLaunchKernelGGL((k_hex_gather<T, 5>), grid, dim3(256), 0, s, g, ye, x, y, mode, dir, bpp, skip); break;
-    }
+    if (evec_small(g, p + 1)) launch_gather_i<T, uint32_t>(p, g, ye, x, y, mode, dir, skip, s);
+    else launch_gather_i<T, size_t>(p, g, ye, x, y, mode, dir, skip, s);
 }
 
 static bool supported(int n1, int q) {
@@ -697,10 +722,16 @@ struct HexOp final : femgpu_op {
         const T *ye = evec.as<T>();
 #define GF(P_)                                                                                              \
     do {                                                                                                    \
-        if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
-        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
+        if (small) GF2(P_, uint32_t);                                                                       \
+        else GF2(P_, size_t);                                                                               \
+    } while (0)
+#define GF2(P_, I_)                                                                                         \
+    do {                                                                                                    \
+        if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_, I_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
+        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
                                 last, part, dir, skip);                                                     \
     } while (0)
+        const bool small = evec_small(g, desc.p + 1);
         switch (desc.p) {
         case 1: GF(1); break;
         case 2: GF(2); break;
@@ -708,6 +739,7 @@ struct HexOp final : femgpu_op {
         case 4: GF(4); break;
         default: GF(5); break;
         }
+#undef GF2
 #undef GF
     }
 
