@@ -303,6 +303,14 @@ __global__ __launch_bounds__(1024) void k_fin_store(const double *part, int nb, 
     if (threadIdx.x == 0) S[slot] = o[0];
 }
 
+// Start of a continued PCG segment: running, no iterations yet, |b| given.
+__global__ void k_set_cont(double *S, int *flags, double bnorm) {
+    S[S_BNORM] = bnorm;
+    S[S_STATUS] = ST_RUNNING;
+    S[S_ITERS] = 0;
+    flags[0] = 0;
+}
+
 static unsigned vgrid(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -395,7 +403,6 @@ template <class T> struct Krylov {
         const T *inv = nullptr;
         if (prec == PREC_JACOBI) inv = build_inv(1e-30);
         else if (prec == PREC_CHEB) inv = vec(W_INV);  // caller prepared it
-        const double brk = mode == MODE_CG ? 1e-32 : 1e-30;
         // r0 = b - A x0
         apply(x, ap, false);
         if (prec == PREC_JACOBI)
@@ -412,6 +419,43 @@ template <class T> struct Krylov {
             hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, (int)S_RZ);
             copy<T>(s, p, z, n);
         }
+        return run(mode, prec, cp, x, inv, rtol, atol, max_iter, res, false);
+    }
+
+    // Continues PCG + Chebyshev-Jacobi on the residual the caller put in
+    // vec(W_R) (e.g. a residual recomputed in higher precision), with x
+    // accumulating the correction.  first: p = z; otherwise the previous search
+    // direction is kept and p = z + beta p, beta = r.z / (previous r.z), so the
+    // Krylov recurrence goes on across the replacement.  Relative residuals are
+    // against bnorm.
+    int solve_cont(bool first, const ChebParams &cp, T *x, double bnorm, double rtol, uint64_t max_iter,
+                   femgpu_solve_result *res) {
+        const unsigned nb = nblk(n);
+        T *r = vec(W_R), *z = vec(W_Z), *p = vec(W_P);
+        const T *inv = vec(W_INV);
+        hipLaunchKernelGGL(k_set_cont, dim3(1), dim3(1), 0, s, S, flags, bnorm);
+        cheb_apply(cp, r, inv, z);
+        if (first) {
+            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, (int)S_RZ);
+            copy<T>(s, p, z, n);
+        } else {
+            hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, flags);
+            hipLaunchKernelGGL(k_pupdate<T>, dim3(vgrid(n)), dim3(256), 0, s, n, (const double *)S, (const T *)z,
+                               p, flags);
+        }
+        return run(MODE_PCG, PREC_CHEB, cp, x, inv, rtol, 0.0, max_iter, res, true);
+    }
+
+    // The PCG iteration loop and its result, shared by solve / solve_cont.
+    // segment: the last iteration stops after the update of x and r (no
+    // preconditioning and no new search direction), so that solve_cont can
+    // go on from a replaced residual with the last search direction.
+    int last_status = ST_RUNNING;
+    int run(int mode, int prec, const ChebParams &cp, T *x, const T *inv, double rtol, double atol,
+            uint64_t max_iter, femgpu_solve_result *res, bool segment) {
+        const unsigned nb = nblk(n);
+        T *r = vec(W_R), *z = vec(W_Z), *p = vec(W_P), *ap = vec(W_AP);
+        const double brk = mode == MODE_CG ? 1e-32 : 1e-30;
         uint64_t it = 0;
         bool stopped = read_flag() != 0;
         const uint64_t check = n < (1u << 20) ? 16 : 4;
@@ -432,6 +476,10 @@ template <class T> struct Krylov {
             hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                                flags, mode, prec == PREC_JACOBI ? 1 : 0, prec != PREC_CHEB ? 1 : 0,
                                rtol, atol, (double)(it + 1));
+            if (segment && it + 1 == max_iter) {
+                ++it;
+                break;
+            }
             if (prec == PREC_CHEB) {
                 cheb_apply(cp, r, inv, z);
                 hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
@@ -446,6 +494,7 @@ template <class T> struct Krylov {
         read_scalars();
         const double *h = c->pinned;
         const int status = (int)h[S_STATUS];
+        last_status = status;
         res->converged = status == ST_CONVERGED;
         res->final_residual_is_relative = mode == MODE_PCG;
         res->error_estimate = 0.0;
@@ -608,6 +657,72 @@ static int ir_refine(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *i
     return 0;
 }
 
+// Continued refinement: one FP32 PCG + Chebyshev-Jacobi run whose recursive
+// residual is replaced by the FP64 residual r = b - A64 u (after the FP32
+// correction has been added to u in FP64) each time it has dropped by the
+// factor delta since the last replacement (or after max_seg iterations).  The
+// search direction survives each replacement (the Krylov recurrence goes on),
+// so the FP32 iterations are not lost to restarts as in ir_refine, and the
+// FP32 recursive residual never has to be accurate far below the residual it
+// started from.  Stops on the FP64 relative residual; a breakdown in a segment
+// restarts with p = z.
+static int ir_continued(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *ir, const void *b,
+                        void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
+                        femgpu_solve_result *res, uint64_t *inner_iterations) {
+    femgpu_ctx *c = op64->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    FG_HIP(hipSetDevice(c->device));
+    Staged st(op64, b, x, mem, 8, 9);
+    Krylov<double> K64(op64);
+    Krylov<float> K32(op32);
+    const size_t n = op64->n;
+    hipStream_t s = c->stream;
+    double *r64 = K64.vec(K64.W_R);
+    float *r32 = K32.vec(K32.W_R), *x32 = K32.vec(K32.W_X);
+    const double *bd = (const double *)st.b;
+    double *xd = (double *)st.x;
+    ChebParams cp = K32.prepare_cheb(&ir->cheb);
+    const double bnorm = std::sqrt(dot_host(c, bd, bd, n, FEMGPU_F64));
+    auto residual = [&]() {
+        op64->apply_dev(xd, r64, false, nullptr);
+        hipLaunchKernelGGL(k_sub_from<double>, dim3(vgrid(n)), dim3(256), 0, s, n, bd, r64);
+        return std::sqrt(dot_host(c, r64, r64, n, FEMGPU_F64));
+    };
+    double rel = bnorm > 0 ? residual() / bnorm : 0.0;
+    const double delta = ir->inner_rtol > 0 && ir->inner_rtol < 1 ? ir->inner_rtol : 1e-2;
+    const uint64_t max_seg = ir->update_interval > 0 ? (uint64_t)ir->update_interval : cfg->max_iter;
+    uint64_t total = 0, updates = 0;
+    bool first = true;
+    double best = rel;
+    int since_best = 0;
+    while (bnorm > 0 && !(rel < cfg->rtol) && total < cfg->max_iter) {
+        convert_f64_to_f32(s, r32, r64, n, 1.0);
+        fill<float>(s, x32, 0.f, n);
+        femgpu_solve_result seg{};
+        const uint64_t len = std::min<uint64_t>(max_seg, cfg->max_iter - total);
+        K32.solve_cont(first, cp, x32, bnorm, std::max(0.5 * cfg->rtol, delta * rel), len, &seg);
+        total += seg.converged ? seg.iterations : len;
+        first = K32.last_status == ST_BREAKDOWN;
+        convert_f32_to_f64_axpy(s, xd, x32, n, 1.0);
+        rel = residual() / bnorm;
+        ++updates;
+        if (!std::isfinite(rel)) break;
+        if (rel < best) { best = rel; since_best = 0; }
+        else if (++since_best >= 8) break;  // no progress over 8 replacements
+    }
+    st.finish();
+    if (inner_iterations) *inner_iterations = total;
+    res->iterations = updates;
+    res->final_residual = rel;
+    res->final_residual_is_relative = 1;
+    res->converged = bnorm == 0 || rel < cfg->rtol;
+    res->error_estimate = 0.0;
+    if (!res->converged)
+        return fail(FEMGPU_ERR_NOT_CONVERGED,
+                    "mixed-precision refinement stopped at relative residual " + std::to_string(rel));
+    return 0;
+}
+
 extern "C" {
 
 int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
@@ -747,6 +862,15 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
                           const void *b, void *x, femgpu_mem mem, const femgpu_solver_config *cfg,
                           femgpu_solve_result *res, uint64_t *inner_iterations) {
     return guard([&] {
+        if (ir && ir->mode == 1) {
+            FG_REQUIRE(op64 && op32 && b && x && res, FEMGPU_ERR_INVALID_ARG, "solve_ir_mixed: NULL argument");
+            FG_REQUIRE(op64->dtype == FEMGPU_F64 && op32->dtype == FEMGPU_F32, FEMGPU_ERR_INVALID_ARG,
+                       "solve_ir_mixed: need an f64 and an f32 operator");
+            FG_REQUIRE(op64->n == op32->n && op64->ctx == op32->ctx, FEMGPU_ERR_DIM_MISMATCH,
+                       "solve_ir_mixed: operators differ in size or context");
+            check_cfg(cfg);
+            return ir_continued(op64, op32, ir, b, x, mem, cfg, res, inner_iterations);
+        }
         return ir_refine(op64, op32, ir, b, x, mem, cfg, res, inner_iterations,
                          [&](Krylov<float> &K32, const ChebParams &cp, const float *r32, float *e32,
                              double t, uint64_t cap, femgpu_solve_result *ir_res) {

@@ -85,7 +85,7 @@ class _ChebConfig(C.Structure):
 
 class _IrConfig(C.Structure):
     _fields_ = [("inner_rtol", D), ("inner_max_iter", U64), ("max_outer", U64),
-                ("cheb", _ChebConfig)]
+                ("cheb", _ChebConfig), ("mode", I), ("update_interval", I)]
 
 
 def _sig(name, args, res=I):
@@ -634,17 +634,23 @@ def estimate_spectral_radius(a, n_iter):
     return rho.value
 
 
-def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0, max_outer=50,
-                   degree=4, ratio=30.0, safety=1.1, power_iters=20, lmax=0.0):
+def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=None, inner_max_iter=0, max_outer=50,
+                   degree=4, ratio=30.0, safety=1.1, power_iters=20, lmax=0.0, mode=1, update_interval=16):
     """FP32-inner / FP64-outer iterative refinement; returns (SolveResult with
-    outer iterations, total inner iterations)."""
+    outer iterations / residual replacements, total inner iterations).
+    mode 1 (default): one FP32 PCG whose residual is replaced by the FP64
+    residual each time it has dropped by inner_rtol (default 1e-2), at most
+    update_interval iterations apart; mode 0: restarted inner solves to
+    inner_rtol (default 1e-4)."""
     cfg = cfg or SolverConfig()
+    if inner_rtol is None:
+        inner_rtol = 1e-2 if mode == 1 else 1e-4
     _check_dims(op64, b, x)
     bp, xp, m = _solve_vectors(op64, b, x)
     r = _SolveResult()
     inner = U64()
     irc = _IrConfig(inner_rtol, inner_max_iter, max_outer,
-                    _ChebConfig(degree, ratio, safety, power_iters, lmax))
+                    _ChebConfig(degree, ratio, safety, power_iters, lmax), mode, update_interval)
     rc = lib.femgpu_solve_ir_mixed(op64.h, op32.h, C.byref(irc), bp, xp, m, C.byref(cfg._c()),
                                    C.byref(r), C.byref(inner))
     if rc not in (OK, ERR_NOT_CONVERGED):
@@ -664,7 +670,7 @@ def solve_ir_learned(op64, op32, net, b, x, cfg=None, inner_rtol=1e-4, inner_max
     inner = U64()
     info = _FcgInfo(patience, truncation, 0, 0, 0, 0, 0)
     irc = _IrConfig(inner_rtol, inner_max_iter, max_outer,
-                    _ChebConfig(degree, ratio, safety, power_iters, lmax))
+                    _ChebConfig(degree, ratio, safety, power_iters, lmax), 0, 0)
     rc = lib.femgpu_solve_ir_learned(op64.h, op32.h, net.h, C.byref(irc), bp, xp, m, C.byref(cfg._c()),
                                      C.byref(r), C.byref(inner), C.byref(info))
     if rc not in (OK, ERR_NOT_CONVERGED):

@@ -243,10 +243,17 @@ int femgpu_estimate_spectral_radius(femgpu_op *op, int n_iter, double *rho);
  * cfg->max_iter bounds the total inner iterations.  Stops early when an outer
  * step does not reduce |r|. */
 typedef struct {
-    double inner_rtol;         /* default 1e-4 */
-    uint64_t inner_max_iter;   /* per inner solve; 0: cfg->max_iter */
-    uint64_t max_outer;        /* default 50 */
+    double inner_rtol;         /* mode 0: inner relative tolerance, default 1e-4;
+                                  mode 1: residual reduction per segment, default 1e-2 */
+    uint64_t inner_max_iter;   /* mode 0: per inner solve; 0: cfg->max_iter */
+    uint64_t max_outer;        /* mode 0: default 50 */
     femgpu_cheb_config cheb;
+    int mode;                  /* 0: restarted inner solves; 1: continued: one FP32 PCG whose
+                                  recursive residual is replaced by the FP64 residual
+                                  b - A64 u (u updated in FP64 first) whenever it has
+                                  dropped by inner_rtol, search direction kept */
+    int update_interval;       /* mode 1: at most this many iterations between
+                                  replacements; 0: no limit */
 } femgpu_ir_config;
 int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config *ir,
                           const void *b, void *x, femgpu_mem mem, const femgpu_solver_config *cfg,

@@ -163,8 +163,31 @@ def test_ir_mixed(fg, fr, ctx):
     op32 = op64.convert(fg.F32)
     b = op64.rhs_random(5)
     x = np.zeros(op64.n_dofs())
-    res, inner = fg.solve_ir_mixed(op64, op32, b, x, fg.SolverConfig(rtol=1e-10), inner_rtol=1e-3)
+    res, inner = fg.solve_ir_mixed(op64, op32, b, x, fg.SolverConfig(rtol=1e-10), inner_rtol=1e-3, mode=0)
     assert res.converged and res.final_residual < 1e-10 and inner > 0
     x64 = np.zeros_like(x)
     r64, _ = fg.solve_pcg_chebyshev(op64, b, x64, fg.SolverConfig(rtol=1e-13, max_iter=5000))
     assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
+
+
+@pytest.mark.parametrize("delta,interval", [(1e-2, 0), (1e-3, 0), (1e-2, 1), (1e-2, 5)])
+def test_ir_mixed_continued(fg, fr, ctx, delta, interval):
+    """Continued refinement (one FP32 PCG, FP64 residual replaced after each
+    reduction by delta, or every `interval` iterations) reaches 1e-10 on the
+    FP64 residual and matches the FP64 solve, with about as many FP32
+    iterations as the FP64 PCG needs."""
+    n, p = 10, 3
+    op64 = _hex_problem(fg, fr, ctx, p, n, dict(kind="grf", seed=1))
+    op32 = op64.convert(fg.F32)
+    b = op64.rhs_random(5)
+    x = np.zeros(op64.n_dofs())
+    res, inner = fg.solve_ir_mixed(op64, op32, b, x, fg.SolverConfig(rtol=1e-10, max_iter=5000), mode=1,
+                                   inner_rtol=delta, update_interval=interval)
+    assert res.converged and res.final_residual < 1e-10
+    assert np.linalg.norm(b - op64.matvec(x)) / np.linalg.norm(b) < 1e-10
+    x64 = np.zeros_like(x)
+    r64, _ = fg.solve_pcg_chebyshev(op64, b, x64, fg.SolverConfig(rtol=1e-13, max_iter=5000))
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
+    xp = np.zeros_like(x)
+    rp, _ = fg.solve_pcg_chebyshev(op64, b, xp, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    assert inner <= 1.3 * rp.iterations + 5, (inner, rp.iterations)
