@@ -65,95 +65,106 @@ static unsigned g1(long long n, int b = 256) {
 // ------------------------------------------------------------ conv fwd --
 // out[v][n] = act(sum_k A[v][k] W[n][k] + bias[n] (+ skip[v][n]))
 // OUTF: 0 -> bf16 output with bias/skip/relu; 1 -> f32 output, plain sum.
-// One wave computes MT tiles of 16 consecutive voxels x all COUT channels on
-// v_mfma_f32_16x16x16bf16_1k, K = 27 CIN in steps of 16.  For CIN >= 16 a
-// k-step is one tap and 16 channels, so the tap, its offset and the channel
-// base are wave-uniform (scalar registers); per voxel tile the six neighbour
-// bits and the linear index are computed once.  CIN = 4 packs four taps per
-// k-step (lane group g = lane / 16 takes tap 4 s + g).
-template <int CIN, int COUT, int MT, int OUTF>
+// Implicit GEMM on v_mfma_f32_16x16x16bf16_1k, K = 27 CIN in steps of 16.
+// A workgroup owns an output tile of 16 x 4 x 4 voxels (x, y, z) and NTB x 16
+// output channels; it stages the tile's input halo brick (18 x 6 x 6 voxels x
+// CIN channels, zero outside the grid) in LDS once, so the 27 taps read LDS
+// instead of re-reading HBM / L2.  Wave w computes z = z0 + w, its four
+// 16-voxel M tiles are the rows y = y0..y0+3.  For CIN >= 16 a k-step is one
+// tap and 16 channels (tap, offset and channel base are wave-uniform); CIN = 4
+// packs four taps per k-step (lane group g = lane / 16 takes tap 4 s + g).
+constexpr int kBX = 16, kBY = 4, kBZ = 4;
+constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
+
+template <int CIN, int COUT, int NTB, int OUTF>
 __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_t *__restrict__ in,
                                                   const uint16_t *__restrict__ W,
                                                   const float *__restrict__ bias,
                                                   const uint16_t *__restrict__ skip,
                                                   void *__restrict__ outp, int relu) {
     static_assert(CIN == 4 || CIN % 16 == 0, "CIN must be 4 or a multiple of 16");
-    constexpr int NT = COUT / 16;
+    static_assert(COUT % (16 * NTB) == 0, "NTB must divide the output tiles");
+    constexpr int NBRICK = kHX * kHY * kHZ;
+    __shared__ __align__(16) uint16_t sIn[NBRICK * CIN];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long V = d.n();
-    const long long vbase = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
-    if (vbase >= V) return;  // uniform per wave; no block barriers in this kernel
-    const long long sy = d.nx, sz = (long long)d.nx * d.ny;
-    long long vv[MT];
-    unsigned nb[MT];  // bit 0: x-1 ok, 1: x+1 ok, 2: y-1, 3: y+1, 4: z-1, 5: z+1, 6: voxel valid
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const long long v = vbase + mt * 16 + (lane & 15);
-        const bool ok = v < V;
-        vv[mt] = ok ? v : 0;
-        const int x = (int)(vv[mt] % d.nx);
-        const long long r = vv[mt] / d.nx;
-        const int y = (int)(r % d.ny), z = (int)(r / d.ny);
-        nb[mt] = (x > 0) | ((x < d.nx - 1) << 1) | ((y > 0) << 2) | ((y < d.ny - 1) << 3) | ((z > 0) << 4) |
-                 ((z < d.nz - 1) << 5) | ((unsigned)ok << 6);
+    const int ntx = (d.nx + kBX - 1) / kBX;
+    const int csplit = COUT / (16 * NTB);  // output-channel groups per tile
+    const int bx = blockIdx.x % ntx, cg = (blockIdx.x / ntx) % csplit;
+    const int x0 = bx * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
+    // ---- stage the halo brick: 8-byte chunks of 4 channels
+    constexpr int C4 = CIN / 4;
+    for (int idx = threadIdx.x; idx < NBRICK * C4; idx += 256) {
+        const int vi = idx / C4, c4 = idx - vi * C4;
+        const int xx = vi % kHX, yy = (vi / kHX) % kHY, zz = vi / (kHX * kHY);
+        const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+        uint2 v = make_uint2(0u, 0u);
+        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
+            v = *reinterpret_cast<const uint2 *>(
+                in + ((size_t)gx + (size_t)d.nx * ((size_t)gy + (size_t)d.ny * gz)) * CIN + c4 * 4);
+        *reinterpret_cast<uint2 *>(sIn + (size_t)vi * CIN + c4 * 4) = v;
     }
-    f32x4 acc[MT][NT];
+    __syncthreads();
+    // ---- MFMA loop
+    f32x4 acc[kBY][NTB];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < kBY; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nt = 0; nt < NTB; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int ksub = (lane >> 4) * 4;
-    const uint16_t *wrow[NT];
+    const int xl = lane & 15;
+    const uint16_t *wrow[NTB];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) wrow[nt] = W + (size_t)(nt * 16 + (lane & 15)) * kpad + ksub;
-    auto tap_ok = [](unsigned m, int dx, int dy, int dz) {
-        return ((m >> 6) & 1) && (dx >= 0 || (m & 1)) && (dx <= 0 || (m & 2)) && (dy >= 0 || (m & 4)) &&
-               (dy <= 0 || (m & 8)) && (dz >= 0 || (m & 16)) && (dz <= 0 || (m & 32));
-    };
+    for (int nt = 0; nt < NTB; ++nt)
+        wrow[nt] = W + (size_t)((cg * NTB + nt) * 16 + (lane & 15)) * kpad + ksub;
     for (int k0 = 0; k0 < kpad; k0 += 16) {
-        bf16x4 b[NT];
+        bf16x4 b[NTB];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) b[nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + k0);
+        for (int nt = 0; nt < NTB; ++nt) b[nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + k0);
         int tap, c0;
         if constexpr (CIN >= 16) {
-            tap = k0 / CIN;            // wave-uniform
+            tap = k0 / CIN;  // wave-uniform
             c0 = k0 - tap * CIN + ksub;
         } else {
-            tap = k0 / 4 + (lane >> 4);  // four taps per k-step
+            tap = k0 / 4 + (lane >> 4);
             c0 = 0;
         }
+        const bool tv = tap < 27;
         const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-        const long long off = dz * sz + dy * sy + dx;
+        const int base = ((wave + 1 + dz) * kHY + 1 + dy) * kHX + xl + 1 + dx;
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
+        for (int mt = 0; mt < kBY; ++mt) {
             bf16x4 a = bf16x4{0, 0, 0, 0};
-            if (tap < 27 && tap_ok(nb[mt], dx, dy, dz))
-                a = *reinterpret_cast<const bf16x4 *>(in + (size_t)(vv[mt] + off) * CIN + c0);
+            if (tv) a = *reinterpret_cast<const bf16x4 *>(sIn + (size_t)(base + mt * kHX) * CIN + c0);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
+            for (int nt = 0; nt < NTB; ++nt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
         }
     }
+    // ---- epilogue: D[m = voxel x (4 (lane/16) + i)][n = lane % 16]
+    const int z = z0 + wave;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < kBY; ++mt) {
+        const int y = y0 + mt;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int n = nt * 16 + (lane & 15);
+        for (int nt = 0; nt < NTB; ++nt) {
+            const int n = (cg * NTB + nt) * 16 + (lane & 15);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const long long v = vbase + mt * 16 + (lane >> 4) * 4 + i;
-                if (v >= V) continue;
+                const int x = x0 + (lane >> 4) * 4 + i;
+                if (x >= d.nx || y >= d.ny || z >= d.nz) continue;
+                const size_t v = (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z);
                 float val = acc[mt][nt][i];
                 if (OUTF) {
-                    reinterpret_cast<float *>(outp)[(size_t)v * COUT + n] = val;
+                    reinterpret_cast<float *>(outp)[v * COUT + n] = val;
                 } else {
                     if (bias) val += bias[n];
-                    if (skip) val += bf2f(skip[(size_t)v * COUT + n]);
+                    if (skip) val += bf2f(skip[v * COUT + n]);
                     if (relu) val = fmaxf(val, 0.f);
-                    reinterpret_cast<uint16_t *>(outp)[(size_t)v * COUT + n] = f2bf(val);
+                    reinterpret_cast<uint16_t *>(outp)[v * COUT + n] = f2bf(val);
                 }
             }
         }
+    }
 }
 
 // ----------------------------------------------------- conv weight grad --
@@ -617,16 +628,22 @@ static void conv_fwd(femgpu_unet *u, int l, const uint16_t *in, const uint16_t *
     const int cin = cout_override ? L.cout : L.cin;  // data-gradient: channels swap
     const int kpad = cout_override ? kpad_of(L.cout) : L.kpad;
     const long long V = L.d.n();
-    // voxel tiles per wave: enough waves to fill the GPU at the coarse levels
-    const int mt = V >= 64ll * 4096 ? 4 : V >= 32ll * 4096 ? 2 : 1;
-    const dim3 grid((unsigned)((V + 64ll * mt - 1) / (64ll * mt))), block(256);
+    const int ntx = (L.d.nx + kBX - 1) / kBX;
     const int r = relu ? 1 : 0;
-#define CF(CI, CO, M, OF) hipLaunchKernelGGL((k_conv_fwd<CI, CO, M, OF>), grid, block, 0, s, L.d, kpad, in, W, bias, skip, out, r)
-#define CM(CI, CO, OF)                    \
-    do {                                  \
-        if (mt == 4) CF(CI, CO, 4, OF);   \
-        else if (mt == 2) CF(CI, CO, 2, OF); \
-        else CF(CI, CO, 1, OF);           \
+    // split the output channels over workgroups at the coarse levels (few tiles)
+    const long long tiles = (long long)ntx * ((L.d.ny + kBY - 1) / kBY) * ((L.d.nz + kBZ - 1) / kBZ);
+    const bool split = tiles < 2048;
+    (void)V;
+#define CF(CI, CO, NB, OF)                                                                                  \
+    do {                                                                                                    \
+        const dim3 grid((unsigned)(ntx * (CO / (16 * NB))), (unsigned)((L.d.ny + kBY - 1) / kBY),           \
+                        (unsigned)((L.d.nz + kBZ - 1) / kBZ));                                              \
+        hipLaunchKernelGGL((k_conv_fwd<CI, CO, NB, OF>), grid, dim3(256), 0, s, L.d, kpad, in, W, bias, skip, out, r); \
+    } while (0)
+#define CM(CI, CO, OF)                        \
+    do {                                      \
+        if (split) CF(CI, CO, 1, OF);         \
+        else CF(CI, CO, CO / 16, OF);         \
     } while (0)
     if (!outf32 && cin == 4 && cout == 16) CM(4, 16, 0);
     else if (!outf32 && cin == 16 && cout == 32) CM(16, 32, 0);
