@@ -9,6 +9,9 @@
 //  - row-per-subgroup (8..64 lanes, shuffle-reduced), used when rows are long
 //    (assembled high-order hex matrices); equal within rounding.
 #include "internal.hpp"
+
+#include <algorithm>
+#include <vector>
 #include "reduce.hpp"
 
 #include <cstdlib>
@@ -98,6 +101,27 @@ __global__ void k_csr_diag(size_t n, const uint64_t *__restrict__ rp,
     for (uint64_t k = rp[r]; k < rp[r + 1]; ++k)
         if (ci[k] == r) { out = v[k]; break; }
     d[r] = out;
+}
+
+// apply_dirichlet_row_zeroing (linalg/src/csr.rs:297-312) for the m
+// constrained rows: zero the row, set its first diagonal entry to 1 and
+// rhs[row] = value.  Rows are distinct (the host keeps each DOF's last value,
+// as the reference's sequential loop does), so the rows are independent.
+template <class T>
+__global__ void k_csr_dirichlet(size_t m, const uint32_t *__restrict__ dofs,
+                                const double *__restrict__ vals, const uint64_t *__restrict__ rp,
+                                const uint32_t *__restrict__ ci, T *__restrict__ v,
+                                T *__restrict__ rhs) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t d = dofs[i];
+    bool diag = false;
+    for (uint64_t k = rp[d]; k < rp[d + 1]; ++k) {
+        const bool first = !diag && ci[k] == d;
+        v[k] = first ? T(1) : T(0);
+        diag = diag || first;
+    }
+    if (rhs) rhs[d] = (T)vals[i];
 }
 
 struct CsrOp final : femgpu_op {
@@ -227,6 +251,74 @@ int femgpu_csr_spmv_add(femgpu_op *op_, double alpha, const void *x, double beta
         FG_HIP(hipGetLastError());
         if (mem == FEMGPU_HOST)
             FG_HIP(hipMemcpyAsync(y, yd, bytes, hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+// apply_dirichlet (space/src/constraints.rs:31-42) on a device CSR matrix.
+int femgpu_csr_apply_dirichlet(femgpu_op *op_, uint64_t n_constrained, const uint32_t *dofs,
+                               const double *values, void *rhs, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<CsrOp *>(op_);
+        FG_REQUIRE(op, FEMGPU_ERR_INVALID_ARG, "csr_apply_dirichlet: not a CSR operator");
+        FG_REQUIRE(n_constrained == 0 || (dofs && values), FEMGPU_ERR_INVALID_ARG,
+                   "csr_apply_dirichlet: NULL dofs or values");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        // last value wins for repeated DOFs, as in the reference's loop
+        std::vector<int64_t> last(op->n, -1);
+        for (uint64_t i = 0; i < n_constrained; ++i) {
+            FG_REQUIRE(dofs[i] < op->n, FEMGPU_ERR_DIM_MISMATCH,
+                       "csr_apply_dirichlet: DOF " + std::to_string(dofs[i]) + " out of range");
+            last[dofs[i]] = (int64_t)i;
+        }
+        std::vector<uint32_t> ud;
+        std::vector<double> uv;
+        for (uint64_t i = 0; i < n_constrained; ++i)
+            if (last[dofs[i]] == (int64_t)i) { ud.push_back(dofs[i]); uv.push_back(values[i]); }
+        const size_t m = ud.size();
+        hipStream_t s = c->stream;
+        const size_t bytes = op->n * op->esize();
+        DevBuf dd(std::max<size_t>(m, 1) * 4), dv(std::max<size_t>(m, 1) * 8);
+        void *rd = rhs;
+        if (rhs && mem == FEMGPU_HOST) {
+            op->stage_y.ensure(bytes);
+            FG_HIP(hipMemcpyAsync(op->stage_y.ptr, rhs, bytes, hipMemcpyHostToDevice, s));
+            rd = op->stage_y.ptr;
+        }
+        if (m) {
+            FG_HIP(hipMemcpyAsync(dd.ptr, ud.data(), m * 4, hipMemcpyHostToDevice, s));
+            FG_HIP(hipMemcpyAsync(dv.ptr, uv.data(), m * 8, hipMemcpyHostToDevice, s));
+            const dim3 grid((unsigned)((m + 255) / 256));
+            if (op->dtype == FEMGPU_F64)
+                hipLaunchKernelGGL(k_csr_dirichlet<double>, grid, dim3(256), 0, s, m, (const uint32_t *)dd.ptr,
+                                   (const double *)dv.ptr, op->rp.as<uint64_t>(), op->ci.as<uint32_t>(),
+                                   op->vals.as<double>(), (double *)rd);
+            else
+                hipLaunchKernelGGL(k_csr_dirichlet<float>, grid, dim3(256), 0, s, m, (const uint32_t *)dd.ptr,
+                                   (const double *)dv.ptr, op->rp.as<uint64_t>(), op->ci.as<uint32_t>(),
+                                   op->vals.as<float>(), (float *)rd);
+            FG_HIP(hipGetLastError());
+        }
+        op->diag_ready = false;  // the cached diagonal no longer holds
+        if (rhs && mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(rhs, rd, bytes, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipStreamSynchronize(s));
+        return 0;
+    });
+}
+
+// Copy of the matrix values (nnz entries of the operator's dtype).
+int femgpu_csr_get_values(femgpu_op *op_, void *values, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<CsrOp *>(op_);
+        FG_REQUIRE(op && values, FEMGPU_ERR_INVALID_ARG, "csr_get_values: not a CSR operator");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        FG_HIP(hipMemcpyAsync(values, op->vals.ptr, op->nnz * op->esize(),
+                              mem == FEMGPU_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->stream));
         FG_HIP(hipStreamSynchronize(c->stream));
         return 0;
     });

@@ -113,6 +113,8 @@ _sig("femgpu_op_destroy", [P])
 _sig("femgpu_op_time_apply", [P, P, P, I, C.POINTER(D), C.POINTER(D)])
 _sig("femgpu_csr_create", [P, U64, U64, P, P, P, I, C.POINTER(P)])
 _sig("femgpu_csr_spmv_add", [P, D, P, D, P, I])
+_sig("femgpu_csr_apply_dirichlet", [P, U64, P, P, P, I])
+_sig("femgpu_csr_get_values", [P, P, I])
 _sig("femgpu_pa_create_hex", [P, C.POINTER(_HexDesc), P, C.POINTER(P)])
 _sig("femgpu_hex_vertices", [I, I, I, D, U64, P])
 _sig("femgpu_pa_hex_rhs_unit", [P, P, I])
@@ -399,6 +401,30 @@ class CsrMatrix(Operator):
         xp, m = _vec(x, self.dtype, self._n, "x")
         yp, _ = _vec(y, self.dtype, self._n, "y")
         _check(lib.femgpu_csr_spmv_add(self.h, alpha, xp, beta, yp, m))
+
+    def apply_dirichlet_row_zeroing(self, dofs, values, rhs=None):
+        """apply_dirichlet_row_zeroing (csr.rs:297-312) for every (dof, value)
+        in order, on the device copy; rhs (host array or DeviceArray) updated."""
+        d = np.ascontiguousarray(dofs, np.uint32)
+        v = np.ascontiguousarray(values, np.float64)
+        if d.shape != v.shape:
+            raise ValueError("constrained_dofs and values must have the same length")
+        rp, m = (None, HOST) if rhs is None else _vec(rhs, self.dtype, self._n, "rhs")
+        _check(lib.femgpu_csr_apply_dirichlet(self.h, len(d), d.ctypes.data, v.ctypes.data, rp, m))
+
+    def values(self):
+        """The nnz matrix values (CsrMatrix::values) as a host array."""
+        out = np.empty(self._nnz, _npdt(self.dtype))
+        _check(lib.femgpu_csr_get_values(self.h, out.ctypes.data, HOST))
+        return out
+
+
+def apply_dirichlet(mat, rhs, constrained_dofs, values):
+    """apply_dirichlet (space/src/constraints.rs:31-42): row zeroing of the
+    constrained rows of `mat`, rhs set to the prescribed values."""
+    if len(constrained_dofs) != len(values):
+        raise ValueError("constrained_dofs and values must have the same length")
+    mat.apply_dirichlet_row_zeroing(constrained_dofs, values, rhs)
 
 
 def _coef_struct(kappa, qp_keep):

@@ -105,6 +105,16 @@ int femgpu_csr_create(femgpu_ctx *ctx, uint64_t nrows, uint64_t ncols, const uin
 /* CsrMatrix::spmv_add: y = alpha*(A x) + beta*y (csr.rs:167-193). */
 int femgpu_csr_spmv_add(femgpu_op *csr, double alpha, const void *x, double beta, void *y,
                         femgpu_mem mem);
+/* apply_dirichlet (space/src/constraints.rs:31-42) = apply_dirichlet_row_zeroing
+ * (csr.rs:297-312) for each (dofs[i], values[i]) in order: zero the row, set
+ * its diagonal entry to 1 (if stored), rhs[dof] = value.  Repeated DOFs: the
+ * last value wins, as in the reference's loop.  rhs (n entries of the matrix
+ * dtype) may be NULL.  A DOF >= nrows returns DIM_MISMATCH (the reference
+ * panics).  Invalidates the cached diagonal. */
+int femgpu_csr_apply_dirichlet(femgpu_op *csr, uint64_t n_constrained, const uint32_t *dofs,
+                               const double *values, void *rhs, femgpu_mem mem);
+/* Copy of the nnz matrix values (CsrMatrix::values). */
+int femgpu_csr_get_values(femgpu_op *csr, void *values, femgpu_mem mem);
 
 /* Coefficient for the hex PA operators (CoeffCtx/ScalarCoeff,
  * assembly/src/coefficient.rs:67-132, evaluated once at set-up into the

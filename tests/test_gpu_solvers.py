@@ -191,3 +191,44 @@ def test_ir_mixed_continued(fg, fr, ctx, delta, interval):
     xp = np.zeros_like(x)
     rp, _ = fg.solve_pcg_chebyshev(op64, b, xp, fg.SolverConfig(rtol=1e-10, max_iter=5000))
     assert inner <= 1.3 * rp.iterations + 5, (inner, rp.iterations)
+
+
+def test_csr_apply_dirichlet(fg, fr, ctx):
+    """apply_dirichlet on the device matrix (constraints.rs:31-42) equals the
+    oracle's row zeroing bitwise (values and rhs), including prescribed
+    non-zero values, a repeated DOF (last value wins) and the cached diagonal;
+    a homogeneous system built this way solves as the oracle's does."""
+    n = 16
+    coords, conn = fr.unit_square_tri(n)
+    N = (n + 1) ** 2
+    rp, ci, v = fr.simplex_assemble_csr(2, 1, coords, conn, conn.copy(), N, fr.FR_DIFFUSION, 1.0, 3)
+    rhs = fr.p1_source_rhs(n, coords, conn, 3)
+    bnd = fr.unit_square_boundary(n)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    d0 = A.diagonal()
+    b = rhs.copy()
+    dofs = np.concatenate([bnd, bnd[:3]])          # repeat three DOFs with new values
+    dvals = np.concatenate([np.zeros(len(bnd)), [7.0, 8.0, 9.0]])
+    fg.apply_dirichlet(A, b, dofs, dvals)
+    last = np.zeros(len(bnd)); last[:3] = [7.0, 8.0, 9.0]
+    vr, br = fr.dirichlet_row_zeroing(rp, ci, v, bnd, rhs, last)
+    assert np.array_equal(A.values(), vr)
+    assert np.array_equal(b, br)
+    d1 = A.diagonal()
+    assert np.all(d1[bnd] == 1.0) and np.array_equal(np.delete(d1, bnd), np.delete(d0, bnd))
+    # row zeroing keeps CG valid only for homogeneous values with x0 = 0 (the
+    # matrix is no longer symmetric): solve the homogeneous system
+    A0 = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    b0 = rhs.copy()
+    fg.apply_dirichlet(A0, b0, bnd, np.zeros(len(bnd)))
+    x = np.zeros(N)
+    res = fg.solve_pcg_jacobi(A0, b0, x, fg.SolverConfig(rtol=1e-10, max_iter=5000))
+    assert res.converged and np.abs(x[bnd]).max() == 0.0
+    vr0, br0 = fr.dirichlet_row_zeroing(rp, ci, v, bnd, rhs)
+    ref = fr.CsrRef(rp, ci, vr0)
+    xo, ro = fr.pcg_jacobi(fr._Op(N, csr=ref), ref.diagonal(), br0, np.zeros(N), rtol=1e-10, max_iter=5000)
+    assert np.abs(x - xo).max() < 1e-10 * np.abs(xo).max()
+    with pytest.raises(fg.FemGpuError):
+        fg.apply_dirichlet(A, None, [N], [0.0])
+    with pytest.raises(ValueError):
+        fg.apply_dirichlet(A, None, [0, 1], [0.0])
