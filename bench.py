@@ -27,8 +27,12 @@ reported and is not part of the solve time).  Each leg reports its solve
 time, iterations, FP64 relative residual and relative error against an FP64
 PCG-Chebyshev solution to rtol / 100.
 
---workload selects the larger configurations (one solve takes seconds to
-tens of seconds there, so use --steps 1 --warmup 0):
+--workload selects the other configurations (one solve takes seconds to
+tens of seconds at the large ones, so use --steps 1 --warmup 0 there):
+  p1-256-csr        2-D Poisson on the unit square, 256^2 quads split into two
+                    triangles each, P1 (66,049 DOFs), assembled CSR (built
+                    here with NumPy/SciPy as the caller would, row-zeroed by
+                    the library's apply_dirichlet), PCG + Jacobi to 1e-10
   q3-96-grf         Q3, 96^3, kappa = exp(GRF) clipped to [1e-2, 1e2], random
                     nodal RHS, rtol 1e-10 (24,137,569 DOFs)
   q4-128-perturbed  Q4, 128^3, vertices perturbed by up to 0.2 h, diffusion +
@@ -50,6 +54,10 @@ sys.path.insert(0, os.path.join(ROOT, "fem-rs_amd"))
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    "p1-256-csr": dict(p=1, n=256, form="csr", rtol=1e-10,
+                       desc="2-D Poisson -lap u = 2 pi^2 sin(pi x) sin(pi y) on the unit square, {n}^2 quads "
+                            "split into 2 triangles, P1 ({N} DOFs, nnz {nnz}), assembled CSR, homogeneous "
+                            "Dirichlet by row zeroing, FP64 PCG + Jacobi to 1e-10"),
     "q2-64-sinprod": dict(p=2, n=64, form="diffusion", kappa="sinprod", amp=0.0, rhs="unit",
                           rtol=1e-8, mass_apply=True,
                           desc="3-D diffusion, {n}^3 hex Q{p} ({N} DOFs), kappa = 1 + 0.5 sin sin sin, "
@@ -85,6 +93,118 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     return ap.parse_args()
+
+
+def poisson_p1_system(n):
+    """Caller-side assembly of the 2-D P1 Poisson system of the unit square
+    (n x n quads, each split into (v0, v1, v3) and (v1, v2, v3) as
+    mesh/src/simplex.rs:408-413 does): CSR stiffness (COO summed to CSR,
+    columns sorted), load vector of f = 2 pi^2 sin(pi x) sin(pi y) with the
+    3-point rule, and the boundary DOFs."""
+    import scipy.sparse as sp
+    m = n + 1
+    j, i = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    xy = np.stack([i.ravel() / n, j.ravel() / n], axis=1)
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    n0 = (jj * m + ii).ravel(); n1 = n0 + 1; n3 = n0 + m; n2 = n3 + 1
+    conn = np.empty((2 * n0.size, 3), np.int64)
+    conn[0::2] = np.stack([n0, n1, n3], 1)
+    conn[1::2] = np.stack([n1, n2, n3], 1)
+    X = xy[conn]                                   # [E, 3, 2]
+    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]], axis=2)  # columns x_k - x_0
+    det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+    JinvT = np.empty_like(J)
+    JinvT[:, 0, 0] = J[:, 1, 1] / det; JinvT[:, 0, 1] = -J[:, 1, 0] / det
+    JinvT[:, 1, 0] = -J[:, 0, 1] / det; JinvT[:, 1, 1] = J[:, 0, 0] / det
+    gref = np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
+    G = np.einsum("eij,aj->eai", JinvT, gref)      # physical gradients [E, 3, 2]
+    Ke = 0.5 * np.abs(det)[:, None, None] * np.einsum("eai,ebi->eab", G, G)
+    rows = np.repeat(conn, 3, axis=1).ravel(); cols = np.tile(conn, (1, 3)).ravel()
+    A = sp.coo_matrix((Ke.ravel(), (rows, cols)), shape=(m * m, m * m)).tocsr()
+    A.sort_indices()
+    qp = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]); qw = np.full(3, 1 / 6)
+    phi = np.stack([1 - qp[:, 0] - qp[:, 1], qp[:, 0], qp[:, 1]], axis=1)   # [q, a]
+    xq = X[:, 0][:, None, :] + np.einsum("eij,qj->eqi", J, qp)
+    f = 2 * np.pi ** 2 * np.sin(np.pi * xq[..., 0]) * np.sin(np.pi * xq[..., 1])
+    fe = np.einsum("q,eq,qa->ea", qw, f * np.abs(det)[:, None], phi)
+    b = np.bincount(conn.ravel(), weights=fe.ravel(), minlength=m * m)
+    bnd = np.where((i.ravel() == 0) | (i.ravel() == n) | (j.ravel() == 0) | (j.ravel() == n))[0]
+    return (A.indptr.astype(np.uint64), A.indices.astype(np.uint32), A.data.astype(np.float64), b,
+            bnd.astype(np.uint32), xy)
+
+
+def run_csr(a, w):
+    """Config-1 workload: assembled CSR, PCG + Jacobi on the GPU."""
+    import femgpu as fg
+    n = a.n or w["n"]
+    rp, ci, v, b_h, bnd, xy = poisson_p1_system(n)
+    ctx = fg.default_context()
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    N, nnz = len(b_h), int(rp[-1])
+    fg.apply_dirichlet(A, b_h, bnd, np.zeros(len(bnd)))
+    b = fg.DeviceArray(ctx, N, np.float64).upload(b_h)
+    x = fg.DeviceArray(ctx, N, np.float64)
+    x0 = fg.DeviceArray(ctx, N, np.float64)
+    cfg = fg.SolverConfig(rtol=w["rtol"], max_iter=5000)
+    xv = fg.DeviceArray(ctx, N, np.float64).upload(np.random.default_rng(0).standard_normal(N))
+    yv = fg.DeviceArray(ctx, N, np.float64)
+    A.time_apply(xv, yv, 5)
+    ms_spmv, _ = A.time_apply(xv, yv, a.apply_reps)
+    spmv_bytes = 12.0 * nnz + 8.0 * (N + 1) + 16.0 * N
+
+    def step():
+        x.copy_from(x0)
+        return fg.solve_pcg_jacobi(A, b, x, cfg)
+
+    for _ in range(a.warmup):
+        step()
+    times, iters = [], []
+    t_all = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = step()
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters.append(res.iterations)
+        if not res.converged:
+            print(f"bench.py: solve did not converge ({res})", file=sys.stderr)
+            sys.exit(1)
+    total = time.perf_counter() - t_all
+    xs = x.download()
+    err = float(np.abs(xs - np.sin(np.pi * xy[:, 0]) * np.sin(np.pi * xy[:, 1])).max())
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import femref as fr
+        ref = fr.CsrRef(rp, ci, A.values())
+        t0 = time.perf_counter()
+        _, ro = fr.pcg_jacobi(fr._Op(N, csr=ref), ref.diagonal(), b_h, np.zeros(N), rtol=w["rtol"], max_iter=5000)
+        dt = time.perf_counter() - t0
+        cpu = {"value": N * ro.iterations / dt / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+               "sample": f"oracle PCG-Jacobi (par_solver.rs semantics) with the reference's 4-way CSR row sum, "
+                         f"full solve, {ro.iterations} iterations ({dt:.2f} s)"}
+    t_med = float(np.median(times))
+    out = {
+        "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
+        "value": round(N * sum(iters) / total / 1e9, 4), "unit": "GDOF/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"].format(n=n, N=N, nnz=nnz), "name": a.workload, "n_dofs": N, "nnz": nnz,
+                   "global_batch": 1, "parallelism": "single GPU"},
+        "solve": {"iterations": int(iters[-1]), "solve_time_ms": round(1e3 * t_med, 4),
+                  "us_per_iteration": round(1e6 * t_med / max(iters[-1], 1), 2),
+                  "rel_residual": res.final_residual, "max_nodal_error_vs_sin_sin": err},
+        "spmv": {"ms": ms_spmv, "bytes": spmv_bytes, "gb_s": round(spmv_bytes / (ms_spmv * 1e-3) / 1e9, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(spmv_bytes / (ms_spmv * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_bytes / (ms_spmv * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "CSR SpMV k_spmv_row<double> (launch-latency-bound at this size)",
+                     "algorithmic_bytes": spmv_bytes},
+        "legs": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
 
 
 def make_operator(fg, ctx, w, n, p):
@@ -224,6 +344,8 @@ def main():
     import femgpu as fg
 
     w = WORKLOADS[a.workload]
+    if w["form"] == "csr":
+        return run_csr(a, w)
     n = a.n or w["n"]
     p = a.p or w["p"]
     q = p + 1

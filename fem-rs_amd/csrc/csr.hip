@@ -9,6 +9,7 @@
 //  - row-per-subgroup (8..64 lanes, shuffle-reduced), used when rows are long
 //    (assembled high-order hex matrices); equal within rounding.
 #include "internal.hpp"
+#include "csr_row.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -17,40 +18,6 @@
 #include <cstdlib>
 
 namespace femgpu {
-
-template <class T>
-__device__ __forceinline__ T row_dot_ref(const uint64_t *__restrict__ rp,
-                                         const uint32_t *__restrict__ ci,
-                                         const T *__restrict__ v, const T *__restrict__ x,
-                                         size_t row);
-
-template <>
-__device__ __forceinline__ double row_dot_ref<double>(const uint64_t *__restrict__ rp,
-                                                      const uint32_t *__restrict__ ci,
-                                                      const double *__restrict__ v,
-                                                      const double *__restrict__ x, size_t row) {
-#pragma clang fp contract(off)
-    const uint64_t s = rp[row], e = rp[row + 1];
-    const uint64_t e4 = s + (e - s) / 4 * 4;
-    double sum = 0.0;
-    uint64_t k = s;
-    for (; k < e4; k += 4)
-        sum += v[k] * x[ci[k]] + v[k + 1] * x[ci[k + 1]] + v[k + 2] * x[ci[k + 2]] +
-               v[k + 3] * x[ci[k + 3]];
-    for (; k < e; ++k) sum += v[k] * x[ci[k]];
-    return sum;
-}
-
-template <>
-__device__ __forceinline__ float row_dot_ref<float>(const uint64_t *__restrict__ rp,
-                                                    const uint32_t *__restrict__ ci,
-                                                    const float *__restrict__ v,
-                                                    const float *__restrict__ x, size_t row) {
-#pragma clang fp contract(off)
-    float sum = 0.0f;
-    for (uint64_t k = rp[row]; k < rp[row + 1]; ++k) sum += v[k] * x[ci[k]];
-    return sum;
-}
 
 // mode 0: y = Ax; 1: y += Ax; 2: y = alpha*Ax + beta*y
 template <class T>
@@ -63,7 +30,7 @@ __global__ __launch_bounds__(256) void k_spmv_row(size_t n, const uint64_t *__re
     if (skip && *skip) return;
     const size_t r = blockIdx.x * (size_t)256 + threadIdx.x;
     if (r >= n) return;
-    const T s = row_dot_ref<T>(rp, ci, v, x, r);
+    const T s = row_dot_ref<T>(rp, ci, v, [x](uint32_t j) { return x[j]; }, r);
     if (mode == 0) y[r] = s;
     else if (mode == 1) y[r] += s;
     else y[r] = alpha * s + beta * y[r];
@@ -158,6 +125,15 @@ struct CsrOp final : femgpu_op {
             launch<double>((const double *)x, (double *)y, acc ? 1 : 0, 1.0, 0.0, skip);
         else
             launch<float>((const float *)x, (float *)y, acc ? 1 : 0, 1.f, 0.f, skip);
+    }
+
+    bool csr_view(CsrView *out) const override {
+        if (sub != 1) return false;
+        out->n = n;
+        out->rp = rp.as<uint64_t>();
+        out->ci = ci.as<uint32_t>();
+        out->vals = vals.ptr;
+        return true;
     }
 
     void diagonal_dev(void *d) override {

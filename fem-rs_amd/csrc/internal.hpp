@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/femgpu.h"
+#include "csr_row.hpp"
 
 namespace femgpu {
 
@@ -111,6 +112,26 @@ struct femgpu_ctx {
 };
 
 // ------------------------------------------------------------ operator --
+namespace femgpu {
+// An instantiated HIP graph of a chunk of solver iterations, valid for the
+// device pointers and parameters in key.
+struct IterGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<const void *> key;
+    std::vector<double> params;
+    IterGraph() = default;
+    IterGraph(const IterGraph &) = delete;
+    IterGraph &operator=(const IterGraph &) = delete;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+        params.clear();
+    }
+    ~IterGraph() { reset(); }
+};
+}  // namespace femgpu
+
 struct femgpu_op {
     femgpu_ctx *ctx = nullptr;
     femgpu_dtype dtype = FEMGPU_F64;
@@ -120,6 +141,7 @@ struct femgpu_op {
     // Solver workspace vectors, grown on demand and reused across solves.
     std::vector<femgpu::DevBuf> ws;
     bool diag_ready = false;  // workspace slot of the diagonal holds diag(A)
+    femgpu::IterGraph pcg_graph;  // cached iteration chunk of the fused PCG-Jacobi
     void *wsvec(size_t i, size_t bytes) {
         if (ws.size() <= i) ws.resize(i + 1);
         ws[i].ensure(bytes);
@@ -150,6 +172,13 @@ struct femgpu_op {
     }
     virtual bool apply_dot_dev(const void *x, void *y, double *part, const int *skip) {
         (void)x; (void)y; (void)part; (void)skip;
+        return false;
+    }
+    // Assembled CSR operators whose SpMV runs one row per thread in the
+    // reference's summation order expose their arrays here, so that the
+    // PCG-Jacobi iteration can form the search direction inside the SpMV.
+    virtual bool csr_view(femgpu::CsrView *out) const {
+        (void)out;
         return false;
     }
     size_t esize() const { return dtype == FEMGPU_F64 ? 8 : 4; }

@@ -37,6 +37,12 @@ enum Prec { PREC_NONE = 0, PREC_JACOBI = 1, PREC_CHEB = 2 };
 static inline unsigned nblk(size_t n) { return (unsigned)red_blocks(n); }
 
 // ------------------------------------------------------------ kernels --
+// a*b + c with one rounding (written out so that every kernel forming the
+// same update rounds it the same way)
+template <class T> __device__ __forceinline__ T fmad(T a, T b, T c);
+template <> __device__ __forceinline__ double fmad<double>(double a, double b, double c) { return fma(a, b, c); }
+template <> __device__ __forceinline__ float fmad<float>(float a, float b, float c) { return fmaf(a, b, c); }
+
 // r = b - ax;  z = inv*r (or r);  p = z;  partials: r.z, b.b, r.r
 template <class T, bool JAC>
 __global__ __launch_bounds__(256) void k_init(size_t n, const T *__restrict__ b,
@@ -85,8 +91,8 @@ __global__ __launch_bounds__(256) void k_update(size_t n, const double *__restri
     const T alpha = (T)S[S_ALPHA];
     double v[2] = {0, 0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        x[i] += alpha * p[i];
-        const T ri = r[i] + (-alpha) * ap[i];
+        x[i] = fmad<T>(alpha, p[i], x[i]);
+        const T ri = fmad<T>(-alpha, ap[i], r[i]);
         r[i] = ri;
         v[0] += (double)ri * (double)ri;
         if (JAC) {
@@ -109,7 +115,7 @@ __global__ void k_pupdate(size_t n, const double *__restrict__ S, const T *__res
     const T beta = (T)S[S_BETA];
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        p[i] = z[i] + beta * p[i];
+        p[i] = fmad<T>(beta, p[i], z[i]);
 }
 
 __device__ __forceinline__ void fin_sum(const double *part, int nb, double *out, int nv) {
@@ -311,6 +317,261 @@ __global__ void k_set_cont(double *S, int *flags, double bnorm) {
     flags[0] = 0;
 }
 
+// ------------------------------------------- fused PCG-Jacobi on CSR --
+// Two kernels per iteration instead of six (SpMV, p.Ap, alpha, update,
+// convergence/beta, p update).  Each 256-thread block finalises the scalars
+// it needs itself from the previous kernel's block partials (rsum256), in
+// exactly the order of the one-block finalisers, so every block sees the same
+// alpha / beta / stopping decision and the iterates are bitwise those of the
+// separate-kernel loop.  The new search direction p_k = z + beta p_{k-1} is
+// formed inside the SpMV (row i reads z_j and p_{k-1,j} of its columns and
+// writes p_k,i into the other buffer of a pair), which removes the p update
+// pass and its dependency.
+//
+// Partial slots: kFzRz[0] / kFzRz[1] hold r.z of even / odd iterations (slot
+// 0 is where k_init leaves r0.z0), kFzRr holds r.r, kFzPap holds p.Ap.
+constexpr int kFzRz[2] = {0, 3};
+constexpr int kFzRr = 4, kFzPap = 5;
+
+// Sums of the nb partials of NS slots, each in fin_sum's order (16
+// wavefront butterflies over 64-entry chunks, then the 16 chunk sums in
+// sequence), computed by a 256-thread block and returned in every thread.
+// All loads are issued before the first reduction.
+//
+// It also returns flags[0] as read by thread 0, the same in every thread of
+// the block: the flag can be set by block 0 of the kernel that is running, so
+// reading it per wavefront could split a block at a barrier.
+template <int NS>
+__device__ __forceinline__ int rsum256(const double *part, int nb, const int (&slot)[NS],
+                                       double (&out)[NS], double (*lds)[16], const int *flags,
+                                       int *stop) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int f = threadIdx.x == 0 ? flags[0] : 0;
+    double v[NS][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = (w + 4 * q) * 64 + lane;
+#pragma unroll
+        for (int t = 0; t < NS; ++t) v[t][q] = i < nb ? part[slot[t] * kRedBlocks + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            const double sum = wave_sum(v[t][q]);
+            if (lane == 0) lds[t][w + 4 * q] = sum;
+        }
+    if (threadIdx.x == 0) *stop = f;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        double acc = 0.0;
+        for (int c = 0; c < 16; ++c) acc += lds[t][c];
+        out[t] = acc;
+    }
+    return *stop;
+}
+
+// flags[0] as read by thread 0, in every thread of the block.
+__device__ __forceinline__ int block_flag(const int *flags, int *stop) {
+    if (threadIdx.x == 0) *stop = flags[0];
+    __syncthreads();
+    return *stop;
+}
+
+// Iteration k, first half.  For k > 1: records iteration k-1 (r.r -> relative
+// residual, the stopping test of par_solver.rs:318-324) and stops there if it
+// has converged; beta = r_{k-1}.z_{k-1} / r_{k-2}.z_{k-2}.  Then
+// p_k = z (k = 1) or z + beta p_{k-1}, Ap_k = A p_k, partial p_k.Ap_k.
+// The loads of a thread's first row (up to kPre entries), which do not depend
+// on beta, are issued before the partial sums are reduced, so the two
+// latency chains overlap.
+constexpr int kPre = 8;
+template <class T>
+__global__ __launch_bounds__(256) void k_pcgj_spmv(size_t n, const uint64_t *__restrict__ rp,
+                                                   const uint32_t *__restrict__ ci,
+                                                   const T *__restrict__ v, const T *__restrict__ z,
+                                                   const T *__restrict__ pold, T *__restrict__ pnew,
+                                                   T *__restrict__ ap, double *__restrict__ part,
+                                                   int nb, double *S, int *flags, int first,
+                                                   int rz_new, int rz_old, double rtol, double atol) {
+    __shared__ double lds[3][16];
+    __shared__ int stop;
+    const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    T beta = 0;
+    // preloaded first row: cnt = its length, or -1 when not preloaded
+    int cnt = -1;
+    T vv[kPre], zz[kPre], pp[kPre], z0 = 0, p0 = 0;
+    if (!first && i0 < n) {
+        const uint64_t s0 = rp[i0], len = rp[i0 + 1] - s0;
+        z0 = z[i0];
+        p0 = pold[i0];
+        if (len <= (uint64_t)kPre) {
+            cnt = (int)len;
+#pragma unroll
+            for (int k = 0; k < kPre; ++k) {
+                if (k < cnt) {
+                    const uint32_t c = ci[s0 + k];
+                    vv[k] = v[s0 + k];
+                    zz[k] = z[c];
+                    pp[k] = pold[c];
+                } else {
+                    vv[k] = zz[k] = pp[k] = 0;
+                }
+            }
+        }
+    }
+    if (first) {
+        if (block_flag(flags, &stop)) return;
+    } else {  // first is uniform: every thread of the block reaches the barriers
+        double o[3];
+        if (rsum256<3>(part, nb, {kFzRr, rz_new, rz_old}, o, lds, flags, &stop)) return;
+        const double rr = o[0], rzn = o[1], rzo = o[2];
+        const double res = sqrt(rr) / S[S_BNORM];
+        const bool conv = res < rtol || sqrt(rr) < atol;
+        const double b = rzn / rzo;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            S[S_RR] = rr;
+            S[S_RES] = res;
+            S[S_ITERS] = S[S_ITERS] + 1.0;
+            if (conv) {
+                S[S_STATUS] = ST_CONVERGED;
+                flags[0] = 1;
+            } else {
+                S[S_BETA] = b;
+                S[S_RZ] = rzn;
+            }
+        }
+        if (conv) return;
+        beta = (T)b;
+    }
+    double d[1] = {0.0};
+    for (size_t i = i0; i < n; i += stride) {
+        T pi, s;
+        if (first) {
+            pi = z[i];
+            s = row_dot_ref<T>(rp, ci, v, [z](uint32_t j) { return z[j]; }, i);
+        } else if (i == i0 && cnt >= 0) {
+            pi = fmad<T>(beta, p0, z0);
+            T xk[kPre];
+#pragma unroll
+            for (int k = 0; k < kPre; ++k) xk[k] = fmad<T>(beta, pp[k], zz[k]);
+            {  // row_dot_ref's order on the preloaded entries, without contraction
+#pragma clang fp contract(off)
+                s = 0;
+                int done = 0;
+                if constexpr (sizeof(T) == 8) {
+                    done = cnt / 4 * 4;
+#pragma unroll
+                    for (int k = 0; k + 4 <= kPre; k += 4)
+                        if (k + 4 <= done)
+                            s += vv[k] * xk[k] + vv[k + 1] * xk[k + 1] + vv[k + 2] * xk[k + 2] +
+                                 vv[k + 3] * xk[k + 3];
+                }
+#pragma unroll
+                for (int k = 0; k < kPre; ++k)
+                    if (k >= done && k < cnt) s += vv[k] * xk[k];
+            }
+        } else {
+            pi = fmad<T>(beta, pold[i], z[i]);
+            s = row_dot_ref<T>(rp, ci, v, [z, pold, beta](uint32_t j) { return fmad<T>(beta, pold[j], z[j]); }, i);
+        }
+        pnew[i] = pi;
+        ap[i] = s;
+        d[0] += (double)pi * (double)s;
+    }
+    block_sum<1, 256>(d);
+    if (threadIdx.x == 0) part[kFzPap * kRedBlocks + blockIdx.x] = d[0];
+}
+
+// Iteration k, second half: alpha = r_{k-1}.z_{k-1} / p.Ap (breakdown below
+// brk), x += alpha p, r -= alpha Ap, z = inv r, partials r.r and r.z.  The
+// first entry's operands are loaded before the partial sums are reduced.
+template <class T>
+__global__ __launch_bounds__(256) void k_pcgj_update(size_t n, T *__restrict__ x,
+                                                     const T *__restrict__ p, T *__restrict__ r,
+                                                     const T *__restrict__ ap,
+                                                     const T *__restrict__ inv, T *__restrict__ z,
+                                                     double *__restrict__ part, int nb, double *S,
+                                                     int *flags, int rz_in, int rz_out, double brk) {
+    __shared__ double lds[2][16];
+    __shared__ int stop;
+    const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    T x0 = 0, p0 = 0, r0 = 0, a0 = 0, q0 = 0;
+    if (i0 < n) {
+        x0 = x[i0];
+        p0 = p[i0];
+        r0 = r[i0];
+        a0 = ap[i0];
+        q0 = inv[i0];
+    }
+    double o[2];
+    if (rsum256<2>(part, nb, {kFzPap, rz_in}, o, lds, flags, &stop)) return;
+    const double pap = o[0], rz = o[1];
+    const bool brkdn = fabs(pap) < brk;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_PAP] = pap;
+        if (brkdn) {
+            S[S_STATUS] = ST_BREAKDOWN;
+            flags[0] = 1;
+        } else {
+            S[S_ALPHA] = rz / pap;
+        }
+    }
+    if (brkdn) return;
+    const T alpha = (T)(rz / pap);
+    double v[2] = {0, 0};
+    for (size_t i = i0; i < n; i += stride) {
+        T xi, pi, ri, ai, qi;
+        if (i == i0) { xi = x0; pi = p0; ri = r0; ai = a0; qi = q0; }
+        else { xi = x[i]; pi = p[i]; ri = r[i]; ai = ap[i]; qi = inv[i]; }
+        x[i] = fmad<T>(alpha, pi, xi);
+        ri = fmad<T>(-alpha, ai, ri);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+        const T zi = qi * ri;
+        z[i] = zi;
+        v[1] += (double)ri * (double)zi;
+    }
+    block_sum<2, 256>(v);
+    if (threadIdx.x == 0) {
+        part[kFzRr * kRedBlocks + blockIdx.x] = v[0];
+        part[rz_out * kRedBlocks + blockIdx.x] = v[1];
+    }
+}
+
+// After the last enqueued iteration: records it and applies the stopping test.
+__global__ __launch_bounds__(256) void k_pcgj_end(const double *part, int nb, double *S, int *flags,
+                                                  double rtol, double atol) {
+    __shared__ double lds[1][16];
+    __shared__ int stop;
+    double o[1];
+    if (rsum256<1>(part, nb, {kFzRr}, o, lds, flags, &stop)) return;
+    const double rr = o[0];
+    if (threadIdx.x) return;
+    const double res = sqrt(rr) / S[S_BNORM];
+    S[S_RR] = rr;
+    S[S_RES] = res;
+    S[S_ITERS] = S[S_ITERS] + 1.0;
+    if (res < rtol || sqrt(rr) < atol) {
+        S[S_STATUS] = ST_CONVERGED;
+        flags[0] = 1;
+    }
+}
+
+// FEMGPU_PCG_FUSED=0 selects the separate-kernel PCG-Jacobi loop (for
+// comparisons); the fused loop is the default.
+static bool fused_pcg_enabled() {
+    const char *e = std::getenv("FEMGPU_PCG_FUSED");
+    return !(e && e[0] == '0');
+}
+// FEMGPU_GRAPHS=0 launches every iteration's kernels directly instead of
+// replaying captured chunks.
+static bool graphs_enabled() {
+    const char *e = std::getenv("FEMGPU_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
 static unsigned vgrid(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -413,6 +674,9 @@ template <class T> struct Krylov {
                                r, z, p, part, prec == PREC_NONE);
         hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                            flags, mode, rtol, atol);
+        CsrView cv;
+        if (mode == MODE_PCG && prec == PREC_JACOBI && op->csr_view(&cv) && fused_pcg_enabled())
+            return run_pcgj_fused(cv, x, inv, rtol, atol, max_iter, res);
         if (prec == PREC_CHEB) {
             // z = M^-1 r0; p = z; rz = r.z
             cheb_apply(cp, r, inv, z);
@@ -420,6 +684,83 @@ template <class T> struct Krylov {
             copy<T>(s, p, z, n);
         }
         return run(mode, prec, cp, x, inv, rtol, atol, max_iter, res, false);
+    }
+
+    // PCG-Jacobi on an operator that exposes its CSR arrays: the two-kernel
+    // iteration (k_pcgj_spmv, k_pcgj_update), same iterates as run().  After
+    // the first iteration, whole chunks of kPcgChunk iterations are replayed
+    // from a HIP graph cached on the operator (launch-bound small systems);
+    // the host reads the stop flag after every chunk.
+    static constexpr uint64_t kPcgChunk = 32;
+    int run_pcgj_fused(const CsrView &cv, T *x, const T *inv, double rtol, double atol,
+                       uint64_t max_iter, femgpu_solve_result *res) {
+        const unsigned nb = nblk(n);
+        T *r = vec(W_R), *z = vec(W_Z), *ap = vec(W_AP);
+        T *pb[2] = {vec(W_P), vec(W_D)};
+        const T *vals = (const T *)cv.vals;
+        const double brk = 1e-30;
+        auto iter = [&](uint64_t k) {
+            T *pn = pb[k & 1];
+            const T *po = pb[(k + 1) & 1];
+            hipLaunchKernelGGL(k_pcgj_spmv<T>, dim3(nb), dim3(256), 0, s, n, cv.rp, cv.ci, vals,
+                               (const T *)z, po, pn, ap, part, (int)nb, S, flags, k == 1 ? 1 : 0,
+                               kFzRz[(k - 1) & 1], kFzRz[k & 1], rtol, atol);
+            hipLaunchKernelGGL(k_pcgj_update<T>, dim3(nb), dim3(256), 0, s, n, x, (const T *)pn, r,
+                               (const T *)ap, inv, z, part, (int)nb, S, flags, kFzRz[(k - 1) & 1],
+                               kFzRz[k & 1], brk);
+        };
+        const bool graphs = graphs_enabled();
+        uint64_t it = 0, since = 0;
+        const uint64_t check = n < (1u << 20) ? kPcgChunk : 8;
+        bool stopped = read_flag() != 0;
+        while (!stopped && it < max_iter) {
+            // a chunk starts at an even k, so its slot parities are fixed
+            if (graphs && (it & 1) && max_iter - it >= kPcgChunk) {
+                FG_HIP(hipGraphLaunch(chunk_graph(cv, x, inv, rtol, atol, iter), s));
+                it += kPcgChunk;
+                since = check;
+            } else {
+                iter(it + 1);
+                FG_HIP(hipGetLastError());
+                ++it;
+                ++since;
+            }
+            if (since >= check || it == max_iter) {
+                stopped = read_flag() != 0;
+                since = 0;
+            }
+        }
+        if (it > 0)
+            hipLaunchKernelGGL(k_pcgj_end, dim3(1), dim3(256), 0, s, (const double *)part, (int)nb, S,
+                               flags, rtol, atol);
+        return finish(MODE_PCG, max_iter, res);
+    }
+
+    // The graph of iterations k = 2 .. kPcgChunk + 1 (any chunk starting at an
+    // even k launches the same kernels), captured once per set of buffers.
+    template <class Iter>
+    hipGraphExec_t chunk_graph(const CsrView &cv, T *x, const T *inv, double rtol, double atol,
+                               Iter &iter) {
+        IterGraph &g = op->pcg_graph;
+        std::vector<const void *> key = {x, inv, vec(W_R), vec(W_Z), vec(W_AP), vec(W_P), vec(W_D),
+                                         part, S, flags, cv.vals, cv.rp, cv.ci};
+        std::vector<double> params = {rtol, atol, (double)n};
+        if (g.exec && g.key == key && g.params == params) return g.exec;
+        g.reset();
+        hipGraph_t graph = nullptr;
+        FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (uint64_t j = 0; j < kPcgChunk; ++j) iter(2 + j);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        FG_HIP(le);
+        FG_HIP(ce);
+        const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) g.exec = nullptr;
+        FG_HIP(ie);
+        g.key = key;
+        g.params = params;
+        return g.exec;
     }
 
     // Continues PCG + Chebyshev-Jacobi on the residual the caller put in
@@ -491,6 +832,11 @@ template <class T> struct Krylov {
             ++it;
             if (it % check == 0 || it == max_iter) stopped = read_flag() != 0;
         }
+        return finish(mode, max_iter, res);
+    }
+
+    // Reads the scalars after a solve and fills the result.
+    int finish(int mode, uint64_t max_iter, femgpu_solve_result *res) {
         read_scalars();
         const double *h = c->pinned;
         const int status = (int)h[S_STATUS];

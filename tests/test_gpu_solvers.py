@@ -232,3 +232,55 @@ def test_csr_apply_dirichlet(fg, fr, ctx):
         fg.apply_dirichlet(A, None, [N], [0.0])
     with pytest.raises(ValueError):
         fg.apply_dirichlet(A, None, [0, 1], [0.0])
+
+
+def _pcg_jacobi_both(fg, ctx, rp, ci, v, b, cfg, monkeypatch):
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEMGPU_PCG_FUSED", fused)
+        A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+        x = np.zeros(len(b))
+        res = fg.solve_pcg_jacobi(A, b, x, cfg)
+        out[fused] = (x, res)
+    return out["1"], out["0"]
+
+
+@pytest.mark.parametrize("max_iter", [5000, 37])
+def test_pcg_jacobi_fused_matches_separate_kernels(fg, fr, ctx, monkeypatch, max_iter):
+    """The two-kernel CSR PCG-Jacobi iteration (search direction formed inside
+    the SpMV, scalars finalised redundantly per workgroup) gives the iterates
+    of the six-kernel loop bit for bit: same x, iteration count and residual,
+    converged (config-1 system at 64^2) and stopped at max_iter."""
+    rp, ci, v, rhs, coords, conn = fr.poisson_p1(64)
+    cfg = fg.SolverConfig(rtol=1e-10, max_iter=max_iter)
+    (xf, rf), (xs, rs) = _pcg_jacobi_both(fg, ctx, rp, ci, v, rhs, cfg, monkeypatch)
+    print(f"max_iter {max_iter}: fused {rf}, separate {rs}")
+    assert rf == rs
+    assert np.array_equal(xf, xs)
+    assert rf.converged == (max_iter == 5000)
+    if max_iter == 37:
+        assert rf.iterations == 37
+
+
+def test_pcg_jacobi_fused_grid_stride(fg, fr, ctx, monkeypatch):
+    """Above 262,144 rows each thread of the fixed 1024-workgroup grid owns
+    several rows; the fused loop still matches the separate kernels bitwise,
+    and the oracle within rounding (1-D Laplacian, 300,001 rows, 60 iterations)."""
+    n = 300001
+    rp = np.zeros(n + 1, np.uint64)
+    cnt = np.full(n, 3); cnt[0] = cnt[-1] = 2
+    rp[1:] = np.cumsum(cnt)
+    ci = np.empty(int(rp[-1]), np.uint32); v = np.empty(int(rp[-1]))
+    rows = np.repeat(np.arange(n), cnt)
+    off = np.arange(int(rp[-1])) - rp[rows].astype(np.int64)
+    first = rows == 0
+    ci[:] = np.where(first, rows + off, rows - 1 + off).astype(np.uint32)
+    v[:] = np.where(ci == rows, 2.0 + 0.01 * (rows % 7), -1.0)
+    b = np.sin(np.arange(n) * 0.001) + 0.5
+    cfg = fg.SolverConfig(rtol=1e-12, max_iter=60)
+    (xf, rf), (xs, rs) = _pcg_jacobi_both(fg, ctx, rp, ci, v, b, cfg, monkeypatch)
+    assert rf == rs and rf.iterations == 60 and not rf.converged
+    assert np.array_equal(xf, xs)
+    ref = fr.CsrRef(rp, ci, v)
+    xo, ro = fr.pcg_jacobi(fr._Op(n, csr=ref), ref.diagonal(), b, np.zeros(n), rtol=1e-12, max_iter=60)
+    assert np.abs(xf - xo).max() < 1e-10 * np.abs(xo).max()
