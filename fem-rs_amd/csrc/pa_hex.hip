@@ -85,6 +85,7 @@ struct HexGrid {
     int nx, ny, nz;
     long long E;
     long long Nx, Ny, Nz;
+    int ne;  // elements per element-kernel workgroup, ne_of(Q)
 };
 
 
@@ -92,11 +93,27 @@ __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long
     return gx == 0 || gx == g.Nx - 1 || gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1;
 }
 
-// Element-vector layout: for each (ez, ey, k, j) an x-row holding every
-// element's N1 local values: [ez][ey][k][j][ex][i].
-__host__ __device__ __forceinline__ size_t evec_index(const HexGrid &g, int N1, int ex, int ey,
-                                                      int ez, int j, int k) {
-    return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.nx * N1 + (size_t)ex * N1;
+// Element-vector layout ("x-merged"): for each element row (ey, ez) and local
+// (k, j) a row of Nx values, one per DOF position gx along x.  Where two
+// elements of one workgroup share an x-face DOF, the row holds their sum
+// (lower element first); everywhere else it holds the single contribution.
+// The first element of a workgroup (e % ne == 0) with ex > 0 stores its
+// i = 0 value in the seam area instead, at seam_index(e / ne, j, k), and the
+// gather adds it.  This keeps Nx instead of nx (P + 1) values per row.
+__host__ __device__ __forceinline__ size_t mrow_index(const HexGrid &g, int N1, int ey, int ez, int j,
+                                                      int k) {
+    return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.Nx;
+}
+__host__ __device__ __forceinline__ size_t seam_base(const HexGrid &g, int N1) {
+    return (size_t)g.nz * g.ny * N1 * N1 * (size_t)g.Nx;
+}
+__host__ __device__ __forceinline__ size_t seam_index(const HexGrid &g, int N1, long long blk, int j,
+                                                      int k) {
+    return seam_base(g, N1) + ((size_t)blk * N1 + k) * N1 + j;
+}
+// entries of the element vector
+static inline size_t evec_count(const HexGrid &g, int N1) {
+    return seam_base(g, N1) + (size_t)((g.E + g.ne - 1) / g.ne) * N1 * N1;
 }
 
 // Quadrature-data layout.  Elements are grouped in blocks of NE = ne_of(Q)
@@ -292,18 +309,19 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
     __syncthreads();
 
-    // ---- X^T + scatter: lane (j,k)
-    if (valid && t < N1 * N1) {
-        const int j = t % N1, k = t / N1;
+    // ---- X^T: lane (j,k) forms its N1 values of the row; the i = 0 value goes
+    // through LDS (sBB is free now) to the lower neighbour in the workgroup,
+    // which writes the sum of the shared x-face DOF.
+    T sv[N1];
+    const bool xt = valid && t < N1 * N1;
+    const int jx = t % N1, kx = t / N1;
+    if (xt) {
         T FG[Q], FB[Q];
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-            FB[a] = sA[le][k][j][a];
-            if constexpr (DIFF) FG[a] = sAg[le][k][j][a];
+            FB[a] = sA[le][kx][jx][a];
+            if constexpr (DIFF) FG[a] = sAg[le][kx][jx][a];
         }
-        // element-vector row (j,k), stored x-row-major: consecutive elements of
-        // an x-row write one contiguous run
-        T *dst = y + evec_index(g, N1, ex, ey, ez, j, k);
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             T s = 0;
@@ -312,18 +330,33 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 s += Bt[a * N1 + i] * FB[a];
                 if constexpr (DIFF) s += Gt[a * N1 + i] * FG[a];
             }
-            dst[i] = s;
+            sv[i] = s;
         }
+        sBB[le][kx][jx][0] = sv[0];
+    }
+    __syncthreads();
+    if (xt) {
+        T *dst = y + mrow_index(g, N1, ey, ez, jx, kx) + (size_t)ex * P;
+        if (ex == 0) dst[0] = sv[0];
+        else if (le == 0) y[seam_index(g, N1, lb, jx, kx)] = sv[0];
+#pragma unroll
+        for (int i = 1; i < P; ++i) dst[i] = sv[i];
+        // the next element is in this workgroup (and row) unless this one ends
+        // the row or the workgroup
+        if (ex < g.nx - 1 && le + 1 < NE) dst[P] = sv[P] + sBB[le + 1][kx][jx][0];
+        else dst[P] = sv[P];
     }
 }
 
 // Element-vector -> DOF values, one x-segment per thread.  Segment (ex, gy,
 // gz) holds the DOFs gx = ex P + i, i = 0..P-1, of DOF row (gy, gz), and the
-// row's last DOF gx = Nx - 1 too when ex = nx - 1.  Each DOF sums its (1, 2,
-// 4 or 8) element contributions in a fixed order -- z candidate, then y, then
-// x, lower element first -- so applies are bitwise reproducible without
-// atomics; the row candidates, divisions and essential-DOF tests are shared
-// by the segment's DOFs.  Essential DOFs take x.  Returns the DOF count.
+// row's last DOF gx = Nx - 1 too when ex = nx - 1.  Each DOF sums its y/z
+// candidate rows (1, 2 or 4) in a fixed order -- z candidate, then y, lower
+// element first -- each row already holding the x-neighbours' sum (lower
+// first; at a workgroup seam the gather forms that sum from the seam value),
+// so applies are bitwise reproducible without atomics.  The row candidates,
+// divisions and essential-DOF tests are shared by the segment's DOFs.
+// Essential DOFs take x.  Returns the DOF count.
 // I: index type of the element vector (uint32_t when it has < 2^31 entries).
 template <class T, int P, class I>
 __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restrict__ ye,
@@ -341,7 +374,6 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
         ce[d][0] = e0 - 1; ci[d][0] = P; cv[d][0] = i0 == 0 && e0 > 0;
         ce[d][1] = e0;     ci[d][1] = i0; cv[d][1] = e0 < ne[d];
     }
-    const I rowlen = (I)g.nx * N1;
     const bool rowbdr = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
 #pragma unroll
     for (int i = 0; i <= P; ++i) out[i] = T(0);
@@ -350,13 +382,16 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             if (!(cv[1][c] && cv[0][b])) continue;
-            const I row = ((((I)ce[1][c] * (I)g.ny + (I)ce[0][b]) * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * rowlen;
-            const T *rp = ye + row + (I)ex * N1;  // this element's N1 values on the row
-            // i = 0: lower neighbour's last value, then this element's first
-            T s0 = out[0];
-            if (ex > 0) s0 += rp[-1];
-            s0 += rp[0];
-            out[0] = s0;
+            const I erow = (I)ce[1][c] * (I)g.ny + (I)ce[0][b];
+            const I row = ((erow * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * (I)g.Nx;
+            const T *rp = ye + row + (I)ex * P;  // this segment's values on the row
+            T m0 = rp[0];
+            if (ex > 0) {
+                const I e = erow * (I)g.nx + (I)ex;
+                if (e % (I)g.ne == 0)
+                    m0 += ye[(I)seam_base(g, N1) + ((e / (I)g.ne) * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]];
+            }
+            out[0] += m0;
 #pragma unroll
             for (int i = 1; i < P; ++i) out[i] += rp[i];
             if (ex == g.nx - 1) out[P] += rp[P];
@@ -623,7 +658,38 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
                         s += (double)qm[qd_index(Q, 1, e, 0, c, t2)] * ph * ph;
                     }
                 }
-        ye[evec_index(g, N1, ex, ey, ez, j, k) + i] = (T)s;
+        ye[(size_t)e * L + l] = (T)s;
+    }
+}
+
+// Per-element values old[e][k][j][i] -> the x-merged element vector, with the
+// element kernel's sums and seams.  One thread per (row, gx).
+template <class T>
+__global__ void k_evec_merge(const HexGrid g, int N1, const T *__restrict__ old, T *__restrict__ ye) {
+    const int P = N1 - 1, L = N1 * N1 * N1;
+    const size_t total = seam_base(g, N1);
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
+         id += (size_t)gridDim.x * blockDim.x) {
+        const long long gx = (long long)(id % (size_t)g.Nx);
+        const size_t row = id / (size_t)g.Nx;
+        const int j = (int)(row % N1), k = (int)((row / N1) % N1);
+        const long long erow = (long long)(row / ((size_t)N1 * N1));
+        auto val = [&](long long ex, int i) { return old[(size_t)(erow * g.nx + ex) * L + (k * N1 + j) * N1 + i]; };
+        long long ex = gx / P;
+        const int i = (int)(gx - ex * P);
+        T v;
+        if (ex == g.nx) v = val(g.nx - 1, P);
+        else if (i != 0 || ex == 0) v = val(ex, i);
+        else {
+            const long long e = erow * g.nx + ex;
+            if (e % g.ne == 0) {
+                v = val(ex - 1, P);
+                ye[seam_index(g, N1, e / g.ne, j, k)] = val(ex, 0);
+            } else {
+                v = val(ex - 1, P) + val(ex, 0);
+            }
+        }
+        ye[id] = v;
     }
 }
 
@@ -662,7 +728,7 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
 }
 
 // element vectors below 2^31 entries use 32-bit index arithmetic in the gathers
-static inline bool evec_small(const HexGrid &g, int n1) { return (double)g.E * n1 * n1 * n1 < 2147483648.0; }
+static inline bool evec_small(const HexGrid &g, int n1) { return (double)evec_count(g, n1) < 2147483648.0; }
 
 template <class T, class I>
 static void launch_gather_i(int p, const HexGrid &g, const T *ye, const T *x, T *y, int mode, int dir,
@@ -693,7 +759,7 @@ struct HexOp final : femgpu_op {
     femgpu_hex_desc desc{};
     HexGrid g{};
     DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
-    DevBuf evec;  // element vector [E][(p+1)^3] between the two apply kernels
+    DevBuf evec;  // x-merged element vector between the two apply kernels
 
     const char *name() const override { return "pa_hex"; }
 
@@ -764,12 +830,17 @@ struct HexOp final : femgpu_op {
 
     template <class T> void diag_t(T *d) {
         hipStream_t s = ctx->stream;
-        hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1((size_t)g.E * (desc.p + 1) * (desc.p + 1) * (desc.p + 1))),
-                           dim3(256), 0, s, g, desc.p + 1, desc.q, Bd.as<double>(), Gd.as<double>(),
-                           qd.as<T>(), qm.as<T>(), desc.form, evec.as<T>());
+        const int n1 = desc.p + 1;
+        const size_t L = (size_t)g.E * n1 * n1 * n1;
+        DevBuf old(L * sizeof(T));
+        hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
+                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), desc.form, old.as<T>());
+        hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(g, n1))), dim3(256), 0, s, g, n1,
+                           (const T *)old.as<T>(), evec.as<T>());
         launch_gather<T>(desc.p, g, evec.as<T>(), d, d, 0, 0, nullptr, s);
         hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, (const T *)d, d, 2,
                            desc.dirichlet, nullptr);
+        FG_HIP(hipStreamSynchronize(s));  // before the temporary is freed
     }
 
     void diagonal_dev(void *d) override {
@@ -806,7 +877,7 @@ struct HexOp final : femgpu_op {
         const size_t es = esize();
         if (desc.form & 2) qd.alloc(6 * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
-        evec.alloc((size_t)g.E * n1 * n1 * n1 * es);
+        evec.alloc(evec_count(g, n1) * es);
         if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
         else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
         FG_HIP(hipStreamSynchronize(s));
@@ -857,6 +928,7 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
         g.E = (long long)d.nx * d.ny * d.nz;
         g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
+        g.ne = ne_of(d.q);
         op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
         FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536 && g.E < (1ll << 31),
                    FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
