@@ -54,7 +54,7 @@ struct Vol {
 
 static inline Vol half(Vol d) { return Vol{(d.nx + 1) / 2, (d.ny + 1) / 2, (d.nz + 1) / 2}; }
 
-static inline int kpad_of(int cin) { return (27 * cin + 15) / 16 * 16; }
+static constexpr int kpad_of(int cin) { return (27 * cin + 15) / 16 * 16; }
 
 static unsigned g1(long long n, int b = 256) {
     long long g = (n + b - 1) / b;
@@ -62,17 +62,37 @@ static unsigned g1(long long n, int b = 256) {
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// Output layer weights w[432] (f32) -> B matrix [16][432] (bf16): row 0 the
+// high parts bf16(w), row 1 the low parts bf16(w - high), rows 2..15 zero.
+__global__ void k_split_out_weights(const float *__restrict__ w, uint16_t *__restrict__ B) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= 16 * 432) return;
+    const int n = id / 432, k = id - n * 432;
+    uint16_t v = 0;
+    if (n < 2) {
+        const uint16_t hi = f2bf(w[k]);
+        v = n == 0 ? hi : f2bf(w[k] - bf2f(hi));
+    }
+    B[id] = v;
+}
+
 // ------------------------------------------------------------ conv fwd --
 // out[v][n] = act(sum_k A[v][k] W[n][k] + bias[n] (+ skip[v][n]))
-// OUTF: 0 -> bf16 output with bias/skip/relu; 1 -> f32 output, plain sum.
+// OUTF: 0 -> bf16 output with bias/skip/relu; 1 -> f32 output, plain sum;
+//       2 -> the 1-channel output layer: W holds the f32 weights split into
+//            bf16 high and low parts (B columns 0 and 1, the rest zero;
+//            k_split_out_weights), so the product keeps the f32 weights to
+//            about 2^-16; out[v] = bias[0] + (column 0 + column 1) (f32 [V]).
 // Implicit GEMM on v_mfma_f32_16x16x16bf16_1k, K = 27 CIN in steps of 16.
 // A workgroup owns an output tile of 16 x 4 x 4 voxels (x, y, z) and NTB x 16
-// output channels; it stages the tile's input halo brick (18 x 6 x 6 voxels x
-// CIN channels, zero outside the grid) in LDS once, so the 27 taps read LDS
-// instead of re-reading HBM / L2.  Wave w computes z = z0 + w, its four
-// 16-voxel M tiles are the rows y = y0..y0+3.  For CIN >= 16 a k-step is one
-// tap and 16 channels (tap, offset and channel base are wave-uniform); CIN = 4
-// packs four taps per k-step (lane group g = lane / 16 takes tap 4 s + g).
+// output channels; it stages the tile's input halo brick (18 x 6 x 6 voxels,
+// zero outside the grid) in LDS 16 channels at a time (20.7 KB whatever CIN
+// is, so the coarse levels keep several workgroups per CU), and the 27 taps
+// read LDS instead of re-reading HBM / L2.  Wave w computes z = z0 + w, its
+// four 16-voxel M tiles are the rows y = y0..y0+3.  For CIN >= 16 a k-step is
+// one tap and the 16 staged channels (tap, offset and channel base are
+// wave-uniform); CIN = 4 packs four taps per k-step (lane group g = lane / 16
+// takes tap 4 s + g).
 constexpr int kBX = 16, kBY = 4, kBZ = 4;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
 
@@ -83,28 +103,21 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_
                                                   const uint16_t *__restrict__ skip,
                                                   void *__restrict__ outp, int relu) {
     static_assert(CIN == 4 || CIN % 16 == 0, "CIN must be 4 or a multiple of 16");
+    static_assert(OUTF != 2 || (CIN == 16 && COUT == 16 && NTB == 1), "output layer is 16 -> 1");
     static_assert(COUT % (16 * NTB) == 0, "NTB must divide the output tiles");
     constexpr int NBRICK = kHX * kHY * kHZ;
-    __shared__ __align__(16) uint16_t sIn[NBRICK * CIN];
+    constexpr int CG = CIN >= 16 ? 16 : CIN;  // channels staged per pass
+    constexpr int NPASS = CIN / CG;
+    constexpr int NCH = NTB * 16;           // output channels of this workgroup
+    constexpr int ESZ = OUTF == 0 ? 2 : 4;  // output element bytes
+    constexpr int SIN_B = NBRICK * CG * 2, SOUT_B = 256 * NCH * ESZ;
+    __shared__ __align__(16) char smem[SIN_B > SOUT_B ? SIN_B : SOUT_B];
+    uint16_t *sIn = reinterpret_cast<uint16_t *>(smem);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ntx = (d.nx + kBX - 1) / kBX;
     const int csplit = COUT / (16 * NTB);  // output-channel groups per tile
     const int bx = blockIdx.x % ntx, cg = (blockIdx.x / ntx) % csplit;
     const int x0 = bx * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
-    // ---- stage the halo brick: 8-byte chunks of 4 channels
-    constexpr int C4 = CIN / 4;
-    for (int idx = threadIdx.x; idx < NBRICK * C4; idx += 256) {
-        const int vi = idx / C4, c4 = idx - vi * C4;
-        const int xx = vi % kHX, yy = (vi / kHX) % kHY, zz = vi / (kHX * kHY);
-        const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
-        uint2 v = make_uint2(0u, 0u);
-        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz)
-            v = *reinterpret_cast<const uint2 *>(
-                in + ((size_t)gx + (size_t)d.nx * ((size_t)gy + (size_t)d.ny * gz)) * CIN + c4 * 4);
-        *reinterpret_cast<uint2 *>(sIn + (size_t)vi * CIN + c4 * 4) = v;
-    }
-    __syncthreads();
-    // ---- MFMA loop
     f32x4 acc[kBY][NTB];
 #pragma unroll
     for (int mt = 0; mt < kBY; ++mt)
@@ -116,53 +129,134 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_
 #pragma unroll
     for (int nt = 0; nt < NTB; ++nt)
         wrow[nt] = W + (size_t)((cg * NTB + nt) * 16 + (lane & 15)) * kpad + ksub;
-    for (int k0 = 0; k0 < kpad; k0 += 16) {
-        bf16x4 b[NTB];
-#pragma unroll
-        for (int nt = 0; nt < NTB; ++nt) b[nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + k0);
-        int tap, c0;
-        if constexpr (CIN >= 16) {
-            tap = k0 / CIN;  // wave-uniform
-            c0 = k0 - tap * CIN + ksub;
-        } else {
-            tap = k0 / 4 + (lane >> 4);
-            c0 = 0;
+    // staging chunks: 16 bytes (8 channels) when 16 channels are staged, else 8
+    constexpr int CK = CG >= 8 ? 8 : 4, NCK = CG / CK;
+    for (int pass = 0; pass < NPASS; ++pass) {
+        // ---- stage the halo brick's channels [pass CG, pass CG + CG)
+        if (pass) __syncthreads();  // the previous pass's reads are done
+        for (int idx = threadIdx.x; idx < NBRICK * NCK; idx += 256) {
+            const int vi = idx / NCK, ck = idx - vi * NCK;
+            const int xx = vi % kHX, yy = (vi / kHX) % kHY, zz = vi / (kHX * kHY);
+            const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+            const bool inb = gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz;
+            const uint16_t *src = in + ((size_t)gx + (size_t)d.nx * ((size_t)gy + (size_t)d.ny * gz)) * CIN + pass * CG + ck * CK;
+            if constexpr (CK == 8) {
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (inb) v = *reinterpret_cast<const uint4 *>(src);
+                *reinterpret_cast<uint4 *>(sIn + (size_t)vi * CG + ck * CK) = v;
+            } else {
+                uint2 v = make_uint2(0u, 0u);
+                if (inb) v = *reinterpret_cast<const uint2 *>(src);
+                *reinterpret_cast<uint2 *>(sIn + (size_t)vi * CG + ck * CK) = v;
+            }
         }
-        const bool tv = tap < 27;
-        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-        const int base = ((wave + 1 + dz) * kHY + 1 + dy) * kHX + xl + 1 + dx;
+        __syncthreads();
+        // ---- MFMA loop over the staged channels, one dz plane (9 taps) at a
+        // time; the plane's weights are loaded together before its MFMAs, so
+        // their latency is paid 3 times per pass rather than 27
+        if constexpr (CIN >= 16) {
+            for (int pz = 0; pz < 3; ++pz) {
+                bf16x4 b[9][NTB];
 #pragma unroll
-        for (int mt = 0; mt < kBY; ++mt) {
-            bf16x4 a = bf16x4{0, 0, 0, 0};
-            if (tv) a = *reinterpret_cast<const bf16x4 *>(sIn + (size_t)(base + mt * kHX) * CIN + c0);
+                for (int t9 = 0; t9 < 9; ++t9)
 #pragma unroll
-            for (int nt = 0; nt < NTB; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
+                    for (int nt = 0; nt < NTB; ++nt)
+                        b[t9][nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + (pz * 9 + t9) * CIN + pass * 16);
+#pragma unroll
+                for (int t9 = 0; t9 < 9; ++t9) {
+                    const int dx = t9 % 3 - 1, dy = t9 / 3 - 1, dz = pz - 1;
+                    const int base = ((wave + 1 + dz) * kHY + 1 + dy) * kHX + xl + 1 + dx;
+#pragma unroll
+                    for (int mt = 0; mt < kBY; ++mt) {
+                        const bf16x4 a = *reinterpret_cast<const bf16x4 *>(sIn + (size_t)(base + mt * kHX) * CG + ksub);
+#pragma unroll
+                        for (int nt = 0; nt < NTB; ++nt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t9][nt], acc[mt][nt], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+            static_assert(CIN != 4 || kpad_of(4) == 112, "CIN = 4 takes 7 k-steps");
+            bf16x4 bw[7][NTB];
+#pragma unroll
+            for (int ks = 0; ks < 7; ++ks)
+#pragma unroll
+                for (int nt = 0; nt < NTB; ++nt) bw[ks][nt] = *reinterpret_cast<const bf16x4 *>(wrow[nt] + ks * 16);
+#pragma unroll
+            for (int ks = 0; ks < 7; ++ks) {
+                const int k0 = ks * 16;
+                bf16x4 b[NTB];
+#pragma unroll
+                for (int nt = 0; nt < NTB; ++nt) b[nt] = bw[ks][nt];
+                const int tap = k0 / 4 + (lane >> 4);
+                const bool tv = tap < 27;
+                const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+                const int base = ((wave + 1 + dz) * kHY + 1 + dy) * kHX + xl + 1 + dx;
+#pragma unroll
+                for (int mt = 0; mt < kBY; ++mt) {
+                    bf16x4 a = bf16x4{0, 0, 0, 0};
+                    if (tv) a = *reinterpret_cast<const bf16x4 *>(sIn + (size_t)(base + mt * kHX) * CG);
+#pragma unroll
+                    for (int nt = 0; nt < NTB; ++nt)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[mt][nt], 0, 0, 0);
+                }
+            }
         }
     }
     // ---- epilogue: D[m = voxel x (4 (lane/16) + i)][n = lane % 16]
-    const int z = z0 + wave;
+    if constexpr (OUTF == 2) {
+        const int z = z0 + wave;
+        const float b0 = bias[0];
 #pragma unroll
-    for (int mt = 0; mt < kBY; ++mt) {
-        const int y = y0 + mt;
-#pragma unroll
-        for (int nt = 0; nt < NTB; ++nt) {
-            const int n = (cg * NTB + nt) * 16 + (lane & 15);
+        for (int mt = 0; mt < kBY; ++mt) {
+            const int y = y0 + mt;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
+                const float lo = __shfl_down(acc[mt][0][i], 1, 64);  // column 1 of the same voxel
                 const int x = x0 + (lane >> 4) * 4 + i;
-                if (x >= d.nx || y >= d.ny || z >= d.nz) continue;
-                const size_t v = (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z);
-                float val = acc[mt][nt][i];
-                if (OUTF) {
-                    reinterpret_cast<float *>(outp)[v * COUT + n] = val;
-                } else {
-                    if (bias) val += bias[n];
-                    if (skip) val += bf2f(skip[v * COUT + n]);
-                    if (relu) val = fmaxf(val, 0.f);
-                    reinterpret_cast<uint16_t *>(outp)[v * COUT + n] = f2bf(val);
+                if ((lane & 15) || x >= d.nx || y >= d.ny || z >= d.nz) continue;
+                reinterpret_cast<float *>(outp)[(size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z)] =
+                    b0 + (acc[mt][0][i] + lo);
+            }
+        }
+        return;
+    } else {
+        // Through LDS, so that global stores are 16-byte chunks: the tile's
+        // voxels [z][y][x] each hold NCH contiguous output channels.
+        __syncthreads();  // every wave is done reading the brick
+#pragma unroll
+        for (int mt = 0; mt < kBY; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTB; ++nt) {
+                const int nl = nt * 16 + (lane & 15), n = cg * NCH + nl;
+                const float bn = (OUTF == 0 && bias) ? bias[n] : 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int vl = (wave * kBY + mt) * kBX + (lane >> 4) * 4 + i;
+                    float val = acc[mt][nt][i];
+                    if constexpr (OUTF == 1) {
+                        reinterpret_cast<float *>(smem)[vl * NCH + nl] = val;
+                    } else {
+                        val += bn;
+                        if (skip) {
+                            const int x = x0 + (lane >> 4) * 4 + i, y = y0 + mt, z = z0 + wave;
+                            if (x < d.nx && y < d.ny && z < d.nz)
+                                val += bf2f(skip[((size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z)) * COUT + n]);
+                        }
+                        if (relu) val = fmaxf(val, 0.f);
+                        reinterpret_cast<uint16_t *>(smem)[vl * NCH + nl] = f2bf(val);
+                    }
                 }
             }
+        __syncthreads();
+        constexpr int CPV = NCH * ESZ / 16;  // 16-byte chunks per voxel
+        for (int id = threadIdx.x; id < 256 * CPV; id += 256) {
+            const int vl = id / CPV, part = id - vl * CPV;
+            const int x = x0 + (vl & 15), y = y0 + ((vl >> 4) & 3), z = z0 + (vl >> 6);
+            if (x >= d.nx || y >= d.ny || z >= d.nz) continue;
+            const size_t v = (size_t)x + (size_t)d.nx * ((size_t)y + (size_t)d.ny * z);
+            *reinterpret_cast<uint4 *>(reinterpret_cast<char *>(outp) + (v * COUT + (size_t)cg * NCH) * ESZ + part * 16) =
+                *reinterpret_cast<const uint4 *>(smem + (size_t)vl * NCH * ESZ + part * 16);
         }
     }
 }
@@ -255,36 +349,8 @@ __global__ void k_wgrad_reduce(const float *__restrict__ part, int nchunk, int c
 }
 
 // ----------------------------------------------- 1-channel output layer --
-// out[v] = b + sum_tap sum_c W[tap*16 + c] d0[v+off][c]     (VALU, f32 out)
-__global__ __launch_bounds__(256) void k_out_fwd(Vol d, const uint16_t *__restrict__ in,
-                                                 const float *__restrict__ W,
-                                                 const float *__restrict__ bptr,
-                                                 float *__restrict__ out) {
-    __shared__ float sW[432];
-    for (int i = threadIdx.x; i < 432; i += 256) sW[i] = W[i];
-    __syncthreads();
-    const long long V = d.n();
-    const long long v = blockIdx.x * 256ll + threadIdx.x;
-    if (v >= V) return;
-    const int x = (int)(v % d.nx);
-    const long long r = v / d.nx;
-    const int y = (int)(r % d.ny), z = (int)(r / d.ny);
-    float s = bptr[0];
-    for (int tap = 0; tap < 27; ++tap) {
-        const int X = x + tap % 3 - 1, Y = y + (tap / 3) % 3 - 1, Z = z + tap / 9 - 1;
-        if (X < 0 || X >= d.nx || Y < 0 || Y >= d.ny || Z < 0 || Z >= d.nz) continue;
-        const uint4 *p = reinterpret_cast<const uint4 *>(in + ((size_t)X + (size_t)d.nx * ((size_t)Y + (size_t)d.ny * Z)) * 16);
-        const uint4 q0 = p[0], q1 = p[1];  // 16 bf16 channels in two 16-byte loads
-        const unsigned w[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            s += sW[tap * 16 + 2 * c] * bf2f((uint16_t)(w[c] & 0xffffu));
-            s += sW[tap * 16 + 2 * c + 1] * bf2f((uint16_t)(w[c] >> 16));
-        }
-    }
-    out[v] = s;
-}
-
+// The forward pass is k_conv_fwd<16, 16, 1, 2> (MFMA, f32 weights split into
+// bf16 high and low parts).
 // dIn[u][c] = sum_tap W[tap*16+c] dout[u - off(tap)]   (f32 out)
 __global__ __launch_bounds__(256) void k_out_bwd_data(Vol d, const float *__restrict__ dout,
                                                       const float *__restrict__ W,
@@ -313,27 +379,43 @@ __global__ __launch_bounds__(256) void k_out_bwd_data(Vol d, const float *__rest
 }
 
 // ----------------------------------------------------- element-wise ops --
-// avg pool 2x2x2 (ceil): out[V1][C] from in[V0][C]  (bf16)
+// avg pool 2x2x2 (ceil): out[V1][C] from in[V0][C]  (bf16).  One thread per
+// coarse voxel and 8-channel chunk (16-byte loads and stores); each channel
+// sums its children in (dz, dy, dx) order.
 __global__ void k_pool(Vol f, Vol c, int C, const uint16_t *__restrict__ in, uint16_t *__restrict__ out) {
-    const long long total = c.n() * C;
+    const int C8 = C / 8;
+    const long long total = c.n() * C8;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int ch = (int)(id % C);
-        const long long v = id / C;
+        const int q = (int)(id % C8);
+        const long long v = id / C8;
         const int x = (int)(v % c.nx);
         const long long r = v / c.nx;
         const int y = (int)(r % c.ny), z = (int)(r / c.ny);
-        float s = 0.f;
+        float s[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] = 0.f;
         int cnt = 0;
         for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
                 for (int dx = 0; dx < 2; ++dx) {
                     const int X = 2 * x + dx, Y = 2 * y + dy, Z = 2 * z + dz;
                     if (X >= f.nx || Y >= f.ny || Z >= f.nz) continue;
-                    s += bf2f(in[((size_t)X + (size_t)f.nx * ((size_t)Y + (size_t)f.ny * Z)) * C + ch]);
+                    const uint4 u = *reinterpret_cast<const uint4 *>(
+                        in + ((size_t)X + (size_t)f.nx * ((size_t)Y + (size_t)f.ny * Z)) * C + q * 8);
+                    const unsigned w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        s[2 * k] += bf2f((uint16_t)(w[k] & 0xffffu));
+                        s[2 * k + 1] += bf2f((uint16_t)(w[k] >> 16));
+                    }
                     ++cnt;
                 }
-        out[id] = f2bf(s / cnt);
+        unsigned o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            o[k] = (unsigned)f2bf(s[2 * k] / cnt) | ((unsigned)f2bf(s[2 * k + 1] / cnt) << 16);
+        *reinterpret_cast<uint4 *>(out + v * C + q * 8) = make_uint4(o[0], o[1], o[2], o[3]);
     }
 }
 
@@ -353,19 +435,33 @@ __global__ void k_unpool_add(Vol f, Vol c, int C, const float *__restrict__ dc, 
     }
 }
 
-// d = relu(up(t) + e): t coarse f32 [Vc][C], e fine bf16, d fine bf16
+// d = relu(up(t) + e): t coarse f32 [Vc][C], e fine bf16, d fine bf16.  One
+// thread per fine voxel and 8-channel chunk (16-byte loads and stores).
 __global__ void k_up_add_relu(Vol f, Vol c, int C, const float *__restrict__ t,
                               const uint16_t *__restrict__ e, uint16_t *__restrict__ d) {
-    const long long total = f.n() * C;
+    const int C8 = C / 8;
+    const long long total = f.n() * C8;
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
          id += (long long)gridDim.x * blockDim.x) {
-        const int ch = (int)(id % C);
-        const long long v = id / C;
+        const int q = (int)(id % C8);
+        const long long v = id / C8;
         const int x = (int)(v % f.nx);
         const long long r = v / f.nx;
         const int y = (int)(r % f.ny), z = (int)(r / f.ny);
-        const float tv = t[((size_t)(x / 2) + (size_t)c.nx * ((size_t)(y / 2) + (size_t)c.ny * (z / 2))) * C + ch];
-        d[id] = f2bf(fmaxf(tv + bf2f(e[id]), 0.f));
+        const float4 *tp = reinterpret_cast<const float4 *>(
+            t + ((size_t)(x / 2) + (size_t)c.nx * ((size_t)(y / 2) + (size_t)c.ny * (z / 2))) * C + q * 8);
+        const float4 t0 = tp[0], t1 = tp[1];
+        const float tv[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+        const uint4 u = *reinterpret_cast<const uint4 *>(e + v * C + q * 8);
+        const unsigned w[4] = {u.x, u.y, u.z, u.w};
+        unsigned o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float lo = fmaxf(tv[2 * k] + bf2f((uint16_t)(w[k] & 0xffffu)), 0.f);
+            const float hi = fmaxf(tv[2 * k + 1] + bf2f((uint16_t)(w[k] >> 16)), 0.f);
+            o[k] = (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+        }
+        *reinterpret_cast<uint4 *>(d + v * C + q * 8) = make_uint4(o[0], o[1], o[2], o[3]);
     }
 }
 
@@ -568,6 +664,7 @@ struct femgpu_unet {
     femgpu::Layer L[6];
     size_t nparam = 0;
     femgpu::DevBuf params, grads, adam_m, adam_v, mask, wbf, wflip, aux;
+    femgpu::DevBuf wout;  // output layer as a 16-column bf16 B matrix: rows 0/1 = high/low parts
     // activations (bf16) and gradients (f32)
     femgpu::DevBuf in, e0, p0, e1, p1, e2, t1, d1b, t0, d0b, out;
     femgpu::DevBuf gout, gd0, ge0, gt0, gd1, ge1, gt1, ge2, gp1, gp0, gtmp, gbf, part, dpart;
@@ -599,6 +696,7 @@ static void unet_alloc(femgpu_unet *u) {
     f(u->params, u->nparam); f(u->grads, u->nparam); f(u->adam_m, u->nparam); f(u->adam_v, u->nparam);
     u->mask.alloc(u->nparam);
     bf(u->wbf, u->nparam);
+    bf(u->wout, 16 * 432);
     size_t maxflip = 0;
     for (int l = 1; l < 5; ++l) maxflip = std::max(maxflip, (size_t)u->L[l].cin * kpad_of(u->L[l].cout));
     bf(u->wflip, maxflip);
@@ -690,6 +788,8 @@ static void chan_sum(femgpu_unet *u, long long V, int C, const uint16_t *dy, con
 static void sync_bf16(femgpu_unet *u) {
     hipLaunchKernelGGL(k_f32_to_bf16, dim3(g1((long long)u->nparam)), dim3(256), 0, u->ctx->stream,
                        (long long)u->nparam, (const float *)u->params.as<float>(), u->wbf.as<uint16_t>());
+    hipLaunchKernelGGL(k_split_out_weights, dim3(g1(16 * 432)), dim3(256), 0, u->ctx->stream,
+                       (const float *)(u->params.as<float>() + u->L[5].woff), u->wout.as<uint16_t>());
 }
 
 static const uint16_t *Wb(femgpu_unet *u, int l) { return u->wbf.as<uint16_t>() + u->L[l].woff; }
@@ -700,21 +800,23 @@ static void forward(femgpu_unet *u) {
     hipStream_t s = u->ctx->stream;
     const Vol D0 = u->d0, D1 = u->d1, D2 = u->d2;
     conv_fwd(u, 0, u->in.as<uint16_t>(), Wb(u, 0), Bf(u, 0), nullptr, u->e0.ptr, true, false);
-    hipLaunchKernelGGL(k_pool, dim3(g1(D1.n() * 16)), dim3(256), 0, s, D0, D1, 16, (const uint16_t *)u->e0.as<uint16_t>(), u->p0.as<uint16_t>());
+    hipLaunchKernelGGL(k_pool, dim3(g1(D1.n() * 2)), dim3(256), 0, s, D0, D1, 16, (const uint16_t *)u->e0.as<uint16_t>(), u->p0.as<uint16_t>());
     conv_fwd(u, 1, u->p0.as<uint16_t>(), Wb(u, 1), Bf(u, 1), nullptr, u->e1.ptr, true, false);
-    hipLaunchKernelGGL(k_pool, dim3(g1(D2.n() * 32)), dim3(256), 0, s, D1, D2, 32, (const uint16_t *)u->e1.as<uint16_t>(), u->p1.as<uint16_t>());
+    hipLaunchKernelGGL(k_pool, dim3(g1(D2.n() * 4)), dim3(256), 0, s, D1, D2, 32, (const uint16_t *)u->e1.as<uint16_t>(), u->p1.as<uint16_t>());
     conv_fwd(u, 2, u->p1.as<uint16_t>(), Wb(u, 2), Bf(u, 2), nullptr, u->e2.ptr, true, false);
     // decoder: t1 = conv(e2) + b (f32), d1 = relu(up(t1) + e1)
     conv_fwd(u, 3, u->e2.as<uint16_t>(), Wb(u, 3), nullptr, nullptr, u->t1.ptr, false, true);
     hipLaunchKernelGGL(k_add_bias_f32, dim3(g1(D2.n() * 32)), dim3(256), 0, s, D2.n(), 32, Bf(u, 3), u->t1.as<float>());
-    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D1.n() * 32)), dim3(256), 0, s, D1, D2, 32, (const float *)u->t1.as<float>(),
+    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D1.n() * 4)), dim3(256), 0, s, D1, D2, 32, (const float *)u->t1.as<float>(),
                        (const uint16_t *)u->e1.as<uint16_t>(), u->d1b.as<uint16_t>());
     conv_fwd(u, 4, u->d1b.as<uint16_t>(), Wb(u, 4), nullptr, nullptr, u->t0.ptr, false, true);
     hipLaunchKernelGGL(k_add_bias_f32, dim3(g1(D1.n() * 16)), dim3(256), 0, s, D1.n(), 16, Bf(u, 4), u->t0.as<float>());
-    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D0.n() * 16)), dim3(256), 0, s, D0, D1, 16, (const float *)u->t0.as<float>(),
+    hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D0.n() * 2)), dim3(256), 0, s, D0, D1, 16, (const float *)u->t0.as<float>(),
                        (const uint16_t *)u->e0.as<uint16_t>(), u->d0b.as<uint16_t>());
-    hipLaunchKernelGGL(k_out_fwd, dim3(g1(D0.n())), dim3(256), 0, s, D0, (const uint16_t *)u->d0b.as<uint16_t>(),
-                       (const float *)(u->params.as<float>() + u->L[5].woff), Bf(u, 5), u->out.as<float>());
+    hipLaunchKernelGGL((k_conv_fwd<16, 16, 1, 2>), dim3((unsigned)((D0.nx + kBX - 1) / kBX), (unsigned)((D0.ny + kBY - 1) / kBY),
+                       (unsigned)((D0.nz + kBZ - 1) / kBZ)), dim3(256), 0, s, D0, 432,
+                       (const uint16_t *)u->d0b.as<uint16_t>(), (const uint16_t *)u->wout.as<uint16_t>(), Bf(u, 5),
+                       (const uint16_t *)nullptr, (void *)u->out.as<float>(), 0);
 }
 
 // backward from u->gout (dL/dout, f32 [V0]); fills u->grads
