@@ -14,8 +14,10 @@ the returned solution (recomputed with a fresh apply), the roofline fraction
 of the PA apply (algorithmic bytes 8 (2N + s E q^3) per apply, s = 6
 diffusion / 7 diffusion + mass, over the HIP-event-timed mean duration of its
 two kernels), the HBM traffic per apply from the committed counter summary
-for that workload (profiles/*_hbm_traffic.md) and a CPU baseline (the
-oracle's sum-factorised PCG-Chebyshev on the host cores, bounded sample).
+for that workload (profiles/*_hbm_traffic.md), the bandwidth of a 1 GiB
+device-to-device copy timed in the same process (the practical streaming
+ceiling, with the apply's fraction of it) and a CPU baseline (the oracle's
+sum-factorised PCG-Chebyshev on the host cores, bounded sample).
 
 "legs" compares, on the same operator and right-hand side, the plain FP64
 solve with (a) FP32 PCG-Chebyshev + FP64 refinement (continued: one FP32 PCG
@@ -38,6 +40,12 @@ tens of seconds at the large ones, so use --steps 1 --warmup 0 there):
   q4-128-perturbed  Q4, 128^3, vertices perturbed by up to 0.2 h, diffusion +
                     mass (rho = 1), kappa = exp(GRF), random RHS, rtol 1e-8
                     (135,005,697 DOFs)
+  q3-96-learned     the learned preconditioner on the Q3 96^3 GRF family: the
+                    U-Net is trained on --train-fields fields (kappa seeds
+                    0, 1, ..., pairs from PCG-Chebyshev iterates) and compared
+                    with PCG + Chebyshev-Jacobi(4) on --test-fields held-out
+                    fields (seeds 256, 257, ...), FP64, to 1e-8; value = N x
+                    iterations / time of the learned solves
 """
 import argparse
 import json
@@ -54,6 +62,10 @@ sys.path.insert(0, os.path.join(ROOT, "fem-rs_amd"))
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8,
+                          desc="learned preconditioner, 3-D diffusion {n}^3 hex Q{p} ({N} DOFs), kappa = exp(GRF) "
+                               "per field, random nodal RHS, Dirichlet, FP64 flexible CG (Chebyshev-Jacobi(4) + "
+                               "U-Net 16/32/64 BF16) vs PCG + Chebyshev-Jacobi(4) to 1e-8 on held-out fields"),
     "p1-256-csr": dict(p=1, n=256, form="csr", rtol=1e-10,
                        desc="2-D Poisson -lap u = 2 pi^2 sin(pi x) sin(pi y) on the unit square, {n}^2 quads "
                             "split into 2 triangles, P1 ({N} DOFs, nnz {nnz}), assembled CSR, homogeneous "
@@ -92,6 +104,8 @@ def parse():
     ap.add_argument("--no-legs", action="store_true", help="skip the mixed / learned comparison")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--train-fields", type=int, default=4, help="q3-96-learned: training fields")
+    ap.add_argument("--test-fields", type=int, default=3, help="q3-96-learned: held-out fields")
     return ap.parse_args()
 
 
@@ -203,6 +217,122 @@ def run_csr(a, w):
                      "algorithmic_bytes": spmv_bytes},
         "legs": None,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def device_copy_gbs(fg, ctx, mib=1024, reps=5):
+    """Bandwidth of the runtime's device-to-device copy of `mib` MiB (read +
+    write bytes over HIP-event time): the practical streaming ceiling that the
+    roofline fraction can be read against."""
+    nc = mib * (1 << 20) // 8
+    src = fg.DeviceArray(ctx, nc, np.float64)
+    dst = fg.DeviceArray(ctx, nc, np.float64)
+    dst.copy_from(src)
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(reps):
+        dst.copy_from(src)
+    ms = ctx.timer_stop() / reps
+    del src, dst
+    return 2.0 * nc * 8 / (ms * 1e-3) / 1e9
+
+
+def run_learned(a, w):
+    """Learned preconditioner on held-out fields of the Q3 GRF family (a
+    reduced form of the 256-field training set: --train-fields fields)."""
+    import femgpu as fg
+    from femgpu import learned
+    n = a.n or w["n"]
+    p = a.p or w["p"]
+    ctx = fg.default_context()
+    steps = a.train_steps
+
+    def field(seed):
+        K = fg.HexPAOperator(p, p + 1, n, form=fg.DIFFUSION, kappa=fg.grf(seed), dirichlet=True, ctx=ctx)
+        b = fg.DeviceArray(ctx, K.n_dofs(), np.float64)
+        K.rhs_random(1000 + seed, b)
+        x = fg.DeviceArray(ctx, K.n_dofs(), np.float64)
+        _, lmax = fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=1))
+        return K, b, lmax
+
+    t0 = time.perf_counter()
+    train_ops, pairs = [], []
+    for sd in range(a.train_fields):
+        K, b, lmax = field(sd)
+        ps = learned.harvest_pairs(K, b.download(), iters=(2, 6, 15, 40, 100), lmax=lmax, tol=1e-10)
+        pairs += [(len(train_ops), r_, e_) for r_, e_ in ps]
+        train_ops.append(K)
+        print(f"bench.py: field {sd}: {len(ps)} pairs", file=sys.stderr, flush=True)
+    t_harvest = time.perf_counter() - t0
+    net = fg.UNet.for_operator(train_ops[0], seed=11)
+    rng = np.random.default_rng(0)
+    losses = []
+    t0 = time.perf_counter()
+    cur = -1
+    for st in range(steps):
+        k, r_, e_ = pairs[rng.integers(len(pairs))]
+        if k != cur:
+            net.aux_from(train_ops[k])
+            cur = k
+        losses.append(net.train_step_energy(train_ops[k], r_, e_, 2e-3 * 0.1 ** (st / max(steps, 1))))
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"bench.py: trained {steps} steps in {t_train:.1f} s", file=sys.stderr, flush=True)
+    for K in train_ops:
+        K.close()
+    del train_ops
+    cfg = fg.SolverConfig(rtol=w["rtol"], max_iter=100000)
+    rows, it_l, t_l_sum = [], 0, 0.0
+    N = 0
+    for j in range(a.test_fields):
+        sd = 256 + j
+        K, b, lmax = field(sd)
+        N = K.n_dofs()
+        x = fg.DeviceArray(ctx, N, np.float64)
+        fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=3), lmax=lmax)  # warm
+        ctx.synchronize()
+        x.upload(np.zeros(N))
+        ctx.synchronize(); t0 = time.perf_counter()
+        rp, _ = fg.solve_pcg_chebyshev(K, b, x, cfg, lmax=lmax)
+        ctx.synchronize(); tp = time.perf_counter() - t0
+        net.aux_from(K)
+        x.upload(np.zeros(N))
+        ctx.synchronize(); t0 = time.perf_counter()
+        rl, info = fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax)
+        ctx.synchronize(); tl = time.perf_counter() - t0
+        r = fg.DeviceArray(ctx, N, np.float64)
+        K.apply_zero(x, r)
+        bs = b.download()
+        relres = float(np.linalg.norm(bs - r.download()) / np.linalg.norm(bs))
+        rows.append({"seed": sd, "pcg_chebyshev_iterations": rp.iterations, "pcg_chebyshev_ms": round(1e3 * tp, 2),
+                     "learned_fcg_iterations": rl.iterations, "learned_fcg_ms": round(1e3 * tl, 2),
+                     "net_corrections": info.net_corrections, "fell_back": info.fell_back,
+                     "learned_rel_residual_fp64": relres,
+                     "iteration_ratio": round(rp.iterations / max(rl.iterations, 1), 3),
+                     "time_ratio": round(tp / tl, 3)})
+        print(f"bench.py: held-out field {sd}: PCG-Chebyshev {rp.iterations} it {1e3 * tp:.0f} ms, "
+              f"learned FCG {rl.iterations} it {1e3 * tl:.0f} ms", file=sys.stderr, flush=True)
+        it_l += rl.iterations
+        t_l_sum += tl
+        K.close()
+        del r, x, b
+    out = {
+        "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
+        "value": round(N * it_l / t_l_sum / 1e9, 4), "unit": "GDOF/s", "n_gpus": 1,
+        "steps": a.test_fields, "warmup": 0, "ms_per_step": round(1e3 * t_l_sum / a.test_fields, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w["desc"].format(n=n, p=p, N=N), "name": a.workload, "n_dofs": N,
+                   "train_fields": a.train_fields, "test_fields": a.test_fields, "global_batch": 1,
+                   "parallelism": "single GPU"},
+        "train": {"pairs": len(pairs), "steps": steps, "harvest_s": round(t_harvest, 1),
+                  "train_s": round(t_train, 1),
+                  "energy_loss_first_last": [round(float(np.mean(losses[:20])), 4),
+                                             round(float(np.mean(losses[-20:])), 4)]},
+        "held_out": rows,
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
 
@@ -346,6 +476,8 @@ def main():
     w = WORKLOADS[a.workload]
     if w["form"] == "csr":
         return run_csr(a, w)
+    if w["form"] == "learned":
+        return run_learned(a, w)
     n = a.n or w["n"]
     p = a.p or w["p"]
     q = p + 1
@@ -370,6 +502,8 @@ def main():
         dst = fg.DeviceArray(ctx, nc, np.float64)
         dst.copy_from(src)  # one copy kernel of known size: reads and writes nc*8 bytes
         del src, dst
+
+    copy_gbs = device_copy_gbs(fg, ctx)
 
     # PA apply rates (device vectors, HIP events)
     xv = dev().upload(np.random.default_rng(0).standard_normal(N))
@@ -458,6 +592,8 @@ def main():
             "kernel": f"PA apply = k_pa_hex<{q},{q},double,{kform}> element kernel + "
                       f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)",
             "algorithmic_bytes": bytes_apply,
+            "device_copy_gb_s": round(copy_gbs, 1),
+            "frac_of_device_copy": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / copy_gbs, 4),
         },
         "legs": leg,
         "cpu_baseline": cpu,
