@@ -540,6 +540,98 @@ __global__ __launch_bounds__(256) void k_pcgj_update(size_t n, T *__restrict__ x
     }
 }
 
+// ------------------------------ fused PCG + Chebyshev-Jacobi iteration --
+// The same finalise-in-every-workgroup scheme for the Chebyshev-preconditioned
+// PCG: k_update_cheb0 takes alpha from the p.Ap partials (slot 0) and the
+// previous r.z partials, updates x and r (partial r.r into slot 1) and starts
+// the Chebyshev polynomial (z = d = inv r / theta); k_pupdate_beta takes beta
+// from the r.z partials of this and the previous iteration and forms p.  The
+// r.z partials alternate between slots kFcRz[0] / kFcRz[1] by iteration.  The
+// one-block k_fin_conv between them keeps the stopping test ahead of the
+// Chebyshev applies.  Iterates are bitwise those of the separate kernels.
+constexpr int kFcRz[2] = {2, 3};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__restrict__ x,
+                                                      const T *__restrict__ p, T *__restrict__ r,
+                                                      const T *__restrict__ ap, const T *__restrict__ inv,
+                                                      T *__restrict__ z, T *__restrict__ d,
+                                                      double *__restrict__ part, int nb, int *flags,
+                                                      int rz_in, double inv_theta, int rz_out, double brk) {
+    __shared__ double lds[2][16];
+    __shared__ int stop;
+    const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    T x0 = 0, p0 = 0, r0 = 0, a0 = 0, q0 = 0;
+    if (i0 < n) {
+        x0 = x[i0];
+        p0 = p[i0];
+        r0 = r[i0];
+        a0 = ap[i0];
+        q0 = inv[i0];
+    }
+    double o[2];
+    if (rsum256<2>(part, nb, {0, rz_in}, o, lds, flags, &stop)) return;
+    const double pap = o[0], rz = o[1];
+    const bool brkdn = fabs(pap) < brk;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_PAP] = pap;
+        if (brkdn) {
+            S[S_STATUS] = ST_BREAKDOWN;
+            flags[0] = 1;
+        } else {
+            S[S_ALPHA] = rz / pap;
+        }
+    }
+    if (brkdn) return;
+    const T alpha = (T)(rz / pap);
+    double v[2] = {0, 0};
+    for (size_t i = i0; i < n; i += stride) {
+        T xi, pi, ri, ai, qi;
+        if (i == i0) { xi = x0; pi = p0; ri = r0; ai = a0; qi = q0; }
+        else { xi = x[i]; pi = p[i]; ri = r[i]; ai = ap[i]; qi = inv[i]; }
+        x[i] = fmad<T>(alpha, pi, xi);
+        ri = fmad<T>(-alpha, ai, ri);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+        const T zi = (T)inv_theta * (qi * ri);
+        z[i] = zi;
+        d[i] = zi;
+        if (rz_out >= 0) v[1] += (double)ri * (double)zi;
+    }
+    block_sum<2, 256>(v);
+    if (threadIdx.x == 0) {
+        part[1 * kRedBlocks + blockIdx.x] = v[0];
+        if (rz_out >= 0) part[rz_out * kRedBlocks + blockIdx.x] = v[1];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pupdate_beta(size_t n, double *S, const T *__restrict__ z,
+                                                      T *__restrict__ p, const double *__restrict__ part,
+                                                      int nb, int *flags, int rz_new, int rz_old) {
+    __shared__ double lds[2][16];
+    __shared__ int stop;
+    const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    T z0 = 0, p0 = 0;
+    if (i0 < n) {
+        z0 = z[i0];
+        p0 = p[i0];
+    }
+    double o[2];
+    if (rsum256<2>(part, nb, {rz_new, rz_old}, o, lds, flags, &stop)) return;
+    const double b = o[0] / o[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S[S_BETA] = b;
+        S[S_RZ] = o[0];
+    }
+    const T beta = (T)b;
+    for (size_t i = i0; i < n; i += stride) {
+        const T zi = i == i0 ? z0 : z[i];
+        const T pi = i == i0 ? p0 : p[i];
+        p[i] = fmad<T>(beta, pi, zi);
+    }
+}
+
 // After the last enqueued iteration: records it and applies the stopping test.
 __global__ __launch_bounds__(256) void k_pcgj_end(const double *part, int nb, double *S, int *flags,
                                                   double rtol, double atol) {
@@ -634,22 +726,26 @@ template <class T> struct Krylov {
         return f;
     }
 
-    // z = cheb(r) with partial r.z into part slot 0 (skippable)
-    void cheb_apply(const ChebParams &cp, const T *r, const T *inv, T *z) {
+    // z = cheb(r) with partial r.z into part slot rz_slot (skippable).  With
+    // from1 the first step (z = d = inv r / theta) has been done by the caller.
+    void cheb_apply(const ChebParams &cp, const T *r, const T *inv, T *z, int rz_slot = 0,
+                    bool from1 = false) {
         T *d = vec(W_D), *t = vec(W_T);
+        double *pz = part + (size_t)rz_slot * kRedBlocks;
         const double theta = 0.5 * (cp.lmax + cp.lmin), delta = 0.5 * (cp.lmax - cp.lmin);
         const double sigma = theta / delta;
         double rho_prev = 1.0 / sigma;
         const unsigned nb = nblk(n);
-        hipLaunchKernelGGL(k_cheb0<T>, dim3(nb), dim3(256), 0, s, n, r, inv, z, d, 1.0 / theta,
-                           cp.degree == 1 ? 1 : 0, part, flags);
+        if (!from1)
+            hipLaunchKernelGGL(k_cheb0<T>, dim3(nb), dim3(256), 0, s, n, r, inv, z, d, 1.0 / theta,
+                               cp.degree == 1 ? 1 : 0, pz, flags);
         for (int k = 1; k < cp.degree; ++k) {
             const double rho = 1.0 / (2.0 * sigma - rho_prev);
             const int last = k == cp.degree - 1 ? 1 : 0;
-            if (!op->apply_cheb_dev(z, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, part, flags)) {
+            if (!op->apply_cheb_dev(z, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, pz, flags)) {
                 apply(z, t, true);
                 hipLaunchKernelGGL(k_chebk<T>, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
-                                   rho * rho_prev, 2.0 * rho / delta, last, part, flags);
+                                   rho * rho_prev, 2.0 * rho / delta, last, pz, flags);
             }
             rho_prev = rho;
         }
@@ -678,9 +774,10 @@ template <class T> struct Krylov {
         if (mode == MODE_PCG && prec == PREC_JACOBI && op->csr_view(&cv) && fused_pcg_enabled())
             return run_pcgj_fused(cv, x, inv, rtol, atol, max_iter, res);
         if (prec == PREC_CHEB) {
-            // z = M^-1 r0; p = z; rz = r.z
-            cheb_apply(cp, r, inv, z);
-            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, (int)S_RZ);
+            // z = M^-1 r0; p = z; rz = r.z (partials in the slot the fused loop reads first)
+            cheb_apply(cp, r, inv, z, kFcRz[0]);
+            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s,
+                               (const double *)(part + (size_t)kFcRz[0] * kRedBlocks), (int)nb, S, (int)S_RZ);
             copy<T>(s, p, z, n);
         }
         return run(mode, prec, cp, x, inv, rtol, atol, max_iter, res, false);
@@ -775,12 +872,13 @@ template <class T> struct Krylov {
         T *r = vec(W_R), *z = vec(W_Z), *p = vec(W_P);
         const T *inv = vec(W_INV);
         hipLaunchKernelGGL(k_set_cont, dim3(1), dim3(1), 0, s, S, flags, bnorm);
-        cheb_apply(cp, r, inv, z);
+        cheb_apply(cp, r, inv, z, kFcRz[0]);
+        const double *prz = part + (size_t)kFcRz[0] * kRedBlocks;
         if (first) {
-            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, (int)S_RZ);
+            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, prz, (int)nb, S, (int)S_RZ);
             copy<T>(s, p, z, n);
         } else {
-            hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, flags);
+            hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, prz, (int)nb, S, flags);
             hipLaunchKernelGGL(k_pupdate<T>, dim3(vgrid(n)), dim3(256), 0, s, n, (const double *)S, (const T *)z,
                                p, flags);
         }
@@ -797,6 +895,8 @@ template <class T> struct Krylov {
         const unsigned nb = nblk(n);
         T *r = vec(W_R), *z = vec(W_Z), *p = vec(W_P), *ap = vec(W_AP);
         const double brk = mode == MODE_CG ? 1e-32 : 1e-30;
+        if (prec == PREC_CHEB && mode == MODE_PCG && fused_pcg_enabled())
+            return run_cheb_fused(cp, x, inv, rtol, atol, max_iter, res, segment);
         uint64_t it = 0;
         bool stopped = read_flag() != 0;
         const uint64_t check = n < (1u << 20) ? 16 : 4;
@@ -833,6 +933,44 @@ template <class T> struct Krylov {
             if (it % check == 0 || it == max_iter) stopped = read_flag() != 0;
         }
         return finish(mode, max_iter, res);
+    }
+
+    // run() for PCG + Chebyshev-Jacobi with the fused vector kernels
+    // (k_update_cheb0, k_pupdate_beta); same iterates, same segment semantics.
+    int run_cheb_fused(const ChebParams &cp, T *x, const T *inv, double rtol, double atol,
+                       uint64_t max_iter, femgpu_solve_result *res, bool segment) {
+        const unsigned nb = nblk(n);
+        T *r = vec(W_R), *z = vec(W_Z), *p = vec(W_P), *ap = vec(W_AP), *d = vec(W_D);
+        const double brk = 1e-30;
+        const double inv_theta = 1.0 / (0.5 * (cp.lmax + cp.lmin));
+        uint64_t it = 0;
+        bool stopped = read_flag() != 0;
+        const uint64_t check = n < (1u << 20) ? 16 : 4;
+        while (!stopped && it < max_iter) {
+            const uint64_t k = it + 1;
+            const int rz_prev = kFcRz[(k - 1) & 1], rz_new = kFcRz[k & 1];
+            if (!op->apply_dot_dev(p, ap, part, flags)) {
+                apply(p, ap, true);
+                hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)p, (const T *)ap,
+                                   part, 0, flags);
+            }
+            hipLaunchKernelGGL(k_update_cheb0<T>, dim3(nb), dim3(256), 0, s, n, S, x, (const T *)p, r,
+                               (const T *)ap, inv, z, d, part, (int)nb, flags, rz_prev, inv_theta,
+                               cp.degree == 1 ? rz_new : -1, brk);
+            hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)(part + kRedBlocks),
+                               (int)nb, S, flags, (int)MODE_PCG, 0, 0, rtol, atol, (double)(it + 1));
+            if (segment && it + 1 == max_iter) {
+                ++it;
+                break;
+            }
+            cheb_apply(cp, r, inv, z, rz_new, true);
+            hipLaunchKernelGGL(k_pupdate_beta<T>, dim3(nb), dim3(256), 0, s, n, S, (const T *)z, p,
+                               (const double *)part, (int)nb, flags, rz_new, rz_prev);
+            FG_HIP(hipGetLastError());
+            ++it;
+            if (it % check == 0 || it == max_iter) stopped = read_flag() != 0;
+        }
+        return finish(MODE_PCG, max_iter, res);
     }
 
     // Reads the scalars after a solve and fills the result.

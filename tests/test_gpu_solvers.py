@@ -284,3 +284,60 @@ def test_pcg_jacobi_fused_grid_stride(fg, fr, ctx, monkeypatch):
     ref = fr.CsrRef(rp, ci, v)
     xo, ro = fr.pcg_jacobi(fr._Op(n, csr=ref), ref.diagonal(), b, np.zeros(n), rtol=1e-12, max_iter=60)
     assert np.abs(xf - xo).max() < 1e-10 * np.abs(xo).max()
+
+
+def _both(monkeypatch, run):
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("FEMGPU_PCG_FUSED", fused)
+        out[fused] = run()
+    return out["1"], out["0"]
+
+
+@pytest.mark.parametrize("degree,max_iter", [(4, 5000), (4, 7), (1, 5000), (2, 5000)])
+def test_pcg_chebyshev_fused_matches_separate_kernels(fg, fr, ctx, monkeypatch, degree, max_iter):
+    """PCG + Chebyshev-Jacobi with the fused vector kernels (alpha, beta
+    finalised in every workgroup, the first Chebyshev step inside the update)
+    equals the separate-kernel loop bit for bit: hex Q2 GRF operator with
+    essential DOFs, converged and stopped at max_iter, degrees 1, 2, 4."""
+    op = _hex_problem(fg, fr, ctx, 2, 9, dict(kind="grf", seed=3))
+    b = op.rhs_random(2)
+    cfg = fg.SolverConfig(rtol=1e-10, max_iter=max_iter)
+
+    def run():
+        x = np.zeros(op.n_dofs())
+        res, lmax = fg.solve_pcg_chebyshev(op, b, x, cfg, degree=degree)
+        return x, res, lmax
+    (xf, rf, lf), (xs, rs, ls) = _both(monkeypatch, run)
+    print(f"degree {degree} max_iter {max_iter}: fused {rf}, separate {rs}")
+    assert lf == ls and rf == rs
+    assert np.array_equal(xf, xs)
+    assert rf.converged == (max_iter == 5000)
+
+
+def test_pcg_chebyshev_fused_csr_and_continued(fg, fr, ctx, monkeypatch):
+    """The fused loop on a CSR operator (config-1 matrix at 48^2, a grid-stride
+    free size) and inside continued mixed-precision refinement (segments that
+    stop before the preconditioner, then resume with the last direction)
+    equals the separate kernels bit for bit."""
+    rp, ci, v, rhs, coords, conn = fr.poisson_p1(48)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    cfg = fg.SolverConfig(rtol=1e-10, max_iter=5000)
+
+    def run_csr():
+        x = np.zeros(len(rhs))
+        res, _ = fg.solve_pcg_chebyshev(A, rhs, x, cfg, degree=3)
+        return x, res
+    (xf, rf), (xs, rs) = _both(monkeypatch, run_csr)
+    assert rf == rs and rf.converged and np.array_equal(xf, xs)
+
+    op64 = _hex_problem(fg, fr, ctx, 3, 8, dict(kind="grf", seed=1))
+    op32 = op64.convert(fg.F32)
+    b = op64.rhs_random(5)
+
+    def run_ir():
+        x = np.zeros(op64.n_dofs())
+        res, inner = fg.solve_ir_mixed(op64, op32, b, x, cfg, mode=1, inner_rtol=1e-2, update_interval=16)
+        return x, res, inner
+    (xf, rf, nf), (xs, rs, ns) = _both(monkeypatch, run_ir)
+    assert rf == rs and nf == ns and rf.converged and np.array_equal(xf, xs)
