@@ -85,7 +85,8 @@ struct HexGrid {
     int nx, ny, nz;
     long long E;
     long long Nx, Ny, Nz;
-    int ne;  // elements per element-kernel workgroup, ne_of(Q)
+    int ne;         // elements per element-kernel workgroup, ne_of(Q)
+    long long Nxp;  // row stride of the x-merged element vector: Nx rounded up to even
 };
 
 
@@ -94,7 +95,8 @@ __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long
 }
 
 // Element-vector layout ("x-merged"): for each element row (ey, ez) and local
-// (k, j) a row of Nx values, one per DOF position gx along x.  Where two
+// (k, j) a row of Nx values (stride Nxp, even, so that even-P segments are
+// 16-byte aligned), one per DOF position gx along x.  Where two
 // elements of one workgroup share an x-face DOF, the row holds their sum
 // (lower element first); everywhere else it holds the single contribution.
 // The first element of a workgroup (e % ne == 0) with ex > 0 stores its
@@ -102,10 +104,10 @@ __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long
 // gather adds it.  This keeps Nx instead of nx (P + 1) values per row.
 __host__ __device__ __forceinline__ size_t mrow_index(const HexGrid &g, int N1, int ey, int ez, int j,
                                                       int k) {
-    return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.Nx;
+    return ((((size_t)ez * g.ny + ey) * N1 + k) * N1 + j) * (size_t)g.Nxp;
 }
 __host__ __device__ __forceinline__ size_t seam_base(const HexGrid &g, int N1) {
-    return (size_t)g.nz * g.ny * N1 * N1 * (size_t)g.Nx;
+    return (size_t)g.nz * g.ny * N1 * N1 * (size_t)g.Nxp;
 }
 __host__ __device__ __forceinline__ size_t seam_index(const HexGrid &g, int N1, long long blk, int j,
                                                       int k) {
@@ -348,6 +350,10 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 }
 
+template <class T> struct Pair;
+template <> struct Pair<double> { using type = double2; };
+template <> struct Pair<float> { using type = float2; };
+
 // Element-vector -> DOF values, one x-segment per thread.  Segment (ex, gy,
 // gz) holds the DOFs gx = ex P + i, i = 0..P-1, of DOF row (gy, gz), and the
 // row's last DOF gx = Nx - 1 too when ex = nx - 1.  Each DOF sums its y/z
@@ -383,9 +389,22 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
         for (int b = 0; b < 2; ++b) {
             if (!(cv[1][c] && cv[0][b])) continue;
             const I erow = (I)ce[1][c] * (I)g.ny + (I)ce[0][b];
-            const I row = ((erow * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * (I)g.Nx;
+            const I row = ((erow * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * (I)g.Nxp;
             const T *rp = ye + row + (I)ex * P;  // this segment's values on the row
-            T m0 = rp[0];
+            T v[P];
+            if constexpr (P % 2 == 0) {  // even row stride and even P: aligned pairs
+                using T2 = typename Pair<T>::type;
+#pragma unroll
+                for (int i = 0; i < P; i += 2) {
+                    const T2 w = *reinterpret_cast<const T2 *>(rp + i);
+                    v[i] = w.x;
+                    v[i + 1] = w.y;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < P; ++i) v[i] = rp[i];
+            }
+            T m0 = v[0];
             if (ex > 0) {
                 const I e = erow * (I)g.nx + (I)ex;
                 if (e % (I)g.ne == 0)
@@ -393,7 +412,7 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
             }
             out[0] += m0;
 #pragma unroll
-            for (int i = 1; i < P; ++i) out[i] += rp[i];
+            for (int i = 1; i < P; ++i) out[i] += v[i];
             if (ex == g.nx - 1) out[P] += rp[P];
         }
     const size_t base = row0 + (size_t)ex * P;
@@ -670,8 +689,9 @@ __global__ void k_evec_merge(const HexGrid g, int N1, const T *__restrict__ old,
     const size_t total = seam_base(g, N1);
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
          id += (size_t)gridDim.x * blockDim.x) {
-        const long long gx = (long long)(id % (size_t)g.Nx);
-        const size_t row = id / (size_t)g.Nx;
+        const long long gx = (long long)(id % (size_t)g.Nxp);
+        const size_t row = id / (size_t)g.Nxp;
+        if (gx >= g.Nx) continue;  // row padding
         const int j = (int)(row % N1), k = (int)((row / N1) % N1);
         const long long erow = (long long)(row / ((size_t)N1 * N1));
         auto val = [&](long long ex, int i) { return old[(size_t)(erow * g.nx + ex) * L + (k * N1 + j) * N1 + i]; };
@@ -929,6 +949,7 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         g.E = (long long)d.nx * d.ny * d.nz;
         g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
         g.ne = ne_of(d.q);
+        g.Nxp = g.Nx + (g.Nx & 1);
         op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
         FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536 && g.E < (1ll << 31),
                    FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
