@@ -109,6 +109,12 @@ int femgpu_malloc(femgpu_ctx *c, size_t bytes, void **dptr) {
         hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
         FG_REQUIRE(e == hipSuccess, FEMGPU_ERR_OUT_OF_MEMORY,
                    "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        // zero-filled, so that a fresh vector is a valid zero initial guess
+        // (the runtime recycles freed blocks with their old contents)
+        if (bytes) {
+            FG_HIP(hipMemsetAsync(*dptr, 0, bytes, c->stream));
+            FG_HIP(hipStreamSynchronize(c->stream));
+        }
         return 0;
     });
 }

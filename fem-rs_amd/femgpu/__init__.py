@@ -248,7 +248,8 @@ def default_context():
 
 
 class DeviceArray:
-    """A 1-D vector resident in HBM (for FEMGPU_DEVICE calls)."""
+    """A 1-D vector resident in HBM (for FEMGPU_DEVICE calls); zero-filled
+    unless data is given."""
 
     def __init__(self, ctx, n, dtype=np.float64, data=None):
         self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)

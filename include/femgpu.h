@@ -62,8 +62,9 @@ const char *femgpu_version(void);
 int femgpu_ctx_create(int device, femgpu_ctx **out);
 int femgpu_ctx_destroy(femgpu_ctx *ctx);
 int femgpu_ctx_synchronize(femgpu_ctx *ctx);
-/* Device memory helpers (for FEMGPU_DEVICE vectors). kind: 0 = H2D, 1 = D2H,
- * 2 = D2D.  Copies are ordered on the context stream and complete on return. */
+/* Device memory helpers (for FEMGPU_DEVICE vectors). femgpu_malloc returns
+ * zero-filled memory.  kind: 0 = H2D, 1 = D2H, 2 = D2D.  Copies are ordered on
+ * the context stream and complete on return. */
 int femgpu_malloc(femgpu_ctx *ctx, size_t bytes, void **dptr);
 int femgpu_free(femgpu_ctx *ctx, void *dptr);
 int femgpu_memcpy(femgpu_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);

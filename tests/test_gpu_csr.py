@@ -83,3 +83,16 @@ def test_spmv_device_arrays_and_errors(fg, fr, ctx):
     # empty matrix
     E = fg.CsrMatrix(np.zeros(1, np.uint64), np.zeros(0, np.uint32), np.zeros(0), ctx=ctx)
     assert E.n_dofs() == 0
+
+
+def test_device_array_zero_filled(fg, ctx):
+    """femgpu_malloc returns zero-filled memory, also when the runtime hands
+    back a block that held data (a fresh DeviceArray is a valid zero initial
+    guess)."""
+    n = 1 << 20
+    for _ in range(3):
+        a = fg.DeviceArray(ctx, n, np.float64, np.full(n, 3.5))
+        a.free()
+        b = fg.DeviceArray(ctx, n, np.float64)
+        assert not np.any(b.download())
+        b.free()
