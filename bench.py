@@ -32,9 +32,9 @@ PCG-Chebyshev solution to rtol / 100.
 --workload selects the other configurations (one solve takes seconds to
 tens of seconds at the large ones, so use --steps 1 --warmup 0 there):
   p1-256-csr        2-D Poisson on the unit square, 256^2 quads split into two
-                    triangles each, P1 (66,049 DOFs), assembled CSR (built
-                    here with NumPy/SciPy as the caller would, row-zeroed by
-                    the library's apply_dirichlet), PCG + Jacobi to 1e-10
+                    triangles each, P1 (66,049 DOFs), CSR and load vector
+                    assembled on the GPU (assemble_bilinear / assemble_linear,
+                    row zeroing by apply_dirichlet), PCG + Jacobi to 1e-10
   q3-96-grf         Q3, 96^3, kappa = exp(GRF) clipped to [1e-2, 1e2], random
                     nodal RHS, rtol 1e-10 (24,137,569 DOFs)
   q4-128-perturbed  Q4, 128^3, vertices perturbed by up to 0.2 h, diffusion +
@@ -109,54 +109,46 @@ def parse():
     return ap.parse_args()
 
 
-def poisson_p1_system(n):
-    """Caller-side assembly of the 2-D P1 Poisson system of the unit square
-    (n x n quads, each split into (v0, v1, v3) and (v1, v2, v3) as
-    mesh/src/simplex.rs:408-413 does): CSR stiffness (COO summed to CSR,
-    columns sorted), load vector of f = 2 pi^2 sin(pi x) sin(pi y) with the
-    3-point rule, and the boundary DOFs."""
-    import scipy.sparse as sp
+def unit_square_mesh(n):
+    """The config-1 mesh, as the caller builds it: the unit square with n x n
+    quads, each split into (v0, v1, v3) and (v1, v2, v3) (mesh/src/simplex.rs:408-413);
+    P1 DOFs are the vertices.  -> (coords f64[(n+1)^2, 2], conn u32[2 n^2, 3],
+    boundary vertices)."""
     m = n + 1
     j, i = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
     xy = np.stack([i.ravel() / n, j.ravel() / n], axis=1)
     jj, ii = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
     n0 = (jj * m + ii).ravel(); n1 = n0 + 1; n3 = n0 + m; n2 = n3 + 1
-    conn = np.empty((2 * n0.size, 3), np.int64)
+    conn = np.empty((2 * n0.size, 3), np.uint32)
     conn[0::2] = np.stack([n0, n1, n3], 1)
     conn[1::2] = np.stack([n1, n2, n3], 1)
-    X = xy[conn]                                   # [E, 3, 2]
-    J = np.stack([X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]], axis=2)  # columns x_k - x_0
-    det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
-    JinvT = np.empty_like(J)
-    JinvT[:, 0, 0] = J[:, 1, 1] / det; JinvT[:, 0, 1] = -J[:, 1, 0] / det
-    JinvT[:, 1, 0] = -J[:, 0, 1] / det; JinvT[:, 1, 1] = J[:, 0, 0] / det
-    gref = np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]])
-    G = np.einsum("eij,aj->eai", JinvT, gref)      # physical gradients [E, 3, 2]
-    Ke = 0.5 * np.abs(det)[:, None, None] * np.einsum("eai,ebi->eab", G, G)
-    rows = np.repeat(conn, 3, axis=1).ravel(); cols = np.tile(conn, (1, 3)).ravel()
-    A = sp.coo_matrix((Ke.ravel(), (rows, cols)), shape=(m * m, m * m)).tocsr()
-    A.sort_indices()
-    qp = np.array([[1 / 6, 1 / 6], [2 / 3, 1 / 6], [1 / 6, 2 / 3]]); qw = np.full(3, 1 / 6)
-    phi = np.stack([1 - qp[:, 0] - qp[:, 1], qp[:, 0], qp[:, 1]], axis=1)   # [q, a]
-    xq = X[:, 0][:, None, :] + np.einsum("eij,qj->eqi", J, qp)
-    f = 2 * np.pi ** 2 * np.sin(np.pi * xq[..., 0]) * np.sin(np.pi * xq[..., 1])
-    fe = np.einsum("q,eq,qa->ea", qw, f * np.abs(det)[:, None], phi)
-    b = np.bincount(conn.ravel(), weights=fe.ravel(), minlength=m * m)
     bnd = np.where((i.ravel() == 0) | (i.ravel() == n) | (j.ravel() == 0) | (j.ravel() == n))[0]
-    return (A.indptr.astype(np.uint64), A.indices.astype(np.uint32), A.data.astype(np.float64), b,
-            bnd.astype(np.uint32), xy)
+    return xy, conn, bnd.astype(np.uint32)
 
 
 def run_csr(a, w):
-    """Config-1 workload: assembled CSR, PCG + Jacobi on the GPU."""
+    """Config-1 workload: the system assembled on the GPU (P1 stiffness, 3-point
+    source load, row zeroing), then PCG + Jacobi on the GPU."""
     import femgpu as fg
     n = a.n or w["n"]
-    rp, ci, v, b_h, bnd, xy = poisson_p1_system(n)
+    xy, conn, bnd = unit_square_mesh(n)
     ctx = fg.default_context()
-    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
-    N, nnz = len(b_h), int(rp[-1])
-    fg.apply_dirichlet(A, b_h, bnd, np.zeros(len(bnd)))
-    b = fg.DeviceArray(ctx, N, np.float64).upload(b_h)
+    N = len(xy)
+
+    def assemble():
+        A = fg.assemble_bilinear(2, 1, xy, conn, conn, N, fg.DIFFUSION, 1.0, 3, ctx=ctx)
+        b = fg.DeviceArray(ctx, N, np.float64)
+        fg.assemble_linear(2, 1, xy, conn, conn, N, fg.SOURCE_SINPROD, (2.0 * np.pi) * np.pi, 3, out=b, ctx=ctx)
+        fg.apply_dirichlet(A, b, bnd, np.zeros(len(bnd)))
+        return A, b
+
+    assemble()  # warm-up (code objects, allocator)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    A, b = assemble()
+    ctx.synchronize()
+    t_asm = time.perf_counter() - t0
+    nnz = A.nnz()
     x = fg.DeviceArray(ctx, N, np.float64)
     x0 = fg.DeviceArray(ctx, N, np.float64)
     cfg = fg.SolverConfig(rtol=w["rtol"], max_iter=5000)
@@ -191,13 +183,18 @@ def run_csr(a, w):
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import femref as fr
-        ref = fr.CsrRef(rp, ci, A.values())
         t0 = time.perf_counter()
-        _, ro = fr.pcg_jacobi(fr._Op(N, csr=ref), ref.diagonal(), b_h, np.zeros(N), rtol=w["rtol"], max_iter=5000)
+        rp, ci, v, rhs, _, _ = fr.poisson_p1(n)
+        t_cpu_asm = time.perf_counter() - t0
+        ref = fr.CsrRef(rp, ci, v)
+        t0 = time.perf_counter()
+        _, ro = fr.pcg_jacobi(fr._Op(N, csr=ref), ref.diagonal(), rhs, np.zeros(N), rtol=w["rtol"], max_iter=5000)
         dt = time.perf_counter() - t0
         cpu = {"value": N * ro.iterations / dt / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
                "sample": f"oracle PCG-Jacobi (par_solver.rs semantics) with the reference's 4-way CSR row sum, "
-                         f"full solve, {ro.iterations} iterations ({dt:.2f} s)"}
+                         f"full solve, {ro.iterations} iterations ({dt:.2f} s); the oracle's assembly of the "
+                         f"same system (element loop, COO sort, row zeroing) took {1e3 * t_cpu_asm:.1f} ms",
+               "assembly_ms": round(1e3 * t_cpu_asm, 2)}
     t_med = float(np.median(times))
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
@@ -206,6 +203,9 @@ def run_csr(a, w):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": w["desc"].format(n=n, N=N, nnz=nnz), "name": a.workload, "n_dofs": N, "nnz": nnz,
                    "global_batch": 1, "parallelism": "single GPU"},
+        "assembly": {"gpu_ms": round(1e3 * t_asm, 2),
+                     "what": "P1 stiffness (element matrices, stable radix sort of the COO triplets, run sums), "
+                             "source load, row zeroing; mesh arrays from the host"},
         "solve": {"iterations": int(iters[-1]), "solve_time_ms": round(1e3 * t_med, 4),
                   "us_per_iteration": round(1e6 * t_med / max(iters[-1], 1), 2),
                   "rel_residual": res.final_residual, "max_nodal_error_vs_sin_sin": err},
@@ -219,23 +219,6 @@ def run_csr(a, w):
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-
-
-def device_copy_gbs(fg, ctx, mib=1024, reps=5):
-    """Bandwidth of the runtime's device-to-device copy of `mib` MiB (read +
-    write bytes over HIP-event time): the practical streaming ceiling that the
-    roofline fraction can be read against."""
-    nc = mib * (1 << 20) // 8
-    src = fg.DeviceArray(ctx, nc, np.float64)
-    dst = fg.DeviceArray(ctx, nc, np.float64)
-    dst.copy_from(src)
-    ctx.synchronize()
-    ctx.timer_start()
-    for _ in range(reps):
-        dst.copy_from(src)
-    ms = ctx.timer_stop() / reps
-    del src, dst
-    return 2.0 * nc * 8 / (ms * 1e-3) / 1e9
 
 
 def run_learned(a, w):

@@ -149,6 +149,25 @@ struct CsrOp final : femgpu_op {
     }
 };
 
+// lanes per row of the SpMV from the mean row length
+static int csr_lanes(uint64_t nrows, uint64_t nnz) {
+    const double avg = nrows ? (double)nnz / nrows : 0.0;
+    int sub = avg <= 16 ? 1 : avg <= 48 ? 8 : avg <= 128 ? 16 : avg <= 400 ? 32 : 64;
+    if (const char *f = std::getenv("FEMGPU_CSR_LANES")) sub = std::atoi(f);
+    if (sub != 1 && sub != 8 && sub != 16 && sub != 32) sub = 64;
+    return sub;
+}
+
+femgpu_op *csr_adopt(femgpu_ctx *c, uint64_t n, uint64_t nnz, DevBuf &&rp, DevBuf &&ci, DevBuf &&vals) {
+    auto op = new CsrOp();
+    op->ctx = c; op->dtype = FEMGPU_F64; op->n = n; op->ncols = n; op->nnz = nnz;
+    op->rp = std::move(rp);
+    op->ci = std::move(ci);
+    op->vals = std::move(vals);
+    op->sub = csr_lanes(n, nnz);
+    return op;
+}
+
 }  // namespace femgpu
 
 using namespace femgpu;
@@ -190,10 +209,7 @@ int femgpu_csr_create(femgpu_ctx *c, uint64_t nrows, uint64_t ncols, const uint6
                 FG_HIP(hipMemcpyAsync(op->vals.ptr, values, nnz * es, hipMemcpyHostToDevice, c->stream));
             }
             FG_HIP(hipStreamSynchronize(c->stream));
-            const double avg = nrows ? (double)nnz / nrows : 0.0;
-            op->sub = avg <= 16 ? 1 : avg <= 48 ? 8 : avg <= 128 ? 16 : avg <= 400 ? 32 : 64;
-            if (const char *f = std::getenv("FEMGPU_CSR_LANES")) op->sub = std::atoi(f);
-            if (op->sub != 1 && op->sub != 8 && op->sub != 16 && op->sub != 32) op->sub = 64;
+            op->sub = csr_lanes(nrows, nnz);
         } catch (...) {
             delete op;
             throw;
@@ -295,6 +311,23 @@ int femgpu_csr_get_values(femgpu_op *op_, void *values, femgpu_mem mem) {
         FG_HIP(hipSetDevice(c->device));
         FG_HIP(hipMemcpyAsync(values, op->vals.ptr, op->nnz * op->esize(),
                               mem == FEMGPU_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+int femgpu_csr_get_pattern(femgpu_op *op_, uint64_t *nnz, uint64_t *row_ptr, uint32_t *col_idx,
+                           femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<CsrOp *>(op_);
+        FG_REQUIRE(op && nnz, FEMGPU_ERR_INVALID_ARG, "csr_get_pattern: not a CSR operator");
+        *nnz = op->nnz;
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const hipMemcpyKind k = mem == FEMGPU_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        if (row_ptr) FG_HIP(hipMemcpyAsync(row_ptr, op->rp.ptr, (op->n + 1) * 8, k, c->stream));
+        if (col_idx && op->nnz) FG_HIP(hipMemcpyAsync(col_idx, op->ci.ptr, op->nnz * 4, k, c->stream));
         FG_HIP(hipStreamSynchronize(c->stream));
         return 0;
     });

@@ -86,6 +86,10 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// A square f64 CsrOp owning device arrays row_ptr u64[n+1], col_idx u32[nnz],
+// values f64[nnz] (csr.hip).
+femgpu_op *csr_adopt(femgpu_ctx *c, uint64_t n, uint64_t nnz, DevBuf &&rp, DevBuf &&ci, DevBuf &&vals);
+
 // Reduction partials: a fixed grid gives a deterministic summation order.
 constexpr int kRedBlocks = 1024;
 // Device scalar slots per context (and the pinned host mirror): 0..63 solver

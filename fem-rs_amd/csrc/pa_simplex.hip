@@ -14,6 +14,7 @@
 // hardware atomics.
 #include "basis.hpp"
 #include "internal.hpp"
+#include "simplex.hpp"
 
 #include <cmath>
 
@@ -94,58 +95,6 @@ static unsigned blocks_for(size_t n) {
     return (unsigned)(b ? b : 1);
 }
 
-// Host-side reference-element tables and per-element geometry.
-struct SimplexSetup {
-    int dim, order, nl, nq;
-    std::vector<double> tab;  // see kernel
-    std::vector<double> w, phi, grad;
-
-    SimplexSetup(int dim_, int order_, int quad_order) : dim(dim_), order(order_) {
-        FG_REQUIRE(dim == 2 || dim == 3, FEMGPU_ERR_INVALID_ARG, "simplex: dim must be 2 or 3");
-        FG_REQUIRE(order >= 1 && order <= 3, FEMGPU_ERR_UNSUPPORTED, "simplex: order must be 1..3");
-        nl = simplex_nlocal(dim, order);
-        double pts[120], wts[40];
-        nq = dim == 2 ? tri_rule(quad_order, pts, wts) : tet_rule(quad_order, pts, wts);
-        w.assign(wts, wts + nq);
-        phi.assign((size_t)nq * nl, 0.0);
-        grad.assign((size_t)nq * nl * dim, 0.0);
-        for (int q = 0; q < nq; ++q)
-            simplex_basis(dim, order, pts + dim * q, &phi[(size_t)q * nl], &grad[(size_t)q * nl * dim]);
-        tab = w;
-        tab.insert(tab.end(), phi.begin(), phi.end());
-        tab.insert(tab.end(), grad.begin(), grad.end());
-    }
-
-    // Affine map of ElementTransformation::from_simplex_nodes
-    // (mesh/src/transformation.rs:34-64): J columns x_k - x_0.
-    double jac(const double *coords, const uint32_t *nodes, double J[3][3], double I[3][3]) const {
-        for (int c = 0; c < dim; ++c)
-            for (int r = 0; r < dim; ++r)
-                J[r][c] = coords[dim * nodes[c + 1] + r] - coords[dim * nodes[0] + r];
-        double det;
-        if (dim == 2) {
-            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-            I[0][0] = J[1][1] / det; I[0][1] = -J[0][1] / det;
-            I[1][0] = -J[1][0] / det; I[1][1] = J[0][0] / det;
-        } else {
-            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-            const double id = 1.0 / det;
-            I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
-            I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-            I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-            I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-            I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-            I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-            I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-            I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-            I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-        }
-        return det;
-    }
-};
-
 struct SimplexOp final : femgpu_op {
     int dim = 2, form = FEMGPU_DIFFUSION, nl = 0, nq = 0;
     size_t ne = 0;
@@ -189,8 +138,8 @@ struct LumpedOp final : femgpu_op {
     }
 };
 
-static void check_mesh(int dim, int order, uint64_t n_nodes, const double *coords, uint64_t n_elem,
-                       const uint32_t *nodes, const uint32_t *edofs, uint64_t n_dofs, int nl) {
+void check_simplex_mesh(int dim, int order, uint64_t n_nodes, const double *coords, uint64_t n_elem,
+                        const uint32_t *nodes, const uint32_t *edofs, uint64_t n_dofs, int nl) {
     FG_REQUIRE(coords && nodes && edofs, FEMGPU_ERR_INVALID_ARG, "simplex: NULL mesh array");
     (void)order; (void)dim;
     for (uint64_t i = 0; i < n_elem * (uint64_t)(dim + 1); ++i)
@@ -224,7 +173,7 @@ int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes
                    "pa_create_simplex: form must be MASS or DIFFUSION");
         SimplexSetup S(dim, order, quad_order);
         FG_REQUIRE(S.nq <= kMaxQp, FEMGPU_ERR_UNSUPPORTED, "pa_create_simplex: too many points");
-        check_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
+        check_simplex_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         FG_HIP(hipSetDevice(c->device));
         auto *op = new SimplexOp();
@@ -291,7 +240,7 @@ int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_node
     return guard([&] {
         FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "lumped_mass_create: NULL argument");
         SimplexSetup S(dim, order, quad_order);
-        check_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
+        check_simplex_mesh(dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, S.nl);
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         FG_HIP(hipSetDevice(c->device));
         // LumpedMassOperator::assemble (partial.rs:228-262): M_ii = sum_q w|J| rho phi_i sum_j phi_j

@@ -115,6 +115,9 @@ _sig("femgpu_csr_create", [P, U64, U64, P, P, P, I, C.POINTER(P)])
 _sig("femgpu_csr_spmv_add", [P, D, P, D, P, I])
 _sig("femgpu_csr_apply_dirichlet", [P, U64, P, P, P, I])
 _sig("femgpu_csr_get_values", [P, P, I])
+_sig("femgpu_csr_get_pattern", [P, C.POINTER(U64), P, P, I])
+_sig("femgpu_csr_assemble_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
+_sig("femgpu_rhs_assemble_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, P, I])
 _sig("femgpu_pa_create_hex", [P, C.POINTER(_HexDesc), P, C.POINTER(P)])
 _sig("femgpu_hex_vertices", [I, I, I, D, U64, P])
 _sig("femgpu_pa_hex_rhs_unit", [P, P, I])
@@ -418,6 +421,65 @@ class CsrMatrix(Operator):
         out = np.empty(self._nnz, _npdt(self.dtype))
         _check(lib.femgpu_csr_get_values(self.h, out.ctypes.data, HOST))
         return out
+
+    def pattern(self):
+        """(row_ptr u64[n+1], col_idx u32[nnz]) as host arrays (csr.rs:81-93)."""
+        nnz = U64()
+        rp = np.empty(self._n + 1, np.uint64)
+        ci = np.empty(self._nnz, np.uint32)
+        _check(lib.femgpu_csr_get_pattern(self.h, C.byref(nnz), rp.ctypes.data, ci.ctypes.data, HOST))
+        return rp, ci
+
+    @classmethod
+    def _adopt(cls, ctx, h):
+        m = cls.__new__(cls)
+        Operator.__init__(m, ctx, h)
+        nnz = U64()
+        _check(lib.femgpu_csr_get_pattern(h, C.byref(nnz), None, None, HOST))
+        m._nnz = int(nnz.value)
+        return m
+
+
+def _mesh_args(dim, order, coords, elem_nodes, elem_dofs):
+    coords = np.ascontiguousarray(coords, np.float64).ravel()
+    nodes = np.ascontiguousarray(elem_nodes, np.uint32).ravel()
+    dofs = np.ascontiguousarray(elem_dofs, np.uint32).ravel()
+    ne = len(nodes) // (dim + 1)
+    return coords, nodes, dofs, ne
+
+
+def assemble_bilinear(dim, order, coords, elem_nodes, elem_dofs, n_dofs, form=DIFFUSION, coef=1.0,
+                      quad_order=3, ctx=None):
+    """Assembler::assemble_bilinear (assembly/src/assembler.rs:147-249) with the
+    diffusion or mass integrator and a constant coefficient, then
+    CooMatrix::into_csr, on the GPU; returns the device CsrMatrix."""
+    ctx = ctx or default_context()
+    coords, nodes, dofs, ne = _mesh_args(dim, order, coords, elem_nodes, elem_dofs)
+    h = P()
+    _check(lib.femgpu_csr_assemble_simplex(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data, ne,
+                                           nodes.ctypes.data, dofs.ctypes.data, n_dofs, form, coef,
+                                           quad_order, C.byref(h)))
+    return CsrMatrix._adopt(ctx, h)
+
+
+SOURCE_CONST, SOURCE_SINPROD = 0, 1
+
+
+def assemble_linear(dim, order, coords, elem_nodes, elem_dofs, n_dofs, source=SOURCE_CONST, value=1.0,
+                    quad_order=3, out=None, ctx=None):
+    """Assembler::assemble_linear with DomainSourceIntegrator
+    (assembly/src/standard/source.rs:44-50) on the GPU: f = value, or
+    value * prod_d sin(pi x_d) for SOURCE_SINPROD.  out: host array (default,
+    returned) or DeviceArray."""
+    ctx = ctx or default_context()
+    coords, nodes, dofs, ne = _mesh_args(dim, order, coords, elem_nodes, elem_dofs)
+    if out is None:
+        out = np.zeros(n_dofs)
+    ptr, m = _vec(out, F64, n_dofs, "b")
+    _check(lib.femgpu_rhs_assemble_simplex(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data, ne,
+                                           nodes.ctypes.data, dofs.ctypes.data, n_dofs, source, value,
+                                           quad_order, ptr, m))
+    return out
 
 
 def apply_dirichlet(mat, rhs, constrained_dofs, values):

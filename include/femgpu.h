@@ -116,6 +116,10 @@ int femgpu_csr_apply_dirichlet(femgpu_op *csr, uint64_t n_constrained, const uin
                                const double *values, void *rhs, femgpu_mem mem);
 /* Copy of the nnz matrix values (CsrMatrix::values). */
 int femgpu_csr_get_values(femgpu_op *csr, void *values, femgpu_mem mem);
+/* The pattern of a CSR operator, CsrMatrix::row_ptr / col_idx (linalg/src/csr.rs:81-93):
+ * nnz always; row_ptr u64[nrows + 1] and col_idx u32[nnz] when not NULL. */
+int femgpu_csr_get_pattern(femgpu_op *csr, uint64_t *nnz, uint64_t *row_ptr, uint32_t *col_idx,
+                           femgpu_mem mem);
 
 /* Coefficient for the hex PA operators (CoeffCtx/ScalarCoeff,
  * assembly/src/coefficient.rs:67-132, evaluated once at set-up into the
@@ -182,6 +186,25 @@ int femgpu_pa_create_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nod
                              const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
                              const uint32_t *elem_dofs, uint64_t n_dofs, int form, double coef,
                              int quad_order, femgpu_op **out);
+/* Assembler::assemble_bilinear (assembly/src/assembler.rs:147-249) with
+ * DiffusionIntegrator (form FEMGPU_DIFFUSION, standard/diffusion.rs:34-55) or
+ * MassIntegrator (FEMGPU_MASS, standard/mass.rs:28-40), constant coefficient,
+ * followed by CooMatrix::into_csr (linalg/src/coo.rs:76-), on the device:
+ * returns a CsrMatrix<f64> operator (columns sorted within rows, duplicates
+ * summed in element order).  Mesh arguments as femgpu_pa_create_simplex. */
+int femgpu_csr_assemble_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                                const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                                const uint32_t *elem_dofs, uint64_t n_dofs, int form, double coef,
+                                int quad_order, femgpu_op **out);
+/* Assembler::assemble_linear with DomainSourceIntegrator
+ * (assembly/src/standard/source.rs:44-50): b_i = sum_e sum_q w |det J| f(x_q)
+ * phi_i(x_q), f = value (FEMGPU_SOURCE_CONST) or value * prod_d sin(pi x_d)
+ * (FEMGPU_SOURCE_SINPROD; config 1 takes value = 2 pi^2).  b: f64[n_dofs]. */
+typedef enum { FEMGPU_SOURCE_CONST = 0, FEMGPU_SOURCE_SINPROD = 1 } femgpu_source_kind;
+int femgpu_rhs_assemble_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                                const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                                const uint32_t *elem_dofs, uint64_t n_dofs, int source, double value,
+                                int quad_order, void *b, femgpu_mem mem);
 /* LumpedMassOperator::assemble (partial.rs:217-262): y = diag(m) x. */
 int femgpu_lumped_mass_create(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
                               const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
