@@ -221,6 +221,23 @@ def run_csr(a, w):
     print(json.dumps(out))
 
 
+def device_copy_gbs(fg, ctx, mib=1024, reps=5):
+    """Bandwidth of the runtime's device-to-device copy of `mib` MiB (read +
+    write bytes over HIP-event time): the practical streaming ceiling that the
+    roofline fraction can be read against."""
+    nc = mib * (1 << 20) // 8
+    src = fg.DeviceArray(ctx, nc, np.float64)
+    dst = fg.DeviceArray(ctx, nc, np.float64)
+    dst.copy_from(src)
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(reps):
+        dst.copy_from(src)
+    ms = ctx.timer_stop() / reps
+    del src, dst
+    return 2.0 * nc * 8 / (ms * 1e-3) / 1e9
+
+
 def run_learned(a, w):
     """Learned preconditioner on held-out fields of the Q3 GRF family (a
     reduced form of the 256-field training set: --train-fields fields)."""
@@ -363,7 +380,7 @@ def cpu_baseline(n, p, iters, lmax):
                       f"problem, {res.iterations} iterations ({dt:.2f} s)"}
 
 
-def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, rtol, a):
+def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, rtol, a, small_op=None):
     from femgpu import learned
     N = K.n_dofs()
     x0 = fg.DeviceArray(ctx, N, np.float64)
@@ -412,17 +429,29 @@ def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, rtol, a):
            "mixed_fp32_ir": mixed, "mixed_fp32_ir_restarted": mixed_r, "reference_rtol": ref_rtol}
     if a.no_learned:
         return out
-    # learned preconditioner: pairs from two other right-hand sides, energy-norm training
+    # learned preconditioner: pairs from two other right-hand sides, energy-norm training.
+    # Above 10M DOFs the network is trained on a 4x coarser mesh of the same family
+    # (the convolutions are resolution-independent) and its weights are moved over.
     t0 = time.perf_counter()
+    Kt = small_op() if big and small_op else K
+    lmax_t = lmax if Kt is K else fg.solve_pcg_chebyshev(Kt, Kt.rhs_random(101), np.zeros(Kt.n_dofs()),
+                                                         fg.SolverConfig(rtol=1e-8, max_iter=1))[1]
     pairs = []
     for sd in (101, 102):
-        pairs += learned.harvest_pairs(K, K.rhs_random(sd), iters=(1, 3, 6, 10, 20, 35, 60), lmax=lmax)
-    net = fg.UNet.for_operator(K, seed=11)
+        pairs += learned.harvest_pairs(Kt, Kt.rhs_random(sd), iters=(1, 3, 6, 10, 20, 35, 60), lmax=lmax_t)
+    net_t = fg.UNet.for_operator(Kt, seed=11)
     rng = np.random.default_rng(0)
     losses = []
     for st in range(a.train_steps):
         r_, e_ = pairs[rng.integers(len(pairs))]
-        losses.append(net.train_step_energy(K, r_, e_, 2e-3 * 0.1 ** (st / max(a.train_steps, 1))))
+        losses.append(net_t.train_step_energy(Kt, r_, e_, 2e-3 * 0.1 ** (st / max(a.train_steps, 1))))
+    if Kt is K:
+        net = net_t
+    else:
+        net = fg.UNet.for_operator(K, seed=11)
+        net.set_params(net_t.get_params())
+        net_t.close()
+        Kt.close()
     t_train = time.perf_counter() - t0
     print(f"bench.py: trained {a.train_steps} steps in {t_train:.1f} s", file=sys.stderr, flush=True)
     (res_l, inner_l, info_l), t_l = timed(
@@ -435,6 +464,7 @@ def legs(fg, ctx, K, b, x, x_plain, lmax, t_plain, it_plain, rtol, a):
           "rel_residual_fp64": relres(xl), "rel_error": err(xl),
           "speedup_vs_plain_fp64": round(t_plain / t_l, 3),
           "train": {"pairs": len(pairs), "steps": a.train_steps, "time_s": round(t_train, 2),
+                    "trained_on": "the operator solved" if Kt is K else "a 4x coarser mesh of the same family",
                     "energy_loss_first_last": [round(float(np.mean(losses[:20])), 4),
                                                round(float(np.mean(losses[-20:])), 4)]}}
     out["mixed_learned"] = ml
@@ -541,7 +571,8 @@ def main():
     ms_step = 1e3 * total / a.steps
     value = N * sum(iters) / total / 1e9
     t_plain = float(np.median(times))
-    leg = None if a.no_legs else legs(fg, ctx, K, b, x, xs, lmax, t_plain, iters[-1], w["rtol"], a)
+    leg = None if a.no_legs else legs(fg, ctx, K, b, x, xs, lmax, t_plain, iters[-1], w["rtol"], a,
+                                      small_op=lambda: make_operator(fg, ctx, w, max(n // 4, 8), p))
     cpu = None
     if not a.no_cpu and a.workload == "q2-64-sinprod":
         cpu = cpu_baseline(n, p, a.cpu_iters, lmax)
