@@ -16,6 +16,7 @@
 #include "internal.hpp"
 #include "simplex.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace femgpu {
@@ -287,6 +288,135 @@ int femgpu_lumped_mass_apply_inverse(femgpu_op *op_, const void *x, void *y, fem
         op->run<double>((const double *)xd, (double *)yd, 2, nullptr);
         FG_HIP(hipGetLastError());
         if (mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(y, yd, bytes, hipMemcpyDeviceToHost, c->stream));
+        FG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------- essential-DOF operator wrapper --
+// The matrix-free essential-BC pattern of examples/src/maxwell.rs:142-197:
+// y = A x0 with x0 = x with the constrained entries zeroed, then y_d = x_d on
+// each constrained DOF d (identity rows, eliminated columns: the operator
+// stays symmetric).  The diagonal is A's with 1 on the constrained DOFs.
+namespace femgpu {
+
+template <class T>
+__global__ void k_zero_rows(size_t m, const uint32_t *__restrict__ dofs, T *__restrict__ v, const int *skip) {
+    if (skip && *skip) return;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < m) v[dofs[i]] = T(0);
+}
+
+template <class T>
+__global__ void k_set_rows(size_t m, const uint32_t *__restrict__ dofs, const T *__restrict__ x, T *__restrict__ y,
+                           int one, const int *skip) {
+    if (skip && *skip) return;
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < m) y[dofs[i]] = one ? T(1) : x[dofs[i]];
+}
+
+template <class T>
+__global__ void k_axpy1(size_t n, const T *__restrict__ t, T *__restrict__ y, const int *skip) {
+    if (skip && *skip) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] += t[i];
+}
+
+struct EssentialOp final : femgpu_op {
+    femgpu_op *inner = nullptr;  // borrowed
+    size_t m = 0;
+    DevBuf dofs, xw, tw;
+
+    const char *name() const override { return "essential"; }
+
+    template <class T> void apply_t(const T *x, T *y, bool acc, const int *skip) {
+        hipStream_t s = ctx->stream;
+        T *w = xw.as<T>();
+        FG_HIP(hipMemcpyAsync(w, x, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (m)
+            hipLaunchKernelGGL(k_zero_rows<T>, dim3(blocks_for(m)), dim3(256), 0, s, m, dofs.as<uint32_t>(), w, skip);
+        T *out = acc ? tw.as<T>() : y;
+        inner->apply_dev(w, out, false, skip);
+        if (m)
+            hipLaunchKernelGGL(k_set_rows<T>, dim3(blocks_for(m)), dim3(256), 0, s, m, dofs.as<uint32_t>(), x, out,
+                               0, skip);
+        if (acc) hipLaunchKernelGGL(k_axpy1<T>, dim3(blocks_for(n)), dim3(256), 0, s, (size_t)n, (const T *)out, y, skip);
+    }
+    void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
+        if (dtype == FEMGPU_F64) apply_t<double>((const double *)x, (double *)y, acc, skip);
+        else apply_t<float>((const float *)x, (float *)y, acc, skip);
+    }
+    void diagonal_dev(void *d) override {
+        inner->diagonal_dev(d);
+        if (!m) return;
+        if (dtype == FEMGPU_F64)
+            hipLaunchKernelGGL(k_set_rows<double>, dim3(blocks_for(m)), dim3(256), 0, ctx->stream, m,
+                               dofs.as<uint32_t>(), (const double *)nullptr, (double *)d, 1, nullptr);
+        else
+            hipLaunchKernelGGL(k_set_rows<float>, dim3(blocks_for(m)), dim3(256), 0, ctx->stream, m,
+                               dofs.as<uint32_t>(), (const float *)nullptr, (float *)d, 1, nullptr);
+    }
+};
+
+}  // namespace femgpu
+
+extern "C" {
+
+int femgpu_essential_create(femgpu_op *inner, uint64_t n_constrained, const uint32_t *dofs, femgpu_op **out) {
+    return guard([&] {
+        FG_REQUIRE(inner && out && (dofs || n_constrained == 0), FEMGPU_ERR_INVALID_ARG,
+                   "essential_create: NULL argument");
+        std::vector<uint32_t> d(dofs, dofs + n_constrained);
+        std::sort(d.begin(), d.end());
+        d.erase(std::unique(d.begin(), d.end()), d.end());
+        for (uint32_t v : d)
+            FG_REQUIRE(v < inner->n, FEMGPU_ERR_INVALID_ARG,
+                       "essential_create: constrained DOF " + std::to_string(v) + " out of range");
+        femgpu_ctx *c = inner->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        auto *op = new EssentialOp();
+        try {
+            op->ctx = c; op->dtype = inner->dtype; op->n = inner->n; op->inner = inner; op->m = d.size();
+            op->dofs.alloc(d.size() * 4);
+            op->xw.alloc(op->n * op->esize());
+            op->tw.alloc(op->n * op->esize());
+            if (!d.empty())
+                FG_HIP(hipMemcpyAsync(op->dofs.ptr, d.data(), d.size() * 4, hipMemcpyHostToDevice, c->stream));
+            FG_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+        return 0;
+    });
+}
+
+int femgpu_essential_mask_rhs(femgpu_op *op_, void *b, femgpu_mem mem) {
+    return guard([&] {
+        auto *op = dynamic_cast<EssentialOp *>(op_);
+        FG_REQUIRE(op && b, FEMGPU_ERR_INVALID_ARG, "essential_mask_rhs: not an essential-DOF operator");
+        femgpu_ctx *c = op->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        const size_t bytes = op->n * op->esize();
+        void *bd = b;
+        if (mem == FEMGPU_HOST) {
+            bd = op->tw.ptr;
+            FG_HIP(hipMemcpyAsync(bd, b, bytes, hipMemcpyHostToDevice, c->stream));
+        }
+        if (op->m) {
+            if (op->dtype == FEMGPU_F64)
+                hipLaunchKernelGGL(k_zero_rows<double>, dim3(blocks_for(op->m)), dim3(256), 0, c->stream, op->m,
+                                   op->dofs.as<uint32_t>(), (double *)bd, nullptr);
+            else
+                hipLaunchKernelGGL(k_zero_rows<float>, dim3(blocks_for(op->m)), dim3(256), 0, c->stream, op->m,
+                                   op->dofs.as<uint32_t>(), (float *)bd, nullptr);
+        }
+        if (mem == FEMGPU_HOST) FG_HIP(hipMemcpyAsync(b, bd, bytes, hipMemcpyDeviceToHost, c->stream));
         FG_HIP(hipStreamSynchronize(c->stream));
         return 0;
     });

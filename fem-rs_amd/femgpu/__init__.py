@@ -126,6 +126,8 @@ _sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
 _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
+_sig("femgpu_essential_create", [P, U64, P, C.POINTER(P)])
+_sig("femgpu_essential_mask_rhs", [P, P, I])
 _sig("femgpu_solve_cg_operator", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
 _sig("femgpu_solve_pcg_jacobi", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
 _sig("femgpu_solve_pcg_chebyshev", [P, C.POINTER(_ChebConfig), P, P, I, C.POINTER(_SolverConfig),
@@ -593,6 +595,25 @@ class PADiffusionOperator(Operator):
         h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
                             elem_dofs, n_dofs, DIFFUSION, float(kappa), quad_order)
         super().__init__(ctx, h)
+
+
+class EssentialBcOperator(Operator):
+    """The matrix-free essential-BC operator of examples/src/maxwell.rs:142-197:
+    A applied to x with the constrained entries zeroed, identity rows on the
+    constrained DOFs.  Keeps a reference to `inner`."""
+
+    def __init__(self, inner, constrained_dofs):
+        d = np.ascontiguousarray(constrained_dofs, np.uint32)
+        h = P()
+        _check(lib.femgpu_essential_create(inner.h, len(d), d.ctypes.data, C.byref(h)))
+        super().__init__(inner.ctx, h)
+        self.inner = inner
+
+    def mask_rhs(self, b):
+        """b with its constrained entries set to zero (in place; host array or DeviceArray)."""
+        ptr, m = _vec(b, self.dtype, self._n, "b")
+        _check(lib.femgpu_essential_mask_rhs(self.h, ptr, m))
+        return b
 
 
 class LumpedMassOperator(Operator):

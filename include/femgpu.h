@@ -205,6 +205,14 @@ int femgpu_rhs_assemble_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_
                                 const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
                                 const uint32_t *elem_dofs, uint64_t n_dofs, int source, double value,
                                 int quad_order, void *b, femgpu_mem mem);
+/* The matrix-free essential-BC pattern of examples/src/maxwell.rs:142-197 as a
+ * reusable operator: y = A x0 (x0 = x with the constrained entries zeroed),
+ * then y_d = x_d on every constrained DOF; diagonal 1 there.  The wrapper
+ * borrows `inner` (which must outlive it; destroy with femgpu_op_destroy).
+ * femgpu_essential_mask_rhs zeroes b on the constrained DOFs. */
+int femgpu_essential_create(femgpu_op *inner, uint64_t n_constrained, const uint32_t *dofs,
+                            femgpu_op **out);
+int femgpu_essential_mask_rhs(femgpu_op *op, void *b, femgpu_mem mem);
 /* LumpedMassOperator::assemble (partial.rs:217-262): y = diag(m) x. */
 int femgpu_lumped_mass_create(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
                               const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,

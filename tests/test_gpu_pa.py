@@ -190,3 +190,42 @@ def test_lumped_mass(fg, fr, ctx):
     y = np.zeros(nd)
     lm.apply_inverse(x, y)
     assert _rel(y, x / ref) < 1e-14
+
+
+def test_essential_bc_operator(fg, fr, ctx):
+    """The matrix-free essential-BC operator (examples/src/maxwell.rs:142-197)
+    around a P2 triangle PA diffusion operator: apply = A applied to x with the
+    constrained entries zeroed, identity rows there (also accumulating);
+    diagonal 1 there; CG on it solves the free block of the assembled matrix
+    with zero constrained values."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    coords, conn, nd, ed = _tri_mesh(fr, 8, 2)
+    K = fg.PADiffusionOperator(2, 2, coords, conn, ed, nd, 1.0, 5, ctx=ctx)
+    rng = np.random.default_rng(5)
+    cons = np.unique(rng.choice(nd, nd // 8, replace=False)).astype(np.uint32)
+    Kb = fg.EssentialBcOperator(K, np.concatenate([cons, cons[:3]]))  # repeats are ignored
+    rp, ci, v = fr.simplex_assemble_csr(2, 2, coords, conn, ed, nd, 2, 1.0, 5)
+    A = sp.csr_matrix((v, ci.astype(np.int64), rp.astype(np.int64)), shape=(nd, nd))
+    x = rng.standard_normal(nd)
+    xm = x.copy(); xm[cons] = 0.0
+    want = A @ xm; want[cons] = x[cons]
+    assert _rel(Kb.matvec(x), want) < 1e-12
+    y0 = rng.standard_normal(nd)
+    y = y0.copy()
+    Kb.apply(x, y)
+    assert _rel(y, y0 + want) < 1e-12
+    dg = A.diagonal().copy(); dg[cons] = 1.0
+    assert _rel(Kb.diagonal(), dg) < 1e-12
+    b = rng.standard_normal(nd)
+    Kb.mask_rhs(b)
+    assert not np.any(b[cons])
+    sol = np.zeros(nd)
+    res = fg.solve_cg_operator(nd, nd, Kb, b, sol, fg.SolverConfig(rtol=1e-12, max_iter=5000))
+    assert res.converged
+    free = np.setdiff1d(np.arange(nd), cons)
+    xf = spl.spsolve(A[free][:, free].tocsc(), b[free])
+    assert np.abs(sol[free] - xf).max() < 1e-9 * np.abs(xf).max()
+    assert not np.any(sol[cons])
+    with pytest.raises(fg.FemGpuError):
+        fg.EssentialBcOperator(K, [nd])
