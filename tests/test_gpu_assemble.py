@@ -100,3 +100,17 @@ def test_assemble_errors(fg, fr, ctx):
         fg.assemble_bilinear(2, 1, coords, bad, bad, 9, fg.DIFFUSION, ctx=ctx)
     with pytest.raises(fg.FemGpuError):
         fg.assemble_bilinear(2, 1, coords, conn, conn, 9, fg.DIFF_MASS, ctx=ctx)
+
+
+def test_matrix_market_of_device_matrix(fg, fr, ctx, tmp_path):
+    """A device-assembled CsrMatrix written as Matrix Market and read back
+    (io/src/matrix_market.rs round trip) is the same matrix, bit for bit."""
+    from femgpu import io as fio
+    coords, conn = fr.unit_square_tri(8)
+    A = fg.assemble_bilinear(2, 1, coords, conn, conn, 81, fg.DIFFUSION, 1.0, 3, ctx=ctx)
+    p = tmp_path / "k.mtx"
+    fio.write_matrix_market(p, A)
+    rp, ci, v, shape = fio.read_matrix_market(p)
+    rpg, cig = A.pattern()
+    assert shape == (81, 81)
+    assert np.array_equal(rp, rpg) and np.array_equal(ci, cig) and np.array_equal(v, A.values())
