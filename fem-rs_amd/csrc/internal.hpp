@@ -86,6 +86,10 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// amg.hip: one V-cycle z = V r on the AMG's stream; its level-0 operator.
+void amg_apply(const femgpu_amg *h, const double *r, double *z, const int *skip);
+femgpu_op *amg_operator(const femgpu_amg *h);
+
 // A square f64 CsrOp owning device arrays row_ptr u64[n+1], col_idx u32[nnz],
 // values f64[nnz] (csr.hip).
 femgpu_op *csr_adopt(femgpu_ctx *c, uint64_t n, uint64_t nnz, DevBuf &&rp, DevBuf &&ci, DevBuf &&vals);

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 
 namespace femgpu {
@@ -32,7 +33,7 @@ enum Scal {
 };
 enum Status { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2 };
 enum Mode { MODE_CG = 0, MODE_PCG = 1 };
-enum Prec { PREC_NONE = 0, PREC_JACOBI = 1, PREC_CHEB = 2 };
+enum Prec { PREC_NONE = 0, PREC_JACOBI = 1, PREC_CHEB = 2, PREC_CB = 3 };  // CB: the pc callback
 
 static inline unsigned nblk(size_t n) { return (unsigned)red_blocks(n); }
 
@@ -686,6 +687,8 @@ template <class T> struct Krylov {
     double *part;
     // workspace slots
     enum { W_R = 0, W_Z, W_P, W_AP, W_INV, W_D, W_T, W_DIAG, W_X, W_B, W_COUNT };
+    // preconditioner for PREC_CB: z = M^-1 r (kernels may honour the stop flag)
+    std::function<void(const T *, T *)> pc;
     T *vec(int slot) { return (T *)op->wsvec(slot, n * sizeof(T)); }
 
     explicit Krylov(femgpu_op *o) : op(o), c(o->ctx), n(o->n), s(o->ctx->stream) {
@@ -778,6 +781,11 @@ template <class T> struct Krylov {
             cheb_apply(cp, r, inv, z, kFcRz[0]);
             hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s,
                                (const double *)(part + (size_t)kFcRz[0] * kRedBlocks), (int)nb, S, (int)S_RZ);
+            copy<T>(s, p, z, n);
+        } else if (prec == PREC_CB) {
+            pc(r, z);
+            hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)r, (const T *)z, part, 0, flags);
+            hipLaunchKernelGGL(k_fin_store, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S, (int)S_RZ);
             copy<T>(s, p, z, n);
         }
         return run(mode, prec, cp, x, inv, rtol, atol, max_iter, res, false);
@@ -915,7 +923,7 @@ template <class T> struct Krylov {
                 hipLaunchKernelGGL((k_update<T, false>), dim3(nb), dim3(256), 0, s, n, (const double *)S,
                                    x, (const T *)p, r, (const T *)ap, inv, z, part, flags);
             hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
-                               flags, mode, prec == PREC_JACOBI ? 1 : 0, prec != PREC_CHEB ? 1 : 0,
+                               flags, mode, prec == PREC_JACOBI ? 1 : 0, prec == PREC_NONE || prec == PREC_JACOBI,
                                rtol, atol, (double)(it + 1));
             if (segment && it + 1 == max_iter) {
                 ++it;
@@ -923,6 +931,12 @@ template <class T> struct Krylov {
             }
             if (prec == PREC_CHEB) {
                 cheb_apply(cp, r, inv, z);
+                hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
+                                   flags);
+            } else if (prec == PREC_CB) {
+                pc(r, z);
+                hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)r, (const T *)z, part, 0,
+                                   flags);
                 hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                                    flags);
             }
@@ -1244,6 +1258,27 @@ int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem me
             return k.solve(MODE_PCG, PREC_JACOBI, ChebParams{}, (const T *)st.b, (T *)st.x, cfg->rtol,
                            cfg->atol, cfg->max_iter, res);
         });
+        st.finish();
+        if (rc == FEMGPU_ERR_NOT_CONVERGED)
+            return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +
+                                " iterations (residual = " + std::to_string(res->final_residual) + ")");
+        return rc;
+    });
+}
+
+int femgpu_solve_amg_cg(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem,
+                        const femgpu_solver_config *cfg, femgpu_solve_result *res) {
+    return guard([&] {
+        FG_REQUIRE(amg && b && x && res, FEMGPU_ERR_INVALID_ARG, "solve_amg_cg: NULL argument");
+        check_cfg(cfg);
+        femgpu_op *op = amg_operator(amg);
+        std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
+        FG_HIP(hipSetDevice(op->ctx->device));
+        Staged st(op, b, x, mem, 8, 9);
+        Krylov<double> k(op);
+        k.pc = [&](const double *r, double *z) { amg_apply(amg, r, z, k.flags); };
+        const int rc = k.solve(MODE_PCG, PREC_CB, ChebParams{}, (const double *)st.b, (double *)st.x, cfg->rtol,
+                               cfg->atol, cfg->max_iter, res);
         st.finish();
         if (rc == FEMGPU_ERR_NOT_CONVERGED)
             return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +

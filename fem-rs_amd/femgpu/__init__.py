@@ -88,6 +88,11 @@ class _IrConfig(C.Structure):
                 ("cheb", _ChebConfig), ("mode", I), ("update_interval", I)]
 
 
+class _AmgConfig(C.Structure):
+    _fields_ = [("theta", D), ("max_levels", I), ("coarse_max", U64), ("smoother_degree", I),
+                ("smoother_ratio", D)]
+
+
 def _sig(name, args, res=I):
     f = getattr(lib, name)
     f.argtypes = args
@@ -127,6 +132,12 @@ _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.PO
 _sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
 _sig("femgpu_essential_create", [P, U64, P, C.POINTER(P)])
+_sig("femgpu_amg_create", [P, P, C.POINTER(P)])
+_sig("femgpu_amg_destroy", [P])
+_sig("femgpu_amg_n_levels", [P, C.POINTER(I)])
+_sig("femgpu_amg_level", [P, I, I, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), P, P, P])
+_sig("femgpu_amg_vcycle", [P, P, P, I])
+_sig("femgpu_solve_amg_cg", [P, P, P, I, P, P])
 _sig("femgpu_essential_mask_rhs", [P, P, I])
 _sig("femgpu_solve_cg_operator", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
 _sig("femgpu_solve_pcg_jacobi", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
@@ -711,6 +722,85 @@ def solve_pcg_jacobi(a, b, x, cfg=None):
         return _result(r)  # par_solve_pcg_jacobi: Ok(converged = false)
     _check(rc)
     return _result(r)
+
+
+class AmgSolver:
+    """AmgSolver (amg/src/lib.rs:157-207): a smoothed-aggregation hierarchy
+    built once (host) for the device CsrMatrix `a`, then PCG with one V-cycle
+    per iteration on the device.  Keeps a reference to `a`."""
+
+    def __init__(self, a, theta=0.08, max_levels=10, coarse_max=400, smoother_degree=2, smoother_ratio=10.0):
+        cfg = _AmgConfig(theta, max_levels, coarse_max, smoother_degree, smoother_ratio)
+        h = P()
+        _check(lib.femgpu_amg_create(a.h, C.byref(cfg), C.byref(h)))
+        self.h, self.a, self.ctx = h, a, a.ctx
+
+    @classmethod
+    def setup(cls, a, **config):
+        return cls(a, **config)
+
+    def n_levels(self):
+        n = I()
+        _check(lib.femgpu_amg_n_levels(self.h, C.byref(n)))
+        return n.value
+
+    def level(self, level, which="A"):
+        """(row_ptr, col_idx, values, (nrows, ncols)) of a level's operator
+        ("A") or prolongator to the next level ("P")."""
+        w = 0 if which == "A" else 1
+        nr, nc, nz = U64(), U64(), U64()
+        _check(lib.femgpu_amg_level(self.h, level, w, C.byref(nr), C.byref(nc), C.byref(nz), None, None, None))
+        rp = np.empty(nr.value + 1, np.uint64)
+        ci = np.empty(nz.value, np.uint32)
+        v = np.empty(nz.value)
+        _check(lib.femgpu_amg_level(self.h, level, w, C.byref(nr), C.byref(nc), C.byref(nz), rp.ctypes.data,
+                                    ci.ctypes.data, v.ctypes.data))
+        return rp, ci, v, (nr.value, nc.value)
+
+    def vcycle(self, b, x=None):
+        """x = one V-cycle applied to b (host arrays or DeviceArrays)."""
+        n = self.a.n_dofs()
+        if x is None:
+            x = np.zeros(n)
+        bp, m = _vec(b, F64, n, "b")
+        xp, m2 = _vec(x, F64, n, "x")
+        if m != m2:
+            raise TypeError("b and x must both be host arrays or both DeviceArrays")
+        _check(lib.femgpu_amg_vcycle(self.h, bp, xp, m))
+        return x
+
+    def solve(self, b, x, cfg=None):
+        """PCG with the V-cycle preconditioner; stopping rule of
+        par_solve_pcg_jacobi (converged = False at max_iter)."""
+        cfg = cfg or SolverConfig()
+        _check_dims(self.a, b, x)
+        bp, xp, m = _solve_vectors(self.a, b, x)
+        r = _SolveResult()
+        rc = lib.femgpu_solve_amg_cg(self.h, bp, xp, m, C.byref(cfg._c()), C.byref(r))
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _check(rc)
+        return _result(r)
+
+    def close(self):
+        if getattr(self, "h", None):
+            if not _EXITING:
+                lib.femgpu_amg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_amg_cg(a, b, x, cfg=None, **amg_config):
+    """solve_amg_cg (amg/src/lib.rs:137-155): build the hierarchy, then AMG-PCG."""
+    s = AmgSolver(a, **amg_config)
+    try:
+        return s.solve(b, x, cfg)
+    finally:
+        s.close()
 
 
 def solve_pcg_chebyshev(a, b, x, cfg=None, degree=4, ratio=30.0, safety=1.1, power_iters=20,

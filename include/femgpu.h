@@ -221,6 +221,34 @@ int femgpu_lumped_mass_create(femgpu_ctx *ctx, int dim, int order, uint64_t n_no
 /* LumpedMassOperator::apply_inverse (partial.rs:269-273). */
 int femgpu_lumped_mass_apply_inverse(femgpu_op *op, const void *x, void *y, femgpu_mem mem);
 
+/* ---------------------------------------------------------------- AMG ---- */
+/* AmgSolver::setup / solve and solve_amg_cg (amg/src/lib.rs:125-207).  The
+ * reference builds linger's AmgHierarchy (absent); this is smoothed
+ * aggregation: strength threshold theta (default 0.08), at most max_levels
+ * levels (10), coarsening until at most coarse_max rows (400, inverted dense),
+ * Chebyshev-Jacobi smoothing of degree smoother_degree (2) on
+ * [lambda_max / smoother_ratio, lambda_max] (ratio 10).  Zero fields take the
+ * defaults.  The hierarchy is built on the host from an f64 CSR operator
+ * (which must outlive it); V-cycles and PCG run on the device. */
+typedef struct femgpu_amg femgpu_amg;
+typedef struct {
+    double theta;
+    int max_levels;
+    uint64_t coarse_max;
+    int smoother_degree;
+    double smoother_ratio;
+} femgpu_amg_config;
+int femgpu_amg_create(femgpu_op *csr, const femgpu_amg_config *cfg, femgpu_amg **out);
+int femgpu_amg_destroy(femgpu_amg *amg);
+/* AmgSolver::n_levels */
+int femgpu_amg_n_levels(const femgpu_amg *amg, int *n);
+/* Level `level`'s operator (which = 0) or prolongator to the next level
+ * (which = 1) as host CSR; arrays may be NULL to query the sizes. */
+int femgpu_amg_level(const femgpu_amg *amg, int level, int which, uint64_t *nrows, uint64_t *ncols,
+                     uint64_t *nnz, uint64_t *row_ptr, uint32_t *col_idx, double *values);
+/* x = one V-cycle applied to b (from x = 0): the preconditioner. */
+int femgpu_amg_vcycle(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem);
+
 /* ------------------------------------------------------------ solvers ---- */
 /* SolverConfig (solver/src/lib.rs:97-113): defaults rtol 1e-8, atol 0,
  * max_iter 1000. */
@@ -254,6 +282,11 @@ int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem m
  * Works for any operator (CSR or PA).  NOT_CONVERGED on max_iter. */
 int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
                             const femgpu_solver_config *cfg, femgpu_solve_result *res);
+/* AmgSolver::solve (amg/src/lib.rs:188-206): PCG with one V-cycle as the
+ * preconditioner, the stopping rule of par_solve_pcg_jacobi (relative
+ * residual; converged = 0 at max_iter), on the AMG's operator. */
+int femgpu_solve_amg_cg(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem,
+                        const femgpu_solver_config *cfg, femgpu_solve_result *res);
 /* PCG as above with a textbook Chebyshev-Jacobi polynomial preconditioner
  * z = p(D^-1 A) D^-1 r of the given degree on [lmax/ratio, lmax], lmax =
  * safety * (power-iteration estimate of the largest eigenvalue of D^-1 A,
