@@ -15,8 +15,11 @@
 //  - P = (I - omega D^-1 A) P_tent, omega = 4 / (3 lambda_max(D^-1 A));
 //  - A_c = P^T A P, until at most coarse_max rows remain; the coarsest level
 //    is inverted (dense) once.
+// A cycle is a V-cycle (one coarse correction per level), a W-cycle (two,
+// the second on the residual the first leaves) or an F-cycle (an F-cycle then
+// a V-cycle on the coarse level).
 // Every product runs row by row with a sparse accumulator and sorted columns,
-// so the hierarchy is deterministic.  The V-cycle runs on the device: per
+// so the hierarchy is deterministic.  The cycle runs on the device: per
 // level a Chebyshev-Jacobi polynomial smoother (the same polynomial before and
 // after the coarse correction, so the cycle is symmetric), residual SpMV,
 // restriction by R = P^T, prolongation by P, and a dense solve at the bottom.
@@ -248,6 +251,7 @@ struct AmgLevel {
     HCsr A, P;  // host copies (P: this level -> next)
     DCsr dA, dP, dR;
     DevBuf inv, b, x, r, t, d, z;
+    DevBuf b2, x2;  // right-hand side and result of a second coarse-level visit (W, F)
     double lmax = 0.0;
 };
 
@@ -257,6 +261,7 @@ struct femgpu_amg {
     femgpu_ctx *ctx = nullptr;
     femgpu_op *op = nullptr;  // borrowed level-0 operator
     femgpu_amg_config cfg{};
+    int cycle = FEMGPU_CYCLE_V;
     std::vector<femgpu::AmgLevel> L;
     femgpu::DevBuf coarse_inv;  // dense inverse of the coarsest operator
 };
@@ -338,7 +343,8 @@ static void amg_build(femgpu_amg *h, HCsr A0) {
         }
         lv.inv.alloc(std::max<size_t>(n, 1) * 8);
         if (n) FG_HIP(hipMemcpyAsync(lv.inv.ptr, inv.data(), n * 8, hipMemcpyHostToDevice, s));
-        for (DevBuf *b : {&lv.b, &lv.x, &lv.r, &lv.t, &lv.d, &lv.z}) b->alloc(std::max<size_t>(n, 1) * 8);
+        for (DevBuf *b : {&lv.b, &lv.x, &lv.r, &lv.t, &lv.d, &lv.z, &lv.b2, &lv.x2})
+            b->alloc(std::max<size_t>(n, 1) * 8);
         if (l + 1 < h->L.size()) {
             lv.dP.upload(lv.P, s);
             lv.dR.upload(transpose(lv.P), s);
@@ -396,8 +402,11 @@ static void amg_smooth(const femgpu_amg *h, const AmgLevel &lv, const double *r,
     }
 }
 
-// x = V(l) b
-static void amg_vcycle(const femgpu_amg *h, size_t l, const double *b, double *x, hipStream_t s, const int *skip) {
+// x = C(l) b for the cycle kind (V, W, F).  A visit of level l uses that
+// level's r, z, t, d as scratch and the next level's b, x (first coarse
+// visit) and b2, x2 (second), so b and x must not be this level's scratch.
+static void amg_cycle(const femgpu_amg *h, size_t l, int kind, const double *b, double *x, hipStream_t s,
+                      const int *skip) {
     const AmgLevel &lv = h->L[l];
     const size_t n = lv.A.n;
     if (l + 1 == h->L.size()) {
@@ -406,18 +415,26 @@ static void amg_vcycle(const femgpu_amg *h, size_t l, const double *b, double *x
     }
     const AmgLevel &nx = h->L[l + 1];
     double *r = lv.r.as<double>(), *z = lv.z.as<double>();
+    double *bc = nx.b.as<double>(), *xc = nx.x.as<double>();
     amg_smooth(h, lv, b, x, s, skip);                           // pre-smoothing from zero
     lv.dA.spmv(x, b, r, 2, s, skip);                            // r = b - A x
-    lv.dR.spmv(r, nullptr, nx.b.as<double>(), 0, s, skip);     // b_c = R r
-    amg_vcycle(h, l + 1, nx.b.as<double>(), nx.x.as<double>(), s, skip);
-    lv.dP.spmv(nx.x.as<double>(), nullptr, x, 1, s, skip);      // x += P x_c
+    lv.dR.spmv(r, nullptr, bc, 0, s, skip);                     // b_c = R r
+    amg_cycle(h, l + 1, kind, bc, xc, s, skip);
+    if (kind != FEMGPU_CYCLE_V && l + 2 < h->L.size()) {        // second coarse visit (exact on the coarsest)
+        const size_t nc = nx.A.n;
+        double *b2 = nx.b2.as<double>(), *x2 = nx.x2.as<double>();
+        nx.dA.spmv(xc, bc, b2, 2, s, skip);                     // b2 = b_c - A_c x_c
+        amg_cycle(h, l + 1, kind == FEMGPU_CYCLE_W ? FEMGPU_CYCLE_W : FEMGPU_CYCLE_V, b2, x2, s, skip);
+        hipLaunchKernelGGL(k_amg_add, dim3(g256(nc)), dim3(256), 0, s, nc, (const double *)x2, xc, skip);
+    }
+    lv.dP.spmv(xc, nullptr, x, 1, s, skip);                     // x += P x_c
     lv.dA.spmv(x, b, r, 2, s, skip);                            // r = b - A x
     amg_smooth(h, lv, r, z, s, skip);                           // post-smoothing
     hipLaunchKernelGGL(k_amg_add, dim3(g256(n)), dim3(256), 0, s, n, (const double *)z, x, skip);
 }
 
 void amg_apply(const femgpu_amg *h, const double *r, double *z, const int *skip) {
-    amg_vcycle(h, 0, r, z, h->ctx->stream, skip);
+    amg_cycle(h, 0, h->cycle, r, z, h->ctx->stream, skip);
 }
 
 femgpu_op *amg_operator(const femgpu_amg *h) { return h->op; }
@@ -442,7 +459,7 @@ int femgpu_amg_create(femgpu_op *op, const femgpu_amg_config *cfg, femgpu_amg **
             h->ctx = c; h->op = op;
             h->cfg.theta = cfg && cfg->theta > 0 ? cfg->theta : 0.08;
             h->cfg.max_levels = cfg && cfg->max_levels > 1 ? cfg->max_levels : 10;
-            h->cfg.coarse_max = cfg && cfg->coarse_max > 0 ? cfg->coarse_max : 400;
+            h->cfg.coarse_max = cfg && cfg->coarse_max > 0 ? cfg->coarse_max : 50;
             h->cfg.smoother_degree = cfg && cfg->smoother_degree > 0 ? cfg->smoother_degree : 2;
             h->cfg.smoother_ratio = cfg && cfg->smoother_ratio > 1 ? cfg->smoother_ratio : 10.0;
             HCsr A;
@@ -496,6 +513,16 @@ int femgpu_amg_level(const femgpu_amg *h, int level, int which, uint64_t *nrows,
         if (row_ptr) std::memcpy(row_ptr, M.rp.data(), (M.n + 1) * 8);
         if (col_idx && !M.ci.empty()) std::memcpy(col_idx, M.ci.data(), M.ci.size() * 4);
         if (values && !M.v.empty()) std::memcpy(values, M.v.data(), M.v.size() * 8);
+        return 0;
+    });
+}
+
+int femgpu_amg_set_cycle(femgpu_amg *h, int cycle) {
+    return guard([&] {
+        FG_REQUIRE(h, FEMGPU_ERR_INVALID_ARG, "amg_set_cycle: NULL argument");
+        FG_REQUIRE(cycle == FEMGPU_CYCLE_V || cycle == FEMGPU_CYCLE_W || cycle == FEMGPU_CYCLE_F,
+                   FEMGPU_ERR_INVALID_ARG, "amg_set_cycle: cycle must be V (0), W (1) or F (2)");
+        h->cycle = cycle;
         return 0;
     });
 }

@@ -137,6 +137,7 @@ _sig("femgpu_amg_destroy", [P])
 _sig("femgpu_amg_n_levels", [P, C.POINTER(I)])
 _sig("femgpu_amg_level", [P, I, I, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), P, P, P])
 _sig("femgpu_amg_vcycle", [P, P, P, I])
+_sig("femgpu_amg_set_cycle", [P, I])
 _sig("femgpu_solve_amg_cg", [P, P, P, I, P, P])
 _sig("femgpu_essential_mask_rhs", [P, P, I])
 _sig("femgpu_solve_cg_operator", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
@@ -724,12 +725,16 @@ def solve_pcg_jacobi(a, b, x, cfg=None):
     return _result(r)
 
 
+CYCLE_V, CYCLE_W, CYCLE_F = 0, 1, 2
+
+
 class AmgSolver:
     """AmgSolver (amg/src/lib.rs:157-207): a smoothed-aggregation hierarchy
-    built once (host) for the device CsrMatrix `a`, then PCG with one V-cycle
-    per iteration on the device.  Keeps a reference to `a`."""
+    built once (host) for the device CsrMatrix `a`, then PCG with one cycle
+    (V by default, see with_cycle) per iteration on the device.  Keeps a
+    reference to `a`."""
 
-    def __init__(self, a, theta=0.08, max_levels=10, coarse_max=400, smoother_degree=2, smoother_ratio=10.0):
+    def __init__(self, a, theta=0.08, max_levels=10, coarse_max=50, smoother_degree=2, smoother_ratio=10.0):
         cfg = _AmgConfig(theta, max_levels, coarse_max, smoother_degree, smoother_ratio)
         h = P()
         _check(lib.femgpu_amg_create(a.h, C.byref(cfg), C.byref(h)))
@@ -738,6 +743,13 @@ class AmgSolver:
     @classmethod
     def setup(cls, a, **config):
         return cls(a, **config)
+
+    def with_cycle(self, cycle):
+        """AmgSolver::with_cycle (amg/src/lib.rs:176-179): "V", "W" or "F"
+        (or CYCLE_V / CYCLE_W / CYCLE_F); returns self."""
+        c = {"V": CYCLE_V, "W": CYCLE_W, "F": CYCLE_F}.get(cycle, cycle) if isinstance(cycle, str) else cycle
+        _check(lib.femgpu_amg_set_cycle(self.h, int(c) if isinstance(c, int) else -1))
+        return self
 
     def n_levels(self):
         n = I()
@@ -758,7 +770,8 @@ class AmgSolver:
         return rp, ci, v, (nr.value, nc.value)
 
     def vcycle(self, b, x=None):
-        """x = one V-cycle applied to b (host arrays or DeviceArrays)."""
+        """x = one cycle (V unless with_cycle chose another) applied to b
+        (host arrays or DeviceArrays)."""
         n = self.a.n_dofs()
         if x is None:
             x = np.zeros(n)
@@ -770,7 +783,7 @@ class AmgSolver:
         return x
 
     def solve(self, b, x, cfg=None):
-        """PCG with the V-cycle preconditioner; stopping rule of
+        """PCG with the cycle as preconditioner; stopping rule of
         par_solve_pcg_jacobi (converged = False at max_iter)."""
         cfg = cfg or SolverConfig()
         _check_dims(self.a, b, x)

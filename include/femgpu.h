@@ -225,7 +225,7 @@ int femgpu_lumped_mass_apply_inverse(femgpu_op *op, const void *x, void *y, femg
 /* AmgSolver::setup / solve and solve_amg_cg (amg/src/lib.rs:125-207).  The
  * reference builds linger's AmgHierarchy (absent); this is smoothed
  * aggregation: strength threshold theta (default 0.08), at most max_levels
- * levels (10), coarsening until at most coarse_max rows (400, inverted dense),
+ * levels (10), coarsening until at most coarse_max rows (50, inverted dense),
  * Chebyshev-Jacobi smoothing of degree smoother_degree (2) on
  * [lambda_max / smoother_ratio, lambda_max] (ratio 10).  Zero fields take the
  * defaults.  The hierarchy is built on the host from an f64 CSR operator
@@ -246,7 +246,12 @@ int femgpu_amg_n_levels(const femgpu_amg *amg, int *n);
  * (which = 1) as host CSR; arrays may be NULL to query the sizes. */
 int femgpu_amg_level(const femgpu_amg *amg, int level, int which, uint64_t *nrows, uint64_t *ncols,
                      uint64_t *nnz, uint64_t *row_ptr, uint32_t *col_idx, double *values);
-/* x = one V-cycle applied to b (from x = 0): the preconditioner. */
+/* AmgSolver::with_cycle (amg/src/lib.rs:176-179): the cycle the
+ * preconditioner runs, V (default), W or F. */
+enum { FEMGPU_CYCLE_V = 0, FEMGPU_CYCLE_W = 1, FEMGPU_CYCLE_F = 2 };
+int femgpu_amg_set_cycle(femgpu_amg *amg, int cycle);
+/* x = one cycle (of the set kind) applied to b (from x = 0): the
+ * preconditioner. */
 int femgpu_amg_vcycle(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem);
 
 /* ------------------------------------------------------------ solvers ---- */

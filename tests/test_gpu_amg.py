@@ -89,3 +89,127 @@ def test_amg_errors(fg, ctx):
     op = fg.HexPAOperator(2, 3, 3, ctx=ctx)
     with pytest.raises(fg.FemGpuError):
         fg.AmgSolver(op)
+
+
+def _p1_l2_error(fr, u, coords, conn):
+    """L2 error of a P1 field against sin(pi x) sin(pi y) with the 5th-order
+    triangle rule (amg/tests/poisson_amg.rs:38-61)."""
+    pts, wts = fr.tri_rule(5)
+    X = coords.reshape(-1, 2)
+    T = conn.reshape(-1, 3).astype(np.int64)
+    x0, x1, x2 = X[T[:, 0]], X[T[:, 1]], X[T[:, 2]]
+    det = np.abs((x1[:, 0] - x0[:, 0]) * (x2[:, 1] - x0[:, 1]) - (x2[:, 0] - x0[:, 0]) * (x1[:, 1] - x0[:, 1]))
+    err = 0.0
+    for (xi, eta), w in zip(pts, wts):
+        uh = (1 - xi - eta) * u[T[:, 0]] + xi * u[T[:, 1]] + eta * u[T[:, 2]]
+        xp = x0 + xi * (x1 - x0) + eta * (x2 - x0)
+        d = uh - np.sin(np.pi * xp[:, 0]) * np.sin(np.pi * xp[:, 1])
+        err += float(np.sum(w * det * d * d))
+    return float(np.sqrt(err))
+
+
+def test_amg_reference_poisson(fg, fr, ctx):
+    """The reference's AMG integration tests (amg/tests/poisson_amg.rs:67-115),
+    rtol 1e-8, atol 1e-10, max_iter 500: on 16x16 P1 Poisson AMG-CG converges
+    with L2 error < 6e-3; one hierarchy with >= 2 levels solves twice; on 64x64
+    AMG-CG needs fewer iterations than plain CG."""
+    cfg = fg.SolverConfig(rtol=1e-8, atol=1e-10, max_iter=500)
+    rp, ci, v, rhs, coords, conn = fr.poisson_p1(16)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    x = np.zeros(len(rhs))
+    res = fg.solve_amg_cg(A, rhs, x, cfg)
+    assert res.converged
+    assert _p1_l2_error(fr, x, coords, conn) < 6e-3
+    amg = fg.AmgSolver.setup(A)
+    assert amg.n_levels() >= 2
+    for _ in range(2):
+        x = np.zeros(len(rhs))
+        assert amg.solve(rhs, x, cfg).converged
+        assert _p1_l2_error(fr, x, coords, conn) < 6e-3
+    amg.close()
+
+    rp, ci, v, rhs, coords, conn = fr.poisson_p1(64)
+    N = len(rhs)
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    x = np.zeros(N)
+    res = fg.solve_amg_cg(A, rhs, x, cfg)
+    _, rc = fr.solve_cg_operator(fr._Op(N, csr=fr.CsrRef(rp, ci, v)), rhs, np.zeros(N), rtol=1e-8, atol=1e-10,
+                                 max_iter=500)
+    print(f"64x64: CG {rc.iterations} it, AMG-CG {res.iterations} it")
+    assert res.converged and res.iterations < rc.iterations
+
+
+def _lap1d(fg, ctx, n):
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+    T.sort_indices()
+    return T, fg.CsrMatrix(T.indptr.astype(np.uint64), T.indices.astype(np.uint32), T.data, ctx=ctx)
+
+
+def test_amg_reference_unit_tests(fg, ctx):
+    """The reference's in-crate AMG tests (amg/src/lib.rs:232-340) on the 1-D
+    Laplacian with b = 1 and the default SolverConfig: V-cycle AMG-CG under 30
+    iterations (n = 100); one hierarchy of >= 2 levels reused three times
+    (n = 80); Chebyshev(degree 3, ratio 3) smoothing under 50 iterations
+    (n = 100); F-cycle under 30 iterations (n = 100); Chebyshev + F-cycle
+    converges (n = 80).  Each solution is checked against a direct solve."""
+    cfg = fg.SolverConfig()
+
+    def check(M, x, res, limit):
+        xe = spl.spsolve(M.tocsc(), np.ones(M.shape[0]))
+        assert res.converged and res.iterations < limit, res.iterations
+        assert np.linalg.norm(x - xe) <= 1e-6 * np.linalg.norm(xe)
+
+    M, A = _lap1d(fg, ctx, 100)
+    x = np.zeros(100)
+    check(M, x, fg.solve_amg_cg(A, np.ones(100), x, cfg), 30)
+    x = np.zeros(100)
+    check(M, x, fg.solve_amg_cg(A, np.ones(100), x, cfg, smoother_degree=3, smoother_ratio=3.0), 50)
+    s = fg.AmgSolver.setup(A).with_cycle("F")
+    x = np.zeros(100)
+    check(M, x, s.solve(np.ones(100), x, cfg), 30)
+    s.close()
+
+    M, A = _lap1d(fg, ctx, 80)
+    s = fg.AmgSolver.setup(A)
+    assert s.n_levels() >= 2
+    for _ in range(3):
+        x = np.zeros(80)
+        check(M, x, s.solve(np.ones(80), x, cfg), 1000)
+    s.close()
+    s = fg.AmgSolver.setup(A, smoother_degree=3, smoother_ratio=3.0).with_cycle(fg.CYCLE_F)
+    x = np.zeros(80)
+    check(M, x, s.solve(np.ones(80), x, cfg), 1000)
+    s.close()
+
+
+def test_amg_cycles(fg, ctx):
+    """W-cycle: symmetric like the V-cycle and contracts at least as well per
+    cycle; F-cycle and W-cycle PCG agree with a direct solve; an unknown cycle
+    kind is refused."""
+    M = _lap2d_interior(120)
+    n = M.shape[0]
+    A = fg.CsrMatrix(M.indptr.astype(np.uint64), M.indices.astype(np.uint32), M.data, ctx=ctx)
+    s = fg.AmgSolver(A)
+    assert s.n_levels() >= 4
+    rng = np.random.default_rng(3)
+    u, w = rng.standard_normal(n), rng.standard_normal(n)
+    xe = spl.spsolve(M.tocsc(), u)
+    contraction = {}
+    for kind in ("V", "W", "F"):
+        s.with_cycle(kind)
+        Vu = s.vcycle(u)
+        contraction[kind] = np.linalg.norm(M @ (xe - Vu)) / np.linalg.norm(u)  # residual after one cycle from 0
+        if kind != "F":
+            Vw = s.vcycle(w)
+            assert abs(w @ Vu - u @ Vw) <= 1e-10 * abs(w @ Vu)
+        x = np.zeros(n)
+        res = s.solve(u, x, fg.SolverConfig(rtol=1e-10, max_iter=300))
+        assert res.converged
+        assert np.linalg.norm(x - xe) <= 1e-8 * np.linalg.norm(xe)
+        print(kind, res.iterations, "it, one-cycle residual", contraction[kind])
+    assert contraction["W"] <= contraction["V"] * 1.05
+    with pytest.raises(fg.FemGpuError):
+        s.with_cycle(7)
+    with pytest.raises(fg.FemGpuError):
+        s.with_cycle("X")
+    s.close()
