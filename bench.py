@@ -11,9 +11,11 @@ value = solve throughput N_dofs * PCG iterations / solve time (GDOF/s).  The
 line also carries the PA apply rates (the operator's form and, for the
 default workload, the mass form; FP64), the solve time, the FP64 residual of
 the returned solution (recomputed with a fresh apply), the roofline fraction
-of the PA apply (algorithmic bytes 8 (2N + s E q^3) per apply, s = 6
-diffusion / 7 diffusion + mass, over the HIP-event-timed mean duration of its
-two kernels), the HBM traffic per apply from the committed counter summary
+of the PA apply (algorithmic bytes 8 (2N + s E q^3) per apply, s = the
+quadrature-data values per Gauss point: 6 diffusion / 7 diffusion + mass on a
+general mesh, 1 / 2 on an axis-aligned box mesh; over the HIP-event-timed mean
+duration of its two kernels; on box meshes also the same operator with
+general quadrature data), the HBM traffic per apply from the committed counter summary
 for that workload (profiles/*_hbm_traffic.md), the bandwidth of a 1 GiB
 device-to-device copy timed in the same process (the practical streaming
 ceiling, with the apply's fraction of it) and a CPU baseline (the oracle's
@@ -363,15 +365,20 @@ def make_operator(fg, ctx, w, n, p):
                             ctx=ctx)
 
 
-def committed_traffic(name, n, p):
+def committed_traffic(name, n, p, qdata_per_point):
     """HBM bytes per apply from the committed counter summary of this
-    workload (its line 'traffic_bytes_per_apply: B'), or None."""
+    workload (its line 'traffic_bytes_per_apply: B'), or None; the summary's
+    'qdata_per_point: s' line must match the operator being measured."""
     path = os.path.join(ROOT, "profiles", f"pa_hex_{name}_hbm_traffic.md")
     w = WORKLOADS[name]
     if not os.path.exists(path) or (n, p) != (w["n"], w["p"]):
         return None
-    m = re.search(r"traffic_bytes_per_apply:\s*([0-9]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    text = open(path).read()
+    m = re.search(r"traffic_bytes_per_apply:\s*([0-9]+)", text)
+    sq = re.search(r"qdata_per_point:\s*([0-9]+)", text)
+    if not m or not sq or int(sq.group(1)) != qdata_per_point:
+        return None
+    return float(m.group(1))
 
 
 def cpu_baseline(n, p, iters, lmax):
@@ -517,7 +524,7 @@ def main():
     N = K.n_dofs()
     E = n ** 3
     q3 = q ** 3
-    s_comp = 6 if w["form"] == "diffusion" else 7
+    s_comp = K.qdata_per_point()  # 6 / 7 general, 1 / 2 on an axis-aligned box mesh
     dev = lambda: fg.DeviceArray(ctx, N, np.float64)
     b = dev()
     if w["rhs"] == "unit":
@@ -543,7 +550,23 @@ def main():
     ms_apply, ms_kern = K.time_apply(xv, yv, a.apply_reps)
     bytes_apply = 8.0 * (2 * N + s_comp * E * q3)
     pa = {"operator_gdofs": round(N / (ms_apply * 1e-3) / 1e9, 3), "operator_ms": ms_apply,
-          "operator_kernel_ms": ms_kern}
+          "operator_kernel_ms": ms_kern, "qdata_per_point": s_comp}
+    if K.cartesian:
+        # the same operator with the general (6-value) quadrature data, for comparison
+        os.environ["FEMGPU_HEX_QDATA"] = "general"
+        try:
+            Kg = make_operator(fg, ctx, w, n, p)
+        finally:
+            del os.environ["FEMGPU_HEX_QDATA"]
+        Kg.time_apply(xv, yv, 5)
+        msg_apply, msg_kern = Kg.time_apply(xv, yv, a.apply_reps)
+        sg = Kg.qdata_per_point()
+        bg = 8.0 * (2 * N + sg * E * q3)
+        pa["general_qdata"] = {"qdata_per_point": sg, "operator_ms": msg_apply, "operator_kernel_ms": msg_kern,
+                               "operator_gdofs": round(N / (msg_apply * 1e-3) / 1e9, 3),
+                               "algorithmic_bytes": bg,
+                               "frac": round(bg / (msg_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        Kg.close()
     if w["mass_apply"]:
         M = fg.HexPAOperator(p, q, n, form=fg.MASS, rho=1.0, dirichlet=False, ctx=ctx)
         msm_apply, msm_kern = M.time_apply(xv, yv, a.apply_reps)
@@ -620,8 +643,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": committed_traffic(a.workload, n, p),
-            "kernel": f"PA apply = k_pa_hex<{q},{q},double,{kform}> element kernel + "
+            "traffic": committed_traffic(a.workload, n, p, s_comp),
+            "kernel": f"PA apply = k_pa_hex<{q},{q},double,{kform},{'true' if K.cartesian else 'false'}> element "
+                      f"kernel ({s_comp} qdata value(s) per Gauss point"
+                      f"{', axis-aligned mesh' if K.cartesian else ''}) + "
                       f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)",
             "algorithmic_bytes": bytes_apply,
             "device_copy_gb_s": round(copy_gbs, 1),

@@ -25,12 +25,21 @@
 //
 // qdata is stored plane-major: plane (comp, qz) holds [E][Q*Q] values, so the
 // Z phase of a wavefront reads fully contiguous, coalesced runs.
+//
+// On an axis-aligned box mesh (vertex x depends on i only, y on j, z on k) J
+// is diagonal and constant per element, D = (w kappa |J|) diag(1/J_xx^2,
+// 1/J_yy^2, 1/J_zz^2) with the three factors depending on ex, ey, ez alone.
+// There ("Cartesian" qdata) one value per point, s = w kappa |J|, is stored
+// with per-axis factor arrays cf = [1/J_xx^2 (nx) | 1/J_yy^2 (ny) | 1/J_zz^2
+// (nz)], a sixth of the general diffusion qdata.
 #include "basis.hpp"
 #include "internal.hpp"
 #include "philox.hpp"
 #include "reduce.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <mutex>
 
 namespace femgpu {
@@ -137,9 +146,9 @@ static inline size_t qd_count(long long E, int Q) {
     return (size_t)((E + NE - 1) / NE) * NE * Q * Q * Q;
 }
 
-template <int N1, int Q, class T, int FORM>
+template <int N1, int Q, class T, int FORM, bool CART>
 __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
-    const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm,
+    const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm, const T *__restrict__ cf,
     const T *__restrict__ x, T *__restrict__ y, const int dirichlet, const int *skip) {
     constexpr int P = N1 - 1;
     constexpr int T2 = Q * Q;
@@ -237,8 +246,14 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             c0[k] = c1[k] = c2[k] = 0;
         }
         constexpr size_t plane = (size_t)NE * T2;
-        const size_t offd = (size_t)lb * 6 * Q * plane + (size_t)le * T2 + t;
+        const size_t offd = (size_t)lb * (CART ? 1 : 6) * Q * plane + (size_t)le * T2 + t;
         const size_t offm = (size_t)lb * Q * plane + (size_t)le * T2 + t;
+        T fx = 0, fy = 0, fz = 0;  // Cartesian per-axis factors of this element
+        if constexpr (DIFF && CART) {
+            fx = cf[ex];
+            fy = cf[g.nx + ey];
+            fz = cf[g.nx + g.ny + ez];
+        }
 #pragma unroll
         for (int c = 0; c < Q; ++c) {
             T v0 = 0, v1 = 0, v2 = 0, vm = 0;
@@ -250,15 +265,22 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     g1 += Bt[c * N1 + k] * gb[k];
                     g2 += Gt[c * N1 + k] * bb[k];
                 }
-                const T d0 = qd[(size_t)(0 * Q + c) * plane + offd];
-                const T d1 = qd[(size_t)(1 * Q + c) * plane + offd];
-                const T d2 = qd[(size_t)(2 * Q + c) * plane + offd];
-                const T d3 = qd[(size_t)(3 * Q + c) * plane + offd];
-                const T d4 = qd[(size_t)(4 * Q + c) * plane + offd];
-                const T d5 = qd[(size_t)(5 * Q + c) * plane + offd];
-                v0 = d0 * g0 + d1 * g1 + d2 * g2;
-                v1 = d1 * g0 + d3 * g1 + d4 * g2;
-                v2 = d2 * g0 + d4 * g1 + d5 * g2;
+                if constexpr (CART) {
+                    const T sq = qd[(size_t)c * plane + offd];
+                    v0 = (sq * fx) * g0;
+                    v1 = (sq * fy) * g1;
+                    v2 = (sq * fz) * g2;
+                } else {
+                    const T d0 = qd[(size_t)(0 * Q + c) * plane + offd];
+                    const T d1 = qd[(size_t)(1 * Q + c) * plane + offd];
+                    const T d2 = qd[(size_t)(2 * Q + c) * plane + offd];
+                    const T d3 = qd[(size_t)(3 * Q + c) * plane + offd];
+                    const T d4 = qd[(size_t)(4 * Q + c) * plane + offd];
+                    const T d5 = qd[(size_t)(5 * Q + c) * plane + offd];
+                    v0 = d0 * g0 + d1 * g1 + d2 * g2;
+                    v1 = d1 * g0 + d3 * g1 + d4 * g2;
+                    v2 = d2 * g0 + d4 * g1 + d5 * g2;
+                }
             }
             if constexpr (MASS) {
                 T uu = 0;
@@ -579,7 +601,7 @@ template <class T>
 __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ xq,
                             const double *__restrict__ wq, const double *__restrict__ coords,
                             CoefDev coef, double rho, int form, T *__restrict__ qd,
-                            T *__restrict__ qm, int mass_only_weight) {
+                            T *__restrict__ qm, int mass_only_weight, int cart) {
     const int Q3 = Q * Q * Q, T2 = Q * Q;
     const size_t total = (size_t)g.E * Q3;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
@@ -616,7 +638,9 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
         const double wd = wq[qx] * wq[qy] * wq[qz] * fabs(det);
         const int t2 = qx + Q * qy;
         if ((form & 1) && qm) qm[qd_index(Q, 1, e, 0, qz, t2)] = (T)(mass_only_weight ? wd : wd * rho);
-        if ((form & 2) && qd) {
+        if ((form & 2) && qd && cart) {
+            qd[qd_index(Q, 1, e, 0, qz, t2)] = (T)(wd * coef_eval(coef, xp, e, qp, Q3));
+        } else if ((form & 2) && qd) {
             const double id = 1.0 / det;
             double I[3][3];
             I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
@@ -646,7 +670,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
 template <class T>
 __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restrict__ B,
                            const double *__restrict__ G, const T *__restrict__ qd,
-                           const T *__restrict__ qm, int form, T *__restrict__ ye) {
+                           const T *__restrict__ qm, const T *__restrict__ cf, int cart, int form,
+                           T *__restrict__ ye) {
     // Element diagonals into the element vector; k_hex_gather then sums them
     // per DOF in a fixed order.
     const int L = N1 * N1 * N1;
@@ -668,7 +693,14 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
                         const double g0 = G[a * N1 + i] * Bj * Bk, g1 = Bi * G[b * N1 + j] * Bk,
                                      g2 = Bi * Bj * G[c * N1 + k];
                         double d[6];
-                        for (int m = 0; m < 6; ++m) d[m] = (double)qd[qd_index(Q, 6, e, m, c, t2)];
+                        if (cart) {
+                            const double sq = (double)qd[qd_index(Q, 1, e, 0, c, t2)];
+                            d[0] = sq * (double)cf[ex]; d[3] = sq * (double)cf[g.nx + ey];
+                            d[5] = sq * (double)cf[g.nx + g.ny + ez];
+                            d[1] = d[2] = d[4] = 0.0;
+                        } else {
+                            for (int m = 0; m < 6; ++m) d[m] = (double)qd[qd_index(Q, 6, e, m, c, t2)];
+                        }
                         s += d[0] * g0 * g0 + d[3] * g1 * g1 + d[5] * g2 * g2 +
                              2.0 * (d[1] * g0 * g1 + d[2] * g0 * g2 + d[4] * g1 * g2);
                     }
@@ -726,21 +758,29 @@ static unsigned grid1(size_t n) {
     return (unsigned)(b ? b : 1);
 }
 
+// cf: Cartesian per-axis factors, or null for general qdata
 template <int N1, int Q, class T>
-static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, const T *x, T *y,
-                        int dir, const int *skip, hipStream_t s) {
+static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, const T *cf, const T *x,
+                        T *y, int dir, const int *skip, hipStream_t s) {
     constexpr int NE = ne_of(Q);
     const dim3 grid((unsigned)((g.E + NE - 1) / NE)), block(NE * Q * Q);
-    if (form == 1) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 1>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
-    else if (form == 2) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
-    else hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3>), grid, block, 0, s, g, qd, qm, x, y, dir, skip);
+    if (form == 1)
+        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 1, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
+    else if (form == 2 && cf)
+        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2, true>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
+    else if (form == 2)
+        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
+    else if (cf)
+        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, true>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
+    else
+        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
 }
 
 template <class T>
-static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm,
+static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm, const T *cf,
                       const T *x, T *y, int dir, const int *skip, hipStream_t s) {
 #define CASE(A, B)                                                                 \
-    if (n1 == A && q == B) { launch_form<A, B, T>(g, form, qd, qm, x, y, dir, skip, s); return true; }
+    if (n1 == A && q == B) { launch_form<A, B, T>(g, form, qd, qm, cf, x, y, dir, skip, s); return true; }
     CASE(2, 2) CASE(2, 3) CASE(3, 3) CASE(3, 4) CASE(4, 4) CASE(4, 5) CASE(5, 5) CASE(5, 6)
     CASE(6, 6)
 #undef CASE
@@ -780,6 +820,13 @@ struct HexOp final : femgpu_op {
     HexGrid g{};
     DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
     DevBuf evec;  // x-merged element vector between the two apply kernels
+    bool cart = false;           // axis-aligned box mesh: Cartesian qdata (one value per point)
+    std::vector<double> cf_host;  // Cartesian per-axis factors [nx | ny | nz]
+    DevBuf cf;                    // the same, in the operator's type
+
+    template <class T> const T *cf_ptr() const { return cart ? (const T *)cf.ptr : nullptr; }
+    // values of quadrature data per Gauss point
+    int qdata_per_point() const { return ((desc.form & 2) ? (cart ? 1 : 6) : 0) + ((desc.form & 1) ? 1 : 0); }
 
     const char *name() const override { return "pa_hex"; }
 
@@ -789,7 +836,8 @@ struct HexOp final : femgpu_op {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
         if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
-        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, evec.as<T>(), dir, skip, s);
+        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
+                     skip, s);
         launch_gather<T>(desc.p, g, evec.as<T>(), x, y, acc ? 1 : 0, dir, skip, s);
         if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
     }
@@ -803,7 +851,8 @@ struct HexOp final : femgpu_op {
                                     int last, T *y, double *part, const int *skip) {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
-        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), x, evec.as<T>(), dir, skip, s);
+        launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
+                     skip, s);
         const dim3 grid((unsigned)red_blocks(n)), block(256);
         const T *ye = evec.as<T>();
 #define GF(P_)                                                                                              \
@@ -854,7 +903,8 @@ struct HexOp final : femgpu_op {
         const size_t L = (size_t)g.E * n1 * n1 * n1;
         DevBuf old(L * sizeof(T));
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
-                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), desc.form, old.as<T>());
+                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, desc.form,
+                           old.as<T>());
         hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(g, n1))), dim3(256), 0, s, g, n1,
                            (const T *)old.as<T>(), evec.as<T>());
         launch_gather<T>(desc.p, g, evec.as<T>(), d, d, 0, 0, nullptr, s);
@@ -878,7 +928,7 @@ struct HexOp final : femgpu_op {
         const size_t total = (size_t)g.E * desc.q * desc.q * desc.q;
         hipLaunchKernelGGL(k_hex_setup<T>, dim3(grid1(total)), dim3(256), 0, ctx->stream, g, desc.q,
                            xq.as<double>(), wq.as<double>(), coords.as<double>(), c, rho, form,
-                           qd_, qm_, mass_weight_only ? 1 : 0);
+                           qd_, qm_, mass_weight_only ? 1 : 0, cart ? 1 : 0);
         FG_HIP(hipGetLastError());
     }
 
@@ -895,8 +945,18 @@ struct HexOp final : femgpu_op {
         FG_HIP(hipMemcpyAsync(Gd.ptr, G, 8 * q * n1, hipMemcpyHostToDevice, s));
         const size_t nq = qd_count(g.E, q);
         const size_t es = esize();
-        if (desc.form & 2) qd.alloc(6 * nq * es);
+        if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
+        if (cart) {
+            cf.alloc(cf_host.size() * es);
+            if (dtype == FEMGPU_F64) {
+                FG_HIP(hipMemcpyAsync(cf.ptr, cf_host.data(), cf_host.size() * 8, hipMemcpyHostToDevice, s));
+            } else {
+                std::vector<float> f(cf_host.begin(), cf_host.end());
+                FG_HIP(hipMemcpyAsync(cf.ptr, f.data(), f.size() * 4, hipMemcpyHostToDevice, s));
+                FG_HIP(hipStreamSynchronize(s));
+            }
+        }
         evec.alloc(evec_count(g, n1) * es);
         if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
         else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
@@ -918,6 +978,31 @@ static void grf_features_host(const femgpu_coef &c, std::vector<double> &w, std:
         }
         b[m] = 2.0 * M_PI * philox_u01(c.seed, 20, (uint64_t)m);
     }
+}
+
+// Axis-aligned box mesh test on the (nx+1)(ny+1)(nz+1) x 3 vertex array:
+// x of vertex (i, j, k) equals x of (i, 0, 0) for all j, k (likewise y, z),
+// and no spacing is zero.  On success f = [1/J_xx^2 | 1/J_yy^2 | 1/J_zz^2]
+// with J_dd = (spacing) / 2 on the reference cube [-1, 1]^3.
+static bool cartesian_factors(int nx, int ny, int nz, const double *X, std::vector<double> &f) {
+    const size_t sx = 1, sy = (size_t)nx + 1, sz = sy * ((size_t)ny + 1);
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                const double *v = X + 3 * (i * sx + j * sy + k * sz);
+                if (v[0] != X[3 * (i * sx)] || v[1] != X[3 * (j * sy) + 1] || v[2] != X[3 * (k * sz) + 2]) return false;
+            }
+    f.assign((size_t)nx + ny + nz, 0.0);
+    const int n[3] = {nx, ny, nz};
+    const size_t st[3] = {sx, sy, sz};
+    size_t o = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < n[a]; ++i) {
+            const double h = 0.5 * (X[3 * ((i + 1) * st[a]) + a] - X[3 * (i * st[a]) + a]);
+            if (!(h != 0.0) || !std::isfinite(h)) return false;
+            f[o++] = 1.0 / (h * h);
+        }
+    return true;
 }
 
 static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *coords_host,
@@ -956,16 +1041,25 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         const size_t nv = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
         hipStream_t s = c->stream;
         op->coords.alloc(nv * 3 * 8);
+        std::vector<double> h;  // host copy of the vertices unless the caller's array is host memory
+        const double *hc = coords_host;
         if (coords_host) {
             FG_HIP(hipMemcpyAsync(op->coords.ptr, coords_host, nv * 24, hipMemcpyHostToDevice, s));
         } else if (coords_dev) {
             FG_HIP(hipMemcpyAsync(op->coords.ptr, coords_dev, nv * 24, hipMemcpyDeviceToDevice, s));
+            h.resize(nv * 3);
+            FG_HIP(hipMemcpyAsync(h.data(), coords_dev, nv * 24, hipMemcpyDeviceToHost, s));
+            hc = h.data();
         } else {
-            std::vector<double> h(nv * 3);
+            h.resize(nv * 3);
             hex_vertices(d.nx, d.ny, d.nz, 0.0, 0, h.data());
             FG_HIP(hipMemcpyAsync(op->coords.ptr, h.data(), nv * 24, hipMemcpyHostToDevice, s));
-            FG_HIP(hipStreamSynchronize(s));
+            hc = h.data();
         }
+        FG_HIP(hipStreamSynchronize(s));
+        const char *qmode = std::getenv("FEMGPU_HEX_QDATA");
+        if ((d.form & 2) && !(qmode && std::strcmp(qmode, "general") == 0))
+            op->cart = cartesian_factors(d.nx, d.ny, d.nz, hc, op->cf_host);
         if ((d.form & 2) && d.kappa.kind == FEMGPU_COEF_QP_ARRAY) {
             const size_t nq = (size_t)g.E * d.q * d.q * d.q;
             op->kqp.alloc(nq * 8);
@@ -1057,6 +1151,15 @@ int femgpu_pa_create_hex(femgpu_ctx *c, const femgpu_hex_desc *d, const double *
     });
 }
 
+int femgpu_pa_hex_qdata_per_point(const femgpu_op *op_, int *values) {
+    return guard([&] {
+        auto *op = dynamic_cast<const HexOp *>(op_);
+        FG_REQUIRE(op && values, FEMGPU_ERR_INVALID_ARG, "pa_hex_qdata_per_point: not a hex operator");
+        *values = op->qdata_per_point();
+        return 0;
+    });
+}
+
 int femgpu_pa_hex_convert(femgpu_op *op_, femgpu_dtype dtype, femgpu_op **out) {
     return guard([&] {
         auto *op = dynamic_cast<HexOp *>(op_);
@@ -1086,14 +1189,16 @@ int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
         if (op->dtype == FEMGPU_F64) {
             op->setup_t<double>(1, 1.0, true, nullptr, tq.as<double>());
             fill<double>(s, ones.as<double>(), 1.0, op->n);
-            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), ones.as<double>(), op->evec.as<double>(), 0, nullptr, s);
+            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), nullptr, ones.as<double>(), op->evec.as<double>(), 0,
+                              nullptr, s);
             launch_gather<double>(op->desc.p, op->g, op->evec.as<double>(), ones.as<double>(), (double *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<double>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const double *)dst, (double *)dst, 3, op->desc.dirichlet, nullptr);
         } else {
             op->setup_t<float>(1, 1.0, true, nullptr, tq.as<float>());
             fill<float>(s, ones.as<float>(), 1.f, op->n);
-            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), ones.as<float>(), op->evec.as<float>(), 0, nullptr, s);
+            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), nullptr, ones.as<float>(), op->evec.as<float>(), 0,
+                             nullptr, s);
             launch_gather<float>(op->desc.p, op->g, op->evec.as<float>(), ones.as<float>(), (float *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<float>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const float *)dst, (float *)dst, 3, op->desc.dirichlet, nullptr);

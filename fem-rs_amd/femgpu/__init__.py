@@ -128,6 +128,7 @@ _sig("femgpu_hex_vertices", [I, I, I, D, U64, P])
 _sig("femgpu_pa_hex_rhs_unit", [P, P, I])
 _sig("femgpu_pa_hex_rhs_random", [P, U64, P, I])
 _sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
+_sig("femgpu_pa_hex_qdata_per_point", [P, C.POINTER(I)])
 _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
@@ -558,6 +559,19 @@ class HexPAOperator(Operator):
     @property
     def n_elements(self):
         return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def qdata_per_point(self):
+        """Quadrature-data values stored per Gauss point: 6 (diffusion,
+        general trilinear mesh) or 1 (diffusion on an axis-aligned box mesh,
+        with per-axis factors), + 1 with the mass term.  FEMGPU_HEX_QDATA=general
+        at creation keeps the general form."""
+        v = I()
+        _check(lib.femgpu_pa_hex_qdata_per_point(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def cartesian(self):
+        return bool(self.form & DIFFUSION) and self.qdata_per_point() <= 2
 
     def convert(self, dtype):
         h = P()

@@ -172,6 +172,13 @@ int femgpu_hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, doubl
 int femgpu_pa_hex_rhs_unit(femgpu_op *op, void *b, femgpu_mem mem);
 /* Seeded uniform [-1,1] nodal field (0 on essential DOFs when dirichlet). */
 int femgpu_pa_hex_rhs_random(femgpu_op *op, uint64_t seed, void *b, femgpu_mem mem);
+/* Values of quadrature data stored per Gauss point: 6 for diffusion on a
+ * general (trilinear) mesh, 1 on an axis-aligned box mesh (vertex x depends
+ * on i only, y on j, z on k; then w kappa |J| is stored with per-axis factors
+ * 1/J_dd^2), plus 1 for the mass term.  Setting FEMGPU_HEX_QDATA=general at
+ * creation keeps the general form on box meshes too.  No counterpart in the
+ * reference (its PA is simplex-only, assembly/src/partial.rs). */
+int femgpu_pa_hex_qdata_per_point(const femgpu_op *op, int *values);
 /* Same-mesh copy of a hex operator in another precision, reusing nothing but
  * the description (for the FP32 inner / FP64 outer solver). */
 int femgpu_pa_hex_convert(femgpu_op *op, femgpu_dtype dtype, femgpu_op **out);

@@ -229,3 +229,41 @@ def test_essential_bc_operator(fg, fr, ctx):
     assert not np.any(sol[cons])
     with pytest.raises(fg.FemGpuError):
         fg.EssentialBcOperator(K, [nd])
+
+
+def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
+    """On an axis-aligned box mesh (uniform, or graded per axis) the operator
+    stores one quadrature value per point (two with mass); its applies and
+    diagonal equal those of the general six-value form (FEMGPU_HEX_QDATA=general)
+    and the oracle's; a perturbed mesh keeps the general form."""
+    rng = np.random.default_rng(4)
+    n = 5
+    # graded per axis: x depends on i only, y on j, z on k
+    ax = [np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for _ in range(3)]
+    ax = [a / a[-1] for a in ax]
+    V = np.array([[ax[0][i], ax[1][j], ax[2][k]] for k in range(n + 1) for j in range(n + 1)
+                  for i in range(n + 1)])
+    for p, form, kappa in ((2, fg.DIFFUSION, "sinprod"), (3, fg.DIFF_MASS, fg.grf(3)), (1, fg.DIFFUSION, 2.5)):
+        q = p + 1
+        for verts in (None, V):
+            K = fg.HexPAOperator(p, q, n, form=form, kappa=kappa, rho=1.3, dirichlet=True, vertices=verts, ctx=ctx)
+            assert K.cartesian and K.qdata_per_point() == (1 if form == fg.DIFFUSION else 2)
+            monkeypatch.setenv("FEMGPU_HEX_QDATA", "general")
+            G = fg.HexPAOperator(p, q, n, form=form, kappa=kappa, rho=1.3, dirichlet=True, vertices=verts, ctx=ctx)
+            monkeypatch.delenv("FEMGPU_HEX_QDATA")
+            assert not G.cartesian and G.qdata_per_point() == (6 if form == fg.DIFFUSION else 7)
+            x = rng.standard_normal(K.n_dofs())
+            yk, yg = K.matvec(x), G.matvec(x)
+            assert np.abs(yk - yg).max() <= 1e-13 * np.abs(yg).max()
+            dk, dg = K.diagonal(), G.diagonal()
+            assert np.abs(dk - dg).max() <= 1e-13 * np.abs(dg).max()
+            K32 = K.convert(fg.F32)
+            assert K32.cartesian
+            y32 = K32.matvec(x.astype(np.float32)).astype(np.float64)
+            assert np.abs(y32 - yg).max() <= 1e-5 * np.abs(yg).max()
+            K.close(); G.close(); K32.close()
+    P = fg.HexPAOperator(2, 3, n, form=fg.DIFFUSION, kappa=1.0, dirichlet=True,
+                         vertices=fg.hex_vertices(n, n, n, amp=0.1, seed=2), ctx=ctx)
+    assert not P.cartesian and P.qdata_per_point() == 6
+    M = fg.HexPAOperator(2, 3, n, form=fg.MASS, rho=1.0, ctx=ctx)
+    assert not M.cartesian and M.qdata_per_point() == 1
