@@ -146,6 +146,41 @@ static inline size_t qd_count(long long E, int Q) {
     return (size_t)((E + NE - 1) / NE) * NE * Q * Q * Q;
 }
 
+// LDS layout of the element kernel's per-element arrays: index order (outer to
+// inner) of A(k, j, a) and B(k, b, a), and the per-element strides SA, SB
+// (>= the array sizes; the extra is padding).  Chosen per (N1, Q, element
+// size) by counting, for each phase's lane -> address map, the bank
+// multiplicity of every 32-lane (8-byte) or 64-lane (4-byte) group on 64
+// banks of 4 bytes, without lowering the workgroups per CU.  Q2 FP64:
+// SQ_LDS_BANK_CONFLICT 6.45M -> 4.33M cycles per apply.
+struct LdsLayout { int a0, a1, a2, b0, b1, b2, SA, SB; };
+constexpr LdsLayout lds_layout(int n1, int q, int es) {
+    if (es == 8) {
+        if (n1 == 2 && q == 3) return {2, 1, 0, 2, 0, 1, 13, 18};
+        if (n1 == 3 && q == 3) return {0, 1, 2, 0, 2, 1, 27, 27};
+        if (n1 == 3 && q == 4) return {0, 2, 1, 1, 0, 2, 36, 48};
+        if (n1 == 4 && q == 4) return {1, 0, 2, 0, 1, 2, 64, 64};
+        if (n1 == 4 && q == 5) return {1, 0, 2, 0, 2, 1, 85, 100};
+        if (n1 == 5 && q == 5) return {0, 1, 2, 0, 2, 1, 125, 125};
+        if (n1 == 5 && q == 6) return {2, 0, 1, 1, 2, 0, 153, 180};
+    } else {
+        if (n1 == 2 && q == 3) return {0, 1, 2, 0, 2, 1, 12, 18};
+        if (n1 == 3 && q == 3) return {1, 0, 2, 0, 2, 1, 27, 27};
+        if (n1 == 3 && q == 4) return {0, 2, 1, 1, 0, 2, 36, 48};
+        if (n1 == 4 && q == 5) return {1, 0, 2, 0, 2, 1, 85, 100};
+        if (n1 == 5 && q == 5) return {0, 1, 2, 0, 2, 1, 125, 125};
+        if (n1 == 5 && q == 6) return {2, 0, 1, 0, 2, 1, 155, 181};
+    }
+    return {0, 1, 2, 0, 1, 2, n1 * n1 * q, n1 * q * q};
+}
+// offset of (i0, i1, i2) with sizes (S0, S1, S2) stored in dimension order P0, P1, P2
+template <int P0, int P1, int P2, int S0, int S1, int S2>
+__device__ __forceinline__ int off3(int i0, int i1, int i2) {
+    const int idx[3] = {i0, i1, i2};
+    constexpr int sz[3] = {S0, S1, S2};
+    return (idx[P0] * sz[P1] + idx[P1]) * sz[P2] + idx[P2];
+}
+
 template <int N1, int Q, class T, int FORM, bool CART>
 __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm, const T *__restrict__ cf,
@@ -157,11 +192,21 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     constexpr bool DIFF = (FORM & 2) != 0;
     constexpr bool MASS = (FORM & 1) != 0;
     constexpr int ND = DIFF ? NE : 1;
-    __shared__ T sA[NE][N1][N1][Q];   // X out (B path), later F_B
-    __shared__ T sAg[ND][N1][N1][Q];  // X out (G path), later F_G
-    __shared__ T sBB[NE][N1][Q][Q];   // Y out BB, later C_zeta+mass
-    __shared__ T sBG[ND][N1][Q][Q];   // Y out BG, later C_xi
-    __shared__ T sGB[ND][N1][Q][Q];   // Y out GB, later C_eta
+    // LDS: per element, A arrays hold (k, j, a) and B arrays (k, b, a) in the
+    // index order and element stride of lds_layout (chosen against bank
+    // conflicts of the five phases' access patterns).
+    constexpr LdsLayout LL = lds_layout(N1, Q, (int)sizeof(T));
+    __shared__ T sA[NE * LL.SA];   // X out (B path), later F_B
+    __shared__ T sAg[ND * LL.SA];  // X out (G path), later F_G
+    __shared__ T sBB[NE * LL.SB];  // Y out BB, later C_zeta+mass
+    __shared__ T sBG[ND * LL.SB];  // Y out BG, later C_xi
+    __shared__ T sGB[ND * LL.SB];  // Y out GB, later C_eta
+    auto iA = [](int e_, int k_, int j_, int a_) {
+        return e_ * LL.SA + off3<LL.a0, LL.a1, LL.a2, N1, N1, Q>(k_, j_, a_);
+    };
+    auto iB = [](int e_, int k_, int b_, int a_) {
+        return e_ * LL.SB + off3<LL.b0, LL.b1, LL.b2, N1, Q, Q>(k_, b_, a_);
+    };
 
     if (skip && *skip) return;
     // Hardware order (consecutive workgroups on different XCDs): measured
@@ -202,8 +247,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 s += Bt[a * N1 + i] * u[i];
                 if constexpr (DIFF) sg += Gt[a * N1 + i] * u[i];
             }
-            sA[le][k][j][a] = s;
-            if constexpr (DIFF) sAg[le][k][j][a] = sg;
+            sA[iA(le, k, j, a)] = s;
+            if constexpr (DIFF) sAg[iA(le, k, j, a)] = sg;
         }
     }
     __syncthreads();
@@ -214,8 +259,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         T A_[N1], Ag_[N1];
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
-            A_[j] = sA[le][k][j][a];
-            if constexpr (DIFF) Ag_[j] = sAg[le][k][j][a];
+            A_[j] = sA[iA(le, k, j, a)];
+            if constexpr (DIFF) Ag_[j] = sAg[iA(le, k, j, a)];
         }
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
@@ -228,8 +273,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     gb += Gt[b * N1 + j] * A_[j];
                 }
             }
-            sBB[le][k][b][a] = bb;
-            if constexpr (DIFF) { sBG[le][k][b][a] = bg; sGB[le][k][b][a] = gb; }
+            sBB[iB(le, k, b, a)] = bb;
+            if constexpr (DIFF) { sBG[iB(le, k, b, a)] = bg; sGB[iB(le, k, b, a)] = gb; }
         }
     }
     __syncthreads();
@@ -241,8 +286,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         T c0[N1], c1[N1], c2[N1];
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            bb[k] = sBB[le][k][b][a];
-            if constexpr (DIFF) { bg[k] = sBG[le][k][b][a]; gb[k] = sGB[le][k][b][a]; }
+            bb[k] = sBB[iB(le, k, b, a)];
+            if constexpr (DIFF) { bg[k] = sBG[iB(le, k, b, a)]; gb[k] = sGB[iB(le, k, b, a)]; }
             c0[k] = c1[k] = c2[k] = 0;
         }
         constexpr size_t plane = (size_t)NE * T2;
@@ -301,8 +346,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         }
 #pragma unroll
         for (int k = 0; k < N1; ++k) {
-            sBB[le][k][b][a] = c2[k];
-            if constexpr (DIFF) { sBG[le][k][b][a] = c0[k]; sGB[le][k][b][a] = c1[k]; }
+            sBB[iB(le, k, b, a)] = c2[k];
+            if constexpr (DIFF) { sBG[iB(le, k, b, a)] = c0[k]; sGB[iB(le, k, b, a)] = c1[k]; }
         }
     }
     __syncthreads();
@@ -313,8 +358,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         T C0[Q], C1[Q], C2[Q];
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
-            C2[b] = sBB[le][k][b][a];
-            if constexpr (DIFF) { C0[b] = sBG[le][k][b][a]; C1[b] = sGB[le][k][b][a]; }
+            C2[b] = sBB[iB(le, k, b, a)];
+            if constexpr (DIFF) { C0[b] = sBG[iB(le, k, b, a)]; C1[b] = sGB[iB(le, k, b, a)]; }
         }
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
@@ -327,8 +372,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     fb += Gt[b * N1 + j] * C1[b];
                 }
             }
-            sA[le][k][j][a] = fb;
-            if constexpr (DIFF) sAg[le][k][j][a] = fg;
+            sA[iA(le, k, j, a)] = fb;
+            if constexpr (DIFF) sAg[iA(le, k, j, a)] = fg;
         }
     }
     __syncthreads();
@@ -343,8 +388,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         T FG[Q], FB[Q];
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
-            FB[a] = sA[le][kx][jx][a];
-            if constexpr (DIFF) FG[a] = sAg[le][kx][jx][a];
+            FB[a] = sA[iA(le, kx, jx, a)];
+            if constexpr (DIFF) FG[a] = sAg[iA(le, kx, jx, a)];
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
@@ -356,7 +401,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             }
             sv[i] = s;
         }
-        sBB[le][kx][jx][0] = sv[0];
+        sBB[iB(le, kx, jx, 0)] = sv[0];
     }
     __syncthreads();
     if (xt) {
@@ -367,7 +412,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         for (int i = 1; i < P; ++i) dst[i] = sv[i];
         // the next element is in this workgroup (and row) unless this one ends
         // the row or the workgroup
-        if (ex < g.nx - 1 && le + 1 < NE) dst[P] = sv[P] + sBB[le + 1][kx][jx][0];
+        if (ex < g.nx - 1 && le + 1 < NE) dst[P] = sv[P] + sBB[iB(le + 1, kx, jx, 0)];
         else dst[P] = sv[P];
     }
 }
