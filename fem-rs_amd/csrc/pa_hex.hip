@@ -140,6 +140,20 @@ __host__ __device__ __forceinline__ size_t qd_index(int Q, int ncomp, long long 
     const long long blk = e / NE, le = e - blk * NE;
     return (((size_t)blk * ncomp + comp) * Q + qz) * (size_t)(NE * T2) + (size_t)le * T2 + t;
 }
+// Thread-per-element layout (k_pa_hex_tpe): groups of 64 consecutive elements
+// (one wavefront), point-major inside a group and element-minor, so every
+// (component, point) load of a wavefront is one contiguous run of 64 values:
+//   index = ((grp * ncomp + comp) * Q^3 + qp) * 64 + (e mod 64),  qp = qx + Q (qy + Q qz)
+__host__ __device__ __forceinline__ size_t qd_index_tpe(int Q, int ncomp, long long e, int comp, int qp) {
+    const long long grp = e >> 6;
+    return (((size_t)grp * ncomp + comp) * (size_t)(Q * Q * Q) + qp) * 64 + (size_t)(e & 63);
+}
+// values per component in either layout (E rounded up to whole blocks)
+static inline size_t qd_count_any(long long E, int Q) {
+    const long long NE = ne_of(Q);
+    const size_t a = (size_t)((E + NE - 1) / NE) * NE, b = (size_t)((E + 63) / 64) * 64;
+    return (a > b ? a : b) * Q * Q * Q;
+}
 // values per component (E rounded up to whole blocks)
 static inline size_t qd_count(long long E, int Q) {
     const long long NE = ne_of(Q);
@@ -417,6 +431,194 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 }
 
+// Thread-per-element diffusion apply for low order (Q2 with 3^3 points): one
+// thread holds its element's N1^3 values in registers through all five
+// contraction phases (no LDS, no barriers), reads its qdata in the
+// wavefront-contiguous qd_index_tpe layout, and writes the same x-merged
+// element vector as k_pa_hex with groups of 64 elements (one wavefront):
+// the x-face value shared with the next element of the wavefront and row
+// comes from that lane by a shuffle (lower element first), and the first
+// element of a wavefront with ex > 0 stores its i = 0 value in the seam
+// area, which the gather adds.
+template <int N1, int Q, class T, bool CART>
+__global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__restrict__ qd,
+                                                    const T *__restrict__ cf, const T *__restrict__ x,
+                                                    T *__restrict__ y, const int dirichlet, const int *skip) {
+    constexpr int P = N1 - 1;
+    constexpr int C = combo(N1, Q);
+    if (skip && *skip) return;
+    const T *Bt = Tab<T>::B(C);
+    const T *Gt = Tab<T>::G(C);
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = e < g.E;
+    const int lane = (int)(threadIdx.x & 63);
+    int ex = 0, ey = 0, ez = 0;
+    if (valid) {
+        ex = (int)(e % g.nx);
+        const long long r = e / g.nx;
+        ey = (int)(r % g.ny);
+        ez = (int)(r / g.ny);
+    }
+    T yv[N1][N1][N1];  // [k][j][i]
+    if (valid) {
+        // ---- X: contract the element's x-lines
+        T A[N1][N1][Q], Ag[N1][N1][Q];
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const long long gy = (long long)ey * P + j, gz = (long long)ez * P + k;
+                const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
+                const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+                T u[N1];
+#pragma unroll
+                for (int i = 0; i < N1; ++i) {
+                    const long long gx = (long long)ex * P + i;
+                    const bool b = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
+                    u[i] = b ? T(0) : x[base + i];
+                }
+#pragma unroll
+                for (int a = 0; a < Q; ++a) {
+                    T s = 0, sg = 0;
+#pragma unroll
+                    for (int i = 0; i < N1; ++i) {
+                        s += Bt[a * N1 + i] * u[i];
+                        sg += Gt[a * N1 + i] * u[i];
+                    }
+                    A[k][j][a] = s;
+                    Ag[k][j][a] = sg;
+                }
+            }
+        // ---- Y
+        T BB[N1][Q][Q], BG[N1][Q][Q], GB[N1][Q][Q];
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    T bb = 0, bg = 0, gb = 0;
+#pragma unroll
+                    for (int j = 0; j < N1; ++j) {
+                        bb += Bt[b * N1 + j] * A[k][j][a];
+                        bg += Bt[b * N1 + j] * Ag[k][j][a];
+                        gb += Gt[b * N1 + j] * A[k][j][a];
+                    }
+                    BB[k][b][a] = bb;
+                    BG[k][b][a] = bg;
+                    GB[k][b][a] = gb;
+                }
+        // ---- Z + pointwise + Z^T, column by column
+        T fx = 0, fy = 0, fz = 0;
+        if constexpr (CART) {
+            fx = cf[ex];
+            fy = cf[g.nx + ey];
+            fz = cf[g.nx + g.ny + ez];
+        }
+#pragma unroll
+        for (int b = 0; b < Q; ++b)
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                T c0[N1], c1[N1], c2[N1];
+#pragma unroll
+                for (int k = 0; k < N1; ++k) c0[k] = c1[k] = c2[k] = 0;
+#pragma unroll
+                for (int c = 0; c < Q; ++c) {
+                    T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+                    for (int k = 0; k < N1; ++k) {
+                        g0 += Bt[c * N1 + k] * BG[k][b][a];
+                        g1 += Bt[c * N1 + k] * GB[k][b][a];
+                        g2 += Gt[c * N1 + k] * BB[k][b][a];
+                    }
+                    const int qp = a + Q * (b + Q * c);
+                    T v0, v1, v2;
+                    if constexpr (CART) {
+                        const T sq = qd[qd_index_tpe(Q, 1, e, 0, qp)];
+                        v0 = (sq * fx) * g0;
+                        v1 = (sq * fy) * g1;
+                        v2 = (sq * fz) * g2;
+                    } else {
+                        const T d0 = qd[qd_index_tpe(Q, 6, e, 0, qp)], d1 = qd[qd_index_tpe(Q, 6, e, 1, qp)];
+                        const T d2 = qd[qd_index_tpe(Q, 6, e, 2, qp)], d3 = qd[qd_index_tpe(Q, 6, e, 3, qp)];
+                        const T d4 = qd[qd_index_tpe(Q, 6, e, 4, qp)], d5 = qd[qd_index_tpe(Q, 6, e, 5, qp)];
+                        v0 = d0 * g0 + d1 * g1 + d2 * g2;
+                        v1 = d1 * g0 + d3 * g1 + d4 * g2;
+                        v2 = d2 * g0 + d4 * g1 + d5 * g2;
+                    }
+#pragma unroll
+                    for (int k = 0; k < N1; ++k) {
+                        c0[k] += Bt[c * N1 + k] * v0;
+                        c1[k] += Bt[c * N1 + k] * v1;
+                        c2[k] += Gt[c * N1 + k] * v2;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    BG[k][b][a] = c0[k];
+                    GB[k][b][a] = c1[k];
+                    BB[k][b][a] = c2[k];
+                }
+            }
+        // ---- Y^T
+        T FB[N1][N1][Q], FG[N1][N1][Q];
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+#pragma unroll
+            for (int a = 0; a < Q; ++a)
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    T fg = 0, fb = 0;
+#pragma unroll
+                    for (int b = 0; b < Q; ++b) {
+                        fb += Bt[b * N1 + j] * BB[k][b][a];
+                        fg += Bt[b * N1 + j] * BG[k][b][a];
+                        fb += Gt[b * N1 + j] * GB[k][b][a];
+                    }
+                    FB[k][j][a] = fb;
+                    FG[k][j][a] = fg;
+                }
+        // ---- X^T
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+#pragma unroll
+            for (int j = 0; j < N1; ++j)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) {
+                    T s = 0;
+#pragma unroll
+                    for (int a = 0; a < Q; ++a) {
+                        s += Bt[a * N1 + i] * FB[k][j][a];
+                        s += Gt[a * N1 + i] * FG[k][j][a];
+                    }
+                    yv[k][j][i] = s;
+                }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+#pragma unroll
+            for (int j = 0; j < N1; ++j)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) yv[k][j][i] = 0;
+    }
+    // ---- write the x-merged rows; every lane takes part in the shuffles
+    const bool has_next = valid && lane < 63 && ex < g.nx - 1 && e + 1 < g.E;
+#pragma unroll
+    for (int k = 0; k < N1; ++k)
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const T next0 = __shfl_down(yv[k][j][0], 1, 64);
+            if (valid) {
+                T *dst = y + mrow_index(g, N1, ey, ez, j, k) + (size_t)ex * P;
+                if (ex == 0) dst[0] = yv[k][j][0];
+                else if (lane == 0) y[seam_index(g, N1, e >> 6, j, k)] = yv[k][j][0];
+#pragma unroll
+                for (int i = 1; i < P; ++i) dst[i] = yv[k][j][i];
+                dst[P] = has_next ? yv[k][j][P] + next0 : yv[k][j][P];
+            }
+        }
+}
+
 template <class T> struct Pair;
 template <> struct Pair<double> { using type = double2; };
 template <> struct Pair<float> { using type = float2; };
@@ -646,7 +848,7 @@ template <class T>
 __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ xq,
                             const double *__restrict__ wq, const double *__restrict__ coords,
                             CoefDev coef, double rho, int form, T *__restrict__ qd,
-                            T *__restrict__ qm, int mass_only_weight, int cart) {
+                            T *__restrict__ qm, int mass_only_weight, int cart, int tpe) {
     const int Q3 = Q * Q * Q, T2 = Q * Q;
     const size_t total = (size_t)g.E * Q3;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
@@ -684,7 +886,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
         const int t2 = qx + Q * qy;
         if ((form & 1) && qm) qm[qd_index(Q, 1, e, 0, qz, t2)] = (T)(mass_only_weight ? wd : wd * rho);
         if ((form & 2) && qd && cart) {
-            qd[qd_index(Q, 1, e, 0, qz, t2)] = (T)(wd * coef_eval(coef, xp, e, qp, Q3));
+            qd[tpe ? qd_index_tpe(Q, 1, e, 0, qp) : qd_index(Q, 1, e, 0, qz, t2)] =
+                (T)(wd * coef_eval(coef, xp, e, qp, Q3));
         } else if ((form & 2) && qd) {
             const double id = 1.0 / det;
             double I[3][3];
@@ -706,7 +909,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
                 s * (I[1][0] * I[2][0] + I[1][1] * I[2][1] + I[1][2] * I[2][2]),
                 s * (I[2][0] * I[2][0] + I[2][1] * I[2][1] + I[2][2] * I[2][2])};
 #pragma unroll
-            for (int comp = 0; comp < 6; ++comp) qd[qd_index(Q, 6, e, comp, qz, t2)] = (T)D[comp];
+            for (int comp = 0; comp < 6; ++comp)
+                qd[tpe ? qd_index_tpe(Q, 6, e, comp, qp) : qd_index(Q, 6, e, comp, qz, t2)] = (T)D[comp];
         }
     }
 }
@@ -715,7 +919,7 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
 template <class T>
 __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restrict__ B,
                            const double *__restrict__ G, const T *__restrict__ qd,
-                           const T *__restrict__ qm, const T *__restrict__ cf, int cart, int form,
+                           const T *__restrict__ qm, const T *__restrict__ cf, int cart, int tpe, int form,
                            T *__restrict__ ye) {
     // Element diagonals into the element vector; k_hex_gather then sums them
     // per DOF in a fixed order.
@@ -739,12 +943,14 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
                                      g2 = Bi * Bj * G[c * N1 + k];
                         double d[6];
                         if (cart) {
-                            const double sq = (double)qd[qd_index(Q, 1, e, 0, c, t2)];
+                            const int qp = t2 + Q * Q * c;
+                            const double sq = (double)qd[tpe ? qd_index_tpe(Q, 1, e, 0, qp) : qd_index(Q, 1, e, 0, c, t2)];
                             d[0] = sq * (double)cf[ex]; d[3] = sq * (double)cf[g.nx + ey];
                             d[5] = sq * (double)cf[g.nx + g.ny + ez];
                             d[1] = d[2] = d[4] = 0.0;
                         } else {
-                            for (int m = 0; m < 6; ++m) d[m] = (double)qd[qd_index(Q, 6, e, m, c, t2)];
+                            for (int m = 0; m < 6; ++m)
+                                d[m] = (double)qd[tpe ? qd_index_tpe(Q, 6, e, m, t2 + Q * Q * c) : qd_index(Q, 6, e, m, c, t2)];
                         }
                         s += d[0] * g0 * g0 + d[3] * g1 * g1 + d[5] * g2 * g2 +
                              2.0 * (d[1] * g0 * g1 + d[2] * g0 * g2 + d[4] * g1 * g2);
@@ -821,9 +1027,16 @@ static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, co
         hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
 }
 
+// tpe: the operator keeps thread-per-element qdata (Q2 diffusion, 3^3 points)
 template <class T>
 static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm, const T *cf,
-                      const T *x, T *y, int dir, const int *skip, hipStream_t s) {
+                      const T *x, T *y, int dir, const int *skip, hipStream_t s, bool tpe = false) {
+    if (tpe && form == 2 && n1 == 3 && q == 3) {
+        const dim3 grid((unsigned)((g.E + 255) / 256)), block(256);
+        if (cf) hipLaunchKernelGGL((k_pa_hex_tpe<3, 3, T, true>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
+        else hipLaunchKernelGGL((k_pa_hex_tpe<3, 3, T, false>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
+        return true;
+    }
 #define CASE(A, B)                                                                 \
     if (n1 == A && q == B) { launch_form<A, B, T>(g, form, qd, qm, cf, x, y, dir, skip, s); return true; }
     CASE(2, 2) CASE(2, 3) CASE(3, 3) CASE(3, 4) CASE(4, 4) CASE(4, 5) CASE(5, 5) CASE(5, 6)
@@ -866,6 +1079,7 @@ struct HexOp final : femgpu_op {
     DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
     DevBuf evec;  // x-merged element vector between the two apply kernels
     bool cart = false;           // axis-aligned box mesh: Cartesian qdata (one value per point)
+    bool tpe = false;            // thread-per-element kernel and qdata layout (Q2 diffusion)
     std::vector<double> cf_host;  // Cartesian per-axis factors [nx | ny | nz]
     DevBuf cf;                    // the same, in the operator's type
 
@@ -882,7 +1096,7 @@ struct HexOp final : femgpu_op {
         const int dir = desc.dirichlet ? 1 : 0;
         if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
-                     skip, s);
+                     skip, s, tpe);
         launch_gather<T>(desc.p, g, evec.as<T>(), x, y, acc ? 1 : 0, dir, skip, s);
         if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
     }
@@ -897,7 +1111,7 @@ struct HexOp final : femgpu_op {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
-                     skip, s);
+                     skip, s, tpe);
         const dim3 grid((unsigned)red_blocks(n)), block(256);
         const T *ye = evec.as<T>();
 #define GF(P_)                                                                                              \
@@ -948,8 +1162,8 @@ struct HexOp final : femgpu_op {
         const size_t L = (size_t)g.E * n1 * n1 * n1;
         DevBuf old(L * sizeof(T));
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
-                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, desc.form,
-                           old.as<T>());
+                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
+                           desc.form, old.as<T>());
         hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(g, n1))), dim3(256), 0, s, g, n1,
                            (const T *)old.as<T>(), evec.as<T>());
         launch_gather<T>(desc.p, g, evec.as<T>(), d, d, 0, 0, nullptr, s);
@@ -973,7 +1187,7 @@ struct HexOp final : femgpu_op {
         const size_t total = (size_t)g.E * desc.q * desc.q * desc.q;
         hipLaunchKernelGGL(k_hex_setup<T>, dim3(grid1(total)), dim3(256), 0, ctx->stream, g, desc.q,
                            xq.as<double>(), wq.as<double>(), coords.as<double>(), c, rho, form,
-                           qd_, qm_, mass_weight_only ? 1 : 0, cart ? 1 : 0);
+                           qd_, qm_, mass_weight_only ? 1 : 0, cart ? 1 : 0, tpe ? 1 : 0);
         FG_HIP(hipGetLastError());
     }
 
@@ -988,8 +1202,11 @@ struct HexOp final : femgpu_op {
         FG_HIP(hipMemcpyAsync(wq.ptr, w, 8 * q, hipMemcpyHostToDevice, s));
         FG_HIP(hipMemcpyAsync(Bd.ptr, B, 8 * q * n1, hipMemcpyHostToDevice, s));
         FG_HIP(hipMemcpyAsync(Gd.ptr, G, 8 * q * n1, hipMemcpyHostToDevice, s));
-        const size_t nq = qd_count(g.E, q);
+        const size_t nq = qd_count_any(g.E, q);
         const size_t es = esize();
+        const char *kmode = std::getenv("FEMGPU_HEX_KERNEL");
+        tpe = n1 == 3 && q == 3 && desc.form == 2 && !(kmode && std::strcmp(kmode, "lds") == 0);
+        if (tpe) g.ne = 64;  // element-vector seams every wavefront
         if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
         if (cart) {
@@ -1002,7 +1219,9 @@ struct HexOp final : femgpu_op {
                 FG_HIP(hipStreamSynchronize(s));
             }
         }
-        evec.alloc(evec_count(g, n1) * es);
+        HexGrid gl = g;
+        gl.ne = ne_of(q);  // the workgroup kernel's grouping (mass-weight applies)
+        evec.alloc(std::max(evec_count(g, n1), evec_count(gl, n1)) * es);
         if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
         else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
         FG_HIP(hipStreamSynchronize(s));
@@ -1230,21 +1449,23 @@ int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
         void *dst = mem == FEMGPU_DEVICE ? b : out.ptr;
         hipStream_t s = c->stream;
         const int n1 = op->desc.p + 1, q = op->desc.q;
+        HexGrid gl = op->g;
+        gl.ne = ne_of(q);  // the workgroup kernel's element-vector grouping
         // b = M_1 1 (mass with weights w|J|), then zero on essential DOFs
         if (op->dtype == FEMGPU_F64) {
             op->setup_t<double>(1, 1.0, true, nullptr, tq.as<double>());
             fill<double>(s, ones.as<double>(), 1.0, op->n);
-            launch_pa<double>(n1, q, op->g, 1, nullptr, tq.as<double>(), nullptr, ones.as<double>(), op->evec.as<double>(), 0,
+            launch_pa<double>(n1, q, gl, 1, nullptr, tq.as<double>(), nullptr, ones.as<double>(), op->evec.as<double>(), 0,
                               nullptr, s);
-            launch_gather<double>(op->desc.p, op->g, op->evec.as<double>(), ones.as<double>(), (double *)dst, 0, 0, nullptr, s);
+            launch_gather<double>(op->desc.p, gl, op->evec.as<double>(), ones.as<double>(), (double *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<double>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const double *)dst, (double *)dst, 3, op->desc.dirichlet, nullptr);
         } else {
             op->setup_t<float>(1, 1.0, true, nullptr, tq.as<float>());
             fill<float>(s, ones.as<float>(), 1.f, op->n);
-            launch_pa<float>(n1, q, op->g, 1, nullptr, tq.as<float>(), nullptr, ones.as<float>(), op->evec.as<float>(), 0,
+            launch_pa<float>(n1, q, gl, 1, nullptr, tq.as<float>(), nullptr, ones.as<float>(), op->evec.as<float>(), 0,
                              nullptr, s);
-            launch_gather<float>(op->desc.p, op->g, op->evec.as<float>(), ones.as<float>(), (float *)dst, 0, 0, nullptr, s);
+            launch_gather<float>(op->desc.p, gl, op->evec.as<float>(), ones.as<float>(), (float *)dst, 0, 0, nullptr, s);
             hipLaunchKernelGGL(k_hex_bc<float>, dim3(grid1(op->n)), dim3(256), 0, s, op->g,
                                (const float *)dst, (float *)dst, 3, op->desc.dirichlet, nullptr);
         }

@@ -267,3 +267,39 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
     assert not P.cartesian and P.qdata_per_point() == 6
     M = fg.HexPAOperator(2, 3, n, form=fg.MASS, rho=1.0, ctx=ctx)
     assert not M.cartesian and M.qdata_per_point() == 1
+
+
+@pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3)])
+def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
+    """Q2 diffusion (3^3 points) runs one thread per element with 64-element
+    element-vector groups; its applies, diagonal and solves equal the workgroup
+    kernel's (FEMGPU_HEX_KERNEL=lds) on meshes whose rows are shorter or longer
+    than a wavefront, box-shaped and perturbed, FP64 and FP32."""
+    rng = np.random.default_rng(11)
+    nx, ny, nz = shape
+    for verts in (None, fg.hex_vertices(nx, ny, nz, amp=0.15, seed=3)):
+        for dirichlet in (True, False):
+            kw = dict(form=fg.DIFFUSION, kappa="sinprod", dirichlet=dirichlet, vertices=verts, ctx=ctx)
+            K = fg.HexPAOperator(2, 3, shape, **kw)
+            monkeypatch.setenv("FEMGPU_HEX_KERNEL", "lds")
+            L = fg.HexPAOperator(2, 3, shape, **kw)
+            monkeypatch.delenv("FEMGPU_HEX_KERNEL")
+            x = rng.standard_normal(K.n_dofs())
+            yk, yl = K.matvec(x), L.matvec(x)
+            assert np.abs(yk - yl).max() <= 1e-13 * np.abs(yl).max()
+            dk, dl = K.diagonal(), L.diagonal()
+            assert np.abs(dk - dl).max() <= 1e-13 * np.abs(dl).max()
+            K32, L32 = K.convert(fg.F32), L.convert(fg.F32)
+            y32, l32 = K32.matvec(x.astype(np.float32)), L32.matvec(x.astype(np.float32))
+            assert np.abs(y32.astype(np.float64) - yl).max() <= 1e-5 * np.abs(yl).max()
+            assert np.abs(l32.astype(np.float64) - yl).max() <= 1e-5 * np.abs(yl).max()
+            if dirichlet:
+                b = K.rhs_unit()
+                assert np.array_equal(b, L.rhs_unit())
+                xk, xl = np.zeros(K.n_dofs()), np.zeros(K.n_dofs())
+                rk, _ = fg.solve_pcg_chebyshev(K, b, xk, fg.SolverConfig(rtol=1e-10, max_iter=2000))
+                rl, _ = fg.solve_pcg_chebyshev(L, b, xl, fg.SolverConfig(rtol=1e-10, max_iter=2000))
+                assert rk.converged and rl.converged and abs(rk.iterations - rl.iterations) <= 1
+                assert np.abs(xk - xl).max() <= 1e-8 * np.abs(xl).max()
+            for op in (K, L, K32, L32):
+                op.close()
