@@ -431,7 +431,9 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 }
 
-// Thread-per-element diffusion apply for low order (Q2 with 3^3 points): one
+// Thread-per-element diffusion apply for low order (Q2 with 3^3 points, used
+// with Cartesian qdata, where the workgroup kernel is bound by its LDS
+// hand-offs rather than by HBM): one
 // thread holds its element's N1^3 values in registers through all five
 // contraction phases (no LDS, no barriers), reads its qdata in the
 // wavefront-contiguous qd_index_tpe layout, and writes the same x-merged
@@ -1205,7 +1207,8 @@ struct HexOp final : femgpu_op {
         const size_t nq = qd_count_any(g.E, q);
         const size_t es = esize();
         const char *kmode = std::getenv("FEMGPU_HEX_KERNEL");
-        tpe = n1 == 3 && q == 3 && desc.form == 2 && !(kmode && std::strcmp(kmode, "lds") == 0);
+        // with the general six-value qdata the workgroup kernel is as fast (both HBM-bound) at higher occupancy
+        tpe = n1 == 3 && q == 3 && desc.form == 2 && cart && !(kmode && std::strcmp(kmode, "lds") == 0);
         if (tpe) g.ne = 64;  // element-vector seams every wavefront
         if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
