@@ -644,8 +644,12 @@ def main():
             "unit": "GB/s",
             "frac": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "traffic": committed_traffic(a.workload, n, p, s_comp),
-            "kernel": f"PA apply = k_pa_hex<{q},{q},double,{kform},{'true' if K.cartesian else 'false'}> element "
-                      f"kernel ({s_comp} qdata value(s) per Gauss point"
+            "kernel": (f"PA apply = k_pa_hex_tpe<{q},{q},double,{'true' if K.cartesian else 'false'}> element kernel "
+                       f"(one thread per element"
+                       if K.element_kernel() == "thread-per-element" else
+                       f"PA apply = k_pa_hex<{q},{q},double,{kform},{'true' if K.cartesian else 'false'}> element "
+                       f"kernel (one workgroup per {256 // (q * q)} elements") +
+                      f"; {s_comp} qdata value(s) per Gauss point"
                       f"{', axis-aligned mesh' if K.cartesian else ''}) + "
                       f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)",
             "algorithmic_bytes": bytes_apply,

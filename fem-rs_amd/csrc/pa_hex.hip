@@ -1427,6 +1427,15 @@ int femgpu_pa_hex_qdata_per_point(const femgpu_op *op_, int *values) {
     });
 }
 
+int femgpu_pa_hex_element_kernel(const femgpu_op *op_, int *kind) {
+    return guard([&] {
+        auto *op = dynamic_cast<const HexOp *>(op_);
+        FG_REQUIRE(op && kind, FEMGPU_ERR_INVALID_ARG, "pa_hex_element_kernel: not a hex operator");
+        *kind = op->tpe ? 1 : 0;
+        return 0;
+    });
+}
+
 int femgpu_pa_hex_convert(femgpu_op *op_, femgpu_dtype dtype, femgpu_op **out) {
     return guard([&] {
         auto *op = dynamic_cast<HexOp *>(op_);

@@ -129,6 +129,7 @@ _sig("femgpu_pa_hex_rhs_unit", [P, P, I])
 _sig("femgpu_pa_hex_rhs_random", [P, U64, P, I])
 _sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
 _sig("femgpu_pa_hex_qdata_per_point", [P, C.POINTER(I)])
+_sig("femgpu_pa_hex_element_kernel", [P, C.POINTER(I)])
 _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
@@ -568,6 +569,13 @@ class HexPAOperator(Operator):
         v = I()
         _check(lib.femgpu_pa_hex_qdata_per_point(self.h, C.byref(v)))
         return v.value
+
+    def element_kernel(self):
+        """"workgroup" (LDS hand-offs between phases) or "thread-per-element"
+        (Q2 diffusion with Cartesian qdata)."""
+        v = I()
+        _check(lib.femgpu_pa_hex_element_kernel(self.h, C.byref(v)))
+        return "thread-per-element" if v.value == 1 else "workgroup"
 
     @property
     def cartesian(self):
