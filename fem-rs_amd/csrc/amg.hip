@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -257,6 +258,15 @@ struct AmgLevel {
 
 }  // namespace femgpu
 
+// A cycle captured as a HIP graph for one (input, output, stop flag, cycle kind).
+struct AmgGraph {
+    const double *r = nullptr;
+    double *z = nullptr;
+    const int *skip = nullptr;
+    int cycle = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct femgpu_amg {
     femgpu_ctx *ctx = nullptr;
     femgpu_op *op = nullptr;  // borrowed level-0 operator
@@ -264,6 +274,11 @@ struct femgpu_amg {
     int cycle = FEMGPU_CYCLE_V;
     std::vector<femgpu::AmgLevel> L;
     femgpu::DevBuf coarse_inv;  // dense inverse of the coarsest operator
+    std::vector<AmgGraph> graphs;  // most recent last, at most kAmgGraphs
+    ~femgpu_amg() {
+        for (AmgGraph &g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 namespace femgpu {
@@ -433,8 +448,48 @@ static void amg_cycle(const femgpu_amg *h, size_t l, int kind, const double *b, 
     hipLaunchKernelGGL(k_amg_add, dim3(g256(n)), dim3(256), 0, s, n, (const double *)z, x, skip);
 }
 
+static bool amg_graphs_enabled() {
+    const char *e = std::getenv("FEMGPU_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+constexpr size_t kAmgGraphs = 8;
+
+// One cycle from r into z.  The cycle's kernels (about a dozen per level, all
+// small on the coarse levels) are captured once per (r, z, skip, cycle kind)
+// into a HIP graph and replayed, so a PCG iteration submits one graph instead
+// of every kernel; the graph records kernels only (no memset or copy nodes).
+// FEMGPU_GRAPHS=0 launches them directly.
 void amg_apply(const femgpu_amg *h, const double *r, double *z, const int *skip) {
-    amg_cycle(h, 0, h->cycle, r, z, h->ctx->stream, skip);
+    hipStream_t s = h->ctx->stream;
+    if (!amg_graphs_enabled()) {
+        amg_cycle(h, 0, h->cycle, r, z, s, skip);
+        return;
+    }
+    auto *hm = const_cast<femgpu_amg *>(h);
+    for (const AmgGraph &g : hm->graphs)
+        if (g.r == r && g.z == z && g.skip == skip && g.cycle == h->cycle) {
+            FG_HIP(hipGraphLaunch(g.exec, s));
+            return;
+        }
+    hipGraph_t graph = nullptr;
+    FG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    amg_cycle(h, 0, h->cycle, r, z, s, skip);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(s, &graph);
+    FG_HIP(le);
+    FG_HIP(ce);
+    AmgGraph g;
+    g.r = r; g.z = z; g.skip = skip; g.cycle = h->cycle;
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    FG_HIP(ie);
+    if (hm->graphs.size() >= kAmgGraphs) {
+        (void)hipGraphExecDestroy(hm->graphs.front().exec);
+        hm->graphs.erase(hm->graphs.begin());
+    }
+    hm->graphs.push_back(g);
+    FG_HIP(hipGraphLaunch(g.exec, s));
 }
 
 femgpu_op *amg_operator(const femgpu_amg *h) { return h->op; }
