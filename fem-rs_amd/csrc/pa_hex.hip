@@ -11,20 +11,28 @@
 // where D = w kappa |J| J^-1 J^-T and M = w rho |J| are the quadrature data
 // ("qdata") computed once at set-up from the trilinear (HexQ1) geometry.
 //
-// Kernel layout (one workgroup = NE elements x Q*Q lanes, 64-wide waves):
+// An apply is two kernels: the element kernel writes an element vector (the
+// x-merged layout below: x-neighbours inside a group summed, no atomics), and
+// k_hex_gather sums it per DOF in a fixed order, so applies are bitwise
+// reproducible.
+//
+// Workgroup element kernel k_pa_hex (one workgroup = NE elements x Q*Q lanes,
+// 64-wide waves):
 //   X  : lane (j,k) reads its x-line of the element from HBM and contracts it
 //        in registers -> LDS
 //   Y  : lane (qx,k) contracts along y in registers -> LDS
 //   Z  : lane (qx,qy) contracts along z, streams its qdata column from HBM,
 //        applies D / M and immediately contracts back along z, all in
 //        registers -> LDS (in place, no hazard: each lane owns its column)
-//   Y^T, X^T as Y, X; X^T scatters with hardware f64/f32 atomics.
+//   Y^T, X^T as Y, X; X^T writes the element's rows of the element vector.
 // Every contraction of a line is done in registers, so LDS carries ~10 values
 // per element point while the 1-D tables come from the scalar cache
-// (compile-time-indexed __constant__ memory).
+// (compile-time-indexed __constant__ memory).  Q2 diffusion with Cartesian
+// qdata instead runs k_pa_hex_tpe, one element per thread with no LDS.
 //
-// qdata is stored plane-major: plane (comp, qz) holds [E][Q*Q] values, so the
-// Z phase of a wavefront reads fully contiguous, coalesced runs.
+// qdata is stored tiled by workgroup and plane-major inside a tile (qd_index),
+// so the Z phase of a wavefront reads contiguous, coalesced runs;
+// k_pa_hex_tpe's layout (qd_index_tpe) is contiguous across its 64 lanes.
 //
 // On an axis-aligned box mesh (vertex x depends on i only, y on j, z on k) J
 // is diagonal and constant per element, D = (w kappa |J|) diag(1/J_xx^2,
