@@ -40,3 +40,30 @@ def test_bench_refuses_several_gpus():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def _bench_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_static_copy", os.path.join(ROOT, "bench.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)  # defines functions only; main() is not run
+    return m
+
+
+def test_committed_traffic_matches_operator_form():
+    """roofline.traffic is read from the committed counter summary only for
+    the workload's own size and quadrature-data form; every committed summary
+    states both figures."""
+    b = _bench_module()
+    for name, w in b.WORKLOADS.items():
+        path = os.path.join(ROOT, "profiles", f"pa_hex_{name}_hbm_traffic.md")
+        if not os.path.exists(path):
+            continue
+        text = open(path).read()
+        assert "traffic_bytes_per_apply:" in text and "qdata_per_point:" in text
+        import re
+        sq = int(re.search(r"qdata_per_point:\s*([0-9]+)", text).group(1))
+        t = b.committed_traffic(name, w["n"], w["p"], sq)
+        assert t is not None and t > 0
+        assert b.committed_traffic(name, w["n"], w["p"], sq + 1) is None  # other qdata form
+        assert b.committed_traffic(name, w["n"] + 1, w["p"], sq) is None  # other size
