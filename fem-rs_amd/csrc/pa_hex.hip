@@ -104,6 +104,7 @@ struct HexGrid {
     long long Nx, Ny, Nz;
     int ne;         // elements per element-kernel workgroup, ne_of(Q)
     long long Nxp;  // row stride of the x-merged element vector: Nx rounded up to even
+    int ymerge;     // k_pa_hex_tpe with nx % 64 == 0: y-faces merged inside 256-element blocks
 };
 
 
@@ -611,6 +612,31 @@ __global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__
 #pragma unroll
                 for (int i = 0; i < N1; ++i) yv[k][j][i] = 0;
     }
+    // ---- y-merge (g.ymerge: nx is a multiple of 64, so a wavefront lies in one
+    // element row and the y-neighbour e + nx is wavefront w + nx/64 of the same
+    // 256-element block when that exists): the upper face j = P goes through
+    // LDS to the next row's wavefront, which adds it to its j = 0 face (lower
+    // element first) and writes it; the lower element writes no j = P rows.
+    bool yhi = false;
+    if (g.ymerge) {
+        __shared__ T sy[4][N1][N1][64];  // [wavefront][k][i][lane], the j = P face
+        const int w = (int)(threadIdx.x >> 6), m = g.nx >> 6;
+        yhi = valid && w + m < 4 && ey < g.ny - 1;
+        const bool ylo = valid && w >= m && ey > 0;
+        if (yhi) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) sy[w][k][i][lane] = yv[k][P][i];
+        }
+        __syncthreads();
+        if (ylo) {
+#pragma unroll
+            for (int k = 0; k < N1; ++k)
+#pragma unroll
+                for (int i = 0; i < N1; ++i) yv[k][0][i] = sy[w - m][k][i][lane] + yv[k][0][i];
+        }
+    }
     // ---- write the x-merged rows; every lane takes part in the shuffles
     const bool has_next = valid && lane < 63 && ex < g.nx - 1 && e + 1 < g.E;
 #pragma unroll
@@ -618,7 +644,7 @@ __global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
             const T next0 = __shfl_down(yv[k][j][0], 1, 64);
-            if (valid) {
+            if (valid && !(j == P && yhi)) {
                 T *dst = y + mrow_index(g, N1, ey, ez, j, k) + (size_t)ex * P;
                 if (ex == 0) dst[0] = yv[k][j][0];
                 else if (lane == 0) y[seam_index(g, N1, e >> 6, j, k)] = yv[k][j][0];
@@ -667,6 +693,12 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             if (!(cv[1][c] && cv[0][b])) continue;
+            if (g.ymerge && b == 0 && cv[0][1]) {
+                // lower y candidate already added into the upper one's j = 0 row when
+                // both elements lie in one 256-element block
+                const size_t e_lo = ((size_t)ce[1][c] * g.ny + (size_t)ce[0][0]) * g.nx + (size_t)ex;
+                if ((e_lo >> 8) == ((e_lo + (size_t)g.nx) >> 8)) continue;
+            }
             const I erow = (I)ce[1][c] * (I)g.ny + (I)ce[0][b];
             const I row = ((erow * N1 + (I)ci[1][c]) * N1 + (I)ci[0][b]) * (I)g.Nxp;
             const T *rp = ye + row + (I)ex * P;  // this segment's values on the row
@@ -1174,9 +1206,11 @@ struct HexOp final : femgpu_op {
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
                            Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
                            desc.form, old.as<T>());
-        hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(g, n1))), dim3(256), 0, s, g, n1,
+        HexGrid gd = g;
+        gd.ymerge = 0;  // k_evec_merge builds the element vector without the y-merge
+        hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(gd, n1))), dim3(256), 0, s, gd, n1,
                            (const T *)old.as<T>(), evec.as<T>());
-        launch_gather<T>(desc.p, g, evec.as<T>(), d, d, 0, 0, nullptr, s);
+        launch_gather<T>(desc.p, gd, evec.as<T>(), d, d, 0, 0, nullptr, s);
         hipLaunchKernelGGL(k_hex_bc<T>, dim3(grid1(n)), dim3(256), 0, s, g, (const T *)d, d, 2,
                            desc.dirichlet, nullptr);
         FG_HIP(hipStreamSynchronize(s));  // before the temporary is freed
@@ -1217,7 +1251,10 @@ struct HexOp final : femgpu_op {
         const char *kmode = std::getenv("FEMGPU_HEX_KERNEL");
         // with the general six-value qdata the workgroup kernel is as fast (both HBM-bound) at higher occupancy
         tpe = n1 == 3 && q == 3 && desc.form == 2 && cart && !(kmode && std::strcmp(kmode, "lds") == 0);
-        if (tpe) g.ne = 64;  // element-vector seams every wavefront
+        if (tpe) {
+            g.ne = 64;                     // element-vector seams every wavefront
+            g.ymerge = g.nx % 64 == 0 ? 1 : 0;  // wavefronts inside element rows
+        }
         if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);
         if (cart) {
@@ -1232,6 +1269,7 @@ struct HexOp final : femgpu_op {
         }
         HexGrid gl = g;
         gl.ne = ne_of(q);  // the workgroup kernel's grouping (mass-weight applies)
+        gl.ymerge = 0;
         evec.alloc(std::max(evec_count(g, n1), evec_count(gl, n1)) * es);
         if (dtype == FEMGPU_F64) setup_t<double>(desc.form, desc.rho, false, qd.as<double>(), qm.as<double>());
         else setup_t<float>(desc.form, desc.rho, false, qd.as<float>(), qm.as<float>());
@@ -1471,6 +1509,7 @@ int femgpu_pa_hex_rhs_unit(femgpu_op *op_, void *b, femgpu_mem mem) {
         const int n1 = op->desc.p + 1, q = op->desc.q;
         HexGrid gl = op->g;
         gl.ne = ne_of(q);  // the workgroup kernel's element-vector grouping
+        gl.ymerge = 0;
         // b = M_1 1 (mass with weights w|J|), then zero on essential DOFs
         if (op->dtype == FEMGPU_F64) {
             op->setup_t<double>(1, 1.0, true, nullptr, tq.as<double>());

@@ -269,12 +269,14 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
     assert not M.cartesian and M.qdata_per_point() == 1
 
 
-@pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3)])
+@pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3), (64, 5, 3), (128, 3, 2), (256, 2, 2)])
 def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
     """Q2 diffusion (3^3 points) runs one thread per element with 64-element
-    element-vector groups; its applies, diagonal and solves equal the workgroup
-    kernel's (FEMGPU_HEX_KERNEL=lds) on meshes whose rows are shorter or longer
-    than a wavefront, box-shaped and perturbed, FP64 and FP32."""
+    element-vector groups (and, when nx is a multiple of 64, y-faces merged
+    inside 256-element blocks); its applies, diagonal and solves equal the
+    workgroup kernel's (FEMGPU_HEX_KERNEL=lds) on meshes whose rows are
+    shorter or longer than a wavefront, with blocks that cross z-planes,
+    box-shaped and perturbed, FP64 and FP32."""
     rng = np.random.default_rng(11)
     nx, ny, nz = shape
     for verts in (None, fg.hex_vertices(nx, ny, nz, amp=0.15, seed=3)):
