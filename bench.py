@@ -216,6 +216,25 @@ def run_csr(a, w):
                "rel_residual": ra.final_residual,
                "speedup_vs_pcg_jacobi": round(t_med / float(np.median(ta)), 2)}
     amg.close()
+    # simplex PA applies (PADiffusionOperator / PAMassOperator, partial.rs:65-209) on a
+    # finer mesh of the same kind: one thread per element, per-element qdata, atomics into y
+    ns = 2048
+    xy2, conn2, _ = unit_square_mesh(ns)
+    N2, E2 = len(xy2), len(conn2)
+    xs2 = fg.DeviceArray(ctx, N2, np.float64).upload(np.random.default_rng(1).standard_normal(N2))
+    ys2 = fg.DeviceArray(ctx, N2, np.float64)
+    simplex = {"mesh": f"{ns}^2 unit square, P1 triangles ({E2} elements, {N2} DOFs)"}
+    for name, Op, kw, s_e in (("diffusion", fg.PADiffusionOperator, {"kappa": 1.0}, 3),
+                              ("mass", fg.PAMassOperator, {"rho": 1.0}, 1)):
+        op = Op(2, 1, xy2, conn2, conn2, N2, quad_order=3, ctx=ctx, **kw)
+        op.time_apply(xs2, ys2, 5)
+        ms, _ = op.time_apply(xs2, ys2, a.apply_reps)
+        byts = 16.0 * N2 + (8.0 * s_e + 12.0) * E2  # x, y; qdata values and 3 DOF indices per element
+        simplex[name] = {"ms": round(ms, 5), "gdofs": round(N2 / (ms * 1e-3) / 1e9, 3),
+                         "algorithmic_bytes": byts, "gb_s": round(byts / (ms * 1e-3) / 1e9, 1),
+                         "frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        op.close()
+    del xs2, ys2
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(N * sum(iters) / total / 1e9, 4), "unit": "GDOF/s", "n_gpus": 1,
@@ -236,6 +255,7 @@ def run_csr(a, w):
                      "kernel": "CSR SpMV k_spmv_row<double> (launch-latency-bound at this size)",
                      "algorithmic_bytes": spmv_bytes},
         "legs": {"amg_pcg": amg_leg},
+        "pa_simplex": simplex,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

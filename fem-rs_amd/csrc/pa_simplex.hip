@@ -23,13 +23,16 @@ namespace femgpu {
 
 constexpr int kMaxNl = 20, kMaxQp = 36;
 
-template <class T, int DIM>
-__global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, int form,
+// NL (local DOFs: 3 / 6 / 10 on triangles, 4 / 10 / 20 on tetrahedra) is a
+// template parameter so that the element's x and y values stay in registers.
+template <class T, int DIM, int NL>
+__global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                                                     const uint32_t *__restrict__ edofs,
                                                     const T *__restrict__ qdata,
                                                     const double *__restrict__ tab,
                                                     const T *__restrict__ x, T *__restrict__ y,
                                                     const int *skip) {
+    constexpr int nl = NL;
     // tab: [nq] weights, [nq][nl] phi, [nq][nl][DIM] grad (reference element)
     __shared__ T s_tab[kMaxQp * (1 + kMaxNl * (1 + 3))];
     if (skip && *skip) return;
@@ -39,7 +42,8 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, i
     const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const T *W = s_tab, *PHI = s_tab + nq, *GR = s_tab + nq + nq * nl;
-    T xe[kMaxNl], ye[kMaxNl];
+    T xe[NL], ye[NL];
+#pragma unroll
     for (int l = 0; l < nl; ++l) { xe[l] = x[edofs[e * nl + l]]; ye[l] = 0; }
     constexpr int NS = DIM == 2 ? 3 : 6;
     if (form == FEMGPU_DIFFUSION) {
@@ -48,6 +52,7 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, i
         for (int q = 0; q < nq; ++q) {
             const T *G = GR + (size_t)q * nl * DIM;
             T gu[3] = {0, 0, 0};
+#pragma unroll
             for (int l = 0; l < nl; ++l)
                 for (int d = 0; d < DIM; ++d) gu[d] += xe[l] * G[l * DIM + d];
             T v[3];
@@ -59,6 +64,7 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, i
                 v[1] = D[1] * gu[0] + D[3] * gu[1] + D[4] * gu[2];
                 v[2] = D[2] * gu[0] + D[4] * gu[1] + D[5] * gu[2];
             }
+#pragma unroll
             for (int i = 0; i < nl; ++i) {
                 T s = 0;
                 for (int d = 0; d < DIM; ++d) s += G[i * DIM + d] * v[d];
@@ -70,11 +76,14 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nl, int nq, i
         for (int q = 0; q < nq; ++q) {
             const T *ph = PHI + (size_t)q * nl;
             T u = 0;
+#pragma unroll
             for (int l = 0; l < nl; ++l) u += ph[l] * xe[l];
             const T wu = W[q] * m * u;
+#pragma unroll
             for (int i = 0; i < nl; ++i) ye[i] += wu * ph[i];
         }
     }
+#pragma unroll
     for (int l = 0; l < nl; ++l) unsafeAtomicAdd(&y[edofs[e * nl + l]], ye[l]);
 }
 
@@ -107,12 +116,18 @@ struct SimplexOp final : femgpu_op {
         hipStream_t s = ctx->stream;
         if (!acc) fill<T>(s, y, T(0), n, skip);
         if (!ne) return;
-        if (dim == 2)
-            hipLaunchKernelGGL((k_pa_simplex<T, 2>), dim3((ne + 255) / 256), dim3(256), 0, s, ne, nl, nq,
-                               form, edofs.as<uint32_t>(), qdata.as<T>(), tab.as<double>(), x, y, skip);
-        else
-            hipLaunchKernelGGL((k_pa_simplex<T, 3>), dim3((ne + 255) / 256), dim3(256), 0, s, ne, nl, nq,
-                               form, edofs.as<uint32_t>(), qdata.as<T>(), tab.as<double>(), x, y, skip);
+        const dim3 grid((unsigned)((ne + 255) / 256)), block(256);
+#define PAS(D_, NL_)                                                                                        \
+    hipLaunchKernelGGL((k_pa_simplex<T, D_, NL_>), grid, block, 0, s, ne, nq, form, edofs.as<uint32_t>(),     \
+                       qdata.as<T>(), tab.as<double>(), x, y, skip)
+        if (dim == 2 && nl == 3) PAS(2, 3);
+        else if (dim == 2 && nl == 6) PAS(2, 6);
+        else if (dim == 2 && nl == 10) PAS(2, 10);
+        else if (dim == 3 && nl == 4) PAS(3, 4);
+        else if (dim == 3 && nl == 10) PAS(3, 10);
+        else if (dim == 3 && nl == 20) PAS(3, 20);
+        else FG_REQUIRE(false, FEMGPU_ERR_UNSUPPORTED, "pa_simplex: unsupported element (dim, local DOFs)");
+#undef PAS
     }
     void apply_dev(const void *x, void *y, bool acc, const int *skip) override {
         if (dtype == FEMGPU_F64) apply_t<double>((const double *)x, (double *)y, acc, skip);
