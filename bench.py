@@ -62,6 +62,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fem-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+
+
+def sumfac_fma_per_element(n1, q, diffusion, mass, cartesian):
+    """Multiply-adds of one element in the sum-factorised apply, as the element
+    kernels perform them: X and Y^T / X^T contractions on the B (and G) paths,
+    the Z contraction with the pointwise product and its transpose."""
+    paths_x = (2 if diffusion else 1)
+    paths_y = (3 if diffusion else 1)
+    x = n1 * n1 * q * n1 * paths_x                  # X: N1^2 lines, Q outputs, N1 inputs
+    y = q * n1 * q * n1 * paths_y                   # Y: (qx, k) lanes, Q outputs, N1 inputs
+    z_col = (3 * n1 if diffusion else 0) + (n1 if mass else 0)            # gradients / value at a point
+    z_col += ((3 if cartesian else 9) if diffusion else 0) + (1 if mass else 0)  # pointwise product
+    z_col += (3 * n1 if diffusion else 0) + (n1 if mass else 0)           # transpose accumulation
+    z = q * q * q * z_col                           # Z: Q^2 columns, Q points each
+    return x + y + z + y + x                        # Y^T and X^T mirror Y and X
 
 WORKLOADS = {
     "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8,
@@ -638,6 +654,7 @@ def main():
     if not a.no_cpu and a.workload == "q2-64-sinprod":
         cpu = cpu_baseline(n, p, a.cpu_iters, lmax)
     kform = 2 if w["form"] == "diffusion" else 3
+    flops_apply = 2.0 * E * sumfac_fma_per_element(p + 1, q, True, kform == 3, K.cartesian)
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(value, 4),
@@ -673,6 +690,8 @@ def main():
                       f"{', axis-aligned mesh' if K.cartesian else ''}) + "
                       f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)",
             "algorithmic_bytes": bytes_apply,
+            "flops_per_apply": flops_apply,
+            "flop_frac_fp64_valu": round(flops_apply / (ms_kern * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS, 4),
             "device_copy_gb_s": round(copy_gbs, 1),
             "frac_of_device_copy": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / copy_gbs, 4),
         },
