@@ -104,7 +104,7 @@ struct HexGrid {
     long long Nx, Ny, Nz;
     int ne;         // elements per element-kernel workgroup, ne_of(Q)
     long long Nxp;  // row stride of the x-merged element vector: Nx rounded up to even
-    int ymerge;     // k_pa_hex_tpe with nx % 64 == 0: y-faces merged inside 256-element blocks
+    int ymerge;     // k_pa_hex_tpe with nx = 64 or 128: y-faces merged inside 256-element blocks
 };
 
 
@@ -612,9 +612,9 @@ __global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__
 #pragma unroll
                 for (int i = 0; i < N1; ++i) yv[k][j][i] = 0;
     }
-    // ---- y-merge (g.ymerge: nx is a multiple of 64, so a wavefront lies in one
-    // element row and the y-neighbour e + nx is wavefront w + nx/64 of the same
-    // 256-element block when that exists): the upper face j = P goes through
+    // ---- y-merge (g.ymerge: nx = 64 or 128, so a wavefront lies in one element
+    // row, rows tile the 256-element block, and the y-neighbour e + nx is
+    // wavefront w + nx/64 of the same block when that exists): the upper face j = P goes through
     // LDS to the next row's wavefront, which adds it to its j = 0 face (lower
     // element first) and writes it; the lower element writes no j = P rows.
     bool yhi = false;
@@ -1253,7 +1253,10 @@ struct HexOp final : femgpu_op {
         tpe = n1 == 3 && q == 3 && desc.form == 2 && cart && !(kmode && std::strcmp(kmode, "lds") == 0);
         if (tpe) {
             g.ne = 64;                     // element-vector seams every wavefront
-            g.ymerge = g.nx % 64 == 0 ? 1 : 0;  // wavefronts inside element rows
+            // rows must tile the 256-element block (nx = 64 or 128): then every
+            // wavefront of a row pairs with the row above alike, and no x-face is
+            // shared by a merged and an unmerged element
+            g.ymerge = (g.nx == 64 || g.nx == 128) ? 1 : 0;
         }
         if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
         if (desc.form & 1) qm.alloc(nq * es);

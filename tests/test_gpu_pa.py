@@ -269,7 +269,8 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
     assert not M.cartesian and M.qdata_per_point() == 1
 
 
-@pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3), (64, 5, 3), (128, 3, 2), (256, 2, 2)])
+@pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3), (64, 5, 3), (128, 3, 2), (192, 3, 2),
+                                   (256, 2, 2)])
 def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
     """Q2 diffusion (3^3 points) runs one thread per element with 64-element
     element-vector groups (and, when nx is a multiple of 64, y-faces merged
