@@ -306,3 +306,27 @@ def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
                 assert np.abs(xk - xl).max() <= 1e-8 * np.abs(xl).max()
             for op in (K, L, K32, L32):
                 op.close()
+
+
+def test_hex_cartesian_mirrored_and_anisotropic(fg, ctx, monkeypatch):
+    """Cartesian qdata on a box mesh whose x axis runs backwards (negative
+    spacing) and whose spacings differ by 100x between axes equals the general
+    form, for the workgroup and the thread-per-element kernels."""
+    rng = np.random.default_rng(21)
+    n = (64, 3, 2)
+    ax = [np.linspace(1.0, 0.0, n[0] + 1), np.linspace(0.0, 0.01, n[1] + 1), np.linspace(0.0, 1.0, n[2] + 1)]
+    V = np.array([[ax[0][i], ax[1][j], ax[2][k]] for k in range(n[2] + 1) for j in range(n[1] + 1)
+                  for i in range(n[0] + 1)])
+    for p in (2, 3):
+        kw = dict(form=fg.DIFFUSION, kappa="sinprod", dirichlet=True, vertices=V, ctx=ctx)
+        K = fg.HexPAOperator(p, p + 1, n, **kw)
+        assert K.cartesian
+        monkeypatch.setenv("FEMGPU_HEX_QDATA", "general")
+        G = fg.HexPAOperator(p, p + 1, n, **kw)
+        monkeypatch.delenv("FEMGPU_HEX_QDATA")
+        assert not G.cartesian
+        x = rng.standard_normal(K.n_dofs())
+        yk, yg = K.matvec(x), G.matvec(x)
+        assert np.abs(yk - yg).max() <= 1e-12 * np.abs(yg).max()
+        assert np.abs(K.diagonal() - G.diagonal()).max() <= 1e-12 * np.abs(G.diagonal()).max()
+        K.close(); G.close()
