@@ -251,3 +251,24 @@ def test_amg_graph_replays(fg, ctx, monkeypatch):
         xs.append((r.iterations, xh))
     assert xs[0][0] == xs[1][0] and np.array_equal(xs[0][1], xs[1][1])
     s.close()
+
+
+def test_amg_single_level_and_tiny(fg, ctx):
+    """A matrix at or below the coarsest size is one level solved by the dense
+    inverse: one cycle is the exact solve and PCG stops after one iteration;
+    a 1 x 1 matrix works too."""
+    for n in (1, 3, 40):
+        T = sp.diags([-1.0, 2.5, -1.0], [-1, 0, 1], shape=(n, n)).tocsr()
+        T.sort_indices()
+        A = fg.CsrMatrix(T.indptr.astype(np.uint64), T.indices.astype(np.uint32), T.data, ctx=ctx)
+        s = fg.AmgSolver(A)
+        assert s.n_levels() == 1
+        b = np.linspace(1.0, 2.0, n)
+        xe = np.linalg.solve(T.toarray(), b)
+        z = s.vcycle(b)
+        assert np.abs(z - xe).max() <= 1e-12 * np.abs(xe).max()
+        x = np.zeros(n)
+        r = s.solve(b, x, fg.SolverConfig(rtol=1e-10))
+        assert r.converged and r.iterations <= 1
+        assert np.abs(x - xe).max() <= 1e-10 * np.abs(xe).max()
+        s.close()
