@@ -235,7 +235,7 @@ def run_csr(a, w):
     # simplex PA applies (PADiffusionOperator / PAMassOperator, partial.rs:65-209) on a
     # finer mesh of the same kind: one thread per element, per-element qdata, atomics into y
     ns = 2048
-    xy2, conn2, _ = unit_square_mesh(ns)
+    xy2, conn2, bnd2 = unit_square_mesh(ns)
     N2, E2 = len(xy2), len(conn2)
     xs2 = fg.DeviceArray(ctx, N2, np.float64).upload(np.random.default_rng(1).standard_normal(N2))
     ys2 = fg.DeviceArray(ctx, N2, np.float64)
@@ -250,7 +250,38 @@ def run_csr(a, w):
                          "algorithmic_bytes": byts, "gb_s": round(byts / (ms * 1e-3) / 1e9, 1),
                          "frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         op.close()
-    del xs2, ys2
+    # CSR SpMV (csr.rs:140-193), the reference Chebyshev smoother and the spectral
+    # radius estimate (amg/src/lib.rs:53-123) at a bandwidth-bound size: the
+    # 2048^2 P1 stiffness matrix, assembled on the GPU
+    Ab = fg.assemble_bilinear(2, 1, xy2, conn2, conn2, N2, fg.DIFFUSION, 1.0, 3, ctx=ctx)
+    Ab.apply_dirichlet_row_zeroing(bnd2, np.zeros(len(bnd2)))  # as config 1: A 1 != 0, rho > 0
+    nnz2 = Ab.nnz()
+    Ab.time_apply(xs2, ys2, 5)
+    ms_sp, _ = Ab.time_apply(xs2, ys2, a.apply_reps)
+    sp_bytes = 12.0 * nnz2 + 8.0 * (N2 + 1) + 16.0 * N2
+    ones = fg.DeviceArray(ctx, N2, np.float64, np.ones(N2))
+    xc = fg.DeviceArray(ctx, N2, np.float64)
+    fg.estimate_spectral_radius(Ab, 2)
+    ctx.synchronize(); t0 = time.perf_counter()
+    rho = fg.estimate_spectral_radius(Ab, 20)
+    ctx.synchronize(); t_rho = (time.perf_counter() - t0) / 20
+    lmax = 1.1 * rho
+    fg.chebyshev_smooth(Ab, xc, ones, lmax / 30.0, lmax, 2)
+    ctx.synchronize(); t0 = time.perf_counter()
+    fg.chebyshev_smooth(Ab, xc, ones, lmax / 30.0, lmax, 20)
+    ctx.synchronize(); t_cheb = (time.perf_counter() - t0) / 20
+    cheb_bytes = sp_bytes + 64.0 * N2  # A x, then b, r, dinv, d, x read and r, d, x written
+    csr_scale = {"matrix": f"2048^2 P1 stiffness assembled on the GPU, Dirichlet rows zeroed ({N2} rows, "
+                           f"{nnz2} nonzeros)",
+                 "spmv": {"ms": round(ms_sp, 5), "algorithmic_bytes": sp_bytes,
+                          "gb_s": round(sp_bytes / (ms_sp * 1e-3) / 1e9, 1),
+                          "frac": round(sp_bytes / (ms_sp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                 "chebyshev_smooth_sweep": {"ms": round(1e3 * t_cheb, 4), "algorithmic_bytes": cheb_bytes,
+                                            "gb_s": round(cheb_bytes / t_cheb / 1e9, 1),
+                                            "frac": round(cheb_bytes / t_cheb / 1e9 / HBM_PEAK_GBS, 4)},
+                 "spectral_radius_iteration_ms": round(1e3 * t_rho, 4), "spectral_radius": rho}
+    Ab.close()
+    del xs2, ys2, ones, xc
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(N * sum(iters) / total / 1e9, 4), "unit": "GDOF/s", "n_gpus": 1,
@@ -272,6 +303,7 @@ def run_csr(a, w):
                      "algorithmic_bytes": spmv_bytes},
         "legs": {"amg_pcg": amg_leg},
         "pa_simplex": simplex,
+        "csr_at_scale": csr_scale,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
