@@ -18,22 +18,28 @@ struct CsrView {
     const void *vals = nullptr;
 };
 
-template <class T, class X>
-__device__ __forceinline__ T row_dot_ref(const uint64_t *__restrict__ rp,
-                                         const uint32_t *__restrict__ ci, const T *__restrict__ v,
-                                         X xat, size_t row) {
+// Entries s .. e-1 of a row; vat(k) / cat(k) return value and column k.
+template <class T, class V, class Cl, class X>
+__device__ __forceinline__ T row_sum_ref(uint64_t s, uint64_t e, V vat, Cl cat, X xat) {
 #pragma clang fp contract(off)
-    const uint64_t s = rp[row], e = rp[row + 1];
     T sum = 0;
     uint64_t k = s;
     if constexpr (sizeof(T) == 8) {
         const uint64_t e4 = s + (e - s) / 4 * 4;
         for (; k < e4; k += 4)
-            sum += v[k] * xat(ci[k]) + v[k + 1] * xat(ci[k + 1]) + v[k + 2] * xat(ci[k + 2]) +
-                   v[k + 3] * xat(ci[k + 3]);
+            sum += vat(k) * xat(cat(k)) + vat(k + 1) * xat(cat(k + 1)) + vat(k + 2) * xat(cat(k + 2)) +
+                   vat(k + 3) * xat(cat(k + 3));
     }
-    for (; k < e; ++k) sum += v[k] * xat(ci[k]);
+    for (; k < e; ++k) sum += vat(k) * xat(cat(k));
     return sum;
+}
+
+template <class T, class X>
+__device__ __forceinline__ T row_dot_ref(const uint64_t *__restrict__ rp,
+                                         const uint32_t *__restrict__ ci, const T *__restrict__ v,
+                                         X xat, size_t row) {
+    return row_sum_ref<T>(rp[row], rp[row + 1], [v](uint64_t k) { return v[k]; },
+                          [ci](uint64_t k) { return ci[k]; }, xat);
 }
 
 }  // namespace femgpu
