@@ -20,12 +20,6 @@
 namespace femgpu {
 
 // mode 0: y = Ax; 1: y += Ax; 2: y = alpha*Ax + beta*y
-// The workgroup's 256 rows hold one contiguous range of entries: when it fits
-// (kSpmvStage entries, 8 per row on average) values and columns are first
-// copied to LDS with coalesced loads, and each lane then sums its row from
-// there in the same order, so a lane's loads no longer wait on one another.
-constexpr int kSpmvStage = 2048;
-
 template <class T>
 __global__ __launch_bounds__(256) void k_spmv_row(size_t n, const uint64_t *__restrict__ rp,
                                                   const uint32_t *__restrict__ ci,
@@ -33,26 +27,13 @@ __global__ __launch_bounds__(256) void k_spmv_row(size_t n, const uint64_t *__re
                                                   const T *__restrict__ x, T *__restrict__ y,
                                                   int mode, T alpha, T beta, const int *skip) {
 #pragma clang fp contract(off)
-    __shared__ T sv[kSpmvStage];
-    __shared__ uint32_t sc[kSpmvStage];
+    __shared__ RowStage<T> st;
     if (skip && *skip) return;
     const size_t r0 = blockIdx.x * (size_t)256;
-    const size_t r1 = r0 + 256 < n ? r0 + 256 : n;
-    const uint64_t k0 = rp[r0], k1 = rp[r1];
-    const bool staged = k1 - k0 <= (uint64_t)kSpmvStage;  // uniform over the workgroup
-    if (staged) {
-        for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
-            sv[k - k0] = v[k];
-            sc[k - k0] = ci[k];
-        }
-    }
-    __syncthreads();
+    stage_rows<T>(st, rp, ci, v, n, r0);
     const size_t r = r0 + threadIdx.x;
     if (r >= n) return;
-    auto xat = [x](uint32_t j) { return x[j]; };
-    const T s = staged ? row_sum_ref<T>(rp[r] - k0, rp[r + 1] - k0, [](uint64_t k) { return sv[k]; },
-                                        [](uint64_t k) { return sc[k]; }, xat)
-                       : row_dot_ref<T>(rp, ci, v, xat, r);
+    const T s = row_staged<T>(st, rp, ci, v, [x](uint32_t j) { return x[j]; }, r);
     if (mode == 0) y[r] = s;
     else if (mode == 1) y[r] += s;
     else y[r] = alpha * s + beta * y[r];

@@ -42,4 +42,50 @@ __device__ __forceinline__ T row_dot_ref(const uint64_t *__restrict__ rp,
                           [ci](uint64_t k) { return ci[k]; }, xat);
 }
 
+// Row kernels with one thread per row, 256 rows per workgroup: the
+// workgroup's rows hold one contiguous range of entries.  When it fits
+// kRowStage entries (8 per row on average), values and columns are first
+// copied to LDS with coalesced loads, and each lane then sums its row from
+// there in row_dot_ref's order, so a lane's loads no longer wait on one
+// another.  Every thread of the workgroup calls stage_rows (it holds a
+// barrier); row_staged then returns row r's sum (r < n).
+constexpr int kRowStage = 2048;
+
+template <class T>
+struct RowStage {
+    T v[kRowStage];
+    uint32_t c[kRowStage];
+    uint64_t k0;
+    bool on;
+};
+
+template <class T>
+__device__ __forceinline__ void stage_rows(RowStage<T> &st, const uint64_t *__restrict__ rp,
+                                           const uint32_t *__restrict__ ci, const T *__restrict__ v,
+                                           size_t n, size_t r0) {
+    const size_t r1 = r0 + 256 < n ? r0 + 256 : n;
+    const uint64_t k0 = rp[r0], k1 = rp[r1];
+    const bool on = k1 - k0 <= (uint64_t)kRowStage;  // uniform over the workgroup
+    if (on)
+        for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+            st.v[k - k0] = v[k];
+            st.c[k - k0] = ci[k];
+        }
+    if (threadIdx.x == 0) {
+        st.k0 = k0;
+        st.on = on;
+    }
+    __syncthreads();
+}
+
+template <class T, class X>
+__device__ __forceinline__ T row_staged(const RowStage<T> &st, const uint64_t *__restrict__ rp,
+                                        const uint32_t *__restrict__ ci, const T *__restrict__ v,
+                                        X xat, size_t r) {
+    if (!st.on) return row_dot_ref<T>(rp, ci, v, xat, r);
+    const uint64_t k0 = st.k0;
+    return row_sum_ref<T>(rp[r] - k0, rp[r + 1] - k0, [&st](uint64_t k) { return st.v[k]; },
+                          [&st](uint64_t k) { return st.c[k]; }, xat);
+}
+
 }  // namespace femgpu
