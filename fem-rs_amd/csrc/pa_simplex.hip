@@ -25,6 +25,16 @@ constexpr int kMaxNl = 20, kMaxQp = 36;
 
 // NL (local DOFs: 3 / 6 / 10 on triangles, 4 / 10 / 20 on tetrahedra) is a
 // template parameter so that the element's x and y values stay in registers.
+//
+// Summation into y: when the workgroup's DOF indices span fewer than kWin
+// entries (neighbouring elements of a well-ordered mesh, e.g. a row of a
+// structured triangle mesh), its elements first add into an LDS window and the
+// workgroup then adds each nonzero window entry to y once, so shared vertices
+// cost one global atomic per workgroup instead of one per element.  Otherwise
+// every element adds its values to y directly.  Either way the order of the
+// additions into y is not fixed (FP64 results agree to rounding).
+constexpr int kWin = 4096;
+
 template <class T, int DIM, int NL>
 __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                                                     const uint32_t *__restrict__ edofs,
@@ -34,57 +44,103 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                                                     const int *skip) {
     constexpr int nl = NL;
     // tab: [nq] weights, [nq][nl] phi, [nq][nl][DIM] grad (reference element)
-    __shared__ T s_tab[kMaxQp * (1 + kMaxNl * (1 + 3))];
+    __shared__ T s_tab[kMaxQp * (1 + NL * (1 + DIM))];
+    __shared__ T win[kWin];
+    __shared__ uint32_t s_lo, s_hi;
     if (skip && *skip) return;
     const int tab_len = nq * (1 + nl * (1 + DIM));
     for (int i = threadIdx.x; i < tab_len; i += blockDim.x) s_tab[i] = (T)tab[i];
-    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_lo = 0xffffffffu;
+        s_hi = 0;
+    }
     const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (e >= ne) return;
-    const T *W = s_tab, *PHI = s_tab + nq, *GR = s_tab + nq + nq * nl;
-    T xe[NL], ye[NL];
+    const bool live = e < ne;
+    uint32_t dof[NL];
+    uint32_t lo = 0xffffffffu, hi = 0;
 #pragma unroll
-    for (int l = 0; l < nl; ++l) { xe[l] = x[edofs[e * nl + l]]; ye[l] = 0; }
-    constexpr int NS = DIM == 2 ? 3 : 6;
-    if (form == FEMGPU_DIFFUSION) {
-        T D[NS];
-        for (int c = 0; c < NS; ++c) D[c] = qdata[e * NS + c];
-        for (int q = 0; q < nq; ++q) {
-            const T *G = GR + (size_t)q * nl * DIM;
-            T gu[3] = {0, 0, 0};
-#pragma unroll
-            for (int l = 0; l < nl; ++l)
-                for (int d = 0; d < DIM; ++d) gu[d] += xe[l] * G[l * DIM + d];
-            T v[3];
-            if (DIM == 2) {
-                v[0] = D[0] * gu[0] + D[1] * gu[1];
-                v[1] = D[1] * gu[0] + D[2] * gu[1];
-            } else {
-                v[0] = D[0] * gu[0] + D[1] * gu[1] + D[2] * gu[2];
-                v[1] = D[1] * gu[0] + D[3] * gu[1] + D[4] * gu[2];
-                v[2] = D[2] * gu[0] + D[4] * gu[1] + D[5] * gu[2];
-            }
-#pragma unroll
-            for (int i = 0; i < nl; ++i) {
-                T s = 0;
-                for (int d = 0; d < DIM; ++d) s += G[i * DIM + d] * v[d];
-                ye[i] += W[q] * s;
-            }
-        }
-    } else {
-        const T m = qdata[e];
-        for (int q = 0; q < nq; ++q) {
-            const T *ph = PHI + (size_t)q * nl;
-            T u = 0;
-#pragma unroll
-            for (int l = 0; l < nl; ++l) u += ph[l] * xe[l];
-            const T wu = W[q] * m * u;
-#pragma unroll
-            for (int i = 0; i < nl; ++i) ye[i] += wu * ph[i];
+    for (int l = 0; l < nl; ++l) {
+        dof[l] = live ? edofs[e * nl + l] : 0u;
+        if (live) {
+            lo = min(lo, dof[l]);
+            hi = max(hi, dof[l]);
         }
     }
+    __syncthreads();
+    if (live) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+    }
+    __syncthreads();
+    const uint32_t wlo = s_lo;
+    const uint32_t span = s_hi - wlo + 1;  // >= 1: every workgroup holds a live element
+    const bool windowed = span <= (uint32_t)kWin;  // uniform over the workgroup
+    if (windowed)
+        for (uint32_t i = threadIdx.x; i < span; i += 256) win[i] = T(0);
+
+    T ye[NL];
 #pragma unroll
-    for (int l = 0; l < nl; ++l) unsafeAtomicAdd(&y[edofs[e * nl + l]], ye[l]);
+    for (int l = 0; l < nl; ++l) ye[l] = 0;
+    if (live) {
+        const T *W = s_tab, *PHI = s_tab + nq, *GR = s_tab + nq + nq * nl;
+        T xe[NL];
+#pragma unroll
+        for (int l = 0; l < nl; ++l) xe[l] = x[dof[l]];
+        constexpr int NS = DIM == 2 ? 3 : 6;
+        if (form == FEMGPU_DIFFUSION) {
+            T D[NS];
+            for (int c = 0; c < NS; ++c) D[c] = qdata[e * NS + c];
+            for (int q = 0; q < nq; ++q) {
+                const T *G = GR + (size_t)q * nl * DIM;
+                T gu[3] = {0, 0, 0};
+#pragma unroll
+                for (int l = 0; l < nl; ++l)
+                    for (int d = 0; d < DIM; ++d) gu[d] += xe[l] * G[l * DIM + d];
+                T v[3];
+                if (DIM == 2) {
+                    v[0] = D[0] * gu[0] + D[1] * gu[1];
+                    v[1] = D[1] * gu[0] + D[2] * gu[1];
+                } else {
+                    v[0] = D[0] * gu[0] + D[1] * gu[1] + D[2] * gu[2];
+                    v[1] = D[1] * gu[0] + D[3] * gu[1] + D[4] * gu[2];
+                    v[2] = D[2] * gu[0] + D[4] * gu[1] + D[5] * gu[2];
+                }
+#pragma unroll
+                for (int i = 0; i < nl; ++i) {
+                    T s = 0;
+                    for (int d = 0; d < DIM; ++d) s += G[i * DIM + d] * v[d];
+                    ye[i] += W[q] * s;
+                }
+            }
+        } else {
+            const T m = qdata[e];
+            for (int q = 0; q < nq; ++q) {
+                const T *ph = PHI + (size_t)q * nl;
+                T u = 0;
+#pragma unroll
+                for (int l = 0; l < nl; ++l) u += ph[l] * xe[l];
+                const T wu = W[q] * m * u;
+#pragma unroll
+                for (int i = 0; i < nl; ++i) ye[i] += wu * ph[i];
+            }
+        }
+    }
+    if (!windowed) {
+        if (live)
+#pragma unroll
+            for (int l = 0; l < nl; ++l) unsafeAtomicAdd(&y[dof[l]], ye[l]);
+        return;
+    }
+    __syncthreads();  // window cleared
+    if (live)
+#pragma unroll
+        for (int l = 0; l < nl; ++l)
+            __hip_atomic_fetch_add(&win[dof[l] - wlo], ye[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < span; i += 256) {
+        const T v = win[i];
+        if (v != T(0) || v != v) unsafeAtomicAdd(&y[wlo + i], v);  // untouched entries stay 0
+    }
 }
 
 template <class T>

@@ -180,6 +180,29 @@ def test_simplex_pa(fg, fr, ctx, dim, order, kind):
     assert _rel(op.diagonal(), fr.CsrRef(rp, ci, v).diagonal()) < 1e-12
 
 
+@pytest.mark.parametrize("renumber", [False, True])
+def test_simplex_pa_window_and_direct(fg, fr, ctx, renumber):
+    """k_pa_simplex sums a workgroup's element values in an LDS window when its
+    DOF indices span at most 4096 entries, else adds them to y directly. A 100^2
+    P1 triangle mesh has 20,000 elements: 79 workgroups of 256, the last one
+    ragged. Its natural numbering puts every workgroup in the window. Randomly
+    renumbered DOFs (10,201 of them) put none there. Both forms, zeroing and
+    accumulating applies."""
+    coords, conn, nd, ed = _tri_mesh(fr, 100, 1)
+    if renumber:
+        ed = np.random.default_rng(3).permutation(nd).astype(np.uint32)[ed]
+    rng = np.random.default_rng(4)
+    for kind, cls in ((1, fg.PAMassOperator), (2, fg.PADiffusionOperator)):
+        op = cls(2, 1, coords, conn, ed, nd, 1.7, 3, ctx=ctx)
+        x = rng.standard_normal(nd)
+        y0 = rng.standard_normal(nd)
+        ref = fr.simplex_pa_apply(2, 1, coords, conn, ed, kind, 1.7, 3, x)
+        assert _rel(op.matvec(x), ref) < 1e-12
+        y = y0.copy()
+        op.apply(x, y)
+        assert _rel(y, fr.simplex_pa_apply(2, 1, coords, conn, ed, kind, 1.7, 3, x, y0)) < 1e-12
+
+
 def test_lumped_mass(fg, fr, ctx):
     coords, conn, nd, ed = _tri_mesh(fr, 4, 1)
     lm = fg.LumpedMassOperator(2, 1, coords, conn, ed, nd, 1.0, 3, ctx=ctx)
