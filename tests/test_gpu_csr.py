@@ -55,6 +55,41 @@ def test_spmv_f32(fg, fr, ctx):
     assert np.abs(y - y64).max() <= 1e-5 * np.abs(y64).max()
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_spmv_staged_and_unstaged_blocks(fg, fr, ctx, f32):
+    """The row kernel copies a 256-row block's entries to LDS when they number at
+    most 2048, else reads them from memory: 2,000 rows averaging under 16
+    entries (so the row kernel runs). Rows 300-399 hold 41 entries each, so
+    block 1 holds 4,468 entries and is not staged. Every 7th row is empty, and
+    the last block is ragged. Bitwise against the oracle in both precisions,
+    for y = A x and y = a A x + b y."""
+    n = 2000
+    rng = np.random.default_rng(11)
+    rp = [0]; ci = []
+    for i in range(n):
+        k = 0 if i % 7 == 0 else 41 if 300 <= i < 400 else 1 + i % 6
+        ci += sorted(rng.choice(n, k, replace=False).tolist())
+        rp.append(len(ci))
+    rp = np.array(rp, np.uint64); ci = np.array(ci, np.uint32)
+    v = rng.standard_normal(len(ci))
+    x = rng.standard_normal(n); y0 = rng.standard_normal(n)
+    if f32:
+        v32, x32, y32 = v.astype(np.float32), x.astype(np.float32), y0.astype(np.float32)
+        A = fg.CsrMatrix(rp, ci, v32, dtype=fg.F32, ctx=ctx)
+        y = np.zeros(n, np.float32)
+        A.spmv(x32, y)
+        assert np.array_equal(y, fr.csr_spmv_f32(rp, ci, v32, x32))
+        return
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    ref = fr.CsrRef(rp, ci, v)
+    y = np.zeros(n)
+    A.spmv(x, y)
+    assert np.array_equal(y, ref.spmv(x))
+    y3 = y0.copy()
+    A.spmv_add(0.7, x, -1.3, y3)
+    assert np.array_equal(y3, ref.spmv_add(0.7, x, -1.3, y0))
+
+
 def test_spmv_long_rows(fg, fr, ctx):
     """Assembled hex Q3 matrix (343 nnz per interior row): sub-wavefront kernel."""
     d = fr.hex_desc(3, 4, (3, 3, 3), kind=fr.FR_DIFF_MASS, kappa=fr.coef_sinprod())
