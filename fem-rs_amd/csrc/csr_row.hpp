@@ -68,8 +68,8 @@ __device__ __forceinline__ void stage_rows(RowStage<T> &st, const uint64_t *__re
     const bool on = k1 - k0 <= (uint64_t)kRowStage;  // uniform over the workgroup
     if (on)
         for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
-            st.v[k - k0] = v[k];
-            st.c[k - k0] = ci[k];
+            st.v[k - k0] = __builtin_nontemporal_load(v + k);
+            st.c[k - k0] = __builtin_nontemporal_load(ci + k);
         }
     if (threadIdx.x == 0) {
         st.k0 = k0;
