@@ -20,20 +20,28 @@
 namespace femgpu {
 
 // mode 0: y = Ax; 1: y += Ax; 2: y = alpha*Ax + beta*y
-template <class T>
+// STAGE: the workgroup's entries go through LDS first (large matrices).
+template <class T, bool STAGE>
 __global__ __launch_bounds__(256) void k_spmv_row(size_t n, const uint64_t *__restrict__ rp,
                                                   const uint32_t *__restrict__ ci,
                                                   const T *__restrict__ v,
                                                   const T *__restrict__ x, T *__restrict__ y,
                                                   int mode, T alpha, T beta, const int *skip) {
 #pragma clang fp contract(off)
-    __shared__ RowStage<T> st;
     if (skip && *skip) return;
     const size_t r0 = blockIdx.x * (size_t)256;
-    stage_rows<T>(st, rp, ci, v, n, r0);
     const size_t r = r0 + threadIdx.x;
-    if (r >= n) return;
-    const T s = row_staged<T>(st, rp, ci, v, [x](uint32_t j) { return x[j]; }, r);
+    auto xat = [x](uint32_t j) { return x[j]; };
+    T s;
+    if constexpr (STAGE) {
+        __shared__ RowStage<T> st;
+        stage_rows<T>(st, rp, ci, v, n, r0);
+        if (r >= n) return;
+        s = row_staged<T>(st, rp, ci, v, xat, r);
+    } else {
+        if (r >= n) return;
+        s = row_dot_ref<T>(rp, ci, v, xat, r);
+    }
     if (mode == 0) y[r] = s;
     else if (mode == 1) y[r] += s;
     else y[r] = alpha * s + beta * y[r];
@@ -98,6 +106,7 @@ struct CsrOp final : femgpu_op {
     uint64_t ncols = 0, nnz = 0;
     DevBuf rp, ci, vals;
     int sub = 1;  // lanes per row (1 = reference-order row kernel)
+    bool stage = false;  // row kernel stages entries in LDS
 
     const char *name() const override { return "csr"; }
 
@@ -109,8 +118,12 @@ struct CsrOp final : femgpu_op {
         const T *v = vals.as<T>();
         if (n == 0) return;
         if (sub == 1) {
-            hipLaunchKernelGGL(k_spmv_row<T>, dim3((n + 255) / 256), dim3(256), 0, s, n, r, c, v, x,
-                               y, mode, alpha, beta, skip);
+            if (stage)
+                hipLaunchKernelGGL((k_spmv_row<T, true>), dim3((n + 255) / 256), dim3(256), 0, s, n, r, c, v, x,
+                                   y, mode, alpha, beta, skip);
+            else
+                hipLaunchKernelGGL((k_spmv_row<T, false>), dim3((n + 255) / 256), dim3(256), 0, s, n, r, c, v,
+                                   x, y, mode, alpha, beta, skip);
             return;
         }
         const size_t threads = n * (size_t)sub;
@@ -161,6 +174,16 @@ static int csr_lanes(uint64_t nrows, uint64_t nnz) {
     return sub;
 }
 
+// LDS staging in the row kernel pays where a launch has many more workgroups
+// than the GPU has CUs (2^19 rows: 2048 workgroups); on smaller matrices its
+// extra pass and barrier only add latency.  FEMGPU_CSR_STAGE=1 / 0 forces it
+// on / off.
+static bool csr_stage(uint64_t nrows) {
+    bool on = nrows >= (1u << 19);
+    if (const char *f = std::getenv("FEMGPU_CSR_STAGE")) on = std::atoi(f) != 0;
+    return on;
+}
+
 femgpu_op *csr_adopt(femgpu_ctx *c, uint64_t n, uint64_t nnz, DevBuf &&rp, DevBuf &&ci, DevBuf &&vals) {
     auto op = new CsrOp();
     op->ctx = c; op->dtype = FEMGPU_F64; op->n = n; op->ncols = n; op->nnz = nnz;
@@ -168,6 +191,7 @@ femgpu_op *csr_adopt(femgpu_ctx *c, uint64_t n, uint64_t nnz, DevBuf &&rp, DevBu
     op->ci = std::move(ci);
     op->vals = std::move(vals);
     op->sub = csr_lanes(n, nnz);
+    op->stage = csr_stage(n);
     return op;
 }
 
@@ -213,6 +237,7 @@ int femgpu_csr_create(femgpu_ctx *c, uint64_t nrows, uint64_t ncols, const uint6
             }
             FG_HIP(hipStreamSynchronize(c->stream));
             op->sub = csr_lanes(nrows, nnz);
+            op->stage = csr_stage(nrows);
         } catch (...) {
             delete op;
             throw;

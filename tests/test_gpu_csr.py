@@ -27,9 +27,13 @@ def test_spmv_tridiag_known_answer(fg, ctx):
     assert A.diagonal().tolist() == [2.0, 2.0, 2.0]  # csr.rs:601-605
 
 
-def test_spmv_config1_bitwise(fg, fr, ctx):
+@pytest.mark.parametrize("stage", ["0", "1"])
+def test_spmv_config1_bitwise(fg, fr, ctx, monkeypatch, stage):
+    """Config-1 matrix, bitwise, with the row kernel's LDS staging off (its default
+    below 2^19 rows) and forced on."""
     rp, ci, v, rhs, _, _ = fr.poisson_p1(256)
     assert len(v) == 460289
+    monkeypatch.setenv("FEMGPU_CSR_STAGE", stage)
     A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
     ref = fr.CsrRef(rp, ci, v)
     x = np.random.default_rng(0).standard_normal(len(rhs))
@@ -41,6 +45,19 @@ def test_spmv_config1_bitwise(fg, fr, ctx):
     A.spmv_add(0.7, x, -1.3, y3)
     assert np.array_equal(y3, ref.spmv_add(0.7, x, -1.3, y2))
     assert np.array_equal(A.diagonal(), ref.diagonal())
+
+
+def test_spmv_large_default_staged(fg, fr, ctx, monkeypatch):
+    """P1 Poisson on 768^2 quads: 591,361 rows, above the 2^19 rows from which the
+    row kernel stages its entries in LDS by default. Bitwise."""
+    monkeypatch.delenv("FEMGPU_CSR_STAGE", raising=False)
+    rp, ci, v, rhs, _, _ = fr.poisson_p1(768)
+    assert len(rhs) == 769 * 769 >= 1 << 19
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    x = np.random.default_rng(5).standard_normal(len(rhs))
+    y = np.zeros_like(x)
+    A.spmv(x, y)
+    assert np.array_equal(y, fr.CsrRef(rp, ci, v).spmv(x))
 
 
 def test_spmv_f32(fg, fr, ctx):
@@ -56,13 +73,15 @@ def test_spmv_f32(fg, fr, ctx):
 
 
 @pytest.mark.parametrize("f32", [False, True])
-def test_spmv_staged_and_unstaged_blocks(fg, fr, ctx, f32):
-    """The row kernel copies a 256-row block's entries to LDS when they number at
-    most 2048, else reads them from memory: 2,000 rows averaging under 16
-    entries (so the row kernel runs). Rows 300-399 hold 41 entries each, so
-    block 1 holds 4,468 entries and is not staged. Every 7th row is empty, and
-    the last block is ragged. Bitwise against the oracle in both precisions,
-    for y = A x and y = a A x + b y."""
+def test_spmv_staged_and_unstaged_blocks(fg, fr, ctx, monkeypatch, f32):
+    """With staging on (forced here; by default from 2^19 rows), the row kernel
+    copies a 256-row block's entries to LDS when they number at most 2048, else
+    reads them from memory: 2,000 rows averaging under 16 entries (so the row
+    kernel runs). Rows 300-399 hold 41 entries each, so block 1 holds 4,468
+    entries and is not staged. Every 7th row is empty, and the last block is
+    ragged. Bitwise against the oracle in both precisions, for y = A x and
+    y = a A x + b y."""
+    monkeypatch.setenv("FEMGPU_CSR_STAGE", "1")
     n = 2000
     rng = np.random.default_rng(11)
     rp = [0]; ci = []
