@@ -716,6 +716,8 @@ def main():
             "kernel": (f"PA apply = k_pa_hex_tpe<{q},{q},double,{'true' if K.cartesian else 'false'}> element kernel "
                        f"(one thread per element"
                        if K.element_kernel() == "thread-per-element" else
+                       "PA apply = k_pa_hex_quad<double> element kernel (one quad of lanes per element"
+                       if K.element_kernel() == "quad-per-element" else
                        f"PA apply = k_pa_hex<{q},{q},double,{kform},{'true' if K.cartesian else 'false'}> element "
                        f"kernel (one workgroup per {256 // (q * q)} elements") +
                       f"; {s_comp} qdata value(s) per Gauss point"

@@ -655,6 +655,181 @@ __global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__
         }
 }
 
+// Value of lane SRC (0..3) of the calling lane's quad (DPP quad_perm broadcast).
+template <int SRC, class T>
+__device__ __forceinline__ T quad_bcast(T v) {
+    constexpr int ctrl = SRC * 0x55;  // quad_perm [SRC, SRC, SRC, SRC]
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, ctrl, 0xf, 0xf, false);
+        const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), ctrl, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xf, 0xf, false));
+    }
+}
+
+// sum over the quad's four lanes of w[l] * (lane l's v), lane 0 first
+template <class T>
+__device__ __forceinline__ T quad_dot(const T (&w)[4], T v) {
+    T s = 0;
+    s += w[0] * quad_bcast<0>(v);
+    s += w[1] * quad_bcast<1>(v);
+    s += w[2] * quad_bcast<2>(v);
+    s += w[3] * quad_bcast<3>(v);
+    return s;
+}
+
+// Quad-per-element diffusion apply for Q3 with 4^3 points and Cartesian qdata
+// (N1 = Q = 4), where the workgroup kernel is bound by its LDS hand-offs and a
+// whole element does not fit one thread's registers: an element is a quad of
+// lanes, lane L holding node layer k = L through the x and y contractions and
+// their transposes, and quadrature layer c = L through the pointwise step.  The
+// z contraction and its transpose exchange the quad's four layers by DPP
+// broadcasts (no LDS, no barriers).  It writes the workgroup kernel's x-merged
+// element vector (seams every 16 elements, one wavefront); the x-face shared
+// with the next element of the wavefront and row comes from that quad by a
+// shuffle (lower element first).  qdata in the qd_index_tpe layout.
+template <class T>
+__global__ __launch_bounds__(256) void k_pa_hex_quad(const HexGrid g, const T *__restrict__ qd,
+                                                     const T *__restrict__ cf, const T *__restrict__ x,
+                                                     T *__restrict__ y, const int dirichlet, const int *skip) {
+    constexpr int N1 = 4, Q = 4, P = 3;
+    constexpr int C = combo(N1, Q);
+    if (skip && *skip) return;
+    const T *Bt = Tab<T>::B(C);
+    const T *Gt = Tab<T>::G(C);
+    const long long e = (long long)blockIdx.x * 64 + (threadIdx.x >> 2);
+    const int L = (int)(threadIdx.x & 3);
+    const int lane = (int)(threadIdx.x & 63);
+    const bool valid = e < g.E;  // uniform over the quad
+    int ex = 0, ey = 0, ez = 0;
+    if (valid) {
+        ex = (int)(e % g.nx);
+        const long long r = e / g.nx;
+        ey = (int)(r % g.ny);
+        ez = (int)(r / g.ny);
+    }
+    T yv[N1][N1];  // [j][i] of layer k = L
+    if (valid) {
+        // ---- X on layer k = L
+        T A[N1][Q], Ag[N1][Q];
+        const long long gz = (long long)ez * P + L;
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            const long long gy = (long long)ey * P + j;
+            const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
+            const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+            T u[N1];
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                const long long gx = (long long)ex * P + i;
+                const bool b = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
+                u[i] = b ? T(0) : x[base + i];
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                T s = 0, sg = 0;
+#pragma unroll
+                for (int i = 0; i < N1; ++i) {
+                    s += Bt[a * N1 + i] * u[i];
+                    sg += Gt[a * N1 + i] * u[i];
+                }
+                A[j][a] = s;
+                Ag[j][a] = sg;
+            }
+        }
+        // ---- Y on layer k = L
+        T BB[Q][Q], BG[Q][Q], GB[Q][Q];  // [b][a]
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                T bb = 0, bg = 0, gb = 0;
+#pragma unroll
+                for (int j = 0; j < N1; ++j) {
+                    bb += Bt[b * N1 + j] * A[j][a];
+                    bg += Bt[b * N1 + j] * Ag[j][a];
+                    gb += Gt[b * N1 + j] * A[j][a];
+                }
+                BB[b][a] = bb;
+                BG[b][a] = bg;
+                GB[b][a] = gb;
+            }
+        // ---- Z (row c = L of the tables) + pointwise + Z^T (column k = L)
+        T Bc[N1], Gc[N1], Bk[Q], Gk[Q];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            Bc[t] = Bt[L * N1 + t];
+            Gc[t] = Gt[L * N1 + t];
+            Bk[t] = Bt[t * N1 + L];
+            Gk[t] = Gt[t * N1 + L];
+        }
+        const T fx = cf[ex], fy = cf[g.nx + ey], fz = cf[g.nx + g.ny + ez];
+#pragma unroll
+        for (int b = 0; b < Q; ++b)
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                const T g0 = quad_dot<T>(Bc, BG[b][a]);
+                const T g1 = quad_dot<T>(Bc, GB[b][a]);
+                const T g2 = quad_dot<T>(Gc, BB[b][a]);
+                const T sq = qd[qd_index_tpe(Q, 1, e, 0, a + Q * (b + Q * L))];
+                const T v0 = (sq * fx) * g0, v1 = (sq * fy) * g1, v2 = (sq * fz) * g2;
+                BG[b][a] = quad_dot<T>(Bk, v0);
+                GB[b][a] = quad_dot<T>(Bk, v1);
+                BB[b][a] = quad_dot<T>(Gk, v2);
+            }
+        // ---- Y^T
+        T FB[N1][Q], FG[N1][Q];
+#pragma unroll
+        for (int a = 0; a < Q; ++a)
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                T fg = 0, fb = 0;
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    fb += Bt[b * N1 + j] * BB[b][a];
+                    fg += Bt[b * N1 + j] * BG[b][a];
+                    fb += Gt[b * N1 + j] * GB[b][a];
+                }
+                FB[j][a] = fb;
+                FG[j][a] = fg;
+            }
+        // ---- X^T
+#pragma unroll
+        for (int j = 0; j < N1; ++j)
+#pragma unroll
+            for (int i = 0; i < N1; ++i) {
+                T s = 0;
+#pragma unroll
+                for (int a = 0; a < Q; ++a) {
+                    s += Bt[a * N1 + i] * FB[j][a];
+                    s += Gt[a * N1 + i] * FG[j][a];
+                }
+                yv[j][i] = s;
+            }
+    } else {
+#pragma unroll
+        for (int j = 0; j < N1; ++j)
+#pragma unroll
+            for (int i = 0; i < N1; ++i) yv[j][i] = 0;
+    }
+    // ---- write the x-merged rows of layer L; every lane takes part in the shuffles
+    const bool has_next = valid && lane < 60 && ex < g.nx - 1 && e + 1 < g.E;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+        const T next0 = __shfl_down(yv[j][0], 4, 64);
+        if (valid) {
+            T *dst = y + mrow_index(g, N1, ey, ez, j, L) + (size_t)ex * P;
+            if (ex == 0) dst[0] = yv[j][0];
+            else if (lane < 4) y[seam_index(g, N1, e >> 4, j, L)] = yv[j][0];
+#pragma unroll
+            for (int i = 1; i < P; ++i) dst[i] = yv[j][i];
+            dst[P] = has_next ? yv[j][P] + next0 : yv[j][P];
+        }
+    }
+}
+
 template <class T> struct Pair;
 template <> struct Pair<double> { using type = double2; };
 template <> struct Pair<float> { using type = float2; };
@@ -1069,7 +1244,9 @@ static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, co
         hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
 }
 
-// tpe: the operator keeps thread-per-element qdata (Q2 diffusion, 3^3 points)
+// tpe: the operator keeps qdata in the qd_index_tpe layout, for the
+// thread-per-element kernel (Q2 diffusion, 3^3 points) or the quad-per-element
+// kernel (Q3 diffusion, 4^3 points, Cartesian qdata)
 template <class T>
 static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm, const T *cf,
                       const T *x, T *y, int dir, const int *skip, hipStream_t s, bool tpe = false) {
@@ -1077,6 +1254,12 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
         const dim3 grid((unsigned)((g.E + 255) / 256)), block(256);
         if (cf) hipLaunchKernelGGL((k_pa_hex_tpe<3, 3, T, true>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
         else hipLaunchKernelGGL((k_pa_hex_tpe<3, 3, T, false>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
+        return true;
+    }
+    if (tpe && form == 2 && n1 == 4 && q == 4 && cf) {
+        FG_REQUIRE(g.ne == 16 && !g.ymerge, FEMGPU_ERR_INVALID_ARG, "pa_hex: quad kernel needs 16-element groups");
+        const dim3 grid((unsigned)((g.E + 63) / 64)), block(256);
+        hipLaunchKernelGGL((k_pa_hex_quad<T>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
         return true;
     }
 #define CASE(A, B)                                                                 \
@@ -1121,7 +1304,8 @@ struct HexOp final : femgpu_op {
     DevBuf coords, qd, qm, kqp, grf_w, grf_b, xq, wq, Bd, Gd;
     DevBuf evec;  // x-merged element vector between the two apply kernels
     bool cart = false;           // axis-aligned box mesh: Cartesian qdata (one value per point)
-    bool tpe = false;            // thread-per-element kernel and qdata layout (Q2 diffusion)
+    bool tpe = false;            // qd_index_tpe qdata layout and a thread-level element kernel
+    bool quad = false;           // that kernel is k_pa_hex_quad (Q3), else k_pa_hex_tpe (Q2)
     std::vector<double> cf_host;  // Cartesian per-axis factors [nx | ny | nz]
     DevBuf cf;                    // the same, in the operator's type
 
@@ -1250,8 +1434,13 @@ struct HexOp final : femgpu_op {
         const size_t es = esize();
         const char *kmode = std::getenv("FEMGPU_HEX_KERNEL");
         // with the general six-value qdata the workgroup kernel is as fast (both HBM-bound) at higher occupancy
-        tpe = n1 == 3 && q == 3 && desc.form == 2 && cart && !(kmode && std::strcmp(kmode, "lds") == 0);
-        if (tpe) {
+        const bool lds_forced = kmode && std::strcmp(kmode, "lds") == 0;
+        tpe = n1 == 3 && q == 3 && desc.form == 2 && cart && !lds_forced;
+        // Q3 with 4^3 points on Cartesian qdata: the quad-per-element kernel, with
+        // the workgroup kernel's 16-element groups (g.ne = ne_of(4) = 16)
+        quad = n1 == 4 && q == 4 && desc.form == 2 && cart && !lds_forced;
+        if (quad) tpe = true;
+        if (tpe && !quad) {
             g.ne = 64;                     // element-vector seams every wavefront
             // rows must tile the 256-element block (nx = 64 or 128): then every
             // wavefront of a row pairs with the row above alike, and no x-face is
@@ -1480,7 +1669,7 @@ int femgpu_pa_hex_element_kernel(const femgpu_op *op_, int *kind) {
     return guard([&] {
         auto *op = dynamic_cast<const HexOp *>(op_);
         FG_REQUIRE(op && kind, FEMGPU_ERR_INVALID_ARG, "pa_hex_element_kernel: not a hex operator");
-        *kind = op->tpe ? 1 : 0;
+        *kind = op->quad ? 2 : op->tpe ? 1 : 0;
         return 0;
     });
 }

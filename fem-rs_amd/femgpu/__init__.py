@@ -571,11 +571,12 @@ class HexPAOperator(Operator):
         return v.value
 
     def element_kernel(self):
-        """"workgroup" (LDS hand-offs between phases) or "thread-per-element"
-        (Q2 diffusion with Cartesian qdata)."""
+        """"workgroup" (LDS hand-offs between phases), "thread-per-element"
+        (Q2 diffusion with Cartesian qdata) or "quad-per-element" (Q3 diffusion,
+        4^3 points, Cartesian qdata)."""
         v = I()
         _check(lib.femgpu_pa_hex_element_kernel(self.h, C.byref(v)))
-        return "thread-per-element" if v.value == 1 else "workgroup"
+        return {1: "thread-per-element", 2: "quad-per-element"}.get(v.value, "workgroup")
 
     @property
     def cartesian(self):

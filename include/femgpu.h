@@ -181,8 +181,9 @@ int femgpu_pa_hex_rhs_random(femgpu_op *op, uint64_t seed, void *b, femgpu_mem m
 int femgpu_pa_hex_qdata_per_point(const femgpu_op *op, int *values);
 /* Element kernel of a hex operator: 0 = one workgroup per group of elements
  * (LDS hand-offs between the contraction phases), 1 = one thread per element
- * (Q2 diffusion, 3^3 points, Cartesian qdata; FEMGPU_HEX_KERNEL=lds at
- * creation selects 0).  No counterpart in the reference. */
+ * (Q2 diffusion, 3^3 points, Cartesian qdata), 2 = one quad of lanes per
+ * element (Q3 diffusion, 4^3 points, Cartesian qdata); FEMGPU_HEX_KERNEL=lds
+ * at creation selects 0.  No counterpart in the reference. */
 int femgpu_pa_hex_element_kernel(const femgpu_op *op, int *kind);
 /* Same-mesh copy of a hex operator in another precision, reusing nothing but
  * the description (for the FP32 inner / FP64 outer solver). */

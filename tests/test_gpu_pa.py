@@ -292,6 +292,51 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
     assert not M.cartesian and M.qdata_per_point() == 1
 
 
+@pytest.mark.parametrize("shape", [(5, 5, 5), (16, 2, 3), (17, 3, 2), (33, 4, 3), (40, 2, 2)])
+def test_hex_q3_quad_per_element(fg, fr, ctx, monkeypatch, shape):
+    """Q3 diffusion with 4^3 points on an axis-aligned mesh runs one quad of lanes
+    per element (z-layers exchanged by DPP inside the quad, 16 elements per
+    wavefront); its applies, diagonal and solves equal the workgroup kernel's
+    (FEMGPU_HEX_KERNEL=lds) on meshes whose rows are shorter or longer than 16
+    elements, with wavefronts that cross rows and z-planes and a ragged last
+    workgroup, FP64 and FP32, with and without essential DOFs."""
+    rng = np.random.default_rng(13)
+    for dirichlet in (True, False):
+        kw = dict(form=fg.DIFFUSION, kappa="sinprod", dirichlet=dirichlet, ctx=ctx)
+        K = fg.HexPAOperator(3, 4, shape, **kw)
+        assert K.element_kernel() == "quad-per-element" and K.qdata_per_point() == 1
+        monkeypatch.setenv("FEMGPU_HEX_KERNEL", "lds")
+        L = fg.HexPAOperator(3, 4, shape, **kw)
+        monkeypatch.delenv("FEMGPU_HEX_KERNEL")
+        assert L.element_kernel() == "workgroup"
+        x = rng.standard_normal(K.n_dofs())
+        yk, yl = K.matvec(x), L.matvec(x)
+        assert np.abs(yk - yl).max() <= 1e-13 * np.abs(yl).max()
+        d = fr.hex_desc(3, 4, shape, kind=fr.FR_DIFFUSION, kappa=fr.coef_sinprod(), dirichlet=dirichlet)
+        Dq, Mq = fr.hex_qdata(d, fr.hex_vertices(*shape, 0.0, 0))
+        assert _rel(yk, fr.hex_apply_sumfac(d, Dq, Mq, x)) < 1e-12
+        y0 = rng.standard_normal(K.n_dofs())
+        ya, yb = y0.copy(), y0.copy()
+        K.apply(x, ya)
+        L.apply(x, yb)
+        assert np.abs(ya - yb).max() <= 1e-13 * np.abs(yb).max()
+        dk, dl = K.diagonal(), L.diagonal()
+        assert np.abs(dk - dl).max() <= 1e-13 * np.abs(dl).max()
+        K32, L32 = K.convert(fg.F32), L.convert(fg.F32)
+        y32 = K32.matvec(x.astype(np.float32)).astype(np.float64)
+        assert np.abs(y32 - yl).max() <= 1e-5 * np.abs(yl).max()
+        if dirichlet:
+            b = K.rhs_unit()
+            assert np.array_equal(b, L.rhs_unit())
+            xk, xl = np.zeros(K.n_dofs()), np.zeros(K.n_dofs())
+            rk, _ = fg.solve_pcg_chebyshev(K, b, xk, fg.SolverConfig(rtol=1e-10, max_iter=2000))
+            rl, _ = fg.solve_pcg_chebyshev(L, b, xl, fg.SolverConfig(rtol=1e-10, max_iter=2000))
+            assert rk.converged and rl.converged and abs(rk.iterations - rl.iterations) <= 1
+            assert np.abs(xk - xl).max() <= 1e-8 * np.abs(xl).max()
+        for op in (K, L, K32, L32):
+            op.close()
+
+
 @pytest.mark.parametrize("shape", [(5, 5, 5), (70, 3, 2), (64, 2, 3), (64, 5, 3), (128, 3, 2), (192, 3, 2),
                                    (256, 2, 2)])
 def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
