@@ -177,14 +177,27 @@ static std::vector<int64_t> aggregate(const HCsr &A, double theta, size_t &nagg)
 
 // ------------------------------------------------------------ kernels --
 // y = A x (mode 0), y += A x (mode 1), y = b - A x (mode 2)
+// STAGE: the workgroup's entries go through LDS first (levels from 2^19 rows;
+// on smaller levels the extra pass and barrier only add latency)
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_amg_spmv(size_t n, const uint64_t *__restrict__ rp,
                                                   const uint32_t *__restrict__ ci, const double *__restrict__ v,
                                                   const double *__restrict__ x, const double *__restrict__ b,
                                                   double *__restrict__ y, int mode, const int *skip) {
     if (skip && *skip) return;
-    const size_t i = blockIdx.x * (size_t)256 + threadIdx.x;
-    if (i >= n) return;
-    const double s = row_dot_ref<double>(rp, ci, v, [x](uint32_t j) { return x[j]; }, i);
+    const size_t r0 = blockIdx.x * (size_t)256;
+    const size_t i = r0 + threadIdx.x;
+    auto xat = [x](uint32_t j) { return x[j]; };
+    double s;
+    if constexpr (STAGE) {
+        __shared__ RowStage<double> st;
+        stage_rows<double>(st, rp, ci, v, n, r0);
+        if (i >= n) return;
+        s = row_staged<double>(st, rp, ci, v, xat, i);
+    } else {
+        if (i >= n) return;
+        s = row_dot_ref<double>(rp, ci, v, xat, i);
+    }
     y[i] = mode == 0 ? s : mode == 1 ? y[i] + s : b[i] - s;
 }
 
@@ -229,9 +242,12 @@ static unsigned g256(size_t n) { return (unsigned)((n + 255) / 256 ? (n + 255) /
 
 struct DCsr {
     size_t n = 0, m = 0, nnz = 0;
+    bool stage = false;  // k_amg_spmv stages entries in LDS (from 2^19 rows; FEMGPU_CSR_STAGE=1/0 forces)
     DevBuf rp, ci, v;
     void upload(const HCsr &H, hipStream_t s) {
         n = H.n; m = H.m; nnz = H.ci.size();
+        stage = n >= (1u << 19);
+        if (const char *f = std::getenv("FEMGPU_CSR_STAGE")) stage = std::atoi(f) != 0;
         rp.alloc((n + 1) * 8);
         ci.alloc(std::max<size_t>(nnz, 1) * 4);
         v.alloc(std::max<size_t>(nnz, 1) * 8);
@@ -242,9 +258,12 @@ struct DCsr {
         }
     }
     void spmv(const double *x, const double *b, double *y, int mode, hipStream_t s, const int *skip) const {
-        if (n)
-            hipLaunchKernelGGL(k_amg_spmv, dim3(g256(n)), dim3(256), 0, s, n, rp.as<uint64_t>(), ci.as<uint32_t>(),
-                               v.as<double>(), x, b, y, mode, skip);
+        if (n && stage)
+            hipLaunchKernelGGL(k_amg_spmv<true>, dim3(g256(n)), dim3(256), 0, s, n, rp.as<uint64_t>(),
+                               ci.as<uint32_t>(), v.as<double>(), x, b, y, mode, skip);
+        else if (n)
+            hipLaunchKernelGGL(k_amg_spmv<false>, dim3(g256(n)), dim3(256), 0, s, n, rp.as<uint64_t>(),
+                               ci.as<uint32_t>(), v.as<double>(), x, b, y, mode, skip);
     }
 };
 

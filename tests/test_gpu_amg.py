@@ -272,3 +272,27 @@ def test_amg_single_level_and_tiny(fg, ctx):
         assert r.converged and r.iterations <= 1
         assert np.abs(x - xe).max() <= 1e-10 * np.abs(xe).max()
         s.close()
+
+
+def test_amg_staged_levels_bitwise(fg, fr, ctx, monkeypatch):
+    """Levels from 2^19 rows run the level SpMV with each workgroup's entries
+    staged in LDS, in the same row order. An AMG-PCG solve on the 768^2 P1
+    Poisson matrix (591,361 rows) with staging on by default equals the one with
+    staging forced off (FEMGPU_CSR_STAGE=0), bit for bit."""
+    rp, ci, v, rhs, _, _ = fr.poisson_p1(768)
+    out = []
+    for stage in ("0", None):
+        if stage is None:
+            monkeypatch.delenv("FEMGPU_CSR_STAGE", raising=False)
+        else:
+            monkeypatch.setenv("FEMGPU_CSR_STAGE", stage)
+        A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+        amg = fg.AmgSolver(A)
+        x = np.zeros(len(rhs))
+        res = amg.solve(rhs, x, fg.SolverConfig(rtol=1e-8, max_iter=200))
+        assert res.converged
+        out.append((res.iterations, x))
+        amg.close()
+        A.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
