@@ -157,10 +157,13 @@ def test_pcg_jacobi_on_pa(fg, fr, ctx):
 
 
 def test_ir_mixed(fg, fr, ctx):
-    """FP32 inner PCG + FP64 refinement reaches 1e-10 and matches the FP64 solve."""
+    """FP32 inner PCG + FP64 refinement reaches 1e-10 and matches the FP64 solve
+    (Q3 with 4^3 points on the unit cube: both precisions run the quad-per-element
+    kernel)."""
     n, p = 10, 3
     op64 = _hex_problem(fg, fr, ctx, p, n, dict(kind="grf", seed=1))
     op32 = op64.convert(fg.F32)
+    assert op64.element_kernel() == op32.element_kernel() == "quad-per-element"
     b = op64.rhs_random(5)
     x = np.zeros(op64.n_dofs())
     res, inner = fg.solve_ir_mixed(op64, op32, b, x, fg.SolverConfig(rtol=1e-10), inner_rtol=1e-3, mode=0)
