@@ -379,9 +379,12 @@ static void amg_build(femgpu_amg *h, HCsr A0) {
         if (n) FG_HIP(hipMemcpyAsync(lv.inv.ptr, inv.data(), n * 8, hipMemcpyHostToDevice, s));
         for (DevBuf *b : {&lv.b, &lv.x, &lv.r, &lv.t, &lv.d, &lv.z, &lv.b2, &lv.x2})
             b->alloc(std::max<size_t>(n, 1) * 8);
+        // R is a named local: the asynchronous copies read it until the sync
+        HCsr R;
         if (l + 1 < h->L.size()) {
+            R = transpose(lv.P);
             lv.dP.upload(lv.P, s);
-            lv.dR.upload(transpose(lv.P), s);
+            lv.dR.upload(R, s);
         }
         FG_HIP(hipStreamSynchronize(s));  // host vectors above go out of scope
     }
@@ -504,6 +507,8 @@ void amg_apply(const femgpu_amg *h, const double *r, double *z, const int *skip)
     (void)hipGraphDestroy(graph);
     FG_HIP(ie);
     if (hm->graphs.size() >= kAmgGraphs) {
+        // the evicted graph may still run from an earlier launch on s
+        FG_HIP(hipStreamSynchronize(s));
         (void)hipGraphExecDestroy(hm->graphs.front().exec);
         hm->graphs.erase(hm->graphs.begin());
     }
@@ -523,8 +528,8 @@ int femgpu_amg_create(femgpu_op *op, const femgpu_amg_config *cfg, femgpu_amg **
     return guard([&] {
         FG_REQUIRE(op && out, FEMGPU_ERR_INVALID_ARG, "amg_create: NULL argument");
         CsrView cv;
-        FG_REQUIRE(op->dtype == FEMGPU_F64 && op->csr_view(&cv), FEMGPU_ERR_INVALID_ARG,
-                   "amg_create: needs an f64 CSR matrix with short rows");
+        FG_REQUIRE(op->dtype == FEMGPU_F64 && op->csr_arrays(&cv), FEMGPU_ERR_INVALID_ARG,
+                   "amg_create: needs an f64 CSR matrix");
         femgpu_ctx *c = op->ctx;
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         FG_HIP(hipSetDevice(c->device));

@@ -143,14 +143,14 @@ struct CsrOp final : femgpu_op {
             launch<float>((const float *)x, (float *)y, acc ? 1 : 0, 1.f, 0.f, skip);
     }
 
-    bool csr_view(CsrView *out) const override {
-        if (sub != 1) return false;
+    bool csr_arrays(CsrView *out) const override {
         out->n = n;
         out->rp = rp.as<uint64_t>();
         out->ci = ci.as<uint32_t>();
         out->vals = vals.ptr;
         return true;
     }
+    bool csr_view(CsrView *out) const override { return sub == 1 && csr_arrays(out); }
 
     void diagonal_dev(void *d) override {
         if (!n) return;

@@ -189,6 +189,12 @@ struct femgpu_op {
         (void)out;
         return false;
     }
+    // The device arrays of any assembled CSR operator, whatever kernel its
+    // SpMV runs (set-up code that copies the matrix, e.g. AMG).
+    virtual bool csr_arrays(femgpu::CsrView *out) const {
+        (void)out;
+        return false;
+    }
     size_t esize() const { return dtype == FEMGPU_F64 ? 8 : 4; }
 };
 

@@ -658,11 +658,15 @@ static bool fused_pcg_enabled() {
     const char *e = std::getenv("FEMGPU_PCG_FUSED");
     return !(e && e[0] == '0');
 }
-// FEMGPU_GRAPHS=0 launches every iteration's kernels directly instead of
-// replaying captured chunks.
+// The fused PCG-Jacobi loop replays captured chunks of iterations from a HIP
+// graph only when FEMGPU_GRAPHS=1.  The replay saved 0.4 us of a 15 us
+// iteration (66K rows); a profiled run (rocprofv3 --kernel-trace) of a solve
+// with the replay on died with a host SIGSEGV inside the runtime's graph
+// launch, which the code here could not be shown to cause, so direct
+// launches are the default.
 static bool graphs_enabled() {
     const char *e = std::getenv("FEMGPU_GRAPHS");
-    return !(e && e[0] == '0');
+    return e && e[0] == '1';
 }
 
 static unsigned vgrid(size_t n) {
@@ -987,6 +991,11 @@ template <class T> struct Krylov {
         return finish(MODE_PCG, max_iter, res);
     }
 
+    // Iterations the last solve completed (x and r updated), whatever its
+    // status: at a breakdown the result reports max_iter, as the reference
+    // does, but fewer ran.  Valid after finish().
+    uint64_t iterations_done() const { return (uint64_t)c->pinned[S_ITERS]; }
+
     // Reads the scalars after a solve and fills the result.
     int finish(int mode, uint64_t max_iter, femgpu_solve_result *res) {
         read_scalars();
@@ -1199,7 +1208,7 @@ static int ir_continued(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_config
         femgpu_solve_result seg{};
         const uint64_t len = std::min<uint64_t>(max_seg, cfg->max_iter - total);
         K32.solve_cont(first, cp, x32, bnorm, std::max(0.5 * cfg->rtol, delta * rel), len, &seg);
-        total += seg.converged ? seg.iterations : len;
+        total += K32.iterations_done();
         first = K32.last_status == ST_BREAKDOWN;
         convert_f32_to_f64_axpy(s, xd, x32, n, 1.0);
         rel = residual() / bnorm;
@@ -1393,7 +1402,10 @@ int femgpu_solve_ir_mixed(femgpu_op *op64, femgpu_op *op32, const femgpu_ir_conf
         return ir_refine(op64, op32, ir, b, x, mem, cfg, res, inner_iterations,
                          [&](Krylov<float> &K32, const ChebParams &cp, const float *r32, float *e32,
                              double t, uint64_t cap, femgpu_solve_result *ir_res) {
-                             return K32.solve(MODE_PCG, PREC_CHEB, cp, r32, e32, t, 0.0, cap, ir_res);
+                             const int rc = K32.solve(MODE_PCG, PREC_CHEB, cp, r32, e32, t, 0.0, cap, ir_res);
+                             // a breakdown reports max_iter; count what ran
+                             ir_res->iterations = K32.iterations_done();
+                             return rc;
                          });
     });
 }

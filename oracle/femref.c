@@ -466,79 +466,135 @@ void fr_hex_qdata(const fr_hex_desc *d, const double *coords, double *D, double 
  * slowest, quadrature.rs:405-419) isoparametric J, J^-T gradients, kappa at the
  * physical point, y_e[i] += w|J| kappa grad u . grad phi_i (+ mass term),
  * scatter-add. */
-void fr_hex_apply_refloop(const fr_hex_desc *d, const double *coords, const double *x, double *y,
-                          int accumulate) {
-    hex_tabs t;
-    hex_tabs_init(d, &t);
-    const int p = d->p, n1 = p + 1, q = d->q, nl = n1 * n1 * n1;
-    const size_t N = fr_hex_ndofs(d);
-    if (!accumulate) memset(y, 0, N * sizeof(double));
-    double *phi = malloc(sizeof(double) * nl), *gr = malloc(sizeof(double) * nl * 3),
-           *gp = malloc(sizeof(double) * nl * 3), *xe = malloc(sizeof(double) * nl),
-           *ye = malloc(sizeof(double) * nl);
-    size_t *gd = malloc(sizeof(size_t) * nl);
-    for (int ez = 0; ez < d->nz; ++ez)
-        for (int ey = 0; ey < d->ny; ++ey)
-            for (int ex = 0; ex < d->nx; ++ex) {
-                double X[8][3];
-                hex_elem_vertices(d, coords, ex, ey, ez, X);
+typedef struct { double *phi, *gr, *gp, *xe, *ye; size_t *gd; } refloop_scratch;
+
+static void refloop_scratch_init(refloop_scratch *w, int nl) {
+    w->phi = malloc(sizeof(double) * nl); w->gr = malloc(sizeof(double) * nl * 3);
+    w->gp = malloc(sizeof(double) * nl * 3); w->xe = malloc(sizeof(double) * nl);
+    w->ye = malloc(sizeof(double) * nl); w->gd = malloc(sizeof(size_t) * nl);
+}
+static void refloop_scratch_free(refloop_scratch *w) {
+    free(w->phi); free(w->gr); free(w->gp); free(w->xe); free(w->ye); free(w->gd);
+}
+
+/* One element of the reference PA loop: gather, per quadrature point J, J^-T
+ * gradients, kappa, element contribution, scatter-add into y. */
+static void refloop_element(const fr_hex_desc *d, const hex_tabs *t, const double *coords,
+                            const double *x, double *y, int ex, int ey, int ez, refloop_scratch *w) {
+    const int n1 = d->p + 1, q = d->q, nl = n1 * n1 * n1;
+    double *phi = w->phi, *gr = w->gr, *gp = w->gp, *xe = w->xe, *ye = w->ye;
+    size_t *gd = w->gd;
+    double X[8][3];
+    hex_elem_vertices(d, coords, ex, ey, ez, X);
+    for (int k = 0; k < n1; ++k)
+        for (int j = 0; j < n1; ++j)
+            for (int i = 0; i < n1; ++i) {
+                int l = i + n1 * (j + n1 * k);
+                gd[l] = hex_gdof(d, ex, ey, ez, i, j, k);
+                xe[l] = (d->dirichlet && hex_is_bdr(d, gd[l])) ? 0.0 : x[gd[l]];
+                ye[l] = 0.0;
+            }
+    for (int a = 0; a < q; ++a)
+        for (int b = 0; b < q; ++b)
+            for (int c = 0; c < q; ++c) {
+                double xi[3] = {t->x[a], t->x[b], t->x[c]};
+                double J[3][3], xp[3], I[3][3];
+                double det = hex_jacobian(X, xi, J, xp);
+                double wq = t->w[a] * t->w[b] * t->w[c] * fabs(det);
+                inv3(J, det, I);
                 for (int k = 0; k < n1; ++k)
                     for (int j = 0; j < n1; ++j)
                         for (int i = 0; i < n1; ++i) {
                             int l = i + n1 * (j + n1 * k);
-                            gd[l] = hex_gdof(d, ex, ey, ez, i, j, k);
-                            xe[l] = (d->dirichlet && hex_is_bdr(d, gd[l])) ? 0.0 : x[gd[l]];
-                            ye[l] = 0.0;
-                        }
-                for (int a = 0; a < q; ++a)
-                    for (int b = 0; b < q; ++b)
-                        for (int c = 0; c < q; ++c) {
-                            double xi[3] = {t.x[a], t.x[b], t.x[c]};
-                            double J[3][3], xp[3], I[3][3];
-                            double det = hex_jacobian(X, xi, J, xp);
-                            double w = t.w[a] * t.w[b] * t.w[c] * fabs(det);
-                            inv3(J, det, I);
-                            for (int k = 0; k < n1; ++k)
-                                for (int j = 0; j < n1; ++j)
-                                    for (int i = 0; i < n1; ++i) {
-                                        int l = i + n1 * (j + n1 * k);
-                                        double Bi = t.B[a * n1 + i], Bj = t.B[b * n1 + j],
-                                               Bk = t.B[c * n1 + k];
-                                        phi[l] = Bi * Bj * Bk;
-                                        gr[3 * l] = t.G[a * n1 + i] * Bj * Bk;
-                                        gr[3 * l + 1] = Bi * t.G[b * n1 + j] * Bk;
-                                        gr[3 * l + 2] = Bi * Bj * t.G[c * n1 + k];
-                                        /* grad_phys = J^-T grad_ref (assembler.rs:128-142) */
-                                        for (int r = 0; r < 3; ++r) {
-                                            double s = 0.0;
-                                            for (int m = 0; m < 3; ++m) s += I[m][r] * gr[3 * l + m];
-                                            gp[3 * l + r] = s;
-                                        }
-                                    }
-                            if (d->kind & FR_DIFFUSION) {
-                                double kap = fr_coef_eval(&d->kappa, t.wf, t.bf, xp);
-                                double gu[3] = {0, 0, 0};
-                                for (int l = 0; l < nl; ++l)
-                                    for (int r = 0; r < 3; ++r) gu[r] += xe[l] * gp[3 * l + r];
-                                for (int l = 0; l < nl; ++l) {
-                                    double dot = 0.0;
-                                    for (int r = 0; r < 3; ++r) dot += gu[r] * gp[3 * l + r];
-                                    ye[l] += w * kap * dot;
-                                }
-                            }
-                            if (d->kind & FR_MASS) {
-                                double u = 0.0;
-                                for (int l = 0; l < nl; ++l) u += phi[l] * xe[l];
-                                for (int l = 0; l < nl; ++l) ye[l] += w * d->rho * phi[l] * u;
+                            double Bi = t->B[a * n1 + i], Bj = t->B[b * n1 + j], Bk = t->B[c * n1 + k];
+                            phi[l] = Bi * Bj * Bk;
+                            gr[3 * l] = t->G[a * n1 + i] * Bj * Bk;
+                            gr[3 * l + 1] = Bi * t->G[b * n1 + j] * Bk;
+                            gr[3 * l + 2] = Bi * Bj * t->G[c * n1 + k];
+                            /* grad_phys = J^-T grad_ref (assembler.rs:128-142) */
+                            for (int r = 0; r < 3; ++r) {
+                                double s = 0.0;
+                                for (int m = 0; m < 3; ++m) s += I[m][r] * gr[3 * l + m];
+                                gp[3 * l + r] = s;
                             }
                         }
-                for (int l = 0; l < nl; ++l)
-                    if (!(d->dirichlet && hex_is_bdr(d, gd[l]))) y[gd[l]] += ye[l];
+                if (d->kind & FR_DIFFUSION) {
+                    double kap = fr_coef_eval(&d->kappa, t->wf, t->bf, xp);
+                    double gu[3] = {0, 0, 0};
+                    for (int l = 0; l < nl; ++l)
+                        for (int r = 0; r < 3; ++r) gu[r] += xe[l] * gp[3 * l + r];
+                    for (int l = 0; l < nl; ++l) {
+                        double dot = 0.0;
+                        for (int r = 0; r < 3; ++r) dot += gu[r] * gp[3 * l + r];
+                        ye[l] += wq * kap * dot;
+                    }
+                }
+                if (d->kind & FR_MASS) {
+                    double u = 0.0;
+                    for (int l = 0; l < nl; ++l) u += phi[l] * xe[l];
+                    for (int l = 0; l < nl; ++l) ye[l] += wq * d->rho * phi[l] * u;
+                }
             }
+    for (int l = 0; l < nl; ++l)
+        if (!(d->dirichlet && hex_is_bdr(d, gd[l]))) y[gd[l]] += ye[l];
+}
+
+/* The reference PA element loop (partial.rs:151-208) generalised to hex Qp:
+ * per element gather, per quadrature point (hex_rule order: first coordinate
+ * slowest, quadrature.rs:405-419) isoparametric J, J^-T gradients, kappa at the
+ * physical point, y_e[i] += w|J| kappa grad u . grad phi_i (+ mass term),
+ * scatter-add.  Serial, elements in lexicographic order, as the reference. */
+void fr_hex_apply_refloop(const fr_hex_desc *d, const double *coords, const double *x, double *y,
+                          int accumulate) {
+    hex_tabs t;
+    hex_tabs_init(d, &t);
+    const int n1 = d->p + 1, nl = n1 * n1 * n1;
+    const size_t N = fr_hex_ndofs(d);
+    if (!accumulate) memset(y, 0, N * sizeof(double));
+    refloop_scratch w;
+    refloop_scratch_init(&w, nl);
+    for (int ez = 0; ez < d->nz; ++ez)
+        for (int ey = 0; ey < d->ny; ++ey)
+            for (int ex = 0; ex < d->nx; ++ex) refloop_element(d, &t, coords, x, y, ex, ey, ez, &w);
     if (d->dirichlet)
         for (size_t g = 0; g < N; ++g)
             if (hex_is_bdr(d, g)) y[g] += x[g];
-    free(phi); free(gr); free(gp); free(xe); free(ye); free(gd);
+    refloop_scratch_free(&w);
+    hex_tabs_free(&t);
+}
+
+/* The same element loop on nthreads OpenMP threads (the CPU baseline's
+ * threaded form of the reference algorithm): the elements are split into the
+ * 8 parity colours (ex, ey, ez mod 2), whose elements share no DOF, and each
+ * colour's elements run in parallel.  Sums into y in another order than the
+ * serial loop, so results agree with it to rounding. */
+void fr_hex_apply_refloop_mt(const fr_hex_desc *d, const double *coords, const double *x, double *y,
+                             int accumulate, int nthreads) {
+    hex_tabs t;
+    hex_tabs_init(d, &t);
+    const int n1 = d->p + 1, nl = n1 * n1 * n1;
+    const size_t N = fr_hex_ndofs(d);
+    if (!accumulate) memset(y, 0, N * sizeof(double));
+    for (int col = 0; col < 8; ++col) {
+        const int px = col & 1, py = (col >> 1) & 1, pz = col >> 2;
+        const int cx = (d->nx - px + 1) / 2, cy = (d->ny - py + 1) / 2, cz = (d->nz - pz + 1) / 2;
+        const long ncol = (long)cx * cy * cz;
+#pragma omp parallel num_threads(nthreads)
+        {
+            refloop_scratch w;
+            refloop_scratch_init(&w, nl);
+#pragma omp for schedule(static)
+            for (long e = 0; e < ncol; ++e) {
+                const int ex = px + 2 * (int)(e % cx), ey = py + 2 * (int)((e / cx) % cy),
+                          ez = pz + 2 * (int)(e / ((long)cx * cy));
+                refloop_element(d, &t, coords, x, y, ex, ey, ez, &w);
+            }
+            refloop_scratch_free(&w);
+        }
+    }
+    if (d->dirichlet)
+        for (size_t g = 0; g < N; ++g)
+            if (hex_is_bdr(d, g)) y[g] += x[g];
     hex_tabs_free(&t);
 }
 

@@ -94,6 +94,10 @@ void fr_hex_qdata(const fr_hex_desc *d, const double *coords, double *D, double 
  * every point, physical gradients J^-T grad_ref, no sum factorisation. */
 void fr_hex_apply_refloop(const fr_hex_desc *d, const double *coords, const double *x,
                           double *y, int accumulate);
+/* fr_hex_apply_refloop on nthreads threads over the 8 element parity colours
+ * (the threaded CPU baseline of the reference algorithm). */
+void fr_hex_apply_refloop_mt(const fr_hex_desc *d, const double *coords, const double *x, double *y,
+                             int accumulate, int nthreads);
 /* y (+)= A x by sum factorisation on stored qdata (no reference counterpart;
  * the optimised CPU path, OpenMP over element colours). */
 void fr_hex_apply_sumfac(const fr_hex_desc *d, const double *D, const double *M,

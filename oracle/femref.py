@@ -90,6 +90,7 @@ def _declare(L):
         "fr_hex_ndofs": (Z, [P]),
         "fr_hex_qdata": (None, [P, P, P, P]),
         "fr_hex_apply_refloop": (None, [P, P, P, P, I]),
+        "fr_hex_apply_refloop_mt": (None, [P, P, P, P, I, I]),
         "fr_hex_apply_sumfac": (None, [P, P, P, P, P, I, I]),
         "fr_hex_apply_sumfac_f32": (None, [P, P, P, P, P, I, I]),
         "fr_hex_diagonal": (None, [P, P, P, P]),
@@ -217,11 +218,16 @@ def hex_qdata(d, coords):
     return Dq, Mq
 
 
-def hex_apply_refloop(d, coords, x, y=None):
+def hex_apply_refloop(d, coords, x, y=None, nthreads=0):
+    """Reference element loop; nthreads > 0 runs the parity-coloured threaded
+    form (same operator, another summation order into y)."""
     x = _f64(x)
     acc = y is not None
     y = np.zeros(hex_ndofs(d)) if y is None else _f64(y).copy()
-    lib().fr_hex_apply_refloop(C.byref(d), _p(_f64(coords)), _p(x), _p(y), 1 if acc else 0)
+    if nthreads > 0:
+        lib().fr_hex_apply_refloop_mt(C.byref(d), _p(_f64(coords)), _p(x), _p(y), 1 if acc else 0, nthreads)
+    else:
+        lib().fr_hex_apply_refloop(C.byref(d), _p(_f64(coords)), _p(x), _p(y), 1 if acc else 0)
     return y
 
 

@@ -129,3 +129,104 @@ def test_grf_statistics(fr):
     for _ in range(20):
         k = fr.coef_eval(coef, rng.random(3))
         assert 1e-2 <= k <= 1e2
+
+
+@pytest.mark.parametrize("p,q", [(1, 2), (2, 3), (3, 4), (4, 5), (2, 4)])
+def test_linear_patch_perturbed(fr, p, q):
+    """Patch test on the perturbed (non-affine) mesh, fixed by the mathematics
+    alone.  For linear u = a.x + c and v = b.x + d (exactly in Q_p on a
+    trilinear mesh) and kappa = rho = 1:
+      v^T K u = int grad u . grad v = a.b |Omega| = a.b (the boundary is the
+        planar unit cube; the integrand a.b |J| has degree <= 2 per reference
+        direction, exact for q >= 2),
+      (K u)_i = int a . grad phi_i = 0 at every interior DOF,
+      v^T M u = int (a.x + c)(b.x + d) (degree <= 4 per direction, q >= 3).
+    A transposed metric (J^-T J^-1 for J^-1 J^-T, the defect of the reference's
+    FemCeed, reed/context.rs:282-284) or a wrong |J| fails the first two."""
+    import hexgeom
+    n = (3, 2, 3)
+    c = fr.hex_vertices(*n, 0.2, 13)
+    X = hexgeom.dof_coords(*n, p, c)
+    rng = np.random.default_rng(100 + p)
+    a, b = rng.standard_normal(3), rng.standard_normal(3)
+    c0, d0 = rng.standard_normal(2)
+    u, v = X @ a + c0, X @ b + d0
+    dk = fr.hex_desc(p, q, n, kind=fr.FR_DIFFUSION, kappa=fr.coef_const(1.0))
+    Dk, Mk = fr.hex_qdata(dk, c)
+    Ku = fr.hex_apply_sumfac(dk, Dk, Mk, u)
+    assert abs(v @ Ku - a @ b) < 1e-12 * (1 + abs(a @ b))
+    inner = ~hexgeom.boundary_mask(*n, p)
+    assert np.abs(Ku[inner]).max() < 1e-12
+    assert np.allclose(fr.hex_apply_refloop(dk, c, u), Ku, rtol=0, atol=1e-13)
+    if q >= 3:
+        dm = fr.hex_desc(p, q, n, kind=fr.FR_MASS, rho=1.0)
+        Dm, Mm = fr.hex_qdata(dm, c)
+        exact = hexgeom.int_linear_product(a, c0, b, d0)
+        assert abs(v @ fr.hex_apply_sumfac(dm, Dm, Mm, u) - exact) < 1e-13 * (1 + abs(exact))
+        dd = fr.hex_desc(p, q, n, kind=fr.FR_DIFF_MASS, kappa=fr.coef_const(1.0), rho=1.0)
+        assert abs(v @ fr.hex_apply_refloop(dd, c, u) - (a @ b + exact)) < 1e-12 * (1 + abs(exact))
+
+
+def test_patch_test_detects_transposed_metric(fr):
+    """The patch test above is sensitive to the metric's orientation: with the
+    qdata's J^-1 J^-T replaced by J^-T J^-1 (built here from the same
+    Jacobians), v^T K u moves away from a.b on the perturbed mesh."""
+    import hexgeom
+    p, q, n = 2, 3, (2, 2, 2)
+    c = fr.hex_vertices(*n, 0.2, 13)
+    X = hexgeom.dof_coords(*n, p, c)
+    a, b = np.array([1.0, 0.3, -0.7]), np.array([0.2, -1.1, 0.5])
+    u, v = X @ a, X @ b
+    # transposed metric on every Gauss point, from the trilinear Jacobian
+    xg, wg = fr.gauss_legendre(q)
+    V = c.reshape(n[2] + 1, n[1] + 1, n[0] + 1, 3)
+    Nx = n[0] * p + 1
+    Ny = n[1] * p + 1
+    B, G = fr.lagrange_1d(p, xg)
+    B, G = B.reshape(q, p + 1), G.reshape(q, p + 1)
+    # trilinear map x = sum_v X_v prod_d (1 + s_d xi_d) / 8, J_ij = dx_i / dxi_j
+    total_ok, total_bad = 0.0, 0.0
+    for ez in range(n[2]):
+        for ey in range(n[1]):
+            for ex in range(n[0]):
+                idx = np.array([[[ (ex * p + i) + Nx * ((ey * p + j) + Ny * (ez * p + k))
+                                   for i in range(p + 1)] for j in range(p + 1)] for k in range(p + 1)])
+                ue, ve = u[idx], v[idx]  # [k][j][i]
+                for cz in range(q):
+                    for cy in range(q):
+                        for cx in range(q):
+                            xi = np.array([xg[cx], xg[cy], xg[cz]])
+                            J = np.zeros((3, 3))
+                            for aa in (0, 1):
+                                for bb in (0, 1):
+                                    for cc in (0, 1):
+                                        w = [(1 + (2 * aa - 1) * xi[0]), (1 + (2 * bb - 1) * xi[1]),
+                                             (1 + (2 * cc - 1) * xi[2])]
+                                        dw = np.array([(2 * aa - 1) * w[1] * w[2], (2 * bb - 1) * w[0] * w[2],
+                                                       (2 * cc - 1) * w[0] * w[1]]) / 8.0
+                                        J += np.outer(V[ez + cc, ey + bb, ex + aa], dw)
+                            gu = np.array([np.einsum("k,j,i,kji", B[cz], B[cy], G[cx], ue),
+                                           np.einsum("k,j,i,kji", B[cz], G[cy], B[cx], ue),
+                                           np.einsum("k,j,i,kji", G[cz], B[cy], B[cx], ue)])
+                            gv = np.array([np.einsum("k,j,i,kji", B[cz], B[cy], G[cx], ve),
+                                           np.einsum("k,j,i,kji", B[cz], G[cy], B[cx], ve),
+                                           np.einsum("k,j,i,kji", G[cz], B[cy], B[cx], ve)])
+                            Ji = np.linalg.inv(J)
+                            wq = wg[cx] * wg[cy] * wg[cz] * abs(np.linalg.det(J))
+                            total_ok += wq * gv @ (Ji @ Ji.T) @ gu
+                            total_bad += wq * gv @ (Ji.T @ Ji) @ gu
+    assert abs(total_ok - a @ b) < 1e-12
+    assert abs(total_bad - a @ b) > 1e-4
+
+
+@pytest.mark.parametrize("dirichlet", [False, True])
+def test_refloop_threaded_matches_serial(fr, dirichlet):
+    """The parity-coloured threaded reference loop (the CPU baseline's
+    threaded form) is the serial loop's operator up to summation order, on
+    odd and even element counts."""
+    d, c = _setup(fr, 2, 3, n=(3, 4, 5), dirichlet=dirichlet)
+    x = np.random.default_rng(4).standard_normal(fr.hex_ndofs(d))
+    y1 = fr.hex_apply_refloop(d, c, x)
+    for nt in (1, 3):
+        y2 = fr.hex_apply_refloop(d, c, x, nthreads=nt)
+        assert np.abs(y1 - y2).max() < 1e-13 * np.abs(y1).max()
