@@ -86,6 +86,10 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(ptr); }
 };
 
+// solvers.hip: device address of the allocation-free solver's scratch array
+// on `device` (which must be current); looked up once per device.
+void *noalloc_scratch(int device);
+
 // amg.hip: one V-cycle z = V r on the AMG's stream; its level-0 operator.
 void amg_apply(const femgpu_amg *h, const double *r, double *z, const int *skip);
 femgpu_op *amg_operator(const femgpu_amg *h);
@@ -117,6 +121,7 @@ struct femgpu_ctx {
     femgpu::DevBuf flags;     // int flags: [0] stop (kernels skip work when set)
     double *pinned = nullptr; // host-visible scalars for read-back
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    void *noalloc_scratch = nullptr;  // this device's scratch array (solvers.hip)
 };
 
 // ------------------------------------------------------------ operator --

@@ -70,6 +70,7 @@ int femgpu_ctx_create(int device, femgpu_ctx **out) {
             FG_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * kScalarSlots, hipHostMallocDefault));
             FG_HIP(hipEventCreate(&c->t0));
             FG_HIP(hipEventCreate(&c->t1));
+            c->noalloc_scratch = noalloc_scratch(device);
         } catch (...) {
             femgpu_ctx_destroy(c);
             throw;

@@ -675,6 +675,33 @@ static unsigned vgrid(size_t n) {
     return (unsigned)(b ? b : 1);
 }
 
+// ------------------------------------------------ allocation-free workspace --
+// The work vectors of femgpu_solve_pcg_jacobi_noalloc: one file-scope device
+// array, which each GPU holds (its own copy, at its own address) from the
+// loading of this code object until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host it is only a stand-in for the
+// symbol): noalloc_scratch() looks up the device address once per device and
+// every use goes through that pointer.
+static __device__ __align__(256) unsigned char scratch[FEMGPU_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[64];
+
+// Address of this device's scratch array; `device` must be current.  Called
+// at context creation, so that the lookup is done before any solve.
+void *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    FG_REQUIRE(device >= 0 && device < 64, FEMGPU_ERR_INVALID_ARG, "noalloc: device index out of range");
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        FG_REQUIRE(e == hipSuccess && ptr, FEMGPU_ERR_HIP,
+                   std::string("noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[device] = ptr;
+    }
+    return g_scratch_dev[device];
+}
+
 // ------------------------------------------------------------ engine --
 struct ChebParams {
     int degree = 4;
@@ -693,7 +720,13 @@ template <class T> struct Krylov {
     enum { W_R = 0, W_Z, W_P, W_AP, W_INV, W_D, W_T, W_DIAG, W_X, W_B, W_COUNT };
     // preconditioner for PREC_CB: z = M^-1 r (kernels may honour the stop flag)
     std::function<void(const T *, T *)> pc;
-    T *vec(int slot) { return (T *)op->wsvec(slot, n * sizeof(T)); }
+    // Caller-provided workspace of W_COUNT vectors, ext_stride entries apart
+    // (the allocation-free entry point), or null: the operator's workspace.
+    T *ext = nullptr;
+    size_t ext_stride = 0;
+    T *vec(int slot) {
+        return ext ? ext + (size_t)slot * ext_stride : (T *)op->wsvec(slot, n * sizeof(T));
+    }
 
     explicit Krylov(femgpu_op *o) : op(o), c(o->ctx), n(o->n), s(o->ctx->stream) {
         S = c->scalars.as<double>();
@@ -707,7 +740,9 @@ template <class T> struct Krylov {
     // diag(A) in the W_DIAG slot, computed once per operator
     T *diag() {
         T *d = vec(W_DIAG);
-        if (!op->diag_ready) {
+        if (ext) {
+            op->diagonal_dev(d);  // the shared workspace keeps no state between calls
+        } else if (!op->diag_ready) {
             op->diagonal_dev(d);
             op->diag_ready = true;
         }
@@ -818,7 +853,7 @@ template <class T> struct Krylov {
                                (const T *)ap, inv, z, part, (int)nb, S, flags, kFzRz[(k - 1) & 1],
                                kFzRz[k & 1], brk);
         };
-        const bool graphs = graphs_enabled();
+        const bool graphs = graphs_enabled() && !ext;  // instantiating a graph allocates
         uint64_t it = 0, since = 0;
         const uint64_t check = n < (1u << 20) ? kPcgChunk : 8;
         bool stopped = read_flag() != 0;
@@ -1078,6 +1113,14 @@ struct Staged {
     }
 };
 
+// Work vectors of a Krylov solve (its workspace slots) and the largest n the
+// allocation-free entry point takes in each precision.
+constexpr int KrylovSlots = Krylov<double>::W_COUNT;
+static size_t noalloc_max_n(femgpu_dtype dt) {
+    const size_t per = (size_t)FEMGPU_NOALLOC_SCRATCH_BYTES / KrylovSlots / (dt == FEMGPU_F64 ? 8 : 4);
+    return per / 64 * 64;
+}
+
 template <class F> static int dispatch(femgpu_op *op, F &&f) {
     if (op->dtype == FEMGPU_F64) { Krylov<double> k(op); return f(k); }
     Krylov<float> k(op);
@@ -1268,6 +1311,40 @@ int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem me
                            cfg->atol, cfg->max_iter, res);
         });
         st.finish();
+        if (rc == FEMGPU_ERR_NOT_CONVERGED)
+            return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +
+                                " iterations (residual = " + std::to_string(res->final_residual) + ")");
+        return rc;
+    });
+}
+
+int femgpu_solve_pcg_jacobi_noalloc(femgpu_op *op, const void *b, void *x, const femgpu_solver_config *cfg,
+                                    femgpu_solve_result *res) {
+    return guard([&] {
+        FG_REQUIRE(op && b && x && res, FEMGPU_ERR_INVALID_ARG, "solve_pcg_jacobi_noalloc: NULL argument");
+        check_cfg(cfg);
+        CsrView cv;
+        FG_REQUIRE(op->csr_arrays(&cv), FEMGPU_ERR_INVALID_ARG,
+                   "solve_pcg_jacobi_noalloc: needs an assembled CSR operator (its apply uses no workspace)");
+        std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
+        FG_HIP(hipSetDevice(op->ctx->device));
+        void *ws = op->ctx->noalloc_scratch;
+        FG_REQUIRE(ws, FEMGPU_ERR_HIP, "solve_pcg_jacobi_noalloc: the context has no scratch address");
+        // W_COUNT work vectors, each starting on a 256-byte boundary
+        const size_t stride = (op->n + 63) / 64 * 64;
+        const size_t need = (size_t)KrylovSlots * stride * op->esize();
+        FG_REQUIRE(need <= (size_t)FEMGPU_NOALLOC_SCRATCH_BYTES, FEMGPU_ERR_INVALID_ARG,
+                   "solve_pcg_jacobi_noalloc: n = " + std::to_string(op->n) + " needs " + std::to_string(need) +
+                       " bytes of workspace, more than the " + std::to_string((size_t)FEMGPU_NOALLOC_SCRATCH_BYTES) +
+                       "-byte scratch array (at most " + std::to_string(noalloc_max_n(op->dtype)) +
+                       " unknowns in this precision)");
+        const int rc = dispatch(op, [&](auto &k) {
+            using T = std::remove_pointer_t<decltype(k.vec(0))>;
+            k.ext = (T *)ws;
+            k.ext_stride = stride;
+            return k.solve(MODE_PCG, PREC_JACOBI, ChebParams{}, (const T *)b, (T *)x, cfg->rtol, cfg->atol,
+                           cfg->max_iter, res);
+        });
         if (rc == FEMGPU_ERR_NOT_CONVERGED)
             return fail(rc, "solver did not converge in " + std::to_string(cfg->max_iter) +
                                 " iterations (residual = " + std::to_string(res->final_residual) + ")");

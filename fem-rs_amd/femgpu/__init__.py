@@ -144,6 +144,7 @@ _sig("femgpu_solve_amg_cg", [P, P, P, I, P, P])
 _sig("femgpu_essential_mask_rhs", [P, P, I])
 _sig("femgpu_solve_cg_operator", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
 _sig("femgpu_solve_pcg_jacobi", [P, P, P, I, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
+_sig("femgpu_solve_pcg_jacobi_noalloc", [P, P, P, C.POINTER(_SolverConfig), C.POINTER(_SolveResult)])
 _sig("femgpu_solve_pcg_chebyshev", [P, C.POINTER(_ChebConfig), P, P, I, C.POINTER(_SolverConfig),
                                     C.POINTER(_SolveResult), C.POINTER(D)])
 _sig("femgpu_chebyshev_smooth", [P, P, P, D, D, I, I])
@@ -744,6 +745,23 @@ def solve_pcg_jacobi(a, b, x, cfg=None):
     rc = lib.femgpu_solve_pcg_jacobi(a.h, bp, xp, m, C.byref(cfg._c()), C.byref(r))
     if rc == ERR_NOT_CONVERGED:
         return _result(r)  # par_solve_pcg_jacobi: Ok(converged = false)
+    _check(rc)
+    return _result(r)
+
+
+def solve_pcg_jacobi_noalloc(a, b, x, cfg=None):
+    """solve_pcg_jacobi for callers that cannot allocate: same result as
+    solve_pcg_jacobi, with the work vectors in the library's fixed per-GPU
+    scratch array (no device allocation in the call).  a is an assembled CSR
+    matrix; b and x are DeviceArrays; at most 838,848 unknowns in FP64."""
+    cfg = cfg or SolverConfig()
+    _check_dims(a, b, x)
+    if not (isinstance(b, DeviceArray) and isinstance(x, DeviceArray)):
+        raise TypeError("solve_pcg_jacobi_noalloc takes DeviceArrays only")
+    r = _SolveResult()
+    rc = lib.femgpu_solve_pcg_jacobi_noalloc(a.h, b.ptr, x.ptr, C.byref(cfg._c()), C.byref(r))
+    if rc == ERR_NOT_CONVERGED:
+        return _result(r)
     _check(rc)
     return _result(r)
 

@@ -300,6 +300,21 @@ int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem m
  * Works for any operator (CSR or PA).  NOT_CONVERGED on max_iter. */
 int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem mem,
                             const femgpu_solver_config *cfg, femgpu_solve_result *res);
+/* No counterpart in the reference: femgpu_solve_pcg_jacobi for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels, iterates and result
+ * as femgpu_solve_pcg_jacobi with FEMGPU_DEVICE vectors, but the call allocates
+ * nothing: its work vectors (10 of n entries) live in a fixed device array of
+ * FEMGPU_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends.
+ * b and x are device pointers; op must be an assembled CSR operator
+ * (femgpu_csr_create / femgpu_csr_assemble_simplex), whose apply needs no
+ * workspace.  n above 838,848 (f64) or 1,677,696 (f32) returns INVALID_ARG.
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents (each call writes every vector it reads first): calls that overlap
+ * on one GPU, from several contexts or threads, race. */
+#define FEMGPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+int femgpu_solve_pcg_jacobi_noalloc(femgpu_op *op, const void *b, void *x,
+                                    const femgpu_solver_config *cfg, femgpu_solve_result *res);
 /* AmgSolver::solve (amg/src/lib.rs:188-206): PCG with one V-cycle as the
  * preconditioner, the stopping rule of par_solve_pcg_jacobi (relative
  * residual; converged = 0 at max_iter), on the AMG's operator. */
