@@ -71,6 +71,7 @@ __device__ __forceinline__ void simplex_geom_dev(const double *__restrict__ coor
 // tab: [nq] weights, [nq][nl] phi, [nq][nl][DIM] reference gradients
 template <int DIM>
 __global__ __launch_bounds__(256) void k_simplex_elem_mat(size_t ne, int nl, int nq, int form, double coef,
+                                                          const double *__restrict__ cq,
                                                           const double *__restrict__ coords,
                                                           const uint32_t *__restrict__ nodes,
                                                           const uint32_t *__restrict__ edofs,
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void k_simplex_elem_mat(size_t ne, int nl, int
         for (int j = 0; j < nl; ++j) {
             double s = 0.0;
             for (int q = 0; q < nq; ++q) {
-                const double wc = (W[q] * adet) * coef;
+                const double wc = (W[q] * adet) * (cq ? cq[e * nq + q] : coef);
                 if (form == FEMGPU_DIFFUSION) {
                     double gi[3], gj[3];
                     for (int r = 0; r < DIM; ++r) {
@@ -236,11 +237,10 @@ static int bits_for(uint64_t n) {
 
 using namespace femgpu;
 
-extern "C" {
-
-int femgpu_csr_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
-                                uint64_t n_elem, const uint32_t *elem_nodes, const uint32_t *elem_dofs,
-                                uint64_t n_dofs, int form, double coef, int quad_order, femgpu_op **out) {
+static int assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
+                            uint64_t n_elem, const uint32_t *elem_nodes, const uint32_t *elem_dofs,
+                            uint64_t n_dofs, int form, double coef, const double *coef_qp, int quad_order,
+                            femgpu_op **out) {
     return guard([&] {
         FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "csr_assemble_simplex: NULL argument");
         FG_REQUIRE(form == FEMGPU_MASS || form == FEMGPU_DIFFUSION, FEMGPU_ERR_INVALID_ARG,
@@ -255,6 +255,11 @@ int femgpu_csr_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_no
         FG_HIP(hipSetDevice(c->device));
         hipStream_t s = c->stream;
         SimplexDev D(c, S, n_nodes, coords, n_elem, elem_nodes, elem_dofs);
+        DevBuf dcq;  // coefficient per (element, quadrature point), when given
+        if (coef_qp && n_elem) {
+            dcq.alloc((size_t)n_elem * S.nq * 8);
+            FG_HIP(hipMemcpyAsync(dcq.ptr, coef_qp, (size_t)n_elem * S.nq * 8, hipMemcpyHostToDevice, s));
+        }
         DevBuf keys(m * 8 + 8), keys2(m * 8 + 8), idx(m * 4 + 4), perm(m * 4 + 4), vals(m * 8 + 8);
         DevBuf head(m * 4 + 4), pos(m * 4 + 4);
         DevBuf rp((n_dofs + 1) * 8);
@@ -264,12 +269,14 @@ int femgpu_csr_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_no
         if (m) {
             if (dim == 2)
                 hipLaunchKernelGGL(k_simplex_elem_mat<2>, dim3(nblocks(n_elem)), dim3(256), 0, s, (size_t)n_elem,
-                                   S.nl, S.nq, form, coef, D.coords.as<double>(), D.nodes.as<uint32_t>(),
+                                   S.nl, S.nq, form, coef, (const double *)dcq.ptr, D.coords.as<double>(),
+                                   D.nodes.as<uint32_t>(),
                                    D.edofs.as<uint32_t>(), D.tab.as<double>(), keys.as<uint64_t>(),
                                    idx.as<uint32_t>(), vals.as<double>());
             else
                 hipLaunchKernelGGL(k_simplex_elem_mat<3>, dim3(nblocks(n_elem)), dim3(256), 0, s, (size_t)n_elem,
-                                   S.nl, S.nq, form, coef, D.coords.as<double>(), D.nodes.as<uint32_t>(),
+                                   S.nl, S.nq, form, coef, (const double *)dcq.ptr, D.coords.as<double>(),
+                                   D.nodes.as<uint32_t>(),
                                    D.edofs.as<uint32_t>(), D.tab.as<double>(), keys.as<uint64_t>(),
                                    idx.as<uint32_t>(), vals.as<double>());
             FG_HIP(hipGetLastError());
@@ -305,6 +312,24 @@ int femgpu_csr_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_no
         *out = csr_adopt(c, n_dofs, nnz, std::move(rp), std::move(ci), std::move(vv));
         return 0;
     });
+}
+
+extern "C" {
+
+int femgpu_csr_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
+                                uint64_t n_elem, const uint32_t *elem_nodes, const uint32_t *elem_dofs,
+                                uint64_t n_dofs, int form, double coef, int quad_order, femgpu_op **out) {
+    return assemble_simplex(c, dim, order, n_nodes, coords, n_elem, elem_nodes, elem_dofs, n_dofs, form, coef,
+                            nullptr, quad_order, out);
+}
+
+int femgpu_csr_assemble_simplex_qp(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
+                                   uint64_t n_elem, const uint32_t *elem_nodes, const uint32_t *elem_dofs,
+                                   uint64_t n_dofs, int form, const double *coef_qp, int quad_order,
+                                   femgpu_op **out) {
+    if (!coef_qp) return fail(FEMGPU_ERR_INVALID_ARG, "csr_assemble_simplex_qp: coef_qp is NULL");
+    return assemble_simplex(c, dim, order, n_nodes, coords, n_elem, elem_nodes, elem_dofs, n_dofs, form, 1.0,
+                            coef_qp, quad_order, out);
 }
 
 int femgpu_rhs_assemble_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,

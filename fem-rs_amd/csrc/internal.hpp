@@ -108,6 +108,8 @@ constexpr int kMultiDotBase = 64, kMultiDotMax = kScalarSlots - kMultiDotBase;
 // energy-loss training uses two from this one on.
 constexpr int kUnetOpSlot = 200;
 constexpr int kRedThreads = 256;
+// Longest residual history a solve records (entries, iteration 0 included).
+constexpr uint64_t kHistMax = 1u << 20;
 
 }  // namespace femgpu
 
@@ -122,6 +124,10 @@ struct femgpu_ctx {
     double *pinned = nullptr; // host-visible scalars for read-back
     hipEvent_t t0 = nullptr, t1 = nullptr;
     void *noalloc_scratch = nullptr;  // this device's scratch array (solvers.hip)
+    // residual history of the last solve that recorded one (print level >= 2)
+    femgpu::DevBuf hist;
+    std::vector<double> hist_host;
+    uint64_t hist_len = 0;
 };
 
 // ------------------------------------------------------------ operator --

@@ -39,6 +39,7 @@ template <class T, int DIM, int NL>
 __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                                                     const uint32_t *__restrict__ edofs,
                                                     const T *__restrict__ qdata,
+                                                    const T *__restrict__ cq,
                                                     const double *__restrict__ tab,
                                                     const T *__restrict__ x, T *__restrict__ y,
                                                     const int *skip) {
@@ -105,11 +106,12 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                     v[1] = D[1] * gu[0] + D[3] * gu[1] + D[4] * gu[2];
                     v[2] = D[2] * gu[0] + D[4] * gu[1] + D[5] * gu[2];
                 }
+                const T wq = cq ? W[q] * cq[e * nq + q] : W[q];
 #pragma unroll
                 for (int i = 0; i < nl; ++i) {
                     T s = 0;
                     for (int d = 0; d < DIM; ++d) s += G[i * DIM + d] * v[d];
-                    ye[i] += W[q] * s;
+                    ye[i] += wq * s;
                 }
             }
         } else {
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(256) void k_pa_simplex(size_t ne, int nq, int form,
                 T u = 0;
 #pragma unroll
                 for (int l = 0; l < nl; ++l) u += ph[l] * xe[l];
-                const T wu = W[q] * m * u;
+                const T wu = (cq ? W[q] * cq[e * nq + q] : W[q]) * m * u;
 #pragma unroll
                 for (int i = 0; i < nl; ++i) ye[i] += wu * ph[i];
             }
@@ -165,6 +167,7 @@ struct SimplexOp final : femgpu_op {
     int dim = 2, form = FEMGPU_DIFFUSION, nl = 0, nq = 0;
     size_t ne = 0;
     DevBuf edofs, qdata, tab, diag;
+    DevBuf cq;  // coefficient per (element, quadrature point), or empty: folded into qdata
 
     const char *name() const override { return "pa_simplex"; }
 
@@ -175,7 +178,7 @@ struct SimplexOp final : femgpu_op {
         const dim3 grid((unsigned)((ne + 255) / 256)), block(256);
 #define PAS(D_, NL_)                                                                                        \
     hipLaunchKernelGGL((k_pa_simplex<T, D_, NL_>), grid, block, 0, s, ne, nq, form, edofs.as<uint32_t>(),     \
-                       qdata.as<T>(), tab.as<double>(), x, y, skip)
+                       qdata.as<T>(), (const T *)cq.ptr, tab.as<double>(), x, y, skip)
         if (dim == 2 && nl == 3) PAS(2, 3);
         else if (dim == 2 && nl == 6) PAS(2, 6);
         else if (dim == 2 && nl == 10) PAS(2, 10);
@@ -233,12 +236,9 @@ template <class T> static void upload(femgpu_ctx *c, DevBuf &b, const std::vecto
 
 using namespace femgpu;
 
-extern "C" {
-
-int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
-                             const double *coords, uint64_t n_elem, const uint32_t *nodes,
-                             const uint32_t *edofs, uint64_t n_dofs, int form, double coef,
-                             int quad_order, femgpu_op **out) {
+static int create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
+                          uint64_t n_elem, const uint32_t *nodes, const uint32_t *edofs, uint64_t n_dofs,
+                          int form, double coef, const double *coef_qp, int quad_order, femgpu_op **out) {
     return guard([&] {
         FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "pa_create_simplex: NULL argument");
         FG_REQUIRE(form == FEMGPU_MASS || form == FEMGPU_DIFFUSION, FEMGPU_ERR_INVALID_ARG,
@@ -258,7 +258,8 @@ int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes
             for (uint64_t e = 0; e < n_elem; ++e) {
                 double J[3][3], I[3][3];
                 const double det = S.jac(coords, nodes + e * (dim + 1), J, I);
-                const double s = coef * std::fabs(det);
+                // with per-point values the coefficient multiplies each point's weight instead
+                const double s = (coef_qp ? 1.0 : coef) * std::fabs(det);
                 double D[6] = {0, 0, 0, 0, 0, 0};
                 if (form == FEMGPU_DIFFUSION) {
                     int k = 0;
@@ -286,10 +287,13 @@ int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes
                             const double ph = S.phi[(size_t)q * S.nl + i];
                             v = s * ph * ph;
                         }
-                        dg[edofs[e * S.nl + i]] += S.w[q] * v;
+                        const double wq = coef_qp ? S.w[q] * coef_qp[e * S.nq + q] : S.w[q];
+                        dg[edofs[e * S.nl + i]] += wq * v;
                     }
             }
             upload<double>(c, op->qdata, qd);
+            if (coef_qp)
+                upload<double>(c, op->cq, std::vector<double>(coef_qp, coef_qp + n_elem * S.nq));
             upload<double>(c, op->tab, S.tab);
             upload<double>(c, op->diag, dg);
             op->edofs.alloc(n_elem * S.nl * 4);
@@ -305,10 +309,9 @@ int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes
     });
 }
 
-int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
-                              const double *coords, uint64_t n_elem, const uint32_t *nodes,
-                              const uint32_t *edofs, uint64_t n_dofs, double rho, int quad_order,
-                              femgpu_op **out) {
+static int create_lumped(femgpu_ctx *c, int dim, int order, uint64_t n_nodes, const double *coords,
+                         uint64_t n_elem, const uint32_t *nodes, const uint32_t *edofs, uint64_t n_dofs,
+                         double rho, const double *rho_qp, int quad_order, femgpu_op **out) {
     return guard([&] {
         FG_REQUIRE(c && out, FEMGPU_ERR_INVALID_ARG, "lumped_mass_create: NULL argument");
         SimplexSetup S(dim, order, quad_order);
@@ -325,7 +328,8 @@ int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_node
                 const double *ph = &S.phi[(size_t)q * S.nl];
                 double sum = 0.0;
                 for (int i = 0; i < S.nl; ++i) sum += ph[i];
-                for (int i = 0; i < S.nl; ++i) dg[edofs[e * S.nl + i]] += w * rho * ph[i] * sum;
+                const double r = rho_qp ? rho_qp[e * S.nq + q] : rho;
+                for (int i = 0; i < S.nl; ++i) dg[edofs[e * S.nl + i]] += w * r * ph[i] * sum;
             }
         }
         auto *op = new LumpedOp();
@@ -337,6 +341,67 @@ int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_node
             throw;
         }
         *out = op;
+        return 0;
+    });
+}
+
+extern "C" {
+
+int femgpu_pa_create_simplex(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                             const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                             const uint32_t *edofs, uint64_t n_dofs, int form, double coef,
+                             int quad_order, femgpu_op **out) {
+    return create_simplex(c, dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, form, coef, nullptr,
+                          quad_order, out);
+}
+
+int femgpu_pa_create_simplex_qp(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                                const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                                const uint32_t *edofs, uint64_t n_dofs, int form, const double *coef_qp,
+                                int quad_order, femgpu_op **out) {
+    if (!coef_qp) return fail(FEMGPU_ERR_INVALID_ARG, "pa_create_simplex_qp: coef_qp is NULL");
+    return create_simplex(c, dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, form, 1.0, coef_qp,
+                          quad_order, out);
+}
+
+int femgpu_lumped_mass_create(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                              const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                              const uint32_t *edofs, uint64_t n_dofs, double rho, int quad_order,
+                              femgpu_op **out) {
+    return create_lumped(c, dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, rho, nullptr,
+                         quad_order, out);
+}
+
+int femgpu_lumped_mass_create_qp(femgpu_ctx *c, int dim, int order, uint64_t n_nodes,
+                                 const double *coords, uint64_t n_elem, const uint32_t *nodes,
+                                 const uint32_t *edofs, uint64_t n_dofs, const double *rho_qp, int quad_order,
+                                 femgpu_op **out) {
+    if (!rho_qp) return fail(FEMGPU_ERR_INVALID_ARG, "lumped_mass_create_qp: rho_qp is NULL");
+    return create_lumped(c, dim, order, n_nodes, coords, n_elem, nodes, edofs, n_dofs, 1.0, rho_qp,
+                         quad_order, out);
+}
+
+int femgpu_simplex_qp_coords(int dim, uint64_t n_nodes, const double *coords, uint64_t n_elem,
+                             const uint32_t *nodes, int quad_order, double *xq, uint64_t *nq) {
+    return guard([&] {
+        FG_REQUIRE(coords && nodes && nq, FEMGPU_ERR_INVALID_ARG, "simplex_qp_coords: NULL argument");
+        SimplexSetup S(dim, 1, quad_order);
+        *nq = (uint64_t)S.nq;
+        if (!xq) return 0;
+        for (uint64_t i = 0; i < n_elem * (uint64_t)(dim + 1); ++i)
+            FG_REQUIRE(nodes[i] < n_nodes, FEMGPU_ERR_INVALID_ARG, "simplex_qp_coords: node index out of range");
+        for (uint64_t e = 0; e < n_elem; ++e) {
+            const uint32_t *v = nodes + e * (dim + 1);
+            for (int q = 0; q < S.nq; ++q) {
+                const double *xi = &S.pts[(size_t)q * dim];
+                for (int r = 0; r < dim; ++r) {
+                    // x = x_0 + J xi, J columns x_k - x_0 (mesh/src/transformation.rs:34-64)
+                    double acc = 0.0;
+                    for (int k = 0; k < dim; ++k) acc += (coords[dim * v[k + 1] + r] - coords[dim * v[0] + r]) * xi[k];
+                    xq[(e * S.nq + q) * dim + r] = coords[dim * v[0] + r] + acc;
+                }
+            }
+        }
         return 0;
     });
 }

@@ -95,6 +95,17 @@ int femgpu_ctx_destroy(femgpu_ctx *c) {
     return 0;
 }
 
+int femgpu_residual_history(femgpu_ctx *c, double *out, uint64_t cap, uint64_t *len) {
+    return guard([&] {
+        FG_REQUIRE(c && len, FEMGPU_ERR_INVALID_ARG, "residual_history: NULL argument");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        *len = c->hist_len;
+        if (out)
+            for (uint64_t k = 0; k < std::min<uint64_t>(cap, c->hist_len); ++k) out[k] = c->hist_host[k];
+        return 0;
+    });
+}
+
 int femgpu_ctx_synchronize(femgpu_ctx *c) {
     return guard([&] {
         FG_REQUIRE(c, FEMGPU_ERR_INVALID_ARG, "ctx is NULL");

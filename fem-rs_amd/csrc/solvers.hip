@@ -310,6 +310,15 @@ __global__ __launch_bounds__(1024) void k_fin_store(const double *part, int nb, 
     if (threadIdx.x == 0) S[slot] = o[0];
 }
 
+// Residual history (print level Iterations and above): after iteration k
+// (k = 0: the initial residual) hist[k] = the residual the stopping test
+// used.  Re-recording an index after the solver has stopped writes the same
+// value again.
+__global__ void k_hist(const double *S, double *hist, uint64_t cap) {
+    const uint64_t k = (uint64_t)S[S_ITERS];
+    if (k < cap) hist[k] = S[S_RES];
+}
+
 // Start of a continued PCG segment: running, no iterations yet, |b| given.
 __global__ void k_set_cont(double *S, int *flags, double bnorm) {
     S[S_BNORM] = bnorm;
@@ -724,6 +733,23 @@ template <class T> struct Krylov {
     // (the allocation-free entry point), or null: the operator's workspace.
     T *ext = nullptr;
     size_t ext_stride = 0;
+    // Reporting (SolverConfig::effective_print_level, solver/src/lib.rs:133-142):
+    // 0 silent, 1 summary line, 2+ per-iteration residuals, recorded on the
+    // device into the context's history buffer and printed after the solve.
+    int level = 0;
+    const char *label = "pcg";
+    uint64_t hist_cap = 0;
+    void record() {
+        if (hist_cap)
+            hipLaunchKernelGGL(k_hist, dim3(1), dim3(1), 0, s, (const double *)S, c->hist.as<double>(), hist_cap);
+    }
+    void start_history(uint64_t max_iter) {
+        hist_cap = 0;
+        if (level < 2) return;
+        hist_cap = std::min<uint64_t>(max_iter + 1, kHistMax);
+        c->hist.ensure(hist_cap * sizeof(double));
+        c->hist_len = 0;
+    }
     T *vec(int slot) {
         return ext ? ext + (size_t)slot * ext_stride : (T *)op->wsvec(slot, n * sizeof(T));
     }
@@ -812,6 +838,8 @@ template <class T> struct Krylov {
                                r, z, p, part, prec == PREC_NONE);
         hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                            flags, mode, rtol, atol);
+        start_history(max_iter);
+        record();
         CsrView cv;
         if (mode == MODE_PCG && prec == PREC_JACOBI && op->csr_view(&cv) && fused_pcg_enabled())
             return run_pcgj_fused(cv, x, inv, rtol, atol, max_iter, res);
@@ -852,8 +880,10 @@ template <class T> struct Krylov {
             hipLaunchKernelGGL(k_pcgj_update<T>, dim3(nb), dim3(256), 0, s, n, x, (const T *)pn, r,
                                (const T *)ap, inv, z, part, (int)nb, S, flags, kFzRz[(k - 1) & 1],
                                kFzRz[k & 1], brk);
+            if (k > 1) record();  // the spmv finalised iteration k - 1
         };
-        const bool graphs = graphs_enabled() && !ext;  // instantiating a graph allocates
+        // instantiating a graph allocates; the history records between the kernels
+        const bool graphs = graphs_enabled() && !ext && !hist_cap;
         uint64_t it = 0, since = 0;
         const uint64_t check = n < (1u << 20) ? kPcgChunk : 8;
         bool stopped = read_flag() != 0;
@@ -874,9 +904,11 @@ template <class T> struct Krylov {
                 since = 0;
             }
         }
-        if (it > 0)
+        if (it > 0) {
             hipLaunchKernelGGL(k_pcgj_end, dim3(1), dim3(256), 0, s, (const double *)part, (int)nb, S,
                                flags, rtol, atol);
+            record();
+        }
         return finish(MODE_PCG, max_iter, res);
     }
 
@@ -964,6 +996,7 @@ template <class T> struct Krylov {
             hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)part, (int)nb, S,
                                flags, mode, prec == PREC_JACOBI ? 1 : 0, prec == PREC_NONE || prec == PREC_JACOBI,
                                rtol, atol, (double)(it + 1));
+            record();
             if (segment && it + 1 == max_iter) {
                 ++it;
                 break;
@@ -1012,6 +1045,7 @@ template <class T> struct Krylov {
                                cp.degree == 1 ? rz_new : -1, brk);
             hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)(part + kRedBlocks),
                                (int)nb, S, flags, (int)MODE_PCG, 0, 0, rtol, atol, (double)(it + 1));
+            record();
             if (segment && it + 1 == max_iter) {
                 ++it;
                 break;
@@ -1031,12 +1065,32 @@ template <class T> struct Krylov {
     // does, but fewer ran.  Valid after finish().
     uint64_t iterations_done() const { return (uint64_t)c->pinned[S_ITERS]; }
 
+    // Copies the recorded residuals to the context and prints what the level asks for.
+    void report(int status, uint64_t iters, double resid, bool relative) {
+        if (level >= 2 && hist_cap) {
+            const uint64_t len = std::min<uint64_t>(iters + 1, hist_cap);
+            c->hist_host.resize(len);
+            FG_HIP(hipMemcpyAsync(c->hist_host.data(), c->hist.ptr, len * sizeof(double), hipMemcpyDeviceToHost, s));
+            FG_HIP(hipStreamSynchronize(s));
+            c->hist_len = len;
+            for (uint64_t k = 1; k < len; ++k)
+                std::fprintf(stderr, "femgpu %s iter %llu: residual = %.3e\n", label, (unsigned long long)k,
+                             c->hist_host[k]);
+        }
+        if (level >= 1)
+            std::fprintf(stderr, "femgpu %s: %s after %llu iterations, %s residual = %.3e\n", label,
+                         status == ST_CONVERGED ? "converged" : status == ST_BREAKDOWN ? "breakdown" : "not converged",
+                         (unsigned long long)iters, relative ? "relative" : "absolute", resid);
+    }
+
     // Reads the scalars after a solve and fills the result.
     int finish(int mode, uint64_t max_iter, femgpu_solve_result *res) {
         read_scalars();
         const double *h = c->pinned;
         const int status = (int)h[S_STATUS];
         last_status = status;
+        if (level >= 1)
+            report(status, (uint64_t)h[S_ITERS], h[S_RES], mode == MODE_PCG);
         res->converged = status == ST_CONVERGED;
         res->final_residual_is_relative = mode == MODE_PCG;
         res->error_estimate = 0.0;
@@ -1130,6 +1184,13 @@ template <class F> static int dispatch(femgpu_op *op, F &&f) {
 static void check_cfg(const femgpu_solver_config *cfg) {
     FG_REQUIRE(cfg, FEMGPU_ERR_INVALID_ARG, "solver: cfg is NULL");
     FG_REQUIRE(cfg->rtol >= 0 && cfg->atol >= 0, FEMGPU_ERR_INVALID_ARG, "solver: negative tolerance");
+}
+
+// SolverConfig::effective_print_level (solver/src/lib.rs:133-142): print_level
+// when it is not Silent, else Iterations when verbose, else Silent.
+static int print_level(const femgpu_solver_config *cfg) {
+    if (cfg->print_level > 0) return cfg->print_level;
+    return cfg->verbose ? 2 : 0;
 }
 
 }  // namespace femgpu
@@ -1285,6 +1346,8 @@ int femgpu_solve_cg_operator(femgpu_op *op, const void *b, void *x, femgpu_mem m
         Staged st(op, b, x, mem, 8, 9);
         const int rc = dispatch(op, [&](auto &k) {
             using T = std::remove_pointer_t<decltype(k.vec(0))>;
+            k.level = print_level(cfg);
+            k.label = "cg_operator";
             return k.solve(MODE_CG, PREC_NONE, ChebParams{}, (const T *)st.b, (T *)st.x, cfg->rtol,
                            cfg->atol, cfg->max_iter, res);
         });
@@ -1307,6 +1370,8 @@ int femgpu_solve_pcg_jacobi(femgpu_op *op, const void *b, void *x, femgpu_mem me
         Staged st(op, b, x, mem, 8, 9);
         const int rc = dispatch(op, [&](auto &k) {
             using T = std::remove_pointer_t<decltype(k.vec(0))>;
+            k.level = print_level(cfg);
+            k.label = "pcg_jacobi";
             return k.solve(MODE_PCG, PREC_JACOBI, ChebParams{}, (const T *)st.b, (T *)st.x, cfg->rtol,
                            cfg->atol, cfg->max_iter, res);
         });
@@ -1363,6 +1428,8 @@ int femgpu_solve_amg_cg(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem,
         Staged st(op, b, x, mem, 8, 9);
         Krylov<double> k(op);
         k.pc = [&](const double *r, double *z) { amg_apply(amg, r, z, k.flags); };
+        k.level = print_level(cfg);
+        k.label = "amg_pcg";
         const int rc = k.solve(MODE_PCG, PREC_CB, ChebParams{}, (const double *)st.b, (double *)st.x, cfg->rtol,
                                cfg->atol, cfg->max_iter, res);
         st.finish();
@@ -1386,6 +1453,8 @@ int femgpu_solve_pcg_chebyshev(femgpu_op *op, const femgpu_cheb_config *cheb, co
             using T = std::remove_pointer_t<decltype(k.vec(0))>;
             ChebParams cp = k.prepare_cheb(cheb);
             if (lmax_used) *lmax_used = cp.lmax;
+            k.level = print_level(cfg);
+            k.label = "pcg_chebyshev";
             return k.solve(MODE_PCG, PREC_CHEB, cp, (const T *)st.b, (T *)st.x, cfg->rtol, cfg->atol,
                            cfg->max_iter, res);
         });

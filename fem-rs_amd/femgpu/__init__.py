@@ -116,6 +116,9 @@ _sig("femgpu_op_dtype", [P, C.POINTER(I)])
 _sig("femgpu_op_diagonal", [P, P, I])
 _sig("femgpu_op_destroy", [P])
 _sig("femgpu_op_time_apply", [P, P, P, I, C.POINTER(D), C.POINTER(D)])
+_APPLY_FN = C.CFUNCTYPE(I, P, P, P, U64)
+_sig("femgpu_op_create_callback", [P, U64, I, I, _APPLY_FN, P, P, C.POINTER(P)])
+_sig("femgpu_residual_history", [P, P, U64, C.POINTER(U64)])
 _sig("femgpu_csr_create", [P, U64, U64, P, P, P, I, C.POINTER(P)])
 _sig("femgpu_csr_spmv_add", [P, D, P, D, P, I])
 _sig("femgpu_csr_apply_dirichlet", [P, U64, P, P, P, I])
@@ -131,6 +134,10 @@ _sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
 _sig("femgpu_pa_hex_qdata_per_point", [P, C.POINTER(I)])
 _sig("femgpu_pa_hex_element_kernel", [P, C.POINTER(I)])
 _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
+_sig("femgpu_pa_create_simplex_qp", [P, I, I, U64, P, U64, P, P, U64, I, P, I, C.POINTER(P)])
+_sig("femgpu_csr_assemble_simplex_qp", [P, I, I, U64, P, U64, P, P, U64, I, P, I, C.POINTER(P)])
+_sig("femgpu_lumped_mass_create_qp", [P, I, I, U64, P, U64, P, P, U64, P, I, C.POINTER(P)])
+_sig("femgpu_simplex_qp_coords", [I, U64, P, U64, P, I, P, C.POINTER(U64)])
 _sig("femgpu_lumped_mass_create", [P, I, I, U64, P, U64, P, P, U64, D, I, C.POINTER(P)])
 _sig("femgpu_lumped_mass_apply_inverse", [P, P, P, I])
 _sig("femgpu_essential_create", [P, U64, P, C.POINTER(P)])
@@ -392,6 +399,53 @@ class Operator:
             pass
 
 
+class CallbackOperator(Operator):
+    """An operator whose apply is a Python function fn(x, y) setting y = A x
+    (the `apply` closure of solve_cg_operator, solver/src/lib.rs:198-208).
+    mem=HOST (default): x and y are numpy views of the library's host staging
+    buffers; mem=DEVICE: fn(x_ptr, y_ptr) gets device addresses (ints) on the
+    context's GPU.  diag: optional diagonal (for Jacobi / Chebyshev).  An
+    exception in fn aborts the solve with FemGpuError."""
+
+    def __init__(self, n, fn, dtype=F64, mem=HOST, diag=None, ctx=None):
+        ctx = ctx or default_context()
+        npt = _npdt(dtype)
+        self._fn, self._mem, self.error = fn, mem, None
+
+        def tramp(user, xp, yp, m):
+            try:
+                if mem == HOST:
+                    x = np.ctypeslib.as_array(C.cast(xp, C.POINTER(C.c_double if npt == np.float64 else C.c_float)),
+                                              shape=(m,))
+                    y = np.ctypeslib.as_array(C.cast(yp, C.POINTER(C.c_double if npt == np.float64 else C.c_float)),
+                                              shape=(m,))
+                    fn(x, y)
+                else:
+                    fn(xp, yp)
+                return 0
+            except Exception as e:  # reported by the failing solve
+                self.error = e
+                return 1
+
+        self._cb = _APPLY_FN(tramp)  # kept alive with the operator
+        d = None if diag is None else np.ascontiguousarray(diag, npt)
+        h = P()
+        _check(lib.femgpu_op_create_callback(ctx.h, int(n), dtype, mem, self._cb, None,
+                                             None if d is None else d.ctypes.data, C.byref(h)))
+        super().__init__(ctx, h)
+
+
+def residual_history(ctx=None):
+    """Residuals of the last solve on ctx that recorded them (SolverConfig
+    with verbose=True or print_level >= 2): entry 0 is the initial residual."""
+    ctx = ctx or default_context()
+    ln = U64()
+    _check(lib.femgpu_residual_history(ctx.h, None, 0, C.byref(ln)))
+    out = np.zeros(ln.value)
+    _check(lib.femgpu_residual_history(ctx.h, out.ctypes.data if ln.value else None, ln.value, C.byref(ln)))
+    return out
+
+
 class CsrMatrix(Operator):
     """CsrMatrix<T> (linalg/src/csr.rs:81-93) resident on the GPU."""
 
@@ -468,14 +522,21 @@ def _mesh_args(dim, order, coords, elem_nodes, elem_dofs):
 def assemble_bilinear(dim, order, coords, elem_nodes, elem_dofs, n_dofs, form=DIFFUSION, coef=1.0,
                       quad_order=3, ctx=None):
     """Assembler::assemble_bilinear (assembly/src/assembler.rs:147-249) with the
-    diffusion or mass integrator and a constant coefficient, then
-    CooMatrix::into_csr, on the GPU; returns the device CsrMatrix."""
+    diffusion or mass integrator, then CooMatrix::into_csr, on the GPU;
+    returns the device CsrMatrix.  coef: a constant, an array (n_elem, nq) of
+    point values, or a function of the physical quadrature points."""
     ctx = ctx or default_context()
+    q = _coef_qp(coef, dim, coords, elem_nodes, quad_order)
     coords, nodes, dofs, ne = _mesh_args(dim, order, coords, elem_nodes, elem_dofs)
     h = P()
-    _check(lib.femgpu_csr_assemble_simplex(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data, ne,
-                                           nodes.ctypes.data, dofs.ctypes.data, n_dofs, form, coef,
-                                           quad_order, C.byref(h)))
+    if q is None:
+        _check(lib.femgpu_csr_assemble_simplex(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data, ne,
+                                               nodes.ctypes.data, dofs.ctypes.data, n_dofs, form, coef,
+                                               quad_order, C.byref(h)))
+    else:
+        _check(lib.femgpu_csr_assemble_simplex_qp(ctx.h, dim, order, len(coords) // dim, coords.ctypes.data,
+                                                  ne, nodes.ctypes.data, dofs.ctypes.data, n_dofs, form,
+                                                  q.ctypes.data, quad_order, C.byref(h)))
     return CsrMatrix._adopt(ctx, h)
 
 
@@ -611,25 +672,65 @@ def _simplex_create(fn, ctx, dim, order, coords, elem_nodes, elem_dofs, n_dofs, 
     return h
 
 
+def simplex_qp_coords(dim, coords, elem_nodes, quad_order):
+    """Physical quadrature points (n_elem, nq, dim): x = x_0 + J xi, the points
+    where ScalarCoeff::eval sees the coefficient (coefficient.rs:67-132)."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    en = np.ascontiguousarray(elem_nodes, np.uint32)
+    ne = len(en) // (dim + 1)
+    nq = U64()
+    _check(lib.femgpu_simplex_qp_coords(dim, len(coords) // dim, coords.ctypes.data, ne, en.ctypes.data,
+                                        quad_order, None, C.byref(nq)))
+    xq = np.zeros((ne, nq.value, dim))
+    _check(lib.femgpu_simplex_qp_coords(dim, len(coords) // dim, coords.ctypes.data, ne, en.ctypes.data,
+                                        quad_order, xq.ctypes.data, C.byref(nq)))
+    return xq
+
+
+def _coef_qp(coef, dim, coords, elem_nodes, quad_order):
+    """None for a constant coefficient; else its values (n_elem * nq, f64) at
+    the quadrature points: coef is an array (n_elem, nq) or a function of the
+    physical points (n_elem, nq, dim) -> (n_elem, nq), like a ScalarCoeff."""
+    if np.isscalar(coef):
+        return None
+    if callable(coef):
+        coef = coef(simplex_qp_coords(dim, coords, elem_nodes, quad_order))
+    return np.ascontiguousarray(np.ravel(coef), np.float64)
+
+
 class PAMassOperator(Operator):
-    """PAMassOperator::new(space, rho, quad_order) (partial.rs:65-131), simplex P1..P3."""
+    """PAMassOperator::new(space, rho, quad_order) (partial.rs:65-131), simplex P1..P3.
+    rho: a constant, an array (n_elem, nq) of point values, or a function of
+    the physical quadrature points (evaluated once, at set-up)."""
 
     def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, rho=1.0, quad_order=3,
                  ctx=None):
         ctx = ctx or default_context()
-        h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
-                            elem_dofs, n_dofs, MASS, float(rho), quad_order)
+        q = _coef_qp(rho, dim, coords, elem_nodes, quad_order)
+        if q is None:
+            h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, MASS, float(rho), quad_order)
+        else:
+            h = _simplex_create(lib.femgpu_pa_create_simplex_qp, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, MASS, q.ctypes.data, quad_order)
         super().__init__(ctx, h)
 
 
 class PADiffusionOperator(Operator):
-    """PADiffusionOperator::new(space, kappa, quad_order) (partial.rs:138-209), simplex P1..P3."""
+    """PADiffusionOperator::new(space, kappa, quad_order) (partial.rs:138-209), simplex P1..P3.
+    kappa: a constant, an array (n_elem, nq) of point values, or a function
+    of the physical quadrature points (partial.rs:137-147, evaluated once)."""
 
     def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, kappa=1.0, quad_order=3,
                  ctx=None):
         ctx = ctx or default_context()
-        h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
-                            elem_dofs, n_dofs, DIFFUSION, float(kappa), quad_order)
+        q = _coef_qp(kappa, dim, coords, elem_nodes, quad_order)
+        if q is None:
+            h = _simplex_create(lib.femgpu_pa_create_simplex, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, DIFFUSION, float(kappa), quad_order)
+        else:
+            h = _simplex_create(lib.femgpu_pa_create_simplex_qp, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, DIFFUSION, q.ctypes.data, quad_order)
         super().__init__(ctx, h)
 
 
@@ -658,8 +759,13 @@ class LumpedMassOperator(Operator):
     def __init__(self, dim, order, coords, elem_nodes, elem_dofs, n_dofs, rho=1.0, quad_order=3,
                  ctx=None):
         ctx = ctx or default_context()
-        h = _simplex_create(lib.femgpu_lumped_mass_create, ctx, dim, order, coords, elem_nodes,
-                            elem_dofs, n_dofs, float(rho), quad_order)
+        q = _coef_qp(rho, dim, coords, elem_nodes, quad_order)
+        if q is None:
+            h = _simplex_create(lib.femgpu_lumped_mass_create, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, float(rho), quad_order)
+        else:
+            h = _simplex_create(lib.femgpu_lumped_mass_create_qp, ctx, dim, order, coords, elem_nodes,
+                                elem_dofs, n_dofs, q.ctypes.data, quad_order)
         super().__init__(ctx, h)
 
     @property
@@ -714,9 +820,24 @@ def _len(v):
 def solve_cg_operator(nrows, ncols, op, b, x, cfg=None):
     """solve_cg_operator (solver/src/lib.rs:198-278): x is the initial guess
     and is overwritten with the solution; returns SolveResult or raises
-    ConvergenceFailed / Linger (breakdown) / DimensionMismatch."""
+    ConvergenceFailed / Linger (breakdown) / DimensionMismatch.  op is a
+    device operator or, as the reference's `apply: F`, a function
+    apply(x, y) setting y = A x on numpy arrays."""
     cfg = cfg or SolverConfig()
-    if nrows != ncols or _len(b) != nrows or _len(x) != ncols or op.n_dofs() != nrows:
+    if nrows != ncols or _len(b) != nrows or _len(x) != ncols:
+        raise DimensionMismatch(nrows, ncols, _len(b))
+    if not isinstance(op, Operator):
+        ctx = b.ctx if isinstance(b, DeviceArray) else None
+        cop = CallbackOperator(nrows, op, ctx=ctx)
+        try:
+            return solve_cg_operator(nrows, ncols, cop, b, x, cfg)
+        except FemGpuError:
+            if cop.error is not None:
+                raise cop.error
+            raise
+        finally:
+            cop.close()
+    if op.n_dofs() != nrows:
         raise DimensionMismatch(nrows, ncols, _len(b))
     bp, xp, m = _solve_vectors(op, b, x)
     r = _SolveResult()

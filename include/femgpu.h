@@ -97,6 +97,25 @@ int femgpu_op_destroy(femgpu_op *op);
 int femgpu_op_time_apply(femgpu_op *op, const void *x, void *y, int reps, double *ms_apply,
                          double *ms_kernel);
 
+/* An operator whose apply is the caller's function: the `apply: F` closure of
+ * solve_cg_operator<F: Fn(&[f64], &mut [f64])> (solver/src/lib.rs:198-208),
+ * for matrix-free operators outside this library (its caller in the
+ * reference: examples/src/maxwell.rs:173-200).  apply(user, x, y, n) must set
+ * y = A x (overwrite) and return 0; a nonzero return aborts the solve or
+ * apply with INVALID_ARG.  mem = FEMGPU_HOST: x and y are host buffers of n
+ * values of dtype owned by the operator (the library copies the device vector
+ * out and the result back; it synchronises its stream first).  mem =
+ * FEMGPU_DEVICE: x and y are device pointers on the context's GPU; work the
+ * callback enqueues on the context's stream (other femgpu calls on the same
+ * context) is ordered, work on other streams must be complete when it
+ * returns.  diag (optional, host, n values, copied) is the operator's
+ * diagonal, which Jacobi and Chebyshev-Jacobi need; without it those
+ * solvers return UNSUPPORTED.  femgpu_op_apply with accumulate = 1 adds the
+ * result to y. */
+typedef int (*femgpu_apply_fn)(void *user, const void *x, void *y, uint64_t n);
+int femgpu_op_create_callback(femgpu_ctx *ctx, uint64_t n, femgpu_dtype dtype, femgpu_mem mem,
+                              femgpu_apply_fn apply, void *user, const void *diag, femgpu_op **out);
+
 /* CsrMatrix<T> (linalg/src/csr.rs:81-93): row_ptr u64[nrows+1], col_idx
  * u32[nnz], values T[nnz] (T = f64 or f32 per dtype), exactly the reference
  * layout.  Arrays are host pointers and are copied. */
@@ -199,6 +218,23 @@ int femgpu_pa_create_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nod
                              const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
                              const uint32_t *elem_dofs, uint64_t n_dofs, int form, double coef,
                              int quad_order, femgpu_op **out);
+/* The coefficient-generic forms: PADiffusionOperator<S, K: ScalarCoeff> and
+ * PAMassOperator<S, C> evaluate kappa.eval(&CoeffCtx) / rho at every
+ * quadrature point of every element (assembly/src/partial.rs:116-117,
+ * 137-147, 186-190; coefficient.rs:67-132).  The caller evaluates its
+ * coefficient at the physical points femgpu_simplex_qp_coords returns and
+ * passes the values: coef_qp[e * nq + q] (f64, copied), q in tri_rule /
+ * tet_rule order for quad_order.  The operator multiplies each point's weight
+ * by its value. */
+int femgpu_pa_create_simplex_qp(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                                const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                                const uint32_t *elem_dofs, uint64_t n_dofs, int form,
+                                const double *coef_qp, int quad_order, femgpu_op **out);
+/* Physical coordinates of the quadrature points, x = x_0 + J xi with the affine
+ * map of mesh/src/transformation.rs:34-64: xq[(e * nq + q) * dim + r]; *nq is
+ * set to the rule's point count (pass xq = NULL to query it).  Host only. */
+int femgpu_simplex_qp_coords(int dim, uint64_t n_nodes, const double *coords, uint64_t n_elem,
+                             const uint32_t *elem_nodes, int quad_order, double *xq, uint64_t *nq);
 /* Assembler::assemble_bilinear (assembly/src/assembler.rs:147-249) with
  * DiffusionIntegrator (form FEMGPU_DIFFUSION, standard/diffusion.rs:34-55) or
  * MassIntegrator (FEMGPU_MASS, standard/mass.rs:28-40), constant coefficient,
@@ -209,6 +245,14 @@ int femgpu_csr_assemble_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_
                                 const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
                                 const uint32_t *elem_dofs, uint64_t n_dofs, int form, double coef,
                                 int quad_order, femgpu_op **out);
+/* femgpu_csr_assemble_simplex with the coefficient given per quadrature point
+ * (DiffusionIntegrator / MassIntegrator with a ScalarCoeff evaluated at each
+ * point, standard/diffusion.rs:34-55, mass.rs:28-40), coef_qp as
+ * femgpu_pa_create_simplex_qp. */
+int femgpu_csr_assemble_simplex_qp(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                                   const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                                   const uint32_t *elem_dofs, uint64_t n_dofs, int form,
+                                   const double *coef_qp, int quad_order, femgpu_op **out);
 /* Assembler::assemble_linear with DomainSourceIntegrator
  * (assembly/src/standard/source.rs:44-50): b_i = sum_e sum_q w |det J| f(x_q)
  * phi_i(x_q), f = value (FEMGPU_SOURCE_CONST) or value * prod_d sin(pi x_d)
@@ -231,6 +275,12 @@ int femgpu_lumped_mass_create(femgpu_ctx *ctx, int dim, int order, uint64_t n_no
                               const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
                               const uint32_t *elem_dofs, uint64_t n_dofs, double rho,
                               int quad_order, femgpu_op **out);
+/* LumpedMassOperator with rho given per quadrature point (rho_qp as coef_qp
+ * of femgpu_pa_create_simplex_qp). */
+int femgpu_lumped_mass_create_qp(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
+                                 const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,
+                                 const uint32_t *elem_dofs, uint64_t n_dofs, const double *rho_qp,
+                                 int quad_order, femgpu_op **out);
 /* LumpedMassOperator::apply_inverse (partial.rs:269-273). */
 int femgpu_lumped_mass_apply_inverse(femgpu_op *op, const void *x, void *y, femgpu_mem mem);
 
@@ -269,7 +319,16 @@ int femgpu_amg_vcycle(femgpu_amg *amg, const void *b, void *x, femgpu_mem mem);
 
 /* ------------------------------------------------------------ solvers ---- */
 /* SolverConfig (solver/src/lib.rs:97-113): defaults rtol 1e-8, atol 0,
- * max_iter 1000. */
+ * max_iter 1000.  print_level is PrintLevel (lib.rs:79-94): 0 Silent,
+ * 1 Summary, 2 Iterations, 3 Debug; the effective level is print_level when
+ * not Silent, else Iterations when verbose (effective_print_level,
+ * lib.rs:133-142).  Summary prints one line to stderr after a solve;
+ * Iterations and Debug also record the residual of every iteration on the
+ * device (no host round trip per iteration) and print them after the solve,
+ * one line each, as par_solve_pcg_jacobi logs them (parallel/src/par_solver.rs
+ * :312-313); femgpu_residual_history returns them.  The CG, PCG-Jacobi,
+ * PCG-Chebyshev and AMG-PCG solvers report; the refinement, flexible-CG and
+ * allocation-free solvers ignore the level. */
 typedef struct {
     double rtol;
     double atol;
@@ -277,6 +336,13 @@ typedef struct {
     int verbose;
     int print_level;
 } femgpu_solver_config;
+
+/* Residual history of the last solve on ctx that recorded one (print level
+ * >= 2): len = iterations + 1 entries (entry 0: the initial residual; then the
+ * residual each iteration's stopping test used, relative for PCG and absolute
+ * for CG, as final_residual), at most 2^20; the first min(cap, len) are
+ * copied to out when out is not NULL. */
+int femgpu_residual_history(femgpu_ctx *ctx, double *out, uint64_t cap, uint64_t *len);
 
 /* SolveResult (solver/src/lib.rs:71-75).  final_residual is absolute for
  * femgpu_solve_cg_operator (as solve_cg_operator) and relative otherwise

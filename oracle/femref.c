@@ -890,8 +890,8 @@ static int simplex_nl(int dim, int order) {
 
 size_t fr_simplex_assemble_csr(int dim, int order, size_t n_nodes, const double *coords,
                                size_t n_elem, const uint32_t *conn, const uint32_t *edofs,
-                               size_t n_dofs, int kind, double coef, int quad_order,
-                               uint64_t *row_ptr, uint32_t *col_idx, double *values) {
+                               size_t n_dofs, int kind, double coef, const double *coef_qp,
+                               int quad_order, uint64_t *row_ptr, uint32_t *col_idx, double *values) {
     (void)n_nodes;
     const int nl = simplex_nl(dim, order), nv = dim + 1;
     double pts[120], wts[40];
@@ -913,7 +913,8 @@ size_t fr_simplex_assemble_csr(int dim, int order, size_t n_nodes, const double 
                     for (int k = 0; k < dim; ++k) s += JIT[r][k] * gr[l * dim + k];
                     gp[l * dim + r] = s;
                 }
-            double wc = w * coef;
+            /* kappa.eval at the point (diffusion.rs:34-55): the caller's value there */
+            double wc = w * (coef_qp ? coef_qp[e * nq + qi] : coef);
             for (int i = 0; i < nl; ++i)
                 for (int j = 0; j < nl; ++j) {
                     if (kind == FR_DIFFUSION) {
@@ -982,7 +983,7 @@ void fr_csr_dirichlet_row_zeroing(size_t nrows, const uint64_t *row_ptr, const u
 
 void fr_simplex_pa_apply(int dim, int order, const double *coords, size_t n_elem,
                          const uint32_t *conn, const uint32_t *edofs, int kind, double coef,
-                         int quad_order, const double *x, double *y, int accumulate) {
+                         const double *coef_qp, int quad_order, const double *x, double *y, int accumulate) {
     const int nl = simplex_nl(dim, order), nv = dim + 1;
     double pts[120], wts[40], phi[20], gr[60], gp[60], xe[20], ye[20];
     int nq = simplex_rule(dim, quad_order, pts, wts);
@@ -997,11 +998,13 @@ void fr_simplex_pa_apply(int dim, int order, const double *coords, size_t n_elem
         for (int l = 0; l < nl; ++l) { xe[l] = x[edofs[e * nl + l]]; ye[l] = 0.0; }
         for (int qi = 0; qi < nq; ++qi) {
             double w = wts[qi] * fabs(det);
+            /* rho / kappa evaluated at the point (partial.rs:116-117, 186-190) */
+            const double cq = coef_qp ? coef_qp[e * nq + qi] : coef;
             fr_simplex_basis(dim, order, pts + dim * qi, phi, gr);
             if (kind == FR_MASS) {
                 double ux = 0.0;
                 for (int l = 0; l < nl; ++l) ux += phi[l] * xe[l];
-                for (int i = 0; i < nl; ++i) ye[i] += w * coef * phi[i] * ux;
+                for (int i = 0; i < nl; ++i) ye[i] += w * cq * phi[i] * ux;
             } else {
                 for (int l = 0; l < nl; ++l)
                     for (int r = 0; r < dim; ++r) {
@@ -1015,7 +1018,7 @@ void fr_simplex_pa_apply(int dim, int order, const double *coords, size_t n_elem
                 for (int i = 0; i < nl; ++i) {
                     double dot = 0.0;
                     for (int r = 0; r < dim; ++r) dot += gu[r] * gp[i * dim + r];
-                    ye[i] += w * coef * dot;
+                    ye[i] += w * cq * dot;
                 }
             }
         }
@@ -1023,9 +1026,30 @@ void fr_simplex_pa_apply(int dim, int order, const double *coords, size_t n_elem
     }
 }
 
+/* Physical quadrature points x = x_0 + J xi (transformation.rs:34-64), the
+ * points where the reference evaluates its coefficients: xq[(e*nq + q)*dim + r].
+ * Returns nq. */
+int fr_simplex_qp_coords(int dim, const double *coords, size_t n_elem, const uint32_t *conn, int quad_order,
+                         double *xq) {
+    double pts[120], wts[40];
+    const int nq = simplex_rule(dim, quad_order, pts, wts), nv = dim + 1;
+    if (!xq) return nq;
+    for (size_t e = 0; e < n_elem; ++e) {
+        double J[3][3], JIT[3][3], x0[3];
+        simplex_geom(dim, coords, conn + e * nv, J, JIT, x0);
+        for (int qi = 0; qi < nq; ++qi)
+            for (int r = 0; r < dim; ++r) {
+                double acc = 0.0;
+                for (int k = 0; k < dim; ++k) acc += J[r][k] * pts[dim * qi + k];
+                xq[(e * nq + qi) * dim + r] = x0[r] + acc;
+            }
+    }
+    return nq;
+}
+
 void fr_simplex_lumped_mass(int dim, int order, const double *coords, size_t n_elem,
                             const uint32_t *conn, const uint32_t *edofs, size_t n_dofs, double rho,
-                            int quad_order, double *diag) {
+                            const double *rho_qp, int quad_order, double *diag) {
     const int nl = simplex_nl(dim, order), nv = dim + 1;
     double pts[120], wts[40], phi[20], gr[60];
     int nq = simplex_rule(dim, quad_order, pts, wts);
@@ -1038,7 +1062,8 @@ void fr_simplex_lumped_mass(int dim, int order, const double *coords, size_t n_e
             fr_simplex_basis(dim, order, pts + dim * qi, phi, gr);
             double s = 0.0;
             for (int l = 0; l < nl; ++l) s += phi[l];
-            for (int i = 0; i < nl; ++i) diag[edofs[e * nl + i]] += w * rho * phi[i] * s;
+            const double r = rho_qp ? rho_qp[e * nq + qi] : rho;
+            for (int i = 0; i < nl; ++i) diag[edofs[e * nl + i]] += w * r * phi[i] * s;
         }
     }
 }

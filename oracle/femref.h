@@ -140,12 +140,12 @@ void fr_unit_square_tri(int n, double *coords, uint32_t *conn);
 /* Assembler::assemble_bilinear(DiffusionIntegrator{kappa}) / MassIntegrator on a
  * P1..P3 triangle or P1..P3 tet mesh (assembler.rs:147-249, 409-429,
  * diffusion.rs:34-55, mass.rs:28-40) into CSR via COO (coo.rs:76-).
- * kind: FR_MASS / FR_DIFFUSION; coef constant.  Returns nnz (col_idx NULL:
- * count only). */
+ * kind: FR_MASS / FR_DIFFUSION; coef constant (or per point: coef_qp[e*nq +
+ * q] when not NULL).  Returns nnz (col_idx NULL: count only). */
 size_t fr_simplex_assemble_csr(int dim, int order, size_t n_nodes, const double *coords,
                                size_t n_elem, const uint32_t *conn, const uint32_t *elem_dofs,
-                               size_t n_dofs, int kind, double coef, int quad_order,
-                               uint64_t *row_ptr, uint32_t *col_idx, double *values);
+                               size_t n_dofs, int kind, double coef, const double *coef_qp,
+                               int quad_order, uint64_t *row_ptr, uint32_t *col_idx, double *values);
 /* Config-1 RHS: DomainSourceIntegrator(2 pi^2 sin(pi x) sin(pi y)) on P1
  * (source.rs:44-50, assembler.rs:434-454). */
 void fr_p1_source_rhs(int n, const double *coords, const uint32_t *conn, int quad_order,
@@ -161,15 +161,21 @@ void fr_csr_dirichlet_row_zeroing(size_t nrows, const uint64_t *row_ptr,
 
 /* --------------------------------------------------- simplex PA apply ---- */
 /* PAMassOperator / PADiffusionOperator::apply (partial.rs:85-130, 154-208):
- * y (+)= A x, element loop with affine Jacobian.  coef constant. */
+ * y (+)= A x, element loop with affine Jacobian.  coef constant, or with
+ * coef_qp (not NULL) the coefficient's value at each quadrature point,
+ * coef_qp[e*nq + q] (the reference evaluates its ScalarCoeff there). */
 void fr_simplex_pa_apply(int dim, int order, const double *coords, size_t n_elem,
                          const uint32_t *conn, const uint32_t *elem_dofs, int kind,
-                         double coef, int quad_order, const double *x, double *y,
+                         double coef, const double *coef_qp, int quad_order, const double *x, double *y,
                          int accumulate);
 /* LumpedMassOperator::assemble (partial.rs:228-262): row-sum lumped diagonal. */
 void fr_simplex_lumped_mass(int dim, int order, const double *coords, size_t n_elem,
                             const uint32_t *conn, const uint32_t *elem_dofs, size_t n_dofs,
-                            double rho, int quad_order, double *diag);
+                            double rho, const double *rho_qp, int quad_order, double *diag);
+/* Physical quadrature points of every element (x = x_0 + J xi); returns nq
+ * (xq NULL: count only). */
+int fr_simplex_qp_coords(int dim, const double *coords, size_t n_elem, const uint32_t *conn, int quad_order,
+                         double *xq);
 /* Conforming global DOF numbering for P1..P3 simplex meshes: vertices first
  * (= node ids), then one block per edge (sorted vertex pair, DOFs ordered from
  * the lower vertex id), then faces, then interiors.  Returns n_dofs and fills

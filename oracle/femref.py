@@ -101,12 +101,13 @@ def _declare(L):
         "fr_csr_spmv_f32": (None, [Z, P, P, P, P, P]),
         "fr_csr_diagonal": (None, [P, P]),
         "fr_unit_square_tri": (None, [I, P, P]),
-        "fr_simplex_assemble_csr": (Z, [I, I, Z, P, Z, P, P, Z, I, D, I, P, P, P]),
+        "fr_simplex_assemble_csr": (Z, [I, I, Z, P, Z, P, P, Z, I, D, P, I, P, P, P]),
+        "fr_simplex_qp_coords": (I, [I, P, Z, P, I, P]),
         "fr_p1_source_rhs": (None, [I, P, P, I, P]),
         "fr_unit_square_boundary": (Z, [I, P]),
         "fr_csr_dirichlet_row_zeroing": (None, [Z, P, P, P, Z, P, P, P]),
-        "fr_simplex_pa_apply": (None, [I, I, P, Z, P, P, I, D, I, P, P, I]),
-        "fr_simplex_lumped_mass": (None, [I, I, P, Z, P, P, Z, D, I, P]),
+        "fr_simplex_pa_apply": (None, [I, I, P, Z, P, P, I, D, P, I, P, P, I]),
+        "fr_simplex_lumped_mass": (None, [I, I, P, Z, P, P, Z, D, P, I, P]),
         "fr_simplex_dofs": (Z, [I, I, Z, Z, P, P]),
         "fr_solve_cg_operator": (I, [Z, P, P, P, P, P, P]),
         "fr_pcg_jacobi": (I, [Z, P, P, P, P, P, P, P]),
@@ -330,13 +331,25 @@ def simplex_dofs(dim, order, n_nodes, conn):
     return nd, ed
 
 
-def simplex_assemble_csr(dim, order, coords, conn, edofs, n_dofs, kind, coef, quad_order):
+def simplex_qp_coords(dim, coords, conn, quad_order):
+    """Physical quadrature points (n_elem, nq, dim), x = x_0 + J xi."""
+    conn = np.ascontiguousarray(conn, np.uint32)
+    ne = len(conn) // (dim + 1)
+    nq = lib().fr_simplex_qp_coords(dim, _p(_f64(coords)), ne, _p(conn), quad_order, None)
+    xq = np.zeros((ne, nq, dim))
+    lib().fr_simplex_qp_coords(dim, _p(_f64(coords)), ne, _p(conn), quad_order, _p(xq))
+    return xq
+
+
+def simplex_assemble_csr(dim, order, coords, conn, edofs, n_dofs, kind, coef, quad_order, coef_qp=None):
+    """coef_qp: optional coefficient values (n_elem, nq) at the quadrature points."""
     coords = _f64(coords); conn = np.ascontiguousarray(conn, np.uint32)
     edofs = np.ascontiguousarray(edofs, np.uint32)
     ne = len(conn) // (dim + 1)
     rp = np.zeros(n_dofs + 1, np.uint64)
+    keep = None if coef_qp is None else _f64(np.ravel(coef_qp))
     args = (dim, order, len(coords) // dim, _p(coords), ne, _p(conn), _p(edofs), n_dofs, kind,
-            coef, quad_order)
+            coef, None if keep is None else _p(keep), quad_order)
     nnz = lib().fr_simplex_assemble_csr(*args, _p(rp), None, None)
     ci = np.zeros(nnz, np.uint32); v = np.zeros(nnz)
     lib().fr_simplex_assemble_csr(*args, _p(rp), _p(ci), _p(v))
@@ -379,23 +392,25 @@ def poisson_p1(n, quad_order=3):
     return rp, ci, v, rhs, coords, conn
 
 
-def simplex_pa_apply(dim, order, coords, conn, edofs, kind, coef, quad_order, x, y=None):
+def simplex_pa_apply(dim, order, coords, conn, edofs, kind, coef, quad_order, x, y=None, coef_qp=None):
     x = _f64(x)
     acc = y is not None
     y = np.zeros(len(x)) if y is None else _f64(y).copy()
     conn = np.ascontiguousarray(conn, np.uint32)
+    keep = None if coef_qp is None else _f64(np.ravel(coef_qp))
     lib().fr_simplex_pa_apply(dim, order, _p(_f64(coords)), len(conn) // (dim + 1), _p(conn),
-                              _p(np.ascontiguousarray(edofs, np.uint32)), kind, coef, quad_order,
-                              _p(x), _p(y), 1 if acc else 0)
+                              _p(np.ascontiguousarray(edofs, np.uint32)), kind, coef,
+                              None if keep is None else _p(keep), quad_order, _p(x), _p(y), 1 if acc else 0)
     return y
 
 
-def simplex_lumped_mass(dim, order, coords, conn, edofs, n_dofs, rho, quad_order):
+def simplex_lumped_mass(dim, order, coords, conn, edofs, n_dofs, rho, quad_order, rho_qp=None):
     d = np.zeros(n_dofs)
     conn = np.ascontiguousarray(conn, np.uint32)
+    keep = None if rho_qp is None else _f64(np.ravel(rho_qp))
     lib().fr_simplex_lumped_mass(dim, order, _p(_f64(coords)), len(conn) // (dim + 1), _p(conn),
                                  _p(np.ascontiguousarray(edofs, np.uint32)), n_dofs, rho,
-                                 quad_order, _p(d))
+                                 None if keep is None else _p(keep), quad_order, _p(d))
     return d
 
 

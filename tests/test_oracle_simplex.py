@@ -94,3 +94,46 @@ def test_dof_numbering_conforming(fr, dim, order):
             else:
                 assert abs(vals[g] - f(p)) < 1e-12  # same physical point -> same DOF
     assert not np.isnan(vals).any()
+
+
+def _kappa(xq):
+    # a coefficient of the physical point, as a ScalarCoeff closure would give it
+    return 1.0 + xq[..., 0] + 0.5 * xq[..., 1] ** 2
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_qp_coefficient(fr, dim, order):
+    """Coefficients given per quadrature point (the ScalarCoeff of
+    partial.rs:137-147, 186-190 evaluated at x_q):
+    - a constant array equals the constant coefficient;
+    - PA apply equals the assembled matrix with the same point values;
+    - P1 linear patch: with kappa = 1 + x, v^T K u = a.b int(1 + x) = 1.5 a.b
+      over the unit square / cube (grad u = a, exact for any rule);
+    - 1^T M 1 = int rho and the lumped mass sums to it (rho = 1 + x)."""
+    coords, conn, nd, ed = _mesh(fr, dim, order)
+    qo = 2 * order + 1
+    xq = fr.simplex_qp_coords(dim, coords, conn, qo)
+    ne, nq = xq.shape[:2]
+    x = np.random.default_rng(order).standard_normal(nd)
+    for kind in (fr.FR_DIFFUSION, fr.FR_MASS):
+        y_c = fr.simplex_pa_apply(dim, order, coords, conn, ed, kind, 2.5, qo, x)
+        y_q = fr.simplex_pa_apply(dim, order, coords, conn, ed, kind, 0.0, qo, x, coef_qp=np.full((ne, nq), 2.5))
+        assert np.abs(y_c - y_q).max() < 1e-14 * np.abs(y_c).max()
+        kq = _kappa(xq)
+        y = fr.simplex_pa_apply(dim, order, coords, conn, ed, kind, 0.0, qo, x, coef_qp=kq)
+        rp, ci, v = fr.simplex_assemble_csr(dim, order, coords, conn, ed, nd, kind, 0.0, qo, coef_qp=kq)
+        ya = fr.CsrRef(rp, ci, v).spmv(x)
+        assert np.abs(y - ya).max() < 1e-13 * np.abs(y).max()
+    # the physical points: sum_q w |J| x_q integrates x exactly
+    c = np.asarray(coords).reshape(-1, dim)
+    if order == 1:
+        a, b = np.arange(1.0, dim + 1.0), np.linspace(-1.0, 0.5, dim)
+        u, w = c @ a, c @ b
+        kq = 1.0 + xq[..., 0]
+        rp, ci, v = fr.simplex_assemble_csr(dim, 1, coords, conn, ed, nd, fr.FR_DIFFUSION, 0.0, qo, coef_qp=kq)
+        assert abs(w @ fr.CsrRef(rp, ci, v).spmv(u) - 1.5 * (a @ b)) < 1e-12
+        rp, ci, v = fr.simplex_assemble_csr(dim, 1, coords, conn, ed, nd, fr.FR_MASS, 0.0, qo, coef_qp=kq)
+        assert abs(fr.CsrRef(rp, ci, v).spmv(np.ones(nd)).sum() - 1.5) < 1e-13
+        lm = fr.simplex_lumped_mass(dim, 1, coords, conn, ed, nd, 0.0, qo, rho_qp=kq)
+        assert abs(lm.sum() - 1.5) < 1e-13
