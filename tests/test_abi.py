@@ -49,3 +49,32 @@ def test_python_mirror_imports():
         assert hasattr(fg, name)
     cfg = fg.SolverConfig()
     assert (cfg.rtol, cfg.atol, cfg.max_iter) == (1e-8, 0.0, 1000)  # lib.rs:104-108
+
+
+def _arity(params):
+    params = params.strip()
+    if params in ("", "void"):
+        return 0
+    depth, n = 0, 1
+    for ch in params:
+        if ch in "(<":
+            depth += 1
+        elif ch in ")>":
+            depth -= 1
+        elif ch == "," and depth == 0:
+            n += 1
+    return n
+
+
+def test_rust_sys_crate_matches_header():
+    """The committed fem-gpu-sys crate declares every C entry point with the
+    header's name and parameter count, and nothing else."""
+    hdr = open(os.path.join(ROOT, "include", "femgpu.h")).read()
+    c_decl = {m.group(1): _arity(m.group(2)) for m in
+              re.finditer(r"^\s*(?:int|const char \*)\s*(femgpu_\w+)\s*\(([^;]*?)\)\s*;", hdr, re.M | re.S)}
+    rs = open(os.path.join(ROOT, "fem-rs_amd", "rust", "fem-gpu-sys", "src", "lib.rs")).read()
+    block = rs[rs.index('extern "C" {'):]
+    r_decl = {m.group(1): _arity(m.group(2)) for m in
+              re.finditer(r"pub fn (femgpu_\w+)\s*\(([^;]*?)\)\s*->", block, re.S)}
+    assert c_decl == r_decl, (set(c_decl) ^ set(r_decl), {k for k in c_decl if c_decl[k] != r_decl.get(k)})
+    assert set(c_decl) == _declared()
