@@ -223,3 +223,44 @@ INST(float)
 #undef INST
 
 }  // namespace femgpu
+
+using namespace femgpu;
+
+extern "C" {
+
+int femgpu_vec_axpby(femgpu_ctx *c, femgpu_dtype dtype, uint64_t n, double alpha, const void *x, double beta,
+                     void *y) {
+    return guard([&] {
+        FG_REQUIRE(c && (n == 0 || (x && y)), FEMGPU_ERR_INVALID_ARG, "vec_axpby: NULL argument");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        if (n == 0) return 0;
+        if (dtype == FEMGPU_F64) axpby<double>(c->stream, (double *)y, alpha, (const double *)x, beta, n);
+        else axpby<float>(c->stream, (float *)y, alpha, (const float *)x, beta, n);
+        FG_HIP(hipGetLastError());
+        return 0;
+    });
+}
+
+int femgpu_vec_convert(femgpu_ctx *c, uint64_t n, femgpu_dtype src_dtype, const void *src, femgpu_dtype dst_dtype,
+                       void *dst) {
+    return guard([&] {
+        FG_REQUIRE(c && (n == 0 || (src && dst)), FEMGPU_ERR_INVALID_ARG, "vec_convert: NULL argument");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        if (n == 0) return 0;
+        if (src_dtype == dst_dtype) {
+            FG_HIP(hipMemcpyAsync(dst, src, n * (src_dtype == FEMGPU_F64 ? 8 : 4), hipMemcpyDeviceToDevice,
+                                  c->stream));
+        } else if (src_dtype == FEMGPU_F64) {
+            convert_f64_to_f32(c->stream, (float *)dst, (const double *)src, n, 1.0);
+        } else {
+            fill<double>(c->stream, (double *)dst, 0.0, n);
+            convert_f32_to_f64_axpy(c->stream, (double *)dst, (const float *)src, n, 1.0);
+        }
+        FG_HIP(hipGetLastError());
+        return 0;
+    });
+}
+
+}  // extern "C"

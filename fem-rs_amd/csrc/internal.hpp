@@ -252,6 +252,8 @@ struct VecArg {
 // hex operator helpers used by the learned preconditioner
 void hex_log_kappa_nodes(femgpu_op *op, float *out_dev);
 bool hex_grid_dims(femgpu_op *op, int *Nx, int *Ny, int *Nz);
+// true when every boundary DOF of the hex operator is essential
+bool hex_dirichlet(femgpu_op *op);
 
 // learned preconditioner (unet.hip): z = |r| N(r/|r|) on device vectors
 bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm);

@@ -1633,6 +1633,11 @@ void hex_log_kappa_nodes(femgpu_op *op_, float *out) {
     FG_HIP(hipGetLastError());
 }
 
+bool hex_dirichlet(femgpu_op *op_) {
+    auto *op = dynamic_cast<HexOp *>(op_);
+    return op && op->desc.dirichlet;
+}
+
 bool hex_grid_dims(femgpu_op *op_, int *Nx, int *Ny, int *Nz) {
     auto *op = dynamic_cast<HexOp *>(op_);
     if (!op) return false;

@@ -648,6 +648,83 @@ __global__ void k_energy_grad(long long V, const float *__restrict__ r, double s
         dout[i] = (float)(coef * ((double)r[i] * sc - gamma * (double)An[i]));
 }
 
+// ------------------------------------------------- coarse placement --
+// With a coarsening factor cf > 1 the network runs on the grid of every cf-th
+// DOF node (Nc = (Nf - 1) / cf + 1 per direction).  Its output reaches the DOF
+// grid by trilinear interpolation P (fine node g takes the coarse nodes
+// around g / cf with weights prod_d (1 - |g_d / cf - i_d|)), and residuals go
+// to the coarse grid by the transpose R = P^T, applied one axis at a time: on
+// an array laid out [outer][n][inner], out[o][i][s] = sum_t (1 - |t| / cf)
+// in[o][cf i + t][s] over |t| < cf.
+template <class TI>
+__global__ void k_restrict_axis(long long outer, int nf, int nc, long long inner, int cf,
+                                const TI *__restrict__ in, float *__restrict__ out) {
+    const long long total = outer * nc * inner;
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < total;
+         id += (long long)gridDim.x * blockDim.x) {
+        const long long sI = id % inner, t2 = id / inner;
+        const int i = (int)(t2 % nc);
+        const long long o = t2 / nc;
+        double acc = 0.0;
+        for (int t = -(cf - 1); t <= cf - 1; ++t) {
+            const int g = cf * i + t;
+            if (g < 0 || g >= nf) continue;
+            acc += (1.0 - (double)abs(t) / cf) * (double)in[(o * nf + g) * inner + sI];
+        }
+        out[id] = (float)acc;
+    }
+}
+
+// z[g] = scale * (P oc)[g] (trilinear interpolation of the coarse values)
+// (bmask: 0 on the boundary of the DOF grid, where every DOF is essential)
+template <class T>
+__global__ void k_prolong(Vol f, Vol c, int cf, const float *__restrict__ oc, double scale, int bmask,
+                          T *__restrict__ z) {
+    const long long V = f.n();
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < V;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int gx = (int)(id % f.nx);
+        const long long r = id / f.nx;
+        const int gy = (int)(r % f.ny), gz = (int)(r / f.ny);
+        if (bmask && (gx == 0 || gy == 0 || gz == 0 || gx == f.nx - 1 || gy == f.ny - 1 || gz == f.nz - 1)) {
+            z[id] = T(0);
+            continue;
+        }
+        // lower coarse node of the cell holding g (the last cell for g = Nf - 1)
+        const int ix = min(gx / cf, max(c.nx - 2, 0)), iy = min(gy / cf, max(c.ny - 2, 0)),
+                  iz = min(gz / cf, max(c.nz - 2, 0));
+        const double tx = (double)(gx - cf * ix) / cf, ty = (double)(gy - cf * iy) / cf,
+                     tz = (double)(gz - cf * iz) / cf;
+        double acc = 0.0;
+        for (int dz = 0; dz < 2; ++dz) {
+            const double wz = dz ? tz : 1.0 - tz;
+            if (wz == 0.0 || iz + dz >= c.nz) continue;
+            for (int dy = 0; dy < 2; ++dy) {
+                const double wy = dy ? ty : 1.0 - ty;
+                if (wy == 0.0 || iy + dy >= c.ny) continue;
+                for (int dx = 0; dx < 2; ++dx) {
+                    const double wx = dx ? tx : 1.0 - tx;
+                    if (wx == 0.0 || ix + dx >= c.nx) continue;
+                    acc += wx * wy * wz * (double)oc[((long long)(iz + dz) * c.ny + (iy + dy)) * c.nx + ix + dx];
+                }
+            }
+        }
+        z[id] = (T)(acc * scale);
+    }
+}
+
+// ac[i] = af[cf i] (the coarse nodes are DOF nodes)
+__global__ void k_inject(Vol f, Vol c, int cf, const float *__restrict__ af, float *__restrict__ ac) {
+    const long long V = c.n();
+    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < V;
+         id += (long long)gridDim.x * blockDim.x) {
+        const int ix = (int)(id % c.nx);
+        const long long r = id / c.nx;
+        const int iy = (int)(r % c.ny), iz = (int)(r / c.ny);
+        ac[id] = af[((long long)(cf * iz) * f.ny + cf * iy) * f.nx + cf * ix];
+    }
+}
+
 // ------------------------------------------------------------ network --
 struct Layer {
     int cin, cout, kpad;
@@ -670,6 +747,12 @@ struct femgpu_unet {
     femgpu::DevBuf gout, gd0, ge0, gt0, gd1, ge1, gt1, ge2, gp1, gp0, gtmp, gbf, part, dpart;
     long long step = 0;
     uint64_t seed = 0;
+    // coarse placement (cf > 1): the DOF grid, its restriction temporaries,
+    // the coarse residual and the prolongated output (f32, DOF grid)
+    int cf = 1;
+    int bmask = 0;  // zero the correction on the DOF grid's boundary (essential DOFs)
+    femgpu::Vol fine{};
+    femgpu::DevBuf rx, rxy, rc, nf;
 };
 
 namespace femgpu {
@@ -880,6 +963,22 @@ static void backward(femgpu_unet *u) {
 
 namespace femgpu {
 
+// R v (v on the DOF grid, type T) into u->rc (f32, network grid): three
+// one-axis passes x, y, z.
+template <class T>
+static void restrict_to_coarse(femgpu_unet *u, const T *v) {
+    hipStream_t s = u->ctx->stream;
+    const Vol f = u->fine, c = u->d0;
+    const long long o1 = (long long)f.nz * f.ny, n1 = o1 * c.nx;
+    hipLaunchKernelGGL(k_restrict_axis<T>, dim3(g1(n1)), dim3(256), 0, s, o1, f.nx, c.nx, 1ll, u->cf, v,
+                       u->rx.as<float>());
+    const long long n2 = (long long)f.nz * c.ny * c.nx;
+    hipLaunchKernelGGL(k_restrict_axis<float>, dim3(g1(n2)), dim3(256), 0, s, (long long)f.nz, f.ny, c.ny,
+                       (long long)c.nx, u->cf, (const float *)u->rx.as<float>(), u->rxy.as<float>());
+    hipLaunchKernelGGL(k_restrict_axis<float>, dim3(g1(c.n())), dim3(256), 0, s, 1ll, f.nz, c.nz,
+                       (long long)c.ny * c.nx, u->cf, (const float *)u->rxy.as<float>(), u->rc.as<float>());
+}
+
 // z = |r|/sqrt(V) N(r sqrt(V)/|r|) written as type T (accumulate: z +=).  Used by
 // the flexible-CG preconditioner; returns false when |r| == 0.
 template <class T>
@@ -887,6 +986,19 @@ static bool unet_apply_t(femgpu_unet *u, const T *r, T *z, double rnorm) {
     hipStream_t s = u->ctx->stream;
     const long long V = u->d0.n();
     if (!(rnorm > 0.0) || !std::isfinite(rnorm)) return false;
+    if (u->cf > 1) {
+        // z = |Rr|/sqrt(Vc) P N(Rr sqrt(Vc)/|Rr|)
+        restrict_to_coarse<T>(u, r);
+        const double rc = std::sqrt(dot_host(u->ctx, u->rc.ptr, u->rc.ptr, V, FEMGPU_F32));
+        if (!(rc > 0.0) || !std::isfinite(rc)) return false;
+        const double scc = std::sqrt((double)V) / rc;
+        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(),
+                           scc, (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
+        forward(u);
+        hipLaunchKernelGGL(k_prolong<T>, dim3(g1(u->fine.n())), dim3(256), 0, s, u->fine, u->d0, u->cf,
+                           (const float *)u->out.as<float>(), 1.0 / scc, u->bmask, z);
+        return true;
+    }
     const double sc = std::sqrt((double)V) / rnorm;
     hipLaunchKernelGGL(k_make_input<T>, dim3(g1(V)), dim3(256), 0, s, V, r, sc, (const float *)u->aux.as<float>(),
                        u->in.as<uint16_t>());
@@ -900,7 +1012,8 @@ bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, dou
     return unet_apply_t<float>(u, (const float *)r, (float *)z, rnorm);
 }
 
-long long unet_voxels(const femgpu_unet *u) { return u->d0.n(); }
+// voxels of the grid the network's input and output live on (the DOF grid)
+long long unet_voxels(const femgpu_unet *u) { return u->cf > 1 ? u->fine.n() : u->d0.n(); }
 femgpu_ctx *unet_ctx(const femgpu_unet *u) { return u->ctx; }
 
 }  // namespace femgpu
@@ -909,9 +1022,12 @@ using namespace femgpu;
 
 extern "C" {
 
-int femgpu_unet_create(femgpu_ctx *c, int nx, int ny, int nz, uint64_t seed, femgpu_unet **out) {
+static int unet_create(femgpu_ctx *c, int nx, int ny, int nz, int cf, uint64_t seed, femgpu_unet **out) {
     return guard([&] {
-        FG_REQUIRE(c && out && nx > 0 && ny > 0 && nz > 0, FEMGPU_ERR_INVALID_ARG, "unet_create: bad arguments");
+        FG_REQUIRE(c && out && nx > 0 && ny > 0 && nz > 0 && cf >= 1, FEMGPU_ERR_INVALID_ARG,
+                   "unet_create: bad arguments");
+        FG_REQUIRE((nx - 1) % cf == 0 && (ny - 1) % cf == 0 && (nz - 1) % cf == 0, FEMGPU_ERR_INVALID_ARG,
+                   "unet_create: grid size - 1 must be a multiple of the coarsening factor");
         FG_REQUIRE((long long)nx * ny * nz < (1ll << 31), FEMGPU_ERR_INVALID_ARG, "unet_create: grid too large");
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         FG_HIP(hipSetDevice(c->device));
@@ -919,7 +1035,15 @@ int femgpu_unet_create(femgpu_ctx *c, int nx, int ny, int nz, uint64_t seed, fem
         try {
             u->ctx = c;
             u->seed = seed;
-            u->d0 = Vol{nx, ny, nz};
+            u->cf = cf;
+            u->fine = Vol{nx, ny, nz};
+            u->d0 = cf > 1 ? Vol{(nx - 1) / cf + 1, (ny - 1) / cf + 1, (nz - 1) / cf + 1} : Vol{nx, ny, nz};
+            if (cf > 1) {
+                u->rx.alloc((size_t)nz * ny * u->d0.nx * 4);
+                u->rxy.alloc((size_t)nz * u->d0.ny * u->d0.nx * 4);
+                u->rc.alloc((size_t)u->d0.n() * 4);
+                u->nf.alloc((size_t)u->fine.n() * 4);
+            }
             u->d1 = half(u->d0);
             u->d2 = half(u->d1);
             unet_layout(u);
@@ -956,6 +1080,23 @@ int femgpu_unet_create(femgpu_ctx *c, int nx, int ny, int nz, uint64_t seed, fem
             throw;
         }
         *out = u;
+        return 0;
+    });
+}
+
+int femgpu_unet_create(femgpu_ctx *c, int nx, int ny, int nz, uint64_t seed, femgpu_unet **out) {
+    return unet_create(c, nx, ny, nz, 1, seed, out);
+}
+
+int femgpu_unet_create_coarse(femgpu_ctx *c, int nx, int ny, int nz, int coarsen, uint64_t seed,
+                              femgpu_unet **out) {
+    return unet_create(c, nx, ny, nz, coarsen, seed, out);
+}
+
+int femgpu_unet_grid(const femgpu_unet *u, int *nx, int *ny, int *nz) {
+    return guard([&] {
+        FG_REQUIRE(u && nx && ny && nz, FEMGPU_ERR_INVALID_ARG, "unet_grid: NULL argument");
+        *nx = u->d0.nx; *ny = u->d0.ny; *nz = u->d0.nz;
         return 0;
     });
 }
@@ -1124,13 +1265,15 @@ int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r,
     return guard([&] {
         FG_REQUIRE(u && op && r && e, FEMGPU_ERR_INVALID_ARG, "unet_train_step_energy: NULL");
         FG_REQUIRE(op->ctx == u->ctx, FEMGPU_ERR_INVALID_ARG, "unet_train_step_energy: different contexts");
-        FG_REQUIRE((long long)op->n == u->d0.n(), FEMGPU_ERR_DIM_MISMATCH,
+        FG_REQUIRE((long long)op->n == unet_voxels(u), FEMGPU_ERR_DIM_MISMATCH,
                    "unet_train_step_energy: network grid and operator size differ");
         std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
         FG_HIP(hipSetDevice(u->ctx->device));
         hipStream_t s = u->ctx->stream;
         femgpu_ctx *c = u->ctx;
-        const long long V = u->d0.n();
+        // V: the DOF grid (the operator's vectors); Vn: the network's grid
+        const long long V = unet_voxels(u), Vn = u->d0.n();
+        const bool coarse = u->cf > 1;
         const float *rd = r, *ed = e;
         DevBuf sr, se;
         if (mem == FEMGPU_HOST) {
@@ -1142,21 +1285,37 @@ int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r,
         const double rn = std::sqrt(dot_host(c, rd, rd, V, FEMGPU_F32));
         FG_REQUIRE(rn > 0 && std::isfinite(rn), FEMGPU_ERR_INVALID_ARG,
                    "unet_train_step_energy: zero or non-finite residual");
-        const double sc = std::sqrt((double)V) / rn;
-        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, rd, sc,
+        // the network input, normalised as in unet_apply_t
+        const float *rin = rd;
+        double sc = std::sqrt((double)V) / rn;
+        if (coarse) {
+            restrict_to_coarse<float>(u, rd);
+            const double rc = std::sqrt(dot_host(c, u->rc.ptr, u->rc.ptr, Vn, FEMGPU_F32));
+            FG_REQUIRE(rc > 0 && std::isfinite(rc), FEMGPU_ERR_INVALID_ARG,
+                       "unet_train_step_energy: residual vanishes on the network's grid");
+            rin = u->rc.as<float>();
+            sc = std::sqrt((double)Vn) / rc;
+        }
+        hipLaunchKernelGGL(k_make_input<float>, dim3(g1(Vn)), dim3(256), 0, s, Vn, rin, sc,
                            (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
         forward(u);
+        // n on the DOF grid: the output itself, or its prolongation
+        const float *nout = u->out.as<float>();
+        if (coarse) {
+            hipLaunchKernelGGL(k_prolong<float>, dim3(g1(V)), dim3(256), 0, s, u->fine, u->d0, u->cf,
+                               (const float *)u->out.as<float>(), 1.0, u->bmask, u->nf.as<float>());
+            nout = u->nf.as<float>();
+        }
         const int nb = red_blocks(V);
         double a = 0, cc = 0, E = 0;
         auto energy = [&](auto tag) {
             using T = decltype(tag);
             T *nT = (T *)op->wsvec(kUnetOpSlot, V * sizeof(T));
             T *AnT = (T *)op->wsvec(kUnetOpSlot + 1, V * sizeof(T));
-            hipLaunchKernelGGL(k_out_scale<T>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->out.as<float>(),
-                               1.0, nT, 0);
+            hipLaunchKernelGGL(k_out_scale<T>, dim3(g1(V)), dim3(256), 0, s, V, nout, 1.0, nT, 0);
             op->apply_dev(nT, AnT, false, nullptr);
-            hipLaunchKernelGGL(k_energy_dots<T>, dim3(nb), dim3(256), 0, s, V, (const float *)u->out.as<float>(),
-                               rd, ed, (const T *)AnT, sc, c->partials.as<double>());
+            hipLaunchKernelGGL(k_energy_dots<T>, dim3(nb), dim3(256), 0, s, V, nout, rd, ed, (const T *)AnT, sc,
+                               c->partials.as<double>());
             std::vector<double> hp(3 * (size_t)kRedBlocks);
             FG_HIP(hipMemcpyAsync(hp.data(), c->partials.ptr, hp.size() * 8, hipMemcpyDeviceToHost, s));
             FG_HIP(hipStreamSynchronize(s));
@@ -1165,8 +1324,15 @@ int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r,
                        FEMGPU_ERR_INVALID_ARG,
                        "unet_train_step_energy: network output or pair gives no positive energy");
             const double gamma = a / cc, coef = -2.0 * a / (cc * E);
+            // d loss / d n on the DOF grid; with the coarse placement it goes to the
+            // network's output through P^T = R
+            float *gn = coarse ? u->nf.as<float>() : u->gout.as<float>();
             hipLaunchKernelGGL(k_energy_grad<T>, dim3(g1(V)), dim3(256), 0, s, V, rd, sc, (const T *)AnT, coef,
-                               gamma, u->gout.as<float>());
+                               gamma, gn);
+            if (coarse) {
+                restrict_to_coarse<float>(u, gn);
+                FG_HIP(hipMemcpyAsync(u->gout.ptr, u->rc.ptr, (size_t)Vn * 4, hipMemcpyDeviceToDevice, s));
+            }
         };
         if (op->dtype == FEMGPU_F64) energy(double{});
         else energy(float{});
@@ -1193,7 +1359,7 @@ int femgpu_unet_apply(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, f
         FG_HIP(hipSetDevice(u->ctx->device));
         femgpu_ctx *c = u->ctx;
         hipStream_t s = c->stream;
-        const long long V = u->d0.n();
+        const long long V = unet_voxels(u);
         const size_t es = dt == FEMGPU_F64 ? 8 : 4;
         const void *rd = r;
         void *zd = z;
@@ -1284,11 +1450,20 @@ extern "C" int femgpu_unet_aux_from_hex(femgpu_unet *u, femgpu_op *hex) {
         FG_REQUIRE(u && hex, FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: NULL");
         int Nx = 0, Ny = 0, Nz = 0;
         FG_REQUIRE(hex_grid_dims(hex, &Nx, &Ny, &Nz), FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: not a hex operator");
-        FG_REQUIRE(Nx == u->d0.nx && Ny == u->d0.ny && Nz == u->d0.nz, FEMGPU_ERR_DIM_MISMATCH,
+        const Vol g = u->cf > 1 ? u->fine : u->d0;
+        FG_REQUIRE(Nx == g.nx && Ny == g.ny && Nz == g.nz, FEMGPU_ERR_DIM_MISMATCH,
                    "unet_aux_from_hex: grid of the network and the operator differ");
         FG_REQUIRE(hex->ctx == u->ctx, FEMGPU_ERR_INVALID_ARG, "unet_aux_from_hex: different contexts");
         std::lock_guard<std::recursive_mutex> lk(u->ctx->mu);
-        hex_log_kappa_nodes(hex, u->aux.as<float>());
+        if (u->cf > 1) {
+            // log kappa at the DOF nodes, then at the coarse nodes (a subset of them)
+            hex_log_kappa_nodes(hex, u->nf.as<float>());
+            hipLaunchKernelGGL(k_inject, dim3(g1(u->d0.n())), dim3(256), 0, u->ctx->stream, u->fine, u->d0, u->cf,
+                               (const float *)u->nf.as<float>(), u->aux.as<float>());
+            u->bmask = hex_dirichlet(hex) ? 1 : 0;
+        } else {
+            hex_log_kappa_nodes(hex, u->aux.as<float>());
+        }
         FG_HIP(hipStreamSynchronize(u->ctx->stream));
         return 0;
     });

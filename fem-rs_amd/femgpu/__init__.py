@@ -109,6 +109,8 @@ _sig("femgpu_malloc", [P, Z, C.POINTER(P)])
 _sig("femgpu_free", [P, P])
 _sig("femgpu_memcpy", [P, P, P, Z, I])
 _sig("femgpu_timer_start", [P])
+_sig("femgpu_vec_axpby", [P, I, U64, D, P, D, P])
+_sig("femgpu_vec_convert", [P, U64, I, P, I, P])
 _sig("femgpu_timer_stop", [P, C.POINTER(D)])
 _sig("femgpu_op_apply", [P, P, P, I, I])
 _sig("femgpu_op_n_dofs", [P, C.POINTER(U64)])
@@ -174,6 +176,8 @@ _sig("femgpu_unet_layer_info", [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), 
 _sig("femgpu_unet_get_params", [P, P])
 _sig("femgpu_unet_set_params", [P, P])
 _sig("femgpu_unet_set_aux", [P, P, I])
+_sig("femgpu_unet_create_coarse", [P, I, I, I, I, U64, C.POINTER(P)])
+_sig("femgpu_unet_grid", [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)])
 _sig("femgpu_unet_aux_from_hex", [P, P])
 _sig("femgpu_unet_forward", [P, P, P, I])
 _sig("femgpu_unet_backward", [P, P, P, I])
@@ -300,6 +304,21 @@ class DeviceArray:
 
     def copy_from(self, other):
         _check(lib.femgpu_memcpy(self.ctx.h, self.ptr, other.ptr, self.nbytes, 2))
+
+    def _dt(self):
+        return F64 if self.dtype == np.float64 else F32
+
+    def axpby(self, alpha, x, beta):
+        """self = alpha x + beta self on the device (femgpu_vec_axpby); returns self."""
+        assert x.n == self.n and x.dtype == self.dtype
+        _check(lib.femgpu_vec_axpby(self.ctx.h, self._dt(), self.n, float(alpha), x.ptr, float(beta), self.ptr))
+        return self
+
+    def astype(self, dtype):
+        """A new DeviceArray with the values converted (f64 <-> f32) on the device."""
+        out = DeviceArray(self.ctx, self.n, dtype)
+        _check(lib.femgpu_vec_convert(self.ctx.h, self.n, self._dt(), self.ptr, out._dt(), out.ptr))
+        return out
 
     def free(self):
         if getattr(self, "ptr", None):
@@ -1068,23 +1087,33 @@ class UNet:
     """3-level 3-D conv U-Net (16/32/64 channels, BF16 on MFMA) on an
     nx*ny*nz grid; see include/femgpu.h for the parameter layout."""
 
-    def __init__(self, shape, seed=0, ctx=None):
+    def __init__(self, shape, seed=0, ctx=None, coarsen=1):
+        """shape: the DOF grid; coarsen > 1 places the network on every
+        coarsen-th node of it (femgpu_unet_create_coarse).  self.V is the
+        network grid's size, self.n_dofs the DOF grid's."""
         self.ctx = ctx or default_context()
         nx, ny, nz = shape
-        self.shape = (nx, ny, nz)
-        self.V = nx * ny * nz
+        self.n_dofs = nx * ny * nz
+        self.coarsen = coarsen
         h = P()
-        _check(lib.femgpu_unet_create(self.ctx.h, nx, ny, nz, seed, C.byref(h)))
+        if coarsen > 1:
+            _check(lib.femgpu_unet_create_coarse(self.ctx.h, nx, ny, nz, coarsen, seed, C.byref(h)))
+        else:
+            _check(lib.femgpu_unet_create(self.ctx.h, nx, ny, nz, seed, C.byref(h)))
         self.h = h
+        g = (I(), I(), I())
+        _check(lib.femgpu_unet_grid(h, *(C.byref(v) for v in g)))
+        self.shape = tuple(v.value for v in g)
+        self.V = self.shape[0] * self.shape[1] * self.shape[2]
         n = U64()
         _check(lib.femgpu_unet_n_params(h, C.byref(n)))
         self.n_params = n.value
 
     @classmethod
-    def for_operator(cls, op, seed=0):
+    def for_operator(cls, op, seed=0, coarsen=1):
         n1 = op.p
         shape = tuple(s * n1 + 1 for s in op.shape)
-        net = cls(shape, seed, op.ctx)
+        net = cls(shape, seed, op.ctx, coarsen=coarsen)
         _check(lib.femgpu_unet_aux_from_hex(net.h, op.h))
         return net
 
@@ -1136,6 +1165,9 @@ class UNet:
         """Adam step on |e - g N(r)|_A^2 / |e|_A^2 (g the Galerkin step); returns
         that relative energy-norm error before the step."""
         loss = D()
+        if isinstance(r, DeviceArray):
+            _check(lib.femgpu_unet_train_step_energy(self.h, op.h, r.ptr, e.ptr, DEVICE, lr, C.byref(loss)))
+            return loss.value
         r = np.ascontiguousarray(r, np.float32); e = np.ascontiguousarray(e, np.float32)
         _check(lib.femgpu_unet_train_step_energy(self.h, op.h, r.ctypes.data, e.ctypes.data, HOST, lr,
                                                  C.byref(loss)))

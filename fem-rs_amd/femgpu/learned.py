@@ -14,7 +14,7 @@ C ABI.
 """
 import numpy as np
 
-from . import SolverConfig, solve_pcg_chebyshev, chebyshev_jacobi_apply, F64
+from . import SolverConfig, solve_pcg_chebyshev, chebyshev_jacobi_apply, F64, F32, DeviceArray
 
 
 def harvest_pairs(op, b, iters=(1, 2, 3, 4, 6, 8, 11, 15, 20, 27, 35), degree=4, ratio=30.0,
@@ -61,3 +61,46 @@ def train(net, pairs, epochs=10, lr=2e-3, seed=0, lr_decay=0.97, op=None):
         hist.append(tot / max(len(pairs), 1))
         lr *= lr_decay
     return hist
+
+
+def harvest_pairs_known(op, seed, iters=(1, 3, 6, 10, 20, 35, 60), degree=4, ratio=30.0, lmax=0.0,
+                        smoothed=True):
+    """Pairs from PCG + Chebyshev-Jacobi iterates of a solve whose solution is
+    known by construction, all on the device: x* = a seeded uniform [-1, 1]
+    nodal field (zero on essential DOFs; the hex operator's rhs_random),
+    b = A x*, x_k the k-th iterate from 0, e_k = x* - x_k, r_k = b - A x_k (then
+    smoothed as femgpu_solve_fcg_learned sees them, as harvest_pairs does).
+    No converged solve is needed, so this scales to the largest operators.
+    -> list of (r, e) float32 DeviceArrays."""
+    ctx = op.ctx
+    n = op.n_dofs()
+    dt = np.float64 if op.dtype == F64 else np.float32
+    xs = DeviceArray(ctx, n, dt)
+    op.rhs_random(seed, xs)
+    b = DeviceArray(ctx, n, dt)
+    op.apply_zero(xs, b)
+    zero = DeviceArray(ctx, n, dt)
+    xk = DeviceArray(ctx, n, dt)
+    rk = DeviceArray(ctx, n, dt)
+    z1 = DeviceArray(ctx, n, dt)
+    t = DeviceArray(ctx, n, dt)
+    pairs = []
+    for k in iters:
+        xk.copy_from(zero)
+        solve_pcg_chebyshev(op, b, xk, SolverConfig(rtol=1e-30, max_iter=k), degree=degree, ratio=ratio,
+                            lmax=lmax)
+        ek = DeviceArray(ctx, n, dt)
+        ek.copy_from(xs)
+        ek.axpby(-1.0, xk, 1.0)                 # e_k = x* - x_k
+        op.apply_zero(xk, rk)
+        rk.axpby(1.0, b, -1.0)                  # r_k = b - A x_k
+        if smoothed:
+            chebyshev_jacobi_apply(op, rk, z1, degree=degree, ratio=ratio, lmax=lmax)
+            op.apply_zero(z1, t)
+            rk.axpby(-1.0, t, 1.0)              # r1 = r_k - A z1
+            ek.axpby(-1.0, z1, 1.0)             # e1 = e_k - z1
+        pairs.append((rk.astype(np.float32), ek.astype(np.float32)))
+        ek.free()
+    for v in (xs, b, zero, xk, rk, z1, t):
+        v.free()
+    return pairs

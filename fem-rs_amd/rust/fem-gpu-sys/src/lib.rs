@@ -171,6 +171,10 @@ extern "C" {
     pub fn femgpu_free(ctx: *mut femgpu_ctx, dptr: *mut c_void) -> c_int;
     pub fn femgpu_memcpy(ctx: *mut femgpu_ctx, dst: *mut c_void, src: *const c_void, bytes: usize,
                          kind: c_int) -> c_int;
+    pub fn femgpu_vec_axpby(ctx: *mut femgpu_ctx, dtype: femgpu_dtype, n: u64, alpha: f64, x: *const c_void,
+                            beta: f64, y: *mut c_void) -> c_int;
+    pub fn femgpu_vec_convert(ctx: *mut femgpu_ctx, n: u64, src_dtype: femgpu_dtype, src: *const c_void,
+                              dst_dtype: femgpu_dtype, dst: *mut c_void) -> c_int;
     pub fn femgpu_timer_start(ctx: *mut femgpu_ctx) -> c_int;
     pub fn femgpu_timer_stop(ctx: *mut femgpu_ctx, ms: *mut f64) -> c_int;
 
@@ -288,6 +292,9 @@ extern "C" {
     // ------------------------------------------- learned preconditioner
     pub fn femgpu_unet_create(ctx: *mut femgpu_ctx, nx: c_int, ny: c_int, nz: c_int, seed: u64,
                               out: *mut *mut femgpu_unet) -> c_int;
+    pub fn femgpu_unet_create_coarse(ctx: *mut femgpu_ctx, nx: c_int, ny: c_int, nz: c_int, coarsen: c_int,
+                                     seed: u64, out: *mut *mut femgpu_unet) -> c_int;
+    pub fn femgpu_unet_grid(net: *const femgpu_unet, nx: *mut c_int, ny: *mut c_int, nz: *mut c_int) -> c_int;
     pub fn femgpu_unet_destroy(net: *mut femgpu_unet) -> c_int;
     pub fn femgpu_unet_n_params(net: *const femgpu_unet, n: *mut u64) -> c_int;
     pub fn femgpu_unet_layer_info(net: *const femgpu_unet, layer: c_int, cin: *mut c_int, cout: *mut c_int,

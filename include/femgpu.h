@@ -68,6 +68,14 @@ int femgpu_ctx_synchronize(femgpu_ctx *ctx);
 int femgpu_malloc(femgpu_ctx *ctx, size_t bytes, void **dptr);
 int femgpu_free(femgpu_ctx *ctx, void *dptr);
 int femgpu_memcpy(femgpu_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+/* Vector helpers on device vectors of the context's GPU, ordered on its
+ * stream (no reference counterpart; Vector<T>::axpy-style BLAS-1 for callers
+ * that keep vectors on the device, linalg/src/vector.rs:39-70):
+ * y = alpha x + beta y, and dst = src converted between f64 and f32. */
+int femgpu_vec_axpby(femgpu_ctx *ctx, femgpu_dtype dtype, uint64_t n, double alpha, const void *x,
+                     double beta, void *y);
+int femgpu_vec_convert(femgpu_ctx *ctx, uint64_t n, femgpu_dtype src_dtype, const void *src,
+                       femgpu_dtype dst_dtype, void *dst);
 /* HIP event timing of everything enqueued between start and stop on the
  * context stream (milliseconds). */
 int femgpu_timer_start(femgpu_ctx *ctx);
@@ -447,6 +455,18 @@ int femgpu_chebyshev_jacobi_apply(femgpu_op *op, const femgpu_cheb_config *cheb,
  * cin + c, tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by bias[cout]. */
 typedef struct femgpu_unet femgpu_unet;
 int femgpu_unet_create(femgpu_ctx *ctx, int nx, int ny, int nz, uint64_t seed, femgpu_unet **out);
+/* The same network placed on a coarser grid: for a DOF grid of nx*ny*nz
+ * points (n - 1 divisible by coarsen) it runs on every coarsen-th node
+ * ((n - 1)/coarsen + 1 per direction).  Residuals reach it by R = P^T and its
+ * output returns to the DOF grid by trilinear interpolation P, so its 3
+ * levels see a 2^3 * coarsen times wider part of the domain.  The training
+ * step, the flexible CG and the refinement take DOF-grid vectors as with
+ * femgpu_unet_create; forward / backward / set_aux work on the network's
+ * grid (femgpu_unet_grid).  With femgpu_unet_aux_from_hex on an operator
+ * whose boundary DOFs are essential, the correction is zero there. */
+int femgpu_unet_create_coarse(femgpu_ctx *ctx, int nx, int ny, int nz, int coarsen, uint64_t seed,
+                              femgpu_unet **out);
+int femgpu_unet_grid(const femgpu_unet *net, int *nx, int *ny, int *nz);
 int femgpu_unet_destroy(femgpu_unet *net);
 int femgpu_unet_n_params(const femgpu_unet *net, uint64_t *n);
 int femgpu_unet_layer_info(const femgpu_unet *net, int layer, int *cin, int *cout, int *kpad,
