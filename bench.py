@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the fem-rs solve path on one MI355X.
 
-Default workload (config 5, the large case):
+Default workload (q4-128-perturbed, the large case):
   3-D diffusion + mass (rho = 1) on the unit cube, 128^3 hex elements with
   interior vertices perturbed by up to 0.2 h (seed 5), H1 Q4 (513^3 =
   135,005,697 DOFs), kappa = exp(GRF) clipped to [1e-2, 1e2] (seed 7), random
@@ -85,7 +85,7 @@ def sumfac_fma_per_element(n1, q, diffusion, mass, cartesian):
     return x + y + z + y + x                        # Y^T and X^T mirror Y and X
 
 WORKLOADS = {
-    "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8,
+    "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8, coarsen=16,
                           desc="learned preconditioner, 3-D diffusion {n}^3 hex Q{p} ({N} DOFs), kappa = exp(GRF) "
                                "per field, random nodal RHS, Dirichlet, FP64 flexible CG (Chebyshev-Jacobi(4) + "
                                "U-Net 16/32/64 BF16) vs PCG + Chebyshev-Jacobi(4) to 1e-8 on held-out fields"),
@@ -133,8 +133,12 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--truncation", type=int, default=16, help="flexible CG: directions kept")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
-    ap.add_argument("--train-fields", type=int, default=4, help="q3-96-learned: training fields")
-    ap.add_argument("--test-fields", type=int, default=3, help="q3-96-learned: held-out fields")
+    ap.add_argument("--train-fields", type=int, default=256, help="q3-96-learned: training fields (seeds 0, 1, ...)")
+    ap.add_argument("--test-fields", type=int, default=32, help="q3-96-learned: held-out fields (seeds 256, ...)")
+    ap.add_argument("--epochs", type=int, default=1, help="q3-96-learned: passes over the training fields")
+    ap.add_argument("--steps-per-field", type=int, default=4, help="q3-96-learned: training steps per field")
+    ap.add_argument("--coarsen", type=int, default=0, help="learned: network grid coarsening (0: the workload's, else the smallest "
+                    "divisor leaving at most 64 intervals per direction)")
     return ap.parse_args()
 
 
@@ -265,7 +269,7 @@ def run_csr(a, w):
     # radius estimate (amg/src/lib.rs:53-123) at a bandwidth-bound size: the
     # 2048^2 P1 stiffness matrix, assembled on the GPU
     Ab = fg.assemble_bilinear(2, 1, xy2, conn2, conn2, N2, fg.DIFFUSION, 1.0, 3, ctx=ctx)
-    Ab.apply_dirichlet_row_zeroing(bnd2, np.zeros(len(bnd2)))  # as config 1: A 1 != 0, rho > 0
+    Ab.apply_dirichlet_row_zeroing(bnd2, np.zeros(len(bnd2)))  # as for the 2-D P1 Poisson system: A 1 != 0, rho > 0
     nnz2 = Ab.nnz()
     Ab.time_apply(xs2, ys2, 5)
     ms_sp, _ = Ab.time_apply(xs2, ys2, a.apply_reps)
@@ -371,14 +375,21 @@ def device_copy_gbs(fg, ctx, mib=1024, reps=5):
 
 
 def run_learned(a, w):
-    """Learned preconditioner on held-out fields of the Q3 GRF family (a
-    reduced form of the 256-field training set: --train-fields fields)."""
+    """The learned preconditioner on the Q3 96^3 GRF family.  The
+    U-Net (placed on the DOF grid coarsened by --coarsen, default as for the
+    other workloads) is trained on --train-fields fields (kappa seeds 0, 1, ...;
+    each field in turn: its operator is built, pairs are harvested from
+    PCG-Chebyshev iterates of a solve with a known solution, --steps-per-field
+    energy-loss steps are taken, and the field is freed) and compared with
+    PCG + Chebyshev-Jacobi(4) on --test-fields held-out fields (seeds 256,
+    257, ...), FP64 flexible CG to 1e-8 judged on the recomputed residual;
+    value = N x iterations / time of the learned solves."""
     import femgpu as fg
     from femgpu import learned
     n = a.n or w["n"]
     p = a.p or w["p"]
     ctx = fg.default_context()
-    steps = a.train_steps
+    say = lambda m: print(f"bench.py: {m}", file=sys.stderr, flush=True)
 
     def field(seed):
         K = fg.HexPAOperator(p, p + 1, n, form=fg.DIFFUSION, kappa=fg.grf(seed), dirichlet=True, ctx=ctx)
@@ -388,34 +399,40 @@ def run_learned(a, w):
         _, lmax = fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=1))
         return K, b, lmax
 
-    t0 = time.perf_counter()
-    train_ops, pairs = [], []
-    for sd in range(a.train_fields):
-        K, b, lmax = field(sd)
-        ps = learned.harvest_pairs(K, b.download(), iters=(2, 6, 15, 40, 100), lmax=lmax, tol=1e-10)
-        pairs += [(len(train_ops), r_, e_) for r_, e_ in ps]
-        train_ops.append(K)
-        print(f"bench.py: field {sd}: {len(ps)} pairs", file=sys.stderr, flush=True)
-    t_harvest = time.perf_counter() - t0
-    net = fg.UNet.for_operator(train_ops[0], seed=11)
+    K0, _, _ = field(0)
+    cf = a.coarsen or w.get("coarsen") or coarsening(K0)
+    net = fg.UNet.for_operator(K0, seed=11, coarsen=cf)
+    K0.close()
     rng = np.random.default_rng(0)
     losses = []
-    t0 = time.perf_counter()
-    cur = -1
-    for st in range(steps):
-        k, r_, e_ = pairs[rng.integers(len(pairs))]
-        if k != cur:
-            net.aux_from(train_ops[k])
-            cur = k
-        losses.append(net.train_step_energy(train_ops[k], r_, e_, 2e-3 * 0.1 ** (st / max(steps, 1))))
-    ctx.synchronize()
-    t_train = time.perf_counter() - t0
-    print(f"bench.py: trained {steps} steps in {t_train:.1f} s", file=sys.stderr, flush=True)
-    for K in train_ops:
-        K.close()
-    del train_ops
+    steps = 0
+    t_harvest = t_train = 0.0
+    t_all = time.perf_counter()
+    for epoch in range(a.epochs):
+        for sd in range(a.train_fields):
+            t0 = time.perf_counter()
+            K, b, lmax = field(sd)
+            pairs = learned.harvest_pairs_known(K, 5000 + 1000 * epoch + sd, iters=(2, 6, 15, 40), lmax=lmax)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            t_harvest += t1 - t0
+            net.aux_from(K)
+            for _ in range(a.steps_per_field):
+                r_, e_ = pairs[rng.integers(len(pairs))]
+                total = a.epochs * a.train_fields * a.steps_per_field
+                losses.append(net.train_step_energy(K, r_, e_, 2e-3 * 0.1 ** (steps / max(total, 1))))
+                steps += 1
+            ctx.synchronize()
+            t_train += time.perf_counter() - t1
+            for r_, e_ in pairs:
+                r_.free(); e_.free()
+            K.close()
+            del b
+            if sd % 32 == 31:
+                say(f"epoch {epoch}: {sd + 1} fields, {steps} steps, loss {np.mean(losses[-64:]):.3f}, "
+                    f"{time.perf_counter() - t_all:.0f} s")
     cfg = fg.SolverConfig(rtol=w["rtol"], max_iter=100000)
-    rows, it_l, t_l_sum = [], 0, 0.0
+    rows, it_l, t_l_sum, it_p, t_p_sum = [], 0, 0.0, 0, 0.0
     N = 0
     for j in range(a.test_fields):
         sd = 256 + j
@@ -423,15 +440,15 @@ def run_learned(a, w):
         N = K.n_dofs()
         x = fg.DeviceArray(ctx, N, np.float64)
         fg.solve_pcg_chebyshev(K, b, x, fg.SolverConfig(rtol=1e-8, max_iter=3), lmax=lmax)  # warm
-        ctx.synchronize()
-        x.upload(np.zeros(N))
+        x0 = fg.DeviceArray(ctx, N, np.float64)
+        x.copy_from(x0)
         ctx.synchronize(); t0 = time.perf_counter()
         rp, _ = fg.solve_pcg_chebyshev(K, b, x, cfg, lmax=lmax)
         ctx.synchronize(); tp = time.perf_counter() - t0
         net.aux_from(K)
-        x.upload(np.zeros(N))
+        x.copy_from(x0)
         ctx.synchronize(); t0 = time.perf_counter()
-        rl, info = fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax)
+        rl, info = fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax, truncation=a.truncation)
         ctx.synchronize(); tl = time.perf_counter() - t0
         r = fg.DeviceArray(ctx, N, np.float64)
         K.apply_zero(x, r)
@@ -443,12 +460,14 @@ def run_learned(a, w):
                      "learned_rel_residual_fp64": relres,
                      "iteration_ratio": round(rp.iterations / max(rl.iterations, 1), 3),
                      "time_ratio": round(tp / tl, 3)})
-        print(f"bench.py: held-out field {sd}: PCG-Chebyshev {rp.iterations} it {1e3 * tp:.0f} ms, "
-              f"learned FCG {rl.iterations} it {1e3 * tl:.0f} ms", file=sys.stderr, flush=True)
+        say(f"held-out field {sd}: PCG-Chebyshev {rp.iterations} it {1e3 * tp:.0f} ms, "
+            f"learned FCG {rl.iterations} it {1e3 * tl:.0f} ms")
         it_l += rl.iterations
         t_l_sum += tl
+        it_p += rp.iterations
+        t_p_sum += tp
         K.close()
-        del r, x, b
+        del r, x, x0, b
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(N * it_l / t_l_sum / 1e9, 4), "unit": "GDOF/s", "n_gpus": 1,
@@ -458,10 +477,17 @@ def run_learned(a, w):
         "config": {"workload": w["desc"].format(n=n, p=p, N=N), "name": a.workload, "n_dofs": N,
                    "train_fields": a.train_fields, "test_fields": a.test_fields, "global_batch": 1,
                    "parallelism": "single GPU"},
-        "train": {"pairs": len(pairs), "steps": steps, "harvest_s": round(t_harvest, 1),
-                  "train_s": round(t_train, 1),
-                  "energy_loss_first_last": [round(float(np.mean(losses[:20])), 4),
-                                             round(float(np.mean(losses[-20:])), 4)]},
+        "train": {"epochs": a.epochs, "steps": steps, "steps_per_field": a.steps_per_field,
+                  "coarsening": cf, "network_grid": list(net.shape), "harvest_s": round(t_harvest, 1),
+                  "train_s": round(t_train, 1), "truncation": a.truncation,
+                  "pairs_from": "PCG-Chebyshev iterates (2, 6, 15, 40) of a solve with b = A x*, x* a seeded "
+                                "uniform field, per training field",
+                  "energy_loss_first_last": [round(float(np.mean(losses[:64])), 4),
+                                             round(float(np.mean(losses[-64:])), 4)]},
+        "held_out_summary": {"pcg_chebyshev_iterations": it_p, "learned_fcg_iterations": it_l,
+                             "pcg_chebyshev_s": round(t_p_sum, 3), "learned_fcg_s": round(t_l_sum, 3),
+                             "iteration_ratio": round(it_p / max(it_l, 1), 3),
+                             "time_ratio": round(t_p_sum / t_l_sum, 3)},
         "held_out": rows,
         "roofline": None,
         "cpu_baseline": None,
@@ -680,7 +706,7 @@ def legs(fg, ctx, K, b, x, lmax, rtol, a):
     # family's FP32 copy with the energy-norm loss on pairs from PCG-Chebyshev iterates
     # of two solves whose solution is known by construction (b = A x*, x* seeded).
     t0 = time.perf_counter()
-    cf = coarsening(K)
+    cf = a.coarsen or coarsening(K)
     pairs = []
     for sd in (101, 102):
         pairs += learned.harvest_pairs_known(K32, sd, lmax=lmax)

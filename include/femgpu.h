@@ -264,7 +264,7 @@ int femgpu_csr_assemble_simplex_qp(femgpu_ctx *ctx, int dim, int order, uint64_t
 /* Assembler::assemble_linear with DomainSourceIntegrator
  * (assembly/src/standard/source.rs:44-50): b_i = sum_e sum_q w |det J| f(x_q)
  * phi_i(x_q), f = value (FEMGPU_SOURCE_CONST) or value * prod_d sin(pi x_d)
- * (FEMGPU_SOURCE_SINPROD; config 1 takes value = 2 pi^2).  b: f64[n_dofs]. */
+ * (FEMGPU_SOURCE_SINPROD; -lap u = f with u = sin sin takes value = 2 pi^2).  b: f64[n_dofs]. */
 typedef enum { FEMGPU_SOURCE_CONST = 0, FEMGPU_SOURCE_SINPROD = 1 } femgpu_source_kind;
 int femgpu_rhs_assemble_simplex(femgpu_ctx *ctx, int dim, int order, uint64_t n_nodes,
                                 const double *coords, uint64_t n_elem, const uint32_t *elem_nodes,

@@ -133,7 +133,7 @@ void fr_csr_spmv_f32(size_t nrows, const uint64_t *row_ptr, const uint32_t *col_
 /* d_i = A_ii by linear row search, 0 if absent (csr.rs:110-120, 246-252). */
 void fr_csr_diagonal(const fr_csr *a, double *d);
 
-/* -------------------------------------------- config 1: P1 Poisson 2-D ---- */
+/* ------------------------------------------------- P1 Poisson 2-D ---- */
 /* SimplexMesh::unit_square_tri(n) (simplex.rs:383-439): coords[2*(j*(n+1)+i)],
  * conn[3*e], squares split [n0,n1,n3],[n1,n2,n3]. */
 void fr_unit_square_tri(int n, double *coords, uint32_t *conn);

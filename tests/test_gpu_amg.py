@@ -69,8 +69,8 @@ def test_amg_vcycle_symmetric_and_pcg(fg, ctx):
     assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-8
 
 
-def test_amg_config1(fg, fr, ctx):
-    """Config 1 (256^2 P1 Poisson, row-zeroed Dirichlet): AMG-PCG reaches 1e-10
+def test_amg_p1_poisson_256(fg, fr, ctx):
+    """256^2 P1 Poisson (row-zeroed Dirichlet): AMG-PCG reaches 1e-10
     and agrees with the oracle's PCG-Jacobi solution."""
     rp, ci, v, rhs, coords, conn = fr.poisson_p1(256)
     N = len(rhs)

@@ -31,7 +31,7 @@ def _tet_mesh(n):
     return coords, np.array(conn, np.uint32)
 
 
-def test_assemble_config1_bitwise(fg, fr, ctx):
+def test_assemble_p1_poisson_256_bitwise(fg, fr, ctx):
     """Config-1 system at 64^2: the P1 stiffness pattern and values equal the
     oracle's bit for bit, the 3-point source load within rounding of sin,
     and after row zeroing PCG-Jacobi gives the oracle's iterations."""

@@ -237,7 +237,7 @@ def _full_size(fg, ctx, p, n, form, amp):
 
 @pytest.mark.parametrize("p,n,amp", [(3, 96, 0.0), (4, 128, 0.2)])
 def test_full_size_properties(fg, ctx, p, n, amp):
-    """Q3 96^3 (config 3's operator) and Q4 128^3 perturbed (config 5's):
+    """Q3 96^3 GRF and Q4 128^3 perturbed diffusion + mass:
     the diffusion operator annihilates constants, the mass operator's
     1^T M 1 is the volume, and the diffusion + mass operator is symmetric and
     linear."""

@@ -28,7 +28,7 @@ def test_spmv_tridiag_known_answer(fg, ctx):
 
 
 @pytest.mark.parametrize("stage", ["0", "1"])
-def test_spmv_config1_bitwise(fg, fr, ctx, monkeypatch, stage):
+def test_spmv_p1_poisson_256_bitwise(fg, fr, ctx, monkeypatch, stage):
     """Config-1 matrix, bitwise, with the row kernel's LDS staging off (its default
     below 2^19 rows) and forced on."""
     rp, ci, v, rhs, _, _ = fr.poisson_p1(256)

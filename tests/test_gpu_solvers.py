@@ -45,8 +45,8 @@ def test_pcg_jacobi_laplacian(fg, ctx):
     assert res.converged and res.iterations < 60
 
 
-def test_config1_poisson(fg, fr, ctx):
-    """Config 1: 256^2 P1 Poisson, PCG-Jacobi to 1e-10 (mfem_ex1_poisson.rs:71-114)."""
+def test_p1_poisson_256(fg, fr, ctx):
+    """256^2 P1 Poisson, PCG-Jacobi to 1e-10 (mfem_ex1_poisson.rs:71-114)."""
     rp, ci, v, rhs, coords, conn = fr.poisson_p1(256)
     A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
     x = np.zeros(len(rhs))

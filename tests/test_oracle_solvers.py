@@ -55,7 +55,7 @@ def test_spmv_4way_order_bitwise(fr):
         assert y[r] == acc
 
 
-def test_config1_matrix(fr):
+def test_p1_poisson_256_matrix(fr):
     n = 256
     rp, ci, v, rhs, coords, conn = fr.poisson_p1(n)
     assert len(rp) - 1 == 66049 and len(v) == 460289
