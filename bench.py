@@ -84,6 +84,27 @@ def sumfac_fma_per_element(n1, q, diffusion, mass, cartesian):
     z = q * q * q * z_col                           # Z: Q^2 columns, Q points each
     return x + y + z + y + x                        # Y^T and X^T mirror Y and X
 
+
+# Floating-point operations per Gauss point of the geometry formed on the fly,
+# counting a multiply-add as 2: the Jacobian's two zeta-dependent columns (6
+# multiply-adds), the rows of det(J) J^-1 as cross products (9 multiplies + 9
+# multiply-adds), det (1 + 2), the refined reciprocal of |det| (1 + 4) and
+# its product with w kappa (1), h = R^T g (3 + 6), v = f R h (3 + 6 + 3), and
+# the lane's set-up from the 8 vertices (about 126 per column of Q points,
+# spread over its points).  The stored form's pointwise product is 9
+# multiply-adds (18).
+def geometry_flops_per_point(q):
+    return 12 + 27 + 5 + 9 + 1 + 15 + 18 + 126.0 / q
+
+
+def apply_flops_per_element(n1, q, mass, mode):
+    """Flops of one element's diffusion (+ mass) apply in the given
+    quadrature-data mode ("general", "cartesian" or "geometry")."""
+    f = 2.0 * sumfac_fma_per_element(n1, q, True, mass, mode == "cartesian")
+    if mode == "geometry":
+        f += q ** 3 * (geometry_flops_per_point(q) - 18.0 + (2.0 if mass else 0.0))
+    return f
+
 WORKLOADS = {
     "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8, coarsen=16,
                           desc="learned preconditioner, 3-D diffusion {n}^3 hex Q{p} ({N} DOFs), kappa = exp(GRF) "
@@ -784,7 +805,8 @@ def main():
     N = K.n_dofs()
     E = n ** 3
     q3 = q ** 3
-    s_comp = K.qdata_per_point()  # 6 / 7 general, 1 / 2 on an axis-aligned box mesh
+    s_comp = K.qdata_per_point()  # 6 / 7 general, 1 / 2 on an axis-aligned box mesh, 1 with the geometry on the fly
+    qmode = K.qdata_mode()
     dev = lambda: fg.DeviceArray(ctx, N, np.float64)
     b = dev()
     if w["rhs"] == "unit":
@@ -807,11 +829,13 @@ def main():
     yv = dev()
     K.time_apply(xv, yv, 5)
     ms_apply, ms_kern = K.time_apply(xv, yv, a.apply_reps)
-    bytes_apply = 8.0 * (2 * N + s_comp * E * q3)
+    # x read and y written once, the stored qdata once, and with the geometry on
+    # the fly the mesh vertices once
+    bytes_apply = 8.0 * (2 * N + s_comp * E * q3) + (24.0 * (n + 1) ** 3 if qmode == "geometry" else 0.0)
     pa = {"operator_gdofs": round(N / (ms_apply * 1e-3) / 1e9, 3), "operator_ms": ms_apply,
-          "operator_kernel_ms": ms_kern, "qdata_per_point": s_comp}
-    if K.cartesian:
-        # the same operator with the general (6-value) quadrature data, for comparison
+          "operator_kernel_ms": ms_kern, "qdata_per_point": s_comp, "qdata_mode": qmode}
+    if qmode != "general":
+        # the same operator with the stored general (6-value) quadrature data, for comparison
         os.environ["FEMGPU_HEX_QDATA"] = "general"
         try:
             Kg = make_operator(fg, ctx, w, n, p)
@@ -874,7 +898,36 @@ def main():
         print(f"bench.py: CPU baseline {cpu['value']} GDOF/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
     kform = 2 if w["form"] == "diffusion" else 3
-    flops_apply = 2.0 * E * sumfac_fma_per_element(p + 1, q, True, kform == 3, K.cartesian)
+    flops_apply = E * apply_flops_per_element(p + 1, q, kform == 3, qmode)
+    kernel = ((f"PA apply = k_pa_hex_tpe<{q},{q},double,{'true' if K.cartesian else 'false'}> element kernel "
+               f"(one thread per element"
+               if K.element_kernel() == "thread-per-element" else
+               "PA apply = k_pa_hex_quad<double> element kernel (one quad of lanes per element"
+               if K.element_kernel() == "quad-per-element" else
+               f"PA apply = k_pa_hex<{q},{q},double,{kform},{ {'general': 0, 'cartesian': 1, 'geometry': 2}[qmode]}> "
+               f"element kernel (one workgroup per {256 // (q * q)} elements") +
+              f"; {s_comp} qdata value(s) per Gauss point"
+              f"{', axis-aligned mesh' if K.cartesian else ''}"
+              f"{', Jacobian formed from the element vertices at every point' if qmode == 'geometry' else ''}) + "
+              f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)")
+    hbm_gbs = bytes_apply / (ms_kern * 1e-3) / 1e9
+    tflops = flops_apply / (ms_kern * 1e-3) / 1e12
+    if qmode == "geometry":
+        # the geometry on the fly trades qdata bytes for FP64 vector work: the
+        # apply is bound by issue, not by HBM
+        roofline = {"bound": "valu", "achieved": round(tflops, 2), "peak": FP64_VALU_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),
+                    "traffic": committed_traffic(a.workload, n, p, s_comp), "kernel": kernel,
+                    "algorithmic_bytes": bytes_apply, "flops_per_apply": flops_apply,
+                    "hbm_achieved_gb_s": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4)}
+    else:
+        roofline = {"bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                    "traffic": committed_traffic(a.workload, n, p, s_comp), "kernel": kernel,
+                    "algorithmic_bytes": bytes_apply, "flops_per_apply": flops_apply,
+                    "flop_frac_fp64_valu": round(tflops / FP64_VALU_PEAK_TFLOPS, 4)}
+    roofline["device_copy_gb_s"] = round(copy_gbs, 1)
+    roofline["frac_of_device_copy"] = round(hbm_gbs / copy_gbs, 4)
     out = {
         "metric": "solve time to 1e-8 rel.res + PA apply GDOF/s; plain FP64 vs mixed+learned",
         "value": round(value, 4),
@@ -896,29 +949,7 @@ def main():
                  "iterations": int(iters[-1]), "ms_per_iteration": round(ms_step / max(iters[-1], 1), 4),
                  "lmax": lmax},
         "pa_apply": pa,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(bytes_apply / (ms_kern * 1e-3) / 1e9, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": committed_traffic(a.workload, n, p, s_comp),
-            "kernel": (f"PA apply = k_pa_hex_tpe<{q},{q},double,{'true' if K.cartesian else 'false'}> element kernel "
-                       f"(one thread per element"
-                       if K.element_kernel() == "thread-per-element" else
-                       "PA apply = k_pa_hex_quad<double> element kernel (one quad of lanes per element"
-                       if K.element_kernel() == "quad-per-element" else
-                       f"PA apply = k_pa_hex<{q},{q},double,{kform},{'true' if K.cartesian else 'false'}> element "
-                       f"kernel (one workgroup per {256 // (q * q)} elements") +
-                      f"; {s_comp} qdata value(s) per Gauss point"
-                      f"{', axis-aligned mesh' if K.cartesian else ''}) + "
-                      f"k_hex_gather<double,{p},unsigned int> DOF gather (HIP events around both)",
-            "algorithmic_bytes": bytes_apply,
-            "flops_per_apply": flops_apply,
-            "flop_frac_fp64_valu": round(flops_apply / (ms_kern * 1e-3) / 1e12 / FP64_VALU_PEAK_TFLOPS, 4),
-            "device_copy_gb_s": round(copy_gbs, 1),
-            "frac_of_device_copy": round(bytes_apply / (ms_kern * 1e-3) / 1e9 / copy_gbs, 4),
-        },
+        "roofline": roofline,
         "legs": leg,
         "cpu_baseline": cpu,
     }

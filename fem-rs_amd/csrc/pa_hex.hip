@@ -58,15 +58,23 @@ constexpr int kCombos = 15;  // n1 = 2..6, q = n1..n1+2
 
 __constant__ double c_tab64[kCombos][2][kTab];
 __constant__ float c_tab32[kCombos][2][kTab];
+// Gauss-Legendre points (on [-1, 1]) and weights for q = 0..7 points: [q][0] points, [q][1] weights
+constexpr int kGaussMax = 8;
+__constant__ double c_gauss64[kGaussMax][2][kGaussMax];
+__constant__ float c_gauss32[kGaussMax][2][kGaussMax];
 
 template <class T> struct Tab;
 template <> struct Tab<double> {
     static __device__ __forceinline__ const double *B(int c) { return c_tab64[c][0]; }
     static __device__ __forceinline__ const double *G(int c) { return c_tab64[c][1]; }
+    static __device__ __forceinline__ const double *X(int q) { return c_gauss64[q][0]; }
+    static __device__ __forceinline__ const double *W(int q) { return c_gauss64[q][1]; }
 };
 template <> struct Tab<float> {
     static __device__ __forceinline__ const float *B(int c) { return c_tab32[c][0]; }
     static __device__ __forceinline__ const float *G(int c) { return c_tab32[c][1]; }
+    static __device__ __forceinline__ const float *X(int q) { return c_gauss32[q][0]; }
+    static __device__ __forceinline__ const float *W(int q) { return c_gauss32[q][1]; }
 };
 
 static std::mutex g_tab_mu;
@@ -93,6 +101,20 @@ static void ensure_tables(int device) {
         }
     FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), h64, sizeof h64));
     FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), h32, sizeof h32));
+    static double g64[kGaussMax][2][kGaussMax];
+    static float g32[kGaussMax][2][kGaussMax];
+    std::memset(g64, 0, sizeof g64);
+    std::memset(g32, 0, sizeof g32);
+    for (int q = 1; q <= 6; ++q) {  // the element kernels' point counts
+        double x[8], w[8];
+        gauss_legendre(q, x, w);
+        for (int i = 0; i < q; ++i) {
+            g64[q][0][i] = x[i]; g64[q][1][i] = w[i];
+            g32[q][0][i] = (float)x[i]; g32[q][1][i] = (float)w[i];
+        }
+    }
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss64), g64, sizeof g64));
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gauss32), g32, sizeof g32));
     if (device < 64) g_tab_done[device] = true;
 }
 
@@ -204,10 +226,32 @@ __device__ __forceinline__ int off3(int i0, int i1, int i2) {
     return (idx[P0] * sz[P1] + idx[P1]) * sz[P2] + idx[P2];
 }
 
-template <int N1, int Q, class T, int FORM, bool CART>
+// 1 / d for d > 0: the hardware reciprocal refined by Newton steps (to about
+// an ulp in either precision), in place of the longer IEEE division sequence.
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
+
+// QM, the quadrature-data mode: 0 general (6 diffusion values per point), 1
+// Cartesian (one value per point and per-axis factors cf), 2 geometry on the
+// fly (qd holds w kappa per point, cf the mesh vertices; J is rebuilt from the
+// element's 8 vertices at every point, and the mass weight is rho w |J|).
+template <int N1, int Q, class T, int FORM, int QM>
 __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm, const T *__restrict__ cf,
-    const T *__restrict__ x, T *__restrict__ y, const int dirichlet, const int *skip) {
+    const T *__restrict__ x, T *__restrict__ y, const int dirichlet, const int *skip, const T rho) {
+    constexpr bool CART = QM == 1;
+    constexpr bool GEO = QM == 2;
+    static_assert(!GEO || (FORM & 2), "geometry on the fly serves the diffusion forms");
     constexpr int P = N1 - 1;
     constexpr int T2 = Q * Q;
     constexpr int NE = ne_of(Q);
@@ -224,6 +268,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     __shared__ T sBB[NE * LL.SB];  // Y out BB, later C_zeta+mass
     __shared__ T sBG[ND * LL.SB];  // Y out BG, later C_xi
     __shared__ T sGB[ND * LL.SB];  // Y out GB, later C_eta
+    __shared__ T sV[GEO ? NE * 24 : 1];  // GEO: element vertices, (i + 2j + 4k) * 3 + component
     auto iA = [](int e_, int k_, int j_, int a_) {
         return e_ * LL.SA + off3<LL.a0, LL.a1, LL.a2, N1, N1, Q>(k_, j_, a_);
     };
@@ -247,6 +292,14 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         const long long r = e / g.nx;
         ey = (int)(r % g.ny);
         ez = (int)(r / g.ny);
+    }
+
+    if constexpr (GEO) {
+        for (int u = t; valid && u < 24; u += T2) {
+            const int v = u / 3, comp = u - 3 * v;
+            const size_t vi = (size_t)ex + (v & 1), vj = (size_t)ey + ((v >> 1) & 1), vk = (size_t)ez + (v >> 2);
+            sV[le * 24 + u] = cf[3 * (vi + (size_t)(g.nx + 1) * (vj + (size_t)(g.ny + 1) * vk)) + comp];
+        }
     }
 
     // ---- X: lane (j,k) owns an x-line
@@ -277,14 +330,18 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     __syncthreads();
 
     // ---- Y: lane (qx,k)
-    if (valid && t < Q * N1) {
+    const bool ylane = valid && t < Q * N1;
+    T A_[N1], Ag_[N1];
+    if (ylane) {
         const int a = t % Q, k = t / Q;
-        T A_[N1], Ag_[N1];
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
             A_[j] = sA[iA(le, k, j, a)];
             if constexpr (DIFF) Ag_[j] = sAg[iA(le, k, j, a)];
         }
+    }
+    if (ylane) {
+        const int a = t % Q, k = t / Q;
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
             T bb = 0, bg = 0, gb = 0;
@@ -314,7 +371,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             c0[k] = c1[k] = c2[k] = 0;
         }
         constexpr size_t plane = (size_t)NE * T2;
-        const size_t offd = (size_t)lb * (CART ? 1 : 6) * Q * plane + (size_t)le * T2 + t;
+        const size_t offd = (size_t)lb * (CART || GEO ? 1 : 6) * Q * plane + (size_t)le * T2 + t;
         const size_t offm = (size_t)lb * Q * plane + (size_t)le * T2 + t;
         T fx = 0, fy = 0, fz = 0;  // Cartesian per-axis factors of this element
         if constexpr (DIFF && CART) {
@@ -322,9 +379,42 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             fy = cf[g.nx + ey];
             fz = cf[g.nx + g.ny + ez];
         }
+        // GEO: the Jacobian columns along this lane's z-column (xi_a, eta_b,
+        // zeta): dX/dxi = Pa + Qa zeta, dX/deta = Pb + Qb zeta, dX/dzeta = Hc
+        // (the trilinear map is linear in each coordinate)
+        T Pa[3] = {0, 0, 0}, Qa[3] = {0, 0, 0}, Pb[3] = {0, 0, 0}, Qb[3] = {0, 0, 0}, Hc[3] = {0, 0, 0};
+        T wab = 0;
+        if constexpr (GEO) {
+            // vertex V(i, j, k) at reference corner (2i - 1, 2j - 1, 2k - 1),
+            // shape functions L0(s) = (1 - s) / 2, L1(s) = (1 + s) / 2 per
+            // direction: dX/dxi = sum_jk (V_1jk - V_0jk) / 2 L_j(eta) L_k(zeta)
+            // = Pa + Qa zeta, and likewise for eta; dX/dzeta does not depend
+            // on zeta
+            const T xa = Tab<T>::X(Q)[a], yb = Tab<T>::X(Q)[b];
+            const T la0 = T(0.5) * (T(1) - xa), la1 = T(0.5) * (T(1) + xa);
+            const T lb0 = T(0.5) * (T(1) - yb), lb1 = T(0.5) * (T(1) + yb);
+            const T *V = sV + le * 24;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                T E[2], F[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const T v00 = V[(0 + 4 * k) * 3 + i], v10 = V[(1 + 4 * k) * 3 + i];
+                    const T v01 = V[(2 + 4 * k) * 3 + i], v11 = V[(3 + 4 * k) * 3 + i];
+                    E[k] = T(0.5) * ((v10 - v00) * lb0 + (v11 - v01) * lb1);
+                    F[k] = T(0.5) * ((v01 - v00) * la0 + (v11 - v10) * la1);
+                }
+                Pa[i] = T(0.5) * (E[0] + E[1]); Qa[i] = T(0.5) * (E[1] - E[0]);
+                Pb[i] = T(0.5) * (F[0] + F[1]); Qb[i] = T(0.5) * (F[1] - F[0]);
+                Hc[i] = T(0.5) * (la0 * (lb0 * (V[4 * 3 + i] - V[0 * 3 + i]) + lb1 * (V[6 * 3 + i] - V[2 * 3 + i])) +
+                                  la1 * (lb0 * (V[5 * 3 + i] - V[1 * 3 + i]) + lb1 * (V[7 * 3 + i] - V[3 * 3 + i])));
+            }
+            wab = (rho * Tab<T>::W(Q)[a]) * Tab<T>::W(Q)[b];
+        }
 #pragma unroll
         for (int c = 0; c < Q; ++c) {
             T v0 = 0, v1 = 0, v2 = 0, vm = 0;
+            T adet = 0;  // GEO: |det J| at this point
             if constexpr (DIFF) {
                 T g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
@@ -338,6 +428,26 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     v0 = (sq * fx) * g0;
                     v1 = (sq * fy) * g1;
                     v2 = (sq * fz) * g2;
+                } else if constexpr (GEO) {
+                    // D = w kappa |J| J^-1 J^-T applied as f R (R^T g): the rows r_d
+                    // of R = det(J) J^-1 are cross products of J's columns, and
+                    // f = w kappa |J| / det^2 = w kappa / |det|
+                    const T zc = Tab<T>::X(Q)[c];
+                    const T ax = Pa[0] + Qa[0] * zc, ay = Pa[1] + Qa[1] * zc, az = Pa[2] + Qa[2] * zc;
+                    const T bx = Pb[0] + Qb[0] * zc, by = Pb[1] + Qb[1] * zc, bz = Pb[2] + Qb[2] * zc;
+                    const T cx = Hc[0], cy = Hc[1], cz = Hc[2];
+                    const T r0x = by * cz - bz * cy, r0y = bz * cx - bx * cz, r0z = bx * cy - by * cx;
+                    const T r1x = cy * az - cz * ay, r1y = cz * ax - cx * az, r1z = cx * ay - cy * ax;
+                    const T r2x = ay * bz - az * by, r2y = az * bx - ax * bz, r2z = ax * by - ay * bx;
+                    const T det = ax * r0x + ay * r0y + az * r0z;
+                    adet = fabs(det);
+                    const T f = qd[(size_t)c * plane + offd] * recip(adet);
+                    const T hx = g0 * r0x + g1 * r1x + g2 * r2x;
+                    const T hy = g0 * r0y + g1 * r1y + g2 * r2y;
+                    const T hz = g0 * r0z + g1 * r1z + g2 * r2z;
+                    v0 = f * (r0x * hx + r0y * hy + r0z * hz);
+                    v1 = f * (r1x * hx + r1y * hy + r1z * hz);
+                    v2 = f * (r2x * hx + r2y * hy + r2z * hz);
                 } else {
                     const T d0 = qd[(size_t)(0 * Q + c) * plane + offd];
                     const T d1 = qd[(size_t)(1 * Q + c) * plane + offd];
@@ -354,7 +464,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 T uu = 0;
 #pragma unroll
                 for (int k = 0; k < N1; ++k) uu += Bt[c * N1 + k] * bb[k];
-                vm = qm[(size_t)c * plane + offm] * uu;
+                if constexpr (GEO) vm = (wab * Tab<T>::W(Q)[c] * adet) * uu;
+                else vm = qm[(size_t)c * plane + offm] * uu;
             }
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
@@ -376,14 +487,17 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     __syncthreads();
 
     // ---- Y^T: lane (qx,k)
-    if (valid && t < Q * N1) {
+    T C0[Q], C1[Q], C2[Q];
+    if (ylane) {
         const int a = t % Q, k = t / Q;
-        T C0[Q], C1[Q], C2[Q];
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
             C2[b] = sBB[iB(le, k, b, a)];
             if constexpr (DIFF) { C0[b] = sBG[iB(le, k, b, a)]; C1[b] = sGB[iB(le, k, b, a)]; }
         }
+    }
+    if (ylane) {
+        const int a = t % Q, k = t / Q;
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
             T fg = 0, fb = 0;
@@ -1065,7 +1179,7 @@ template <class T>
 __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ xq,
                             const double *__restrict__ wq, const double *__restrict__ coords,
                             CoefDev coef, double rho, int form, T *__restrict__ qd,
-                            T *__restrict__ qm, int mass_only_weight, int cart, int tpe) {
+                            T *__restrict__ qm, int mass_only_weight, int cart, int tpe, int geo) {
     const int Q3 = Q * Q * Q, T2 = Q * Q;
     const size_t total = (size_t)g.E * Q3;
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total;
@@ -1102,7 +1216,10 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
         const double wd = wq[qx] * wq[qy] * wq[qz] * fabs(det);
         const int t2 = qx + Q * qy;
         if ((form & 1) && qm) qm[qd_index(Q, 1, e, 0, qz, t2)] = (T)(mass_only_weight ? wd : wd * rho);
-        if ((form & 2) && qd && cart) {
+        if ((form & 2) && qd && geo) {
+            // geometry on the fly: the element kernel forms |J| J^-1 J^-T itself
+            qd[qd_index(Q, 1, e, 0, qz, t2)] = (T)(wq[qx] * wq[qy] * wq[qz] * coef_eval(coef, xp, e, qp, Q3));
+        } else if ((form & 2) && qd && cart) {
             qd[tpe ? qd_index_tpe(Q, 1, e, 0, qp) : qd_index(Q, 1, e, 0, qz, t2)] =
                 (T)(wd * coef_eval(coef, xp, e, qp, Q3));
         } else if ((form & 2) && qd) {
@@ -1213,6 +1330,11 @@ __global__ void k_evec_merge(const HexGrid g, int N1, const T *__restrict__ old,
     }
 }
 
+__global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 template <class T>
 __global__ void k_random_field(size_t n, uint64_t seed, T *__restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -1226,22 +1348,23 @@ static unsigned grid1(size_t n) {
     return (unsigned)(b ? b : 1);
 }
 
-// cf: Cartesian per-axis factors, or null for general qdata
+// qm_mode (k_pa_hex's QM): 0 general qdata, 1 Cartesian (cf: per-axis
+// factors), 2 geometry on the fly (cf: mesh vertices in T, rho: the constant
+// mass coefficient)
 template <int N1, int Q, class T>
-static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, const T *cf, const T *x,
-                        T *y, int dir, const int *skip, hipStream_t s) {
+static void launch_form(const HexGrid &g, int form, int qm_mode, const T *qd, const T *qm, const T *cf,
+                        const T *x, T *y, int dir, const int *skip, T rho, hipStream_t s) {
     constexpr int NE = ne_of(Q);
     const dim3 grid((unsigned)((g.E + NE - 1) / NE)), block(NE * Q * Q);
-    if (form == 1)
-        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 1, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
-    else if (form == 2 && cf)
-        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2, true>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
-    else if (form == 2)
-        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 2, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
-    else if (cf)
-        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, true>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
-    else
-        hipLaunchKernelGGL((k_pa_hex<N1, Q, T, 3, false>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip);
+#define L_(F_, M_) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, F_, M_>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip, rho)
+    if (form == 1) L_(1, 0);
+    else if (form == 2 && qm_mode == 1) L_(2, 1);
+    else if (form == 2 && qm_mode == 2) L_(2, 2);
+    else if (form == 2) L_(2, 0);
+    else if (qm_mode == 1) L_(3, 1);
+    else if (qm_mode == 2) L_(3, 2);
+    else L_(3, 0);
+#undef L_
 }
 
 // tpe: the operator keeps qdata in the qd_index_tpe layout, for the
@@ -1249,7 +1372,8 @@ static void launch_form(const HexGrid &g, int form, const T *qd, const T *qm, co
 // kernel (Q3 diffusion, 4^3 points, Cartesian qdata)
 template <class T>
 static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, const T *qm, const T *cf,
-                      const T *x, T *y, int dir, const int *skip, hipStream_t s, bool tpe = false) {
+                      const T *x, T *y, int dir, const int *skip, hipStream_t s, bool tpe = false,
+                      bool geo = false, T rho = T(0)) {
     if (tpe && form == 2 && n1 == 3 && q == 3) {
         const dim3 grid((unsigned)((g.E + 255) / 256)), block(256);
         if (cf) hipLaunchKernelGGL((k_pa_hex_tpe<3, 3, T, true>), grid, block, 0, s, g, qd, cf, x, y, dir, skip);
@@ -1263,7 +1387,10 @@ static bool launch_pa(int n1, int q, const HexGrid &g, int form, const T *qd, co
         return true;
     }
 #define CASE(A, B)                                                                 \
-    if (n1 == A && q == B) { launch_form<A, B, T>(g, form, qd, qm, cf, x, y, dir, skip, s); return true; }
+    if (n1 == A && q == B) {                                                       \
+        launch_form<A, B, T>(g, form, geo ? 2 : cf ? 1 : 0, qd, qm, cf, x, y, dir, skip, rho, s); \
+        return true;                                                               \
+    }
     CASE(2, 2) CASE(2, 3) CASE(3, 3) CASE(3, 4) CASE(4, 4) CASE(4, 5) CASE(5, 5) CASE(5, 6)
     CASE(6, 6)
 #undef CASE
@@ -1306,12 +1433,19 @@ struct HexOp final : femgpu_op {
     bool cart = false;           // axis-aligned box mesh: Cartesian qdata (one value per point)
     bool tpe = false;            // qd_index_tpe qdata layout and a thread-level element kernel
     bool quad = false;           // that kernel is k_pa_hex_quad (Q3), else k_pa_hex_tpe (Q2)
+    bool geo = false;            // geometry on the fly: qd = w kappa per point, J from the vertices
+    int geo_req = -1;            // requested at creation (-1: from FEMGPU_HEX_QDATA)
     std::vector<double> cf_host;  // Cartesian per-axis factors [nx | ny | nz]
-    DevBuf cf;                    // the same, in the operator's type
+    DevBuf cf;                    // the same, in the operator's type; with geo, the vertices in that type
 
-    template <class T> const T *cf_ptr() const { return cart ? (const T *)cf.ptr : nullptr; }
-    // values of quadrature data per Gauss point
-    int qdata_per_point() const { return ((desc.form & 2) ? (cart ? 1 : 6) : 0) + ((desc.form & 1) ? 1 : 0); }
+    template <class T> const T *cf_ptr() const { return cart || geo ? (const T *)cf.ptr : nullptr; }
+    // values of quadrature data per Gauss point (stored, read by every apply)
+    int qdata_per_point() const {
+        if (geo) return 1;
+        return ((desc.form & 2) ? (cart ? 1 : 6) : 0) + ((desc.form & 1) ? 1 : 0);
+    }
+    // 0 general, 1 Cartesian, 2 geometry on the fly
+    int qdata_mode() const { return geo ? 2 : cart ? 1 : 0; }
 
     const char *name() const override { return "pa_hex"; }
 
@@ -1322,7 +1456,7 @@ struct HexOp final : femgpu_op {
         const int dir = desc.dirichlet ? 1 : 0;
         if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
-                     skip, s, tpe);
+                     skip, s, tpe, geo, (T)desc.rho);
         launch_gather<T>(desc.p, g, evec.as<T>(), x, y, acc ? 1 : 0, dir, skip, s);
         if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
     }
@@ -1337,7 +1471,7 @@ struct HexOp final : femgpu_op {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
-                     skip, s, tpe);
+                     skip, s, tpe, geo, (T)desc.rho);
         const dim3 grid((unsigned)red_blocks(n)), block(256);
         const T *ye = evec.as<T>();
 #define GF(P_)                                                                                              \
@@ -1387,8 +1521,19 @@ struct HexOp final : femgpu_op {
         const int n1 = desc.p + 1;
         const size_t L = (size_t)g.E * n1 * n1 * n1;
         DevBuf old(L * sizeof(T));
+        DevBuf gqd, gqm;  // geo: general qdata, built for the diagonal only
+        const T *dqd = qd.as<T>(), *dqm = qm.as<T>();
+        if (geo) {
+            const size_t nq = qd_count(g.E, desc.q);
+            gqd.alloc(6 * nq * sizeof(T));
+            if (desc.form & 1) gqm.alloc(nq * sizeof(T));
+            geo = false;  // setup_t builds the stored general form
+            setup_t<T>(desc.form, desc.rho, false, gqd.as<T>(), gqm.as<T>());
+            geo = true;
+            dqd = gqd.as<T>(); dqm = gqm.as<T>();
+        }
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
-                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
+                           Gd.as<double>(), dqd, dqm, (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
                            desc.form, old.as<T>());
         HexGrid gd = g;
         gd.ymerge = 0;  // k_evec_merge builds the element vector without the y-merge
@@ -1415,7 +1560,7 @@ struct HexOp final : femgpu_op {
         const size_t total = (size_t)g.E * desc.q * desc.q * desc.q;
         hipLaunchKernelGGL(k_hex_setup<T>, dim3(grid1(total)), dim3(256), 0, ctx->stream, g, desc.q,
                            xq.as<double>(), wq.as<double>(), coords.as<double>(), c, rho, form,
-                           qd_, qm_, mass_weight_only ? 1 : 0, cart ? 1 : 0, tpe ? 1 : 0);
+                           qd_, qm_, mass_weight_only ? 1 : 0, cart ? 1 : 0, tpe ? 1 : 0, geo ? 1 : 0);
         FG_HIP(hipGetLastError());
     }
 
@@ -1432,6 +1577,13 @@ struct HexOp final : femgpu_op {
         FG_HIP(hipMemcpyAsync(Gd.ptr, G, 8 * q * n1, hipMemcpyHostToDevice, s));
         const size_t nq = qd_count_any(g.E, q);
         const size_t es = esize();
+        // Geometry on the fly on meshes that are not axis-aligned boxes (diffusion
+        // forms): faster than streaming the six-value qdata in both precisions
+        // (Q4 128^3 perturbed: 4.21 -> 3.33 ms FP64, 2.42 -> 1.85 ms FP32) and a
+        // seventh of its memory.  FEMGPU_HEX_QDATA=general keeps stored qdata.
+        const char *qmode = std::getenv("FEMGPU_HEX_QDATA");
+        const bool want_geo = geo_req >= 0 ? geo_req == 1 : !(qmode && std::strcmp(qmode, "general") == 0);
+        geo = (desc.form & 2) && !cart && want_geo;
         const char *kmode = std::getenv("FEMGPU_HEX_KERNEL");
         // with the general six-value qdata the workgroup kernel is as fast (both HBM-bound) at higher occupancy
         const bool lds_forced = kmode && std::strcmp(kmode, "lds") == 0;
@@ -1447,8 +1599,20 @@ struct HexOp final : femgpu_op {
             // shared by a merged and an unmerged element
             g.ymerge = (g.nx == 64 || g.nx == 128) ? 1 : 0;
         }
-        if (desc.form & 2) qd.alloc((cart ? 1 : 6) * nq * es);
-        if (desc.form & 1) qm.alloc(nq * es);
+        if (desc.form & 2) qd.alloc((cart || geo ? 1 : 6) * nq * es);
+        if ((desc.form & 1) && !geo) qm.alloc(nq * es);
+        if (geo) {
+            // the vertices in the operator's type, for the element kernel
+            const size_t nv3 = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1) * 3;
+            cf.alloc(nv3 * es);
+            if (dtype == FEMGPU_F64) {
+                FG_HIP(hipMemcpyAsync(cf.ptr, coords.ptr, nv3 * 8, hipMemcpyDeviceToDevice, s));
+            } else {
+                hipLaunchKernelGGL(k_to_f32, dim3(grid1(nv3)), dim3(256), 0, s, nv3, coords.as<double>(),
+                                   cf.as<float>());
+                FG_HIP(hipGetLastError());
+            }
+        }
         if (cart) {
             cf.alloc(cf_host.size() * es);
             if (dtype == FEMGPU_F64) {
@@ -1510,8 +1674,11 @@ static bool cartesian_factors(int nx, int ny, int nz, const double *X, std::vect
     return true;
 }
 
+// geo_req: -1 takes the quadrature-data mode from FEMGPU_HEX_QDATA, 0 / 1
+// asks for stored qdata / the geometry on the fly (a converted copy keeps its
+// source's mode)
 static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *coords_host,
-                       const void *coords_dev, const void *kqp_dev) {
+                       const void *coords_dev, const void *kqp_dev, int geo_req = -1) {
     FG_REQUIRE(d.nx > 0 && d.ny > 0 && d.nz > 0, FEMGPU_ERR_INVALID_ARG, "pa_create_hex: empty mesh");
     FG_REQUIRE(d.p >= 1 && supported(d.p + 1, d.q), FEMGPU_ERR_UNSUPPORTED,
                "pa_create_hex: unsupported (p, q) = (" + std::to_string(d.p) + ", " +
@@ -1534,6 +1701,7 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
     auto *op = new HexOp();
     try {
         op->ctx = c; op->dtype = d.dtype; op->desc = d; op->desc.kappa.qp_values = nullptr;
+        op->geo_req = geo_req;
         HexGrid &g = op->g;
         g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
         g.E = (long long)d.nx * d.ny * d.nz;
@@ -1670,6 +1838,15 @@ int femgpu_pa_hex_qdata_per_point(const femgpu_op *op_, int *values) {
     });
 }
 
+int femgpu_pa_hex_qdata_mode(const femgpu_op *op_, int *mode) {
+    return guard([&] {
+        auto *op = dynamic_cast<const HexOp *>(op_);
+        FG_REQUIRE(op && mode, FEMGPU_ERR_INVALID_ARG, "pa_hex_qdata_mode: not a hex operator");
+        *mode = op->qdata_mode();
+        return 0;
+    });
+}
+
 int femgpu_pa_hex_element_kernel(const femgpu_op *op_, int *kind) {
     return guard([&] {
         auto *op = dynamic_cast<const HexOp *>(op_);
@@ -1686,7 +1863,7 @@ int femgpu_pa_hex_convert(femgpu_op *op_, femgpu_dtype dtype, femgpu_op **out) {
         std::lock_guard<std::recursive_mutex> lk(op->ctx->mu);
         femgpu_hex_desc d = op->desc;
         d.dtype = dtype;
-        *out = make_hex(op->ctx, d, nullptr, op->coords.ptr, op->kqp.ptr);
+        *out = make_hex(op->ctx, d, nullptr, op->coords.ptr, op->kqp.ptr, op->geo ? 1 : 0);
         return 0;
     });
 }

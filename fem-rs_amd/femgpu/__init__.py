@@ -134,6 +134,7 @@ _sig("femgpu_pa_hex_rhs_unit", [P, P, I])
 _sig("femgpu_pa_hex_rhs_random", [P, U64, P, I])
 _sig("femgpu_pa_hex_convert", [P, I, C.POINTER(P)])
 _sig("femgpu_pa_hex_qdata_per_point", [P, C.POINTER(I)])
+_sig("femgpu_pa_hex_qdata_mode", [P, C.POINTER(I)])
 _sig("femgpu_pa_hex_element_kernel", [P, C.POINTER(I)])
 _sig("femgpu_pa_create_simplex", [P, I, I, U64, P, U64, P, P, U64, I, D, I, C.POINTER(P)])
 _sig("femgpu_pa_create_simplex_qp", [P, I, I, U64, P, U64, P, P, U64, I, P, I, C.POINTER(P)])
@@ -643,13 +644,23 @@ class HexPAOperator(Operator):
         return self.shape[0] * self.shape[1] * self.shape[2]
 
     def qdata_per_point(self):
-        """Quadrature-data values stored per Gauss point: 6 (diffusion,
-        general trilinear mesh) or 1 (diffusion on an axis-aligned box mesh,
-        with per-axis factors), + 1 with the mass term.  FEMGPU_HEX_QDATA=general
-        at creation keeps the general form."""
+        """Quadrature-data values stored per Gauss point: 1 (w kappa) with
+        the geometry formed on the fly (diffusion on a mesh that is not an
+        axis-aligned box, the default), 1 (+ 1 with the mass term) on an
+        axis-aligned box mesh with per-axis factors, 6 (+ 1) for the stored
+        general form (FEMGPU_HEX_QDATA=general at creation)."""
         v = I()
         _check(lib.femgpu_pa_hex_qdata_per_point(self.h, C.byref(v)))
         return v.value
+
+    def qdata_mode(self):
+        """"general" (D = w kappa |J| J^-1 J^-T per point; mass-only
+        operators, or FEMGPU_HEX_QDATA=general at creation), "cartesian"
+        (axis-aligned box mesh) or "geometry" (w kappa per point, J formed
+        from the vertices by every apply; the default elsewhere)."""
+        v = I()
+        _check(lib.femgpu_pa_hex_qdata_mode(self.h, C.byref(v)))
+        return {0: "general", 1: "cartesian", 2: "geometry"}[v.value]
 
     def element_kernel(self):
         """"workgroup" (LDS hand-offs between phases), "thread-per-element"
@@ -661,7 +672,7 @@ class HexPAOperator(Operator):
 
     @property
     def cartesian(self):
-        return bool(self.form & DIFFUSION) and self.qdata_per_point() <= 2
+        return bool(self.form & DIFFUSION) and self.qdata_mode() == "cartesian"
 
     def convert(self, dtype):
         h = P()

@@ -211,6 +211,7 @@ extern "C" {
     pub fn femgpu_pa_hex_rhs_unit(op: *mut femgpu_op, b: *mut c_void, mem: femgpu_mem) -> c_int;
     pub fn femgpu_pa_hex_rhs_random(op: *mut femgpu_op, seed: u64, b: *mut c_void, mem: femgpu_mem) -> c_int;
     pub fn femgpu_pa_hex_qdata_per_point(op: *const femgpu_op, values: *mut c_int) -> c_int;
+    pub fn femgpu_pa_hex_qdata_mode(op: *const femgpu_op, mode: *mut c_int) -> c_int;
     pub fn femgpu_pa_hex_element_kernel(op: *const femgpu_op, kind: *mut c_int) -> c_int;
     pub fn femgpu_pa_hex_convert(op: *mut femgpu_op, dtype: femgpu_dtype, out: *mut *mut femgpu_op) -> c_int;
 

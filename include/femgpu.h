@@ -199,13 +199,24 @@ int femgpu_hex_vertices(int nx, int ny, int nz, double amp, uint64_t seed, doubl
 int femgpu_pa_hex_rhs_unit(femgpu_op *op, void *b, femgpu_mem mem);
 /* Seeded uniform [-1,1] nodal field (0 on essential DOFs when dirichlet). */
 int femgpu_pa_hex_rhs_random(femgpu_op *op, uint64_t seed, void *b, femgpu_mem mem);
-/* Values of quadrature data stored per Gauss point: 6 for diffusion on a
- * general (trilinear) mesh, 1 on an axis-aligned box mesh (vertex x depends
- * on i only, y on j, z on k; then w kappa |J| is stored with per-axis factors
- * 1/J_dd^2), plus 1 for the mass term.  Setting FEMGPU_HEX_QDATA=general at
- * creation keeps the general form on box meshes too.  No counterpart in the
- * reference (its PA is simplex-only, assembly/src/partial.rs). */
+/* Values of quadrature data stored per Gauss point: 1 (w kappa) with the
+ * geometry formed on the fly (the default for the diffusion forms on meshes
+ * that are not axis-aligned boxes); 1 on an axis-aligned box mesh (vertex x
+ * depends on i only, y on j, z on k; then w kappa |J| is stored with per-axis
+ * factors 1/J_dd^2), plus 1 for the mass term; 6 (+ 1) for the stored general
+ * form, which FEMGPU_HEX_QDATA=general at creation selects on every mesh.  No
+ * counterpart in the reference (its PA is simplex-only,
+ * assembly/src/partial.rs). */
 int femgpu_pa_hex_qdata_per_point(const femgpu_op *op, int *values);
+/* Quadrature-data mode of a hex operator: 0 = general (D = w kappa |J|
+ * J^-1 J^-T stored per point), 1 = Cartesian (axis-aligned box mesh), 2 =
+ * geometry on the fly (w kappa stored per point; the element kernel forms J
+ * from the element's 8 vertices at every point, and the mass weight rho w |J|
+ * with the constant rho).  Diffusion forms take 1 on axis-aligned box meshes
+ * and 2 elsewhere; FEMGPU_HEX_QDATA=general at creation selects 0; the
+ * mass-only form stores w rho |J| (0).  A converted copy keeps its source's
+ * mode.  No counterpart in the reference. */
+int femgpu_pa_hex_qdata_mode(const femgpu_op *op, int *mode);
 /* Element kernel of a hex operator: 0 = one workgroup per group of elements
  * (LDS hand-offs between the contraction phases), 1 = one thread per element
  * (Q2 diffusion, 3^3 points, Cartesian qdata), 2 = one quad of lanes per

@@ -31,11 +31,22 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+@pytest.fixture(params=["default", "general"])
+def qdata(request, monkeypatch):
+    """Runs a test with the operators' default quadrature data (the geometry
+    formed on the fly on these perturbed meshes) and with the stored general
+    form (FEMGPU_HEX_QDATA=general)."""
+    if request.param == "general":
+        monkeypatch.setenv("FEMGPU_HEX_QDATA", "general")
+    return request.param
+
+
 @pytest.mark.parametrize("p,q", HEX_CASES)
 @pytest.mark.parametrize("form", [1, 2, 3])
-def test_hex_apply_f64(fg, fr, ctx, p, q, form):
+def test_hex_apply_f64(fg, fr, ctx, qdata, p, q, form):
     n = (3, 2, 4)
     op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, form)
+    assert op.qdata_mode() == ("geometry" if qdata == "default" and form != 1 else "general")
     N = op.n_dofs()
     assert N == fr.hex_ndofs(d)
     x = np.random.default_rng(p * 10 + q).standard_normal(N)
@@ -59,7 +70,7 @@ MULTI_TILE = [(1, 2, (17, 9, 5)), (2, 3, (16, 9, 7)), (2, 3, (15, 5, 9)), (3, 4,
 @pytest.mark.parametrize("p,q,n", MULTI_TILE)
 @pytest.mark.parametrize("form", [1, 2, 3])
 @pytest.mark.parametrize("dirichlet", [False, True])
-def test_hex_apply_multi_tile(fg, fr, ctx, p, q, n, form, dirichlet):
+def test_hex_apply_multi_tile(fg, fr, ctx, qdata, p, q, n, form, dirichlet):
     op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, form, dirichlet=dirichlet)
     N = op.n_dofs()
     x = np.random.default_rng(p + 7 * form).standard_normal(N)
@@ -74,7 +85,7 @@ def test_hex_apply_multi_tile(fg, fr, ctx, p, q, n, form, dirichlet):
 
 
 @pytest.mark.parametrize("p,q", [(2, 3), (3, 4), (4, 5)])
-def test_hex_dirichlet_diag_rhs(fg, fr, ctx, p, q):
+def test_hex_dirichlet_diag_rhs(fg, fr, ctx, qdata, p, q):
     n = (4, 3, 3)
     op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, 3, dirichlet=True)
     x = np.random.default_rng(1).standard_normal(op.n_dofs())
@@ -85,7 +96,7 @@ def test_hex_dirichlet_diag_rhs(fg, fr, ctx, p, q):
     assert _rel(op.matvec(x), fr.hex_apply_refloop(d, c, x)) < 1e-12
 
 
-def test_hex_grf_and_qp_array(fg, fr, ctx):
+def test_hex_grf_and_qp_array(fg, fr, ctx, qdata):
     n = (3, 3, 3)
     op, d, c, Dq, Mq = _hex(fg, fr, ctx, 3, 4, n, 2, kappa=dict(kind="grf", seed=11))
     x = np.random.default_rng(2).standard_normal(op.n_dofs())
@@ -101,7 +112,7 @@ def test_hex_grf_and_qp_array(fg, fr, ctx):
 
 
 @pytest.mark.parametrize("p,q", [(2, 3), (3, 4), (4, 5)])
-def test_hex_f32(fg, fr, ctx, p, q):
+def test_hex_f32(fg, fr, ctx, qdata, p, q):
     n = (4, 4, 3)
     op, d, c, Dq, Mq = _hex(fg, fr, ctx, p, q, n, 3, dtype=fg.F32)
     x = np.random.default_rng(7).standard_normal(op.n_dofs()).astype(np.float32)
@@ -258,7 +269,8 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
     """On an axis-aligned box mesh (uniform, or graded per axis) the operator
     stores one quadrature value per point (two with mass); its applies and
     diagonal equal those of the general six-value form (FEMGPU_HEX_QDATA=general)
-    and the oracle's; a perturbed mesh keeps the general form."""
+    and the oracle's; a perturbed mesh forms its geometry on the fly, or keeps
+    the stored general form when asked to."""
     rng = np.random.default_rng(4)
     n = 5
     # graded per axis: x depends on i only, y on j, z on k
@@ -287,7 +299,12 @@ def test_hex_cartesian_qdata(fg, fr, ctx, monkeypatch):
             K.close(); G.close(); K32.close()
     P = fg.HexPAOperator(2, 3, n, form=fg.DIFFUSION, kappa=1.0, dirichlet=True,
                          vertices=fg.hex_vertices(n, n, n, amp=0.1, seed=2), ctx=ctx)
-    assert not P.cartesian and P.qdata_per_point() == 6
+    assert not P.cartesian and P.qdata_mode() == "geometry" and P.qdata_per_point() == 1
+    monkeypatch.setenv("FEMGPU_HEX_QDATA", "general")
+    P = fg.HexPAOperator(2, 3, n, form=fg.DIFFUSION, kappa=1.0, dirichlet=True,
+                         vertices=fg.hex_vertices(n, n, n, amp=0.1, seed=2), ctx=ctx)
+    monkeypatch.delenv("FEMGPU_HEX_QDATA")
+    assert P.qdata_mode() == "general" and P.qdata_per_point() == 6
     M = fg.HexPAOperator(2, 3, n, form=fg.MASS, rho=1.0, ctx=ctx)
     assert not M.cartesian and M.qdata_per_point() == 1
 
