@@ -258,6 +258,8 @@ bool hex_dirichlet(femgpu_op *op);
 // learned preconditioner (unet.hip): z = |r| N(r/|r|) on device vectors
 bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm);
 long long unet_voxels(const femgpu_unet *u);
+// true when the network runs on a coarsened copy of the DOF grid
+bool unet_coarse(const femgpu_unet *u);
 femgpu_ctx *unet_ctx(const femgpu_unet *u);
 
 // hex mesh generator (host) shared by the C ABI and set-up code

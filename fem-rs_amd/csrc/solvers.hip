@@ -1568,6 +1568,144 @@ namespace femgpu {
 // scaled by its Galerkin step (the network's output scale does not matter).
 constexpr int kFcgTruncMax = 64;
 
+// ---- fused flexible-CG vector kernels (partials on the fixed red_blocks grid) ----
+constexpr int kFcgPack = 16;  // directions per k_fcg_pq pass
+template <class T> struct FcgDirs {
+    const T *p[kFcgPack];
+    const T *q[kFcgPack];
+    double b[kFcgPack];
+    int m;
+};
+
+// r1 = r - az1; partials of r1.r1
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_resid(size_t n, const T *__restrict__ r, const T *__restrict__ az1,
+                                                   T *__restrict__ r1, double *__restrict__ part) {
+    double v[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const T d = r[i] - az1[i];
+        r1[i] = d;
+        v[0] += (double)d * (double)d;
+    }
+    block_sum<1, 256>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// partials of nn.An (row 0) and nn.r1 (row 1)
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_dots2(size_t n, const T *__restrict__ nn, const T *__restrict__ An,
+                                                   const T *__restrict__ r1, double *__restrict__ part) {
+    double v[2] = {0.0, 0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double a = (double)nn[i];
+        v[0] += a * (double)An[i];
+        v[1] += a * (double)r1[i];
+    }
+    block_sum<2, 256>(v);
+    if (threadIdx.x == 0) { part[blockIdx.x] = v[0]; part[kRedBlocks + blockIdx.x] = v[1]; }
+}
+
+// partials of (z1 + g nn).q_j, j < m <= kPack (row j); nn unused when g == 0
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_zq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
+                                                double g, VecPack<T> qs, double *__restrict__ part) {
+    double v[kPack];
+#pragma unroll
+    for (int j = 0; j < kPack; ++j) v[j] = 0.0;
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double z = g != 0.0 ? (double)(T)((double)z1[i] + g * (double)nn[i]) : (double)z1[i];
+#pragma unroll
+        for (int j = 0; j < kPack; ++j)
+            if (j < qs.m) v[j] += z * (double)qs.v[j][i];
+    }
+    block_sum<kPack, 256>(v);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < qs.m; ++j) part[(size_t)j * kRedBlocks + blockIdx.x] = v[j];
+}
+
+// p = base_p + sum_j b_j p_j, q = base_q + sum_j b_j q_j with base (z1 + g nn,
+// az1 + g An) on the first pass and (p, q) after; on the last pass partials of
+// p.q (row 0) and p.r (row 1)
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
+                                                const T *__restrict__ az1, const T *__restrict__ An, double g,
+                                                const T *__restrict__ r, FcgDirs<T> d, T *p, T *q, int first,
+                                                int last, double *__restrict__ part) {
+    double v[2] = {0.0, 0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        double sp, sq;
+        if (first) {
+            // the preconditioned residual z and A z, rounded to T as the
+            // unfused form stored them
+            sp = g != 0.0 ? (double)(T)((double)z1[i] + g * (double)nn[i]) : (double)z1[i];
+            sq = g != 0.0 ? (double)(T)((double)az1[i] + g * (double)An[i]) : (double)az1[i];
+        } else {
+            sp = (double)p[i];
+            sq = (double)q[i];
+        }
+#pragma unroll
+        for (int j = 0; j < kFcgPack; ++j)
+            if (j < d.m) {
+                sp += d.b[j] * (double)d.p[j][i];
+                sq += d.b[j] * (double)d.q[j][i];
+            }
+        const T pt = (T)sp, qt = (T)sq;
+        p[i] = pt;
+        q[i] = qt;
+        if (last) {
+            v[0] += (double)pt * (double)qt;
+            v[1] += (double)pt * (double)r[i];
+        }
+    }
+    if (last) {
+        block_sum<2, 256>(v);
+        if (threadIdx.x == 0) { part[blockIdx.x] = v[0]; part[kRedBlocks + blockIdx.x] = v[1]; }
+    }
+}
+
+// x += a p, r -= a q; partials of r.r
+template <class T>
+__global__ __launch_bounds__(256) void k_fcg_xr(size_t n, double a, const T *__restrict__ p, const T *__restrict__ q,
+                                                T *__restrict__ x, T *__restrict__ r, double *__restrict__ part) {
+    double v[1] = {0.0};
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        x[i] = (T)((double)x[i] + a * (double)p[i]);
+        const T ri = (T)((double)r[i] - a * (double)q[i]);
+        r[i] = ri;
+        v[0] += (double)ri * (double)ri;
+    }
+    block_sum<1, 256>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// out[j] = sum of partial row j (fixed order), j < m
+__global__ __launch_bounds__(1024) void k_fcg_fin(const double *__restrict__ partials, int nb, int m,
+                                                  double *__restrict__ out) {
+    for (int j = 0; j < m; ++j) {
+        double v[1] = {threadIdx.x < (unsigned)nb ? partials[(size_t)j * kRedBlocks + threadIdx.x] : 0.0};
+        block_sum<1, 1024>(v);
+        if (threadIdx.x == 0) out[j] = v[0];
+        __syncthreads();
+    }
+}
+
+// the m sums of the partial rows, copied to the host (one synchronisation)
+static void fcg_fin_host(femgpu_ctx *c, int nb, int m, double *out) {
+    double *o = c->scalars.as<double>() + kMultiDotBase;
+    hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(1024), 0, c->stream, (const double *)c->partials.as<double>(), nb,
+                       m, o);
+    FG_HIP(hipMemcpyAsync(c->pinned + kMultiDotBase, o, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    FG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->pinned + kMultiDotBase, sizeof(double) * m);
+}
+
+// Flexible CG (Notay, SIAM J. Sci. Comput. 22 (2000)) truncated to the last
+// `trunc` directions (A-orthogonalisation window), with a two-level
+// preconditioner: Chebyshev-Jacobi smoothing z1 = C r, then the network's
+// correction of the smooth residual r1 = r - A z1 that the smoother leaves,
+// scaled by its Galerkin step (the network's output scale does not matter).
+// z = z1 + gamma n and A z = A z1 + gamma A n are never stored: the fused
+// kernels above read their parts.
 template <class T>
 static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, const T *b, T *x,
                        double rtol, double atol, uint64_t max_iter, int patience, int trunc,
@@ -1576,12 +1714,16 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     hipStream_t s = K.s;
     const size_t n = K.n;
     const femgpu_dtype dt = sizeof(T) == 8 ? FEMGPU_F64 : FEMGPU_F32;
-    T *r = K.vec(K.W_R), *z = K.vec(K.W_Z), *az = K.vec(K.W_AP);
+    T *r = K.vec(K.W_R), *tmp = K.vec(K.W_AP);
     T *nn = K.vec(12), *An = K.vec(13), *r1 = K.vec(14), *z1 = K.vec(15), *az1 = K.vec(16);
     std::vector<T *> P(trunc), Q(trunc);
     std::vector<double> pq_hist(trunc, 1.0), beta(trunc);
     for (int j = 0; j < trunc; ++j) { P[j] = K.vec(20 + 2 * j); Q[j] = K.vec(21 + 2 * j); }
     const T *inv = K.vec(K.W_INV);
+    const int nb = red_blocks(n);
+    const dim3 grid((unsigned)nb), block(256);
+    double *part = c->partials.as<double>();
+    const bool coarse = unet_coarse(net);
     auto dot = [&](const T *a, const T *bb) { return dot_host(c, a, bb, n, dt); };
     FG_HIP(hipMemsetAsync(K.flags, 0, sizeof(int), s));
     // Two-level preconditioner, z = B r, together with A z:
@@ -1589,33 +1731,34 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     //   r1 = r - A z1         (the smooth residual the smoother leaves)
     //   n  = N(r1), gamma = n.r1 / n.An   (network correction, Galerkin-scaled)
     //   z  = z1 + gamma n,    A z = A z1 + gamma A n
-    // Without a usable network output z = C r.
-    auto precond = [&](const T *rin, T *zout, T *azout) {
+    // Without a usable network output gamma = 0, z = C r.
+    double gamma = 0.0;
+    auto precond = [&](const T *rin) {
+        gamma = 0.0;
         K.cheb_apply(cp, rin, inv, z1);
         K.apply(z1, az1, false);
-        copy<T>(s, r1, rin, n);
-        axpby<T>(s, r1, -1.0, az1, 1.0, n);          // r1 = r - A z1
-        copy<T>(s, zout, z1, n);
-        copy<T>(s, azout, az1, n);
-        const double rn = std::sqrt(dot(r1, r1));
+        hipLaunchKernelGGL(k_fcg_resid<T>, grid, block, 0, s, n, rin, (const T *)az1, r1, part);
+        double rn = 1.0;  // the coarse placement scales by |R r1| itself
+        if (!coarse) {
+            fcg_fin_host(c, nb, 1, &rn);
+            rn = std::sqrt(rn);
+        }
         if (!unet_apply_dev(net, r1, nn, dt, rn)) return;
         K.apply(nn, An, false);
-        const void *da[2] = {nn, nn}, *db[2] = {An, r1};
+        hipLaunchKernelGGL(k_fcg_dots2<T>, grid, block, 0, s, n, (const T *)nn, (const T *)An, (const T *)r1, part);
         double d[2];
-        dots_host(c, 2, da, db, n, dt, d);
+        fcg_fin_host(c, nb, 2, d);
         const double nAn = d[0], nr = d[1];
         if (!(nAn > 0.0) || !std::isfinite(nAn) || !std::isfinite(nr)) return;
-        const double gamma = nr / nAn;
-        axpby<T>(s, zout, gamma, nn, 1.0, n);
-        axpby<T>(s, azout, gamma, An, 1.0, n);
+        gamma = nr / nAn;
         ++info->net_corrections;
     };
     info->fcg_iterations = 0; info->fallback_iterations = 0; info->fell_back = 0; info->fallback_reason = 0;
     info->net_corrections = 0;
     auto true_residual = [&]() {
-        K.apply(x, az, false);
+        K.apply(x, tmp, false);
         copy<T>(s, r, b, n);
-        axpby<T>(s, r, -1.0, az, 1.0, n);              // r = b - A x
+        axpby<T>(s, r, -1.0, tmp, 1.0, n);              // r = b - A x
         return std::sqrt(dot(r, r));
     };
     double rr = true_residual();
@@ -1628,7 +1771,7 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     }
     double best = rr;
     int since_best = 0, reason = 0, replacements = 0;
-    precond(r, z, az);
+    precond(r);
     uint64_t it = 0, stored = 0;  // stored: directions kept so far (<= trunc)
     for (; it < max_iter; ++it) {
         // p = z - sum_j (z.q_j / p_j.q_j) p_j and q = A p = A z - sum_j (...) q_j
@@ -1636,27 +1779,39 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
         const int slot = (int)(it % trunc);
         T *p = P[slot], *qk = Q[slot];
         const int m = (int)std::min<uint64_t>(stored, (uint64_t)trunc - 1);
-        std::vector<const T *> vp(m), vq(m);
-        for (int j = 0; j < m; ++j) {
-            const int sj = (int)((it - 1 - j) % trunc);
-            vp[j] = P[sj]; vq[j] = Q[sj];
+        std::vector<int> sj(m);
+        for (int j = 0; j < m; ++j) sj[j] = (int)((it - 1 - j) % trunc);
+        // z.q_j, kPack directions per pass
+        for (int j0 = 0; j0 < m; j0 += kPack) {
+            VecPack<T> pk{};
+            pk.m = std::min(kPack, m - j0);
+            for (int j = 0; j < pk.m; ++j) pk.v[j] = Q[sj[j0 + j]];
+            hipLaunchKernelGGL(k_fcg_zq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, gamma, pk, part);
+            fcg_fin_host(c, nb, pk.m, beta.data() + j0);
         }
-        multidot_host<T>(c, z, vq.data(), m, n, beta.data());
-        for (int j = 0; j < m; ++j) beta[j] = -beta[j] / pq_hist[(int)((it - 1 - j) % trunc)];
-        multiaxpy<T>(s, p, z, vp.data(), beta.data(), m, n);
-        multiaxpy<T>(s, qk, az, vq.data(), beta.data(), m, n);
-        const void *ea[2] = {p, p}, *eb[2] = {qk, r};
+        for (int j = 0; j < m; ++j) beta[j] = -beta[j] / pq_hist[sj[j]];
+        // p, q and p.q, p.r, kFcgPack directions per pass
+        const int passes = std::max(1, (m + kFcgPack - 1) / kFcgPack);
+        for (int ps = 0; ps < passes; ++ps) {
+            FcgDirs<T> dd{};
+            const int j0 = ps * kFcgPack;
+            dd.m = std::max(0, std::min(kFcgPack, m - j0));
+            for (int j = 0; j < dd.m; ++j) { dd.p[j] = P[sj[j0 + j]]; dd.q[j] = Q[sj[j0 + j]]; dd.b[j] = beta[j0 + j]; }
+            hipLaunchKernelGGL(k_fcg_pq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const T *)az1,
+                               (const T *)An, gamma, (const T *)r, dd, p, qk, ps == 0 ? 1 : 0,
+                               ps == passes - 1 ? 1 : 0, part);
+        }
         double e2[2];
-        dots_host(c, 2, ea, eb, n, dt, e2);
+        fcg_fin_host(c, nb, 2, e2);
         const double pq = e2[0];
         if (!std::isfinite(pq)) { reason = 2; break; }
         if (!(pq > 0.0)) { reason = 3; break; }
         pq_hist[slot] = pq;
         if (stored < (uint64_t)trunc) ++stored;
         const double alpha = e2[1] / pq;
-        axpby<T>(s, x, alpha, p, 1.0, n);
-        axpby<T>(s, r, -alpha, qk, 1.0, n);
-        rr = std::sqrt(dot(r, r));
+        hipLaunchKernelGGL(k_fcg_xr<T>, grid, block, 0, s, n, alpha, (const T *)p, (const T *)qk, x, r, part);
+        fcg_fin_host(c, nb, 1, &rr);
+        rr = std::sqrt(rr);
         if (!std::isfinite(rr)) { reason = 2; break; }
         if (rr / bnorm < rtol || rr < atol) {
             // The recurrence residual says converged: confirm with b - A x and,
@@ -1671,7 +1826,7 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
         }
         if (rr < best) { best = rr; since_best = 0; }
         else if (++since_best >= patience) { reason = 1; ++it; break; }
-        precond(r, z, az);
+        precond(r);
     }
     info->fcg_iterations = it;
     if (!reason) {

@@ -1014,6 +1014,7 @@ bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, dou
 
 // voxels of the grid the network's input and output live on (the DOF grid)
 long long unet_voxels(const femgpu_unet *u) { return u->cf > 1 ? u->fine.n() : u->d0.n(); }
+bool unet_coarse(const femgpu_unet *u) { return u->cf > 1; }
 femgpu_ctx *unet_ctx(const femgpu_unet *u) { return u->ctx; }
 
 }  // namespace femgpu
