@@ -1891,7 +1891,7 @@ int femgpu_solve_fcg_learned(femgpu_op *op, femgpu_unet *net, const femgpu_cheb_
         FG_HIP(hipSetDevice(op->ctx->device));
         Staged st(op, b, x, mem, 17, 18);
         const int patience = info->patience > 0 ? info->patience : 20;
-        const int trunc = info->truncation > 0 ? (info->truncation < kFcgTruncMax ? info->truncation : kFcgTruncMax) : 16;
+        const int trunc = info->truncation > 0 ? (info->truncation < kFcgTruncMax ? info->truncation : kFcgTruncMax) : 8;
         const int rc = dispatch(op, [&](auto &k) {
             using T = std::remove_pointer_t<decltype(k.vec(0))>;
             ChebParams cp = k.prepare_cheb(cheb);
@@ -1916,7 +1916,7 @@ int femgpu_solve_ir_learned(femgpu_op *op64, femgpu_op *op32, femgpu_unet *net,
                    "solve_ir_learned: network grid and operator size differ");
         FG_REQUIRE(unet_ctx(net) == op32->ctx, FEMGPU_ERR_INVALID_ARG, "solve_ir_learned: different contexts");
         const int patience = info->patience > 0 ? info->patience : 20;
-        const int trunc = info->truncation > 0 ? std::min(info->truncation, kFcgTruncMax) : 16;
+        const int trunc = info->truncation > 0 ? std::min(info->truncation, kFcgTruncMax) : 8;
         femgpu_fcg_info tot = *info;
         tot.fcg_iterations = tot.fallback_iterations = tot.net_corrections = 0;
         tot.fell_back = 0; tot.fallback_reason = 0;

@@ -1064,7 +1064,7 @@ def solve_ir_mixed(op64, op32, b, x, cfg=None, inner_rtol=None, inner_max_iter=0
 
 
 def solve_ir_learned(op64, op32, net, b, x, cfg=None, inner_rtol=1e-4, inner_max_iter=0, max_outer=50,
-                     degree=4, ratio=30.0, safety=1.1, power_iters=20, patience=20, truncation=16,
+                     degree=4, ratio=30.0, safety=1.1, power_iters=20, patience=20, truncation=8,
                      lmax=0.0):
     """FP64 refinement around the FP32 flexible CG with the network; returns
     (SolveResult with outer iterations, total inner iterations, FcgInfo totals)."""
@@ -1223,7 +1223,7 @@ class FcgInfo:
 
 
 def solve_fcg_learned(a, net, b, x, cfg=None, degree=4, ratio=30.0, safety=1.1, power_iters=20,
-                      lmax=0.0, patience=20, truncation=16):
+                      lmax=0.0, patience=20, truncation=8):
     """Flexible CG whose preconditioner is Chebyshev-Jacobi smoothing followed by
     the network's Galerkin-scaled correction of the remaining residual; falls
     back to plain PCG-Chebyshev on stagnation.  Returns (SolveResult, FcgInfo)."""

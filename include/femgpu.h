@@ -521,7 +521,7 @@ int femgpu_unet_load(femgpu_unet *net, const char *path);
  * residual recomputed in the operator's precision. */
 typedef struct {
     int patience;              /* iterations without a new best residual; default 20 */
-    int truncation;            /* directions kept for A-orthogonalisation; default 16, max 64 */
+    int truncation;            /* directions kept for A-orthogonalisation; default 8, max 64 */
     uint64_t fcg_iterations;   /* out: iterations done in FCG */
     uint64_t fallback_iterations; /* out: PCG iterations after a fallback */
     int fell_back;             /* out: 1 if the fallback ran */
