@@ -245,3 +245,42 @@ def test_simplex_coefficient_per_point(fg, fr, ctx, dim, order):
         assert abs((c @ b) @ K.matvec(c @ a) - 1.5 * (a @ b)) < 1e-12
         M = fg.PAMassOperator(dim, 1, coords, conn, ed, nd, rho=lambda p: 1.0 + p[..., 0], quad_order=qo, ctx=ctx)
         assert abs(M.matvec(np.ones(nd)).sum() - 1.5) < 1e-13
+
+
+def test_ir_continued_recovers_from_inner_breakdown(fg, ctx):
+    """Continued refinement whose FP32 PCG breaks down (p^T A p = 0: the FP32
+    operator's closure returns zeros for the fifth search direction) restarts
+    the FP32 recursion from the replaced FP64 residual and still reaches 1e-10;
+    the reported inner count is the iterations that ran (comparable with the
+    run without the breakdown), not the iteration budget."""
+    n = 300
+    rp, ci, v, apply = _lap1d(n)
+    A64 = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    b = np.ones(n)
+    cfg = fg.SolverConfig(rtol=1e-10, max_iter=20000)
+    counts = {}
+    for sabotage in (False, True):
+        calls = []
+
+        def apply32(xx, yy):
+            calls.append(1)
+            if sabotage and len(calls) == 5:
+                yy[:] = 0.0
+                return
+            t = np.empty(n)
+            apply(xx.astype(np.float64), t)
+            yy[:] = t.astype(np.float32)
+        A32 = fg.CallbackOperator(n, apply32, dtype=fg.F32, diag=np.full(n, 2.0), ctx=ctx)
+        x = np.zeros(n)
+        # degree 1: the preconditioner applies no A, so the k-th closure call is
+        # the k-th iteration's A p
+        res, inner = fg.solve_ir_mixed(A64, A32, b, x, cfg, degree=1, lmax=2.0, mode=1, update_interval=0)
+        assert res.converged, (sabotage, res, inner)
+        y = np.empty(n)
+        apply(x, y)
+        assert np.linalg.norm(y - b) / np.linalg.norm(b) < 1e-10
+        counts[sabotage] = inner
+        A32.close()
+    print(f"inner iterations: {counts[False]} without, {counts[True]} with the breakdown")
+    assert counts[True] < 3 * counts[False] + 50
+    assert counts[True] < cfg.max_iter // 4
