@@ -32,6 +32,14 @@ namespace femgpu {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Convolutions with CIN >= 16 take two taps per MFMA with gfx950's
+// v_mfma_f32_16x16x32_bf16 (lane l holds A[l % 16][8 (l / 16) + i] and
+// B[8 (l / 16) + i][l % 16], i < 8; D as for 16x16x16), instead of one tap per
+// v_mfma_f32_16x16x16bf16_1k.
+constexpr bool kConvDoubleK = true;
 
 __host__ __device__ inline uint16_t f2bf(float f) {
     uint32_t u;
@@ -154,7 +162,43 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_
         // ---- MFMA loop over the staged channels, one dz plane (9 taps) at a
         // time; the plane's weights are loaded together before its MFMAs, so
         // their latency is paid 3 times per pass rather than 27
-        if constexpr (CIN >= 16) {
+        if constexpr (CIN >= 16 && kConvDoubleK) {
+            // K = 32 per MFMA: lane group g = lane / 16 takes tap 2 pp + g / 2 of
+            // the plane and staged channels 8 (g % 2) .. + 7; a plane's 9 taps
+            // are 5 pairs, the last one padded with zeros
+            const int g4 = lane >> 4, half = (g4 & 1) * 8, tsel = g4 >> 1;
+            const u32x4 zero4 = {0u, 0u, 0u, 0u};
+            for (int pz = 0; pz < 3; ++pz) {
+                bf16x8 b[5][NTB];
+#pragma unroll
+                for (int pp = 0; pp < 5; ++pp) {
+                    const int t9 = 2 * pp + tsel;
+#pragma unroll
+                    for (int nt = 0; nt < NTB; ++nt) {
+                        u32x4 w = zero4;
+                        if (t9 < 9)
+                            w = *reinterpret_cast<const u32x4 *>(wrow[nt] - ksub + half + (pz * 9 + t9) * CIN + pass * 16);
+                        b[pp][nt] = __builtin_bit_cast(bf16x8, w);
+                    }
+                }
+#pragma unroll
+                for (int pp = 0; pp < 5; ++pp) {
+                    const int t9 = 2 * pp + tsel;
+                    const bool tv = t9 < 9;
+                    const int dx = t9 % 3 - 1, dy = t9 / 3 - 1, dz = pz - 1;
+                    const int base = ((wave + 1 + dz) * kHY + 1 + dy) * kHX + xl + 1 + dx;
+#pragma unroll
+                    for (int mt = 0; mt < kBY; ++mt) {
+                        u32x4 av = zero4;
+                        if (tv) av = *reinterpret_cast<const u32x4 *>(sIn + (size_t)(base + mt * kHX) * CG + half);
+                        const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+                        for (int nt = 0; nt < NTB; ++nt)
+                            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pp][nt], acc[mt][nt], 0, 0, 0);
+                    }
+                }
+            }
+        } else if constexpr (CIN >= 16) {
             for (int pz = 0; pz < 3; ++pz) {
                 bf16x4 b[9][NTB];
 #pragma unroll
