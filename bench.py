@@ -158,6 +158,9 @@ def parse():
     ap.add_argument("--test-fields", type=int, default=32, help="q3-96-learned: held-out fields (seeds 256, ...)")
     ap.add_argument("--epochs", type=int, default=1, help="q3-96-learned: passes over the training fields")
     ap.add_argument("--steps-per-field", type=int, default=4, help="q3-96-learned: training steps per field")
+    ap.add_argument("--unet-passes", type=int, default=0,
+                    help="time only the U-Net: this many forward + backward passes on a --unet-grid^3 grid")
+    ap.add_argument("--unet-grid", type=int, default=129)
     ap.add_argument("--coarsen", type=int, default=0, help="learned: network grid coarsening (0: the workload's, else the smallest "
                     "divisor leaving at most 64 intervals per direction)")
     return ap.parse_args()
@@ -784,6 +787,38 @@ def legs(fg, ctx, K, b, x, lmax, rtol, a):
     return out
 
 
+def run_unet(a):
+    """The learned preconditioner's network alone: --unet-passes forward +
+    backward passes of the 3-level U-Net (16/32/64 channels) on a --unet-grid^3
+    grid after 3 warm-up forwards, timed on the host around synchronised
+    passes.  FEMGPU_CONV_K16=1 selects one tap per MFMA in the convolutions
+    (for comparison with the default two)."""
+    import femgpu as fg
+    ctx = fg.default_context()
+    n = a.unet_grid
+    net = fg.UNet((n, n, n), seed=3, ctx=ctx)
+    x = np.random.default_rng(0).standard_normal(net.V).astype(np.float32)
+    dout = np.random.default_rng(1).standard_normal(net.V).astype(np.float32)
+    for _ in range(3):
+        net.forward(x)
+    ctx.synchronize()
+    tf = tb = 0.0
+    for _ in range(a.unet_passes):
+        t0 = time.perf_counter()
+        net.forward(x)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        net.backward(dout)
+        ctx.synchronize()
+        tf += t1 - t0
+        tb += time.perf_counter() - t1
+    k = max(a.unet_passes, 1)
+    print(json.dumps({"unet_grid": [n, n, n], "passes": a.unet_passes,
+                      "conv_mfma": "16x16x16" if os.environ.get("FEMGPU_CONV_K16") == "1" else "16x16x32",
+                      "forward_ms_incl_host_copies": round(1e3 * tf / k, 3),
+                      "backward_ms_incl_host_copies": round(1e3 * tb / k, 3)}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -792,6 +827,8 @@ def main():
         sys.exit(2)
     import femgpu as fg
 
+    if a.unet_passes > 0:
+        return run_unet(a)
     w = WORKLOADS[a.workload]
     if w["form"] == "csr":
         return run_csr(a, w)

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 
@@ -38,8 +39,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // Convolutions with CIN >= 16 take two taps per MFMA with gfx950's
 // v_mfma_f32_16x16x32_bf16 (lane l holds A[l % 16][8 (l / 16) + i] and
 // B[8 (l / 16) + i][l % 16], i < 8; D as for 16x16x16), instead of one tap per
-// v_mfma_f32_16x16x16bf16_1k.
-constexpr bool kConvDoubleK = true;
+// v_mfma_f32_16x16x16bf16_1k (DK = false; FEMGPU_CONV_K16=1 selects it, for
+// comparison).
 
 __host__ __device__ inline uint16_t f2bf(float f) {
     uint32_t u;
@@ -104,7 +105,7 @@ __global__ void k_split_out_weights(const float *__restrict__ w, uint16_t *__res
 constexpr int kBX = 16, kBY = 4, kBZ = 4;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
 
-template <int CIN, int COUT, int NTB, int OUTF>
+template <int CIN, int COUT, int NTB, int OUTF, bool DK>
 __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_t *__restrict__ in,
                                                   const uint16_t *__restrict__ W,
                                                   const float *__restrict__ bias,
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Vol d, int kpad, const uint16_
         // ---- MFMA loop over the staged channels, one dz plane (9 taps) at a
         // time; the plane's weights are loaded together before its MFMAs, so
         // their latency is paid 3 times per pass rather than 27
-        if constexpr (CIN >= 16 && kConvDoubleK) {
+        if constexpr (CIN >= 16 && DK) {
             // K = 32 per MFMA: lane group g = lane / 16 takes tap 2 pp + g / 2 of
             // the plane and staged channels 8 (g % 2) .. + 7; a plane's 9 taps
             // are 5 pairs, the last one padded with zeros
@@ -845,6 +846,12 @@ static int wgrad_chunks(long long V) {
     return (int)(c < 1 ? 1 : c);
 }
 
+// false when FEMGPU_CONV_K16=1 (read once): one tap per MFMA, for comparison
+static bool conv_double_k() {
+    static const bool dk = !(std::getenv("FEMGPU_CONV_K16") && std::getenv("FEMGPU_CONV_K16")[0] == '1');
+    return dk;
+}
+
 static void conv_fwd(femgpu_unet *u, int l, const uint16_t *in, const uint16_t *W, const float *bias,
                      const uint16_t *skip, void *out, bool relu, bool outf32, int cout_override = 0) {
     const Layer &L = u->L[l];
@@ -858,12 +865,14 @@ static void conv_fwd(femgpu_unet *u, int l, const uint16_t *in, const uint16_t *
     // split the output channels over workgroups at the coarse levels (few tiles)
     const long long tiles = (long long)ntx * ((L.d.ny + kBY - 1) / kBY) * ((L.d.nz + kBZ - 1) / kBZ);
     const bool split = tiles < 2048;
+    const bool dk = conv_double_k();
     (void)V;
 #define CF(CI, CO, NB, OF)                                                                                  \
     do {                                                                                                    \
         const dim3 grid((unsigned)(ntx * (CO / (16 * NB))), (unsigned)((L.d.ny + kBY - 1) / kBY),           \
                         (unsigned)((L.d.nz + kBZ - 1) / kBZ));                                              \
-        hipLaunchKernelGGL((k_conv_fwd<CI, CO, NB, OF>), grid, dim3(256), 0, s, L.d, kpad, in, W, bias, skip, out, r); \
+        if (dk) hipLaunchKernelGGL((k_conv_fwd<CI, CO, NB, OF, true>), grid, dim3(256), 0, s, L.d, kpad, in, W, bias, skip, out, r); \
+        else hipLaunchKernelGGL((k_conv_fwd<CI, CO, NB, OF, false>), grid, dim3(256), 0, s, L.d, kpad, in, W, bias, skip, out, r); \
     } while (0)
 #define CM(CI, CO, OF)                        \
     do {                                      \
@@ -940,10 +949,16 @@ static void forward(femgpu_unet *u) {
     hipLaunchKernelGGL(k_add_bias_f32, dim3(g1(D1.n() * 16)), dim3(256), 0, s, D1.n(), 16, Bf(u, 4), u->t0.as<float>());
     hipLaunchKernelGGL(k_up_add_relu, dim3(g1(D0.n() * 2)), dim3(256), 0, s, D0, D1, 16, (const float *)u->t0.as<float>(),
                        (const uint16_t *)u->e0.as<uint16_t>(), u->d0b.as<uint16_t>());
-    hipLaunchKernelGGL((k_conv_fwd<16, 16, 1, 2>), dim3((unsigned)((D0.nx + kBX - 1) / kBX), (unsigned)((D0.ny + kBY - 1) / kBY),
-                       (unsigned)((D0.nz + kBZ - 1) / kBZ)), dim3(256), 0, s, D0, 432,
-                       (const uint16_t *)u->d0b.as<uint16_t>(), (const uint16_t *)u->wout.as<uint16_t>(), Bf(u, 5),
-                       (const uint16_t *)nullptr, (void *)u->out.as<float>(), 0);
+    const dim3 og((unsigned)((D0.nx + kBX - 1) / kBX), (unsigned)((D0.ny + kBY - 1) / kBY),
+                  (unsigned)((D0.nz + kBZ - 1) / kBZ));
+    if (conv_double_k())
+        hipLaunchKernelGGL((k_conv_fwd<16, 16, 1, 2, true>), og, dim3(256), 0, s, D0, 432,
+                           (const uint16_t *)u->d0b.as<uint16_t>(), (const uint16_t *)u->wout.as<uint16_t>(), Bf(u, 5),
+                           (const uint16_t *)nullptr, (void *)u->out.as<float>(), 0);
+    else
+        hipLaunchKernelGGL((k_conv_fwd<16, 16, 1, 2, false>), og, dim3(256), 0, s, D0, 432,
+                           (const uint16_t *)u->d0b.as<uint16_t>(), (const uint16_t *)u->wout.as<uint16_t>(), Bf(u, 5),
+                           (const uint16_t *)nullptr, (void *)u->out.as<float>(), 0);
 }
 
 // backward from u->gout (dL/dout, f32 [V0]); fills u->grads
