@@ -139,33 +139,45 @@ def forward(params, x0, aux, shape, emulate_bf16=True):
     return S["out"].ravel(), S
 
 
-def backward(params, S, dout):
-    """FP32 parameter gradient of sum(dout * out) given forward state S."""
+def backward(params, S, dout, emulate_bf16=False):
+    """FP32 parameter gradient of sum(dout * out) given forward state S.
+    emulate_bf16: round to bf16 where the GPU's backward does -- each hidden
+    layer's output gradient before its weight gradient, bias gradient and data
+    gradient, and that layer's weights in the data gradient; the output
+    layer's weight gradient takes bf16(dout), its bias and data gradients the
+    f32 dout and weights."""
+    q = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float32))
     Ws = unpack(params)
     D0 = S["e0"].shape[1:]
     g_out = np.asarray(dout, np.float32).reshape((1,) + D0)
     grads = [None] * 6
     dx, dw, db = fr.conv3d_bwd(S["d0"], Ws[5][0], g_out)
+    if emulate_bf16:
+        _, dw, _ = fr.conv3d_bwd(S["d0"], Ws[5][0], q(g_out))
     grads[5] = (dw, db)
+
+    def layer(x, l, g, need_dx=True):
+        gq = q(g)
+        return fr.conv3d_bwd(x, q(Ws[l][0]), gq, need_dx=need_dx)
     g = dx * (S["d0"] > 0)
     ge0 = g.copy()
     gt0 = sum_children(g, S["t0"].shape[1:])
-    dx, dw, db = fr.conv3d_bwd(S["d1"], Ws[4][0], gt0)
+    dx, dw, db = layer(S["d1"], 4, gt0)
     grads[4] = (dw, db)
     g1 = dx * (S["d1"] > 0)
     ge1 = g1.copy()
     gt1 = sum_children(g1, S["t1"].shape[1:])
-    dx, dw, db = fr.conv3d_bwd(S["e2"], Ws[3][0], gt1)
+    dx, dw, db = layer(S["e2"], 3, gt1)
     grads[3] = (dw, db)
     g2 = dx * (S["e2"] > 0)
-    dx, dw, db = fr.conv3d_bwd(S["p1"], Ws[2][0], g2)
+    dx, dw, db = layer(S["p1"], 2, g2)
     grads[2] = (dw, db)
     ge1 += unpool(dx, S["e1"].shape)
     g1b = ge1 * (S["e1"] > 0)
-    dx, dw, db = fr.conv3d_bwd(S["p0"], Ws[1][0], g1b)
+    dx, dw, db = layer(S["p0"], 1, g1b)
     grads[1] = (dw, db)
     ge0 += unpool(dx, S["e0"].shape)
     g0 = ge0 * (S["e0"] > 0)
-    _, dw, db = fr.conv3d_bwd(S["in"], Ws[0][0], g0, need_dx=False)
+    _, dw, db = layer(S["in"], 0, g0, need_dx=False)
     grads[0] = (dw, db)
     return pack_grad(grads)

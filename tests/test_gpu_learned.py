@@ -56,13 +56,20 @@ def test_unet_backward_matches_cpu(fg, small):
     net.forward(x0)
     g = net.backward(dout)
     _, S = U.forward(p, x0, aux, shape, emulate_bf16=True)
-    gr = U.backward(p, S, dout)
+    # the restatement rounds to bf16 where the GPU's backward does, so what is
+    # left is accumulation order and the few ReLU masks that it flips: the
+    # forward pass's 1e-2 bar; pure FP32 gradients stay within the BF16 band
+    gr = U.backward(p, S, dout, emulate_bf16=True)
+    gr32 = U.backward(p, S, dout)
     lay, _ = U.layout()
     for l, L in enumerate(lay):
         w = slice(L["woff"], L["woff"] + L["cout"] * L["kpad"])
         b = slice(L["boff"], L["boff"] + L["cout"])
-        assert _rel(g[w], gr[w]) < 5e-2, ("W", l, _rel(g[w], gr[w]))
-        assert _rel(g[b], gr[b]) < 5e-2, ("b", l, _rel(g[b], gr[b]))
+        print(f"layer {l}: dW {_rel(g[w], gr[w]):.2e} (FP32 {_rel(g[w], gr32[w]):.2e}), "
+              f"db {_rel(g[b], gr[b]):.2e} (FP32 {_rel(g[b], gr32[b]):.2e})")
+        assert _rel(g[w], gr[w]) < 1e-2, ("W", l, _rel(g[w], gr[w]))
+        assert _rel(g[b], gr[b]) < 1e-2, ("b", l, _rel(g[b], gr[b]))
+        assert _rel(g[w], gr32[w]) < 5e-2 and _rel(g[b], gr32[b]) < 5e-2
 
 
 def test_unet_train_and_checkpoint(fg, ctx, tmp_path):

@@ -31,6 +31,14 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def _iters_close(i, j):
+    """Iteration counts of two solves whose operators differ only in rounding
+    (1e-13): the stopping test can fall a few iterations apart when the
+    residual curve is flat near the tolerance, so the bar is 3 iterations or
+    2%; the solutions themselves are compared to 1e-8."""
+    return abs(i - j) <= max(3, 0.02 * max(i, j))
+
+
 @pytest.fixture(params=["default", "general"])
 def qdata(request, monkeypatch):
     """Runs a test with the operators' default quadrature data (the geometry
@@ -348,7 +356,7 @@ def test_hex_q3_quad_per_element(fg, fr, ctx, monkeypatch, shape):
             xk, xl = np.zeros(K.n_dofs()), np.zeros(K.n_dofs())
             rk, _ = fg.solve_pcg_chebyshev(K, b, xk, fg.SolverConfig(rtol=1e-10, max_iter=2000))
             rl, _ = fg.solve_pcg_chebyshev(L, b, xl, fg.SolverConfig(rtol=1e-10, max_iter=2000))
-            assert rk.converged and rl.converged and abs(rk.iterations - rl.iterations) <= 1
+            assert rk.converged and rl.converged and _iters_close(rk.iterations, rl.iterations)
             assert np.abs(xk - xl).max() <= 1e-8 * np.abs(xl).max()
         for op in (K, L, K32, L32):
             op.close()
@@ -387,7 +395,7 @@ def test_hex_q2_thread_per_element(fg, fr, ctx, monkeypatch, shape):
                 xk, xl = np.zeros(K.n_dofs()), np.zeros(K.n_dofs())
                 rk, _ = fg.solve_pcg_chebyshev(K, b, xk, fg.SolverConfig(rtol=1e-10, max_iter=2000))
                 rl, _ = fg.solve_pcg_chebyshev(L, b, xl, fg.SolverConfig(rtol=1e-10, max_iter=2000))
-                assert rk.converged and rl.converged and abs(rk.iterations - rl.iterations) <= 1
+                assert rk.converged and rl.converged and _iters_close(rk.iterations, rl.iterations)
                 assert np.abs(xk - xl).max() <= 1e-8 * np.abs(xl).max()
             for op in (K, L, K32, L32):
                 op.close()
