@@ -284,14 +284,19 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const T *Gt = Tab<T>::G(C);
     const int le = threadIdx.x / T2;
     const int t = threadIdx.x - le * T2;
-    const long long e = (long long)lb * NE + le;
+    // E < 2^31 (checked at creation), so 32-bit unsigned division is exact
+    // and avoids the 64-bit division sequence
+    const unsigned eu = lb * (unsigned)NE + (unsigned)le;
+    const long long e = (long long)eu;
     const bool valid = e < g.E;
     int ex = 0, ey = 0, ez = 0;
     if (valid) {
-        ex = (int)(e % g.nx);
-        const long long r = e / g.nx;
-        ey = (int)(r % g.ny);
-        ez = (int)(r / g.ny);
+        const unsigned nxu = (unsigned)g.nx, nyu = (unsigned)g.ny;
+        const unsigned r = eu / nxu;
+        ex = (int)(eu - r * nxu);
+        const unsigned r2 = r / nyu;
+        ey = (int)(r - r2 * nyu);
+        ez = (int)r2;
     }
 
     if constexpr (GEO) {
@@ -578,11 +583,13 @@ __global__ __launch_bounds__(256) void k_pa_hex_tpe(const HexGrid g, const T *__
     const bool valid = e < g.E;
     const int lane = (int)(threadIdx.x & 63);
     int ex = 0, ey = 0, ez = 0;
-    if (valid) {
-        ex = (int)(e % g.nx);
-        const long long r = e / g.nx;
-        ey = (int)(r % g.ny);
-        ez = (int)(r / g.ny);
+    if (valid) {  // E < 2^31: 32-bit unsigned division is exact
+        const unsigned eu = (unsigned)e, nxu = (unsigned)g.nx, nyu = (unsigned)g.ny;
+        const unsigned r = eu / nxu;
+        ex = (int)(eu - r * nxu);
+        const unsigned r2 = r / nyu;
+        ey = (int)(r - r2 * nyu);
+        ez = (int)r2;
     }
     T yv[N1][N1][N1];  // [k][j][i]
     if (valid) {
@@ -818,11 +825,13 @@ __global__ __launch_bounds__(256) void k_pa_hex_quad(const HexGrid g, const T *_
     const int lane = (int)(threadIdx.x & 63);
     const bool valid = e < g.E;  // uniform over the quad
     int ex = 0, ey = 0, ez = 0;
-    if (valid) {
-        ex = (int)(e % g.nx);
-        const long long r = e / g.nx;
-        ey = (int)(r % g.ny);
-        ez = (int)(r / g.ny);
+    if (valid) {  // E < 2^31: 32-bit unsigned division is exact
+        const unsigned eu = (unsigned)e, nxu = (unsigned)g.nx, nyu = (unsigned)g.ny;
+        const unsigned r = eu / nxu;
+        ex = (int)(eu - r * nxu);
+        const unsigned r2 = r / nyu;
+        ey = (int)(r - r2 * nyu);
+        ez = (int)r2;
     }
     T yv[N1][N1];  // [j][i] of layer k = L
     if (valid) {
