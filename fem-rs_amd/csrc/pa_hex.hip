@@ -1076,15 +1076,15 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
 // Gather fused with the Chebyshev step that consumes it (the solver's k_chebk):
 // t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z
 // over the fixed red_blocks(n) x 256 grid-stride grid of segments.
-template <class T, int P, class I>
-__global__ __launch_bounds__(256) void k_hex_gather_cheb(
+template <class T, int P, class I, int BS>
+__global__ __launch_bounds__(BS) void k_hex_gather_cheb(
     const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
     const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2, int last,
     double *__restrict__ part, int dirichlet, const int *skip) {
     if (skip && *skip) return;
     const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
     double v[1] = {0.0};
-    for (size_t sid = blockIdx.x * (size_t)256 + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * 256) {
+    for (size_t sid = blockIdx.x * (size_t)BS + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * BS) {
         int ex, gy, gz;
         seg_coords(g, sid, ex, gy, gz);
         const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
@@ -1103,22 +1103,22 @@ __global__ __launch_bounds__(256) void k_hex_gather_cheb(
         }
     }
     if (last) {
-        block_sum<1, 256>(v);
+        block_sum<1, BS>(v);
         if (threadIdx.x == 0) part[blockIdx.x] = v[0];
     }
 }
 
 // Gather fused with the dot product that follows it (the solver's k_dot):
 // y = A x and partials of x.y in slot 0, on the solver's reduction grid.
-template <class T, int P, class I>
-__global__ __launch_bounds__(256) void k_hex_gather_dot(const HexGrid g, const T *__restrict__ ye,
+template <class T, int P, class I, int BS>
+__global__ __launch_bounds__(BS) void k_hex_gather_dot(const HexGrid g, const T *__restrict__ ye,
                                                         const T *__restrict__ x, T *__restrict__ y,
                                                         double *__restrict__ part, int dirichlet,
                                                         const int *skip) {
     if (skip && *skip) return;
     const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
     double v[1] = {0.0};
-    for (size_t sid = blockIdx.x * (size_t)256 + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * 256) {
+    for (size_t sid = blockIdx.x * (size_t)BS + threadIdx.x; sid < nseg; sid += (size_t)gridDim.x * BS) {
         int ex, gy, gz;
         seg_coords(g, sid, ex, gy, gz);
         const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_hex_gather_dot(const HexGrid g, const T
             v[0] += (double)x[base + i] * (double)t[i];
         }
     }
-    block_sum<1, 256>(v);
+    block_sum<1, BS>(v);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
@@ -1481,8 +1481,19 @@ struct HexOp final : femgpu_op {
         const int dir = desc.dirichlet ? 1 : 0;
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
                      skip, s, tpe, geo, (T)desc.rho);
-        const dim3 grid((unsigned)red_blocks(n)), block(256);
+        const dim3 grid((unsigned)red_blocks(n));
         const T *ye = evec.as<T>();
+        // Threads per block on the fixed red_blocks(n) grid (the reduction
+        // partials stay one per block): 1024 from 2^24 segments on (Q4 128^3:
+        // 3.5% faster per PCG-Chebyshev iteration than 256, more waves in
+        // flight for the dependent loads), 256 below (Q2 64^3: 1% faster);
+        // FEMGPU_GATHER_BS=256/512/1024 overrides, for comparison
+        static const int gbs_env = [] {
+            const char *e = std::getenv("FEMGPU_GATHER_BS");
+            return e ? std::atoi(e) : 0;
+        }();
+        const double nseg = (double)g.nx * (double)g.Ny * (double)g.Nz;
+        const int gbs = gbs_env ? gbs_env : (nseg >= 16777216.0 ? 1024 : 256);
 #define GF(P_)                                                                                              \
     do {                                                                                                    \
         if (small) GF2(P_, uint32_t);                                                                       \
@@ -1490,8 +1501,15 @@ struct HexOp final : femgpu_op {
     } while (0)
 #define GF2(P_, I_)                                                                                         \
     do {                                                                                                    \
-        if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_, I_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
-        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
+        if (gbs == 512) GF3(P_, I_, 512);                                                                  \
+        else if (gbs == 1024) GF3(P_, I_, 1024);                                                           \
+        else GF3(P_, I_, 256);                                                                             \
+    } while (0)
+#define GF3(P_, I_, BS_)                                                                                    \
+    do {                                                                                                    \
+        const dim3 block(BS_);                                                                              \
+        if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_, I_, BS_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
+        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
                                 last, part, dir, skip);                                                     \
     } while (0)
         const bool small = evec_small(g, desc.p + 1);
@@ -1502,6 +1520,7 @@ struct HexOp final : femgpu_op {
         case 4: GF(4); break;
         default: GF(5); break;
         }
+#undef GF3
 #undef GF2
 #undef GF
     }
