@@ -151,7 +151,7 @@ def parse():
     ap.add_argument("--tight-rtol", type=float, default=1e-12,
                     help="legs: tolerance of the continued solves (error target 1e-10)")
     ap.add_argument("--ref-rtol", type=float, default=1e-13, help="legs: tolerance of the FP64 reference")
-    ap.add_argument("--train-steps", type=int, default=600)
+    ap.add_argument("--train-steps", type=int, default=1200)
     ap.add_argument("--truncation", type=int, default=8, help="flexible CG: directions kept")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--train-fields", type=int, default=256, help="q3-96-learned: training fields (seeds 0, 1, ...)")
