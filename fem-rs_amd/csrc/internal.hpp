@@ -257,6 +257,8 @@ bool hex_dirichlet(femgpu_op *op);
 
 // learned preconditioner (unet.hip): z = |r| N(r/|r|) on device vectors
 bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm);
+// the coarse placement's correction with |R r|^2 kept in *nrm2 on the device (no read-back)
+void unet_apply_coarse_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double *nrm2);
 long long unet_voxels(const femgpu_unet *u);
 // true when the network runs on a coarsened copy of the DOF grid
 bool unet_coarse(const femgpu_unet *u);

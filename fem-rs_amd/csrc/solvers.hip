@@ -1569,12 +1569,27 @@ namespace femgpu {
 constexpr int kFcgTruncMax = 64;
 
 // ---- fused flexible-CG vector kernels (partials on the fixed red_blocks grid) ----
+// The solver's scalars live in a small device state (FcgState slots); the
+// one-block kernels k_fcgd_* finalise the partials into it, and the vector
+// kernels read gamma, beta and alpha from it, so an iteration reads back one
+// status record instead of every dot product.
+enum FcgState {
+    F_GAMMA = 0, F_ALPHA, F_RR, F_BEST, F_SINCE, F_STATUS, F_NET, F_RC2,
+    F_BETA = 8,                  // kFcgTruncMax coefficients
+    F_PQ = F_BETA + kFcgTruncMax,  // p_j.q_j by history slot
+    F_COUNT = F_PQ + kFcgTruncMax
+};
+enum FcgStatus { FS_RUN = 0, FS_CONV = 1, FS_NONFINITE = 2, FS_PQ = 3, FS_STAG = 4 };
+
 constexpr int kFcgPack = 16;  // directions per k_fcg_pq pass
 template <class T> struct FcgDirs {
     const T *p[kFcgPack];
     const T *q[kFcgPack];
-    double b[kFcgPack];
-    int m;
+    int m, j0;  // directions in this pass, index of the first one's coefficient
+};
+struct FcgSlots {
+    int s[kPack];  // history slots of the pack's directions
+    int m, j0;
 };
 
 // r1 = r - az1; partials of r1.r1
@@ -1605,10 +1620,13 @@ __global__ __launch_bounds__(256) void k_fcg_dots2(size_t n, const T *__restrict
     if (threadIdx.x == 0) { part[blockIdx.x] = v[0]; part[kRedBlocks + blockIdx.x] = v[1]; }
 }
 
-// partials of (z1 + g nn).q_j, j < m <= kPack (row j); nn unused when g == 0
+// partials of (z1 + g nn).q_j, j < m <= kPack (row j), g = st[F_GAMMA]; nn
+// unused when g == 0
 template <class T>
 __global__ __launch_bounds__(256) void k_fcg_zq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
-                                                double g, VecPack<T> qs, double *__restrict__ part) {
+                                                const double *__restrict__ st, VecPack<T> qs,
+                                                double *__restrict__ part) {
+    const double g = st[F_GAMMA];
     double v[kPack];
 #pragma unroll
     for (int j = 0; j < kPack; ++j) v[j] = 0.0;
@@ -1623,20 +1641,25 @@ __global__ __launch_bounds__(256) void k_fcg_zq(size_t n, const T *__restrict__ 
         for (int j = 0; j < qs.m; ++j) part[(size_t)j * kRedBlocks + blockIdx.x] = v[j];
 }
 
-// p = base_p + sum_j b_j p_j, q = base_q + sum_j b_j q_j with base (z1 + g nn,
-// az1 + g An) on the first pass and (p, q) after; on the last pass partials of
-// p.q (row 0) and p.r (row 1)
+// p = base_p + sum_j beta_j p_j, q = base_q + sum_j beta_j q_j with base
+// (z1 + g nn, az1 + g An) on the first pass and (p, q) after; on the last pass
+// partials of p.q (row 0) and p.r (row 1)
 template <class T>
 __global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
-                                                const T *__restrict__ az1, const T *__restrict__ An, double g,
-                                                const T *__restrict__ r, FcgDirs<T> d, T *p, T *q, int first,
-                                                int last, double *__restrict__ part) {
+                                                const T *__restrict__ az1, const T *__restrict__ An,
+                                                const double *__restrict__ st, const T *__restrict__ r,
+                                                FcgDirs<T> d, T *p, T *q, int first, int last,
+                                                double *__restrict__ part) {
+    const double g = st[F_GAMMA];
+    double b[kFcgPack];
+#pragma unroll
+    for (int j = 0; j < kFcgPack; ++j) b[j] = j < d.m ? st[F_BETA + d.j0 + j] : 0.0;
     double v[2] = {0.0, 0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         double sp, sq;
         if (first) {
-            // the preconditioned residual z and A z, rounded to T as the
-            // unfused form stored them
+            // the preconditioned residual z and A z, rounded to T as a stored
+            // z would be
             sp = g != 0.0 ? (double)(T)((double)z1[i] + g * (double)nn[i]) : (double)z1[i];
             sq = g != 0.0 ? (double)(T)((double)az1[i] + g * (double)An[i]) : (double)az1[i];
         } else {
@@ -1646,8 +1669,8 @@ __global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ 
 #pragma unroll
         for (int j = 0; j < kFcgPack; ++j)
             if (j < d.m) {
-                sp += d.b[j] * (double)d.p[j][i];
-                sq += d.b[j] * (double)d.q[j][i];
+                sp += b[j] * (double)d.p[j][i];
+                sq += b[j] * (double)d.q[j][i];
             }
         const T pt = (T)sp, qt = (T)sq;
         p[i] = pt;
@@ -1663,10 +1686,14 @@ __global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ 
     }
 }
 
-// x += a p, r -= a q; partials of r.r
+// x += alpha p, r -= alpha q (alpha = st[F_ALPHA]); partials of r.r.  Nothing
+// when the iteration has already stopped (p.q <= 0 or not finite).
 template <class T>
-__global__ __launch_bounds__(256) void k_fcg_xr(size_t n, double a, const T *__restrict__ p, const T *__restrict__ q,
-                                                T *__restrict__ x, T *__restrict__ r, double *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_fcg_xr(size_t n, const double *__restrict__ st, const T *__restrict__ p,
+                                                const T *__restrict__ q, T *__restrict__ x, T *__restrict__ r,
+                                                double *__restrict__ part) {
+    if (st[F_STATUS] != FS_RUN) return;
+    const double a = st[F_ALPHA];
     double v[1] = {0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         x[i] = (T)((double)x[i] + a * (double)p[i]);
@@ -1699,13 +1726,75 @@ static void fcg_fin_host(femgpu_ctx *c, int nb, int m, double *out) {
     std::memcpy(out, c->pinned + kMultiDotBase, sizeof(double) * m);
 }
 
+// ---- one-block finalisers into the state (fixed summation order) ----
+__global__ __launch_bounds__(1024) void k_fcgd_init(double *st, double best) {
+    for (int i = threadIdx.x; i < F_COUNT; i += 1024) st[i] = i >= F_PQ ? 1.0 : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) st[F_BEST] = best;
+}
+// after the true-residual check of a recurrence that claimed convergence
+__global__ void k_fcgd_resume(double *st, double rr, double best, double since) {
+    st[F_RR] = rr; st[F_BEST] = best; st[F_SINCE] = since; st[F_STATUS] = FS_RUN;
+}
+__global__ void k_fcgd_no_net(double *st) { st[F_GAMMA] = 0.0; }
+// gamma = n.r1 / n.An, or 0 when that is unusable (or the network's input
+// scale |R r1|^2 in st[F_RC2] is, with use_rc2)
+__global__ __launch_bounds__(1024) void k_fcgd_gamma(const double *part, int nb, double *st, int use_rc2) {
+    __shared__ double o[2];
+    fin_sum(part, nb, o, 2);
+    if (threadIdx.x) return;
+    const double nAn = o[0], nr = o[1], rc2 = st[F_RC2];
+    const bool ok = nAn > 0.0 && isfinite(nAn) && isfinite(nr) && (!use_rc2 || (rc2 > 0.0 && isfinite(rc2)));
+    st[F_GAMMA] = ok ? nr / nAn : 0.0;
+    if (ok) st[F_NET] += 1.0;
+}
+// beta[j0 + j] = -(z.q_j) / (p_j.q_j), j < m
+__global__ __launch_bounds__(1024) void k_fcgd_beta(const double *part, int nb, FcgSlots sl, double *st) {
+    __shared__ double o[kPack];
+    fin_sum(part, nb, o, sl.m);
+    if (threadIdx.x) return;
+    for (int j = 0; j < sl.m; ++j) st[F_BETA + sl.j0 + j] = -o[j] / st[F_PQ + sl.s[j]];
+}
+// alpha = p.r / p.q and p.q into the history slot; the stop codes of a
+// non-finite or non-positive p.q
+__global__ __launch_bounds__(1024) void k_fcgd_alpha(const double *part, int nb, int slot, double *st) {
+    __shared__ double o[2];
+    fin_sum(part, nb, o, 2);
+    if (threadIdx.x) return;
+    const double pq = o[0], pr = o[1];
+    if (!isfinite(pq)) { st[F_STATUS] = FS_NONFINITE; return; }
+    if (!(pq > 0.0)) { st[F_STATUS] = FS_PQ; return; }
+    st[F_PQ + slot] = pq;
+    st[F_ALPHA] = pr / pq;
+}
+// |r|, the stopping test on the recurrence residual and the stagnation count
+__global__ __launch_bounds__(1024) void k_fcgd_conv(const double *part, int nb, double *st, double bnorm,
+                                                    double rtol, double atol, int patience) {
+    __shared__ double o[1];
+    fin_sum(part, nb, o, 1);
+    if (threadIdx.x) return;
+    if (st[F_STATUS] != FS_RUN) return;
+    const double rr = sqrt(o[0]);
+    st[F_RR] = rr;
+    if (!isfinite(rr)) { st[F_STATUS] = FS_NONFINITE; return; }
+    if (rr / bnorm < rtol || rr < atol) { st[F_STATUS] = FS_CONV; return; }
+    if (rr < st[F_BEST]) {
+        st[F_BEST] = rr;
+        st[F_SINCE] = 0.0;
+    } else {
+        st[F_SINCE] += 1.0;
+        if (st[F_SINCE] >= patience) st[F_STATUS] = FS_STAG;
+    }
+}
+
 // Flexible CG (Notay, SIAM J. Sci. Comput. 22 (2000)) truncated to the last
 // `trunc` directions (A-orthogonalisation window), with a two-level
 // preconditioner: Chebyshev-Jacobi smoothing z1 = C r, then the network's
 // correction of the smooth residual r1 = r - A z1 that the smoother leaves,
 // scaled by its Galerkin step (the network's output scale does not matter).
 // z = z1 + gamma n and A z = A z1 + gamma A n are never stored: the fused
-// kernels above read their parts.
+// kernels above read their parts.  One status read-back per iteration (two
+// with a network on the full DOF grid, which needs |r1| on the host).
 template <class T>
 static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, const T *b, T *x,
                        double rtol, double atol, uint64_t max_iter, int patience, int trunc,
@@ -1717,13 +1806,19 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     T *r = K.vec(K.W_R), *tmp = K.vec(K.W_AP);
     T *nn = K.vec(12), *An = K.vec(13), *r1 = K.vec(14), *z1 = K.vec(15), *az1 = K.vec(16);
     std::vector<T *> P(trunc), Q(trunc);
-    std::vector<double> pq_hist(trunc, 1.0), beta(trunc);
     for (int j = 0; j < trunc; ++j) { P[j] = K.vec(20 + 2 * j); Q[j] = K.vec(21 + 2 * j); }
     const T *inv = K.vec(K.W_INV);
     const int nb = red_blocks(n);
     const dim3 grid((unsigned)nb), block(256);
     double *part = c->partials.as<double>();
     const bool coarse = unet_coarse(net);
+    DevBuf stb(sizeof(double) * F_COUNT);
+    double *st = stb.as<double>();
+    double *hst = c->pinned + kMultiDotBase;  // read-back of the first 8 state slots
+    auto read_state = [&]() {
+        FG_HIP(hipMemcpyAsync(hst, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+        FG_HIP(hipStreamSynchronize(s));
+    };
     auto dot = [&](const T *a, const T *bb) { return dot_host(c, a, bb, n, dt); };
     FG_HIP(hipMemsetAsync(K.flags, 0, sizeof(int), s));
     // Two-level preconditioner, z = B r, together with A z:
@@ -1732,26 +1827,23 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     //   n  = N(r1), gamma = n.r1 / n.An   (network correction, Galerkin-scaled)
     //   z  = z1 + gamma n,    A z = A z1 + gamma A n
     // Without a usable network output gamma = 0, z = C r.
-    double gamma = 0.0;
     auto precond = [&](const T *rin) {
-        gamma = 0.0;
         K.cheb_apply(cp, rin, inv, z1);
         K.apply(z1, az1, false);
         hipLaunchKernelGGL(k_fcg_resid<T>, grid, block, 0, s, n, rin, (const T *)az1, r1, part);
-        double rn = 1.0;  // the coarse placement scales by |R r1| itself
-        if (!coarse) {
+        if (coarse) {
+            unet_apply_coarse_dev(net, r1, nn, dt, st + F_RC2);  // the scale stays on the device
+        } else {
+            double rn;
             fcg_fin_host(c, nb, 1, &rn);
-            rn = std::sqrt(rn);
+            if (!unet_apply_dev(net, r1, nn, dt, std::sqrt(rn))) {
+                hipLaunchKernelGGL(k_fcgd_no_net, dim3(1), dim3(1), 0, s, st);
+                return;
+            }
         }
-        if (!unet_apply_dev(net, r1, nn, dt, rn)) return;
         K.apply(nn, An, false);
         hipLaunchKernelGGL(k_fcg_dots2<T>, grid, block, 0, s, n, (const T *)nn, (const T *)An, (const T *)r1, part);
-        double d[2];
-        fcg_fin_host(c, nb, 2, d);
-        const double nAn = d[0], nr = d[1];
-        if (!(nAn > 0.0) || !std::isfinite(nAn) || !std::isfinite(nr)) return;
-        gamma = nr / nAn;
-        ++info->net_corrections;
+        hipLaunchKernelGGL(k_fcgd_gamma, dim3(1), dim3(1024), 0, s, (const double *)part, nb, st, coarse ? 1 : 0);
     };
     info->fcg_iterations = 0; info->fallback_iterations = 0; info->fell_back = 0; info->fallback_reason = 0;
     info->net_corrections = 0;
@@ -1769,8 +1861,12 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
         res->converged = 1; res->iterations = 0; res->final_residual = 0.0;
         return FEMGPU_OK;
     }
-    double best = rr;
-    int since_best = 0, reason = 0, replacements = 0;
+    hipLaunchKernelGGL(k_fcgd_init, dim3(1), dim3(1024), 0, s, st, rr);
+    int reason = 0, replacements = 0;
+    auto finish_info = [&]() {
+        read_state();
+        info->net_corrections = (uint64_t)hst[F_NET];
+    };
     precond(r);
     uint64_t it = 0, stored = 0;  // stored: directions kept so far (<= trunc)
     for (; it < max_iter; ++it) {
@@ -1781,54 +1877,58 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
         const int m = (int)std::min<uint64_t>(stored, (uint64_t)trunc - 1);
         std::vector<int> sj(m);
         for (int j = 0; j < m; ++j) sj[j] = (int)((it - 1 - j) % trunc);
-        // z.q_j, kPack directions per pass
-        for (int j0 = 0; j0 < m; j0 += kPack) {
+        for (int j0 = 0; j0 < m; j0 += kPack) {  // z.q_j, kPack directions per pass
             VecPack<T> pk{};
-            pk.m = std::min(kPack, m - j0);
-            for (int j = 0; j < pk.m; ++j) pk.v[j] = Q[sj[j0 + j]];
-            hipLaunchKernelGGL(k_fcg_zq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, gamma, pk, part);
-            fcg_fin_host(c, nb, pk.m, beta.data() + j0);
+            FcgSlots sl{};
+            pk.m = sl.m = std::min(kPack, m - j0);
+            sl.j0 = j0;
+            for (int j = 0; j < pk.m; ++j) { pk.v[j] = Q[sj[j0 + j]]; sl.s[j] = sj[j0 + j]; }
+            hipLaunchKernelGGL(k_fcg_zq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const double *)st,
+                               pk, part);
+            hipLaunchKernelGGL(k_fcgd_beta, dim3(1), dim3(1024), 0, s, (const double *)part, nb, sl, st);
         }
-        for (int j = 0; j < m; ++j) beta[j] = -beta[j] / pq_hist[sj[j]];
-        // p, q and p.q, p.r, kFcgPack directions per pass
         const int passes = std::max(1, (m + kFcgPack - 1) / kFcgPack);
-        for (int ps = 0; ps < passes; ++ps) {
+        for (int ps = 0; ps < passes; ++ps) {  // p, q and p.q, p.r, kFcgPack directions per pass
             FcgDirs<T> dd{};
-            const int j0 = ps * kFcgPack;
-            dd.m = std::max(0, std::min(kFcgPack, m - j0));
-            for (int j = 0; j < dd.m; ++j) { dd.p[j] = P[sj[j0 + j]]; dd.q[j] = Q[sj[j0 + j]]; dd.b[j] = beta[j0 + j]; }
+            dd.j0 = ps * kFcgPack;
+            dd.m = std::max(0, std::min(kFcgPack, m - dd.j0));
+            for (int j = 0; j < dd.m; ++j) { dd.p[j] = P[sj[dd.j0 + j]]; dd.q[j] = Q[sj[dd.j0 + j]]; }
             hipLaunchKernelGGL(k_fcg_pq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const T *)az1,
-                               (const T *)An, gamma, (const T *)r, dd, p, qk, ps == 0 ? 1 : 0,
+                               (const T *)An, (const double *)st, (const T *)r, dd, p, qk, ps == 0 ? 1 : 0,
                                ps == passes - 1 ? 1 : 0, part);
         }
-        double e2[2];
-        fcg_fin_host(c, nb, 2, e2);
-        const double pq = e2[0];
-        if (!std::isfinite(pq)) { reason = 2; break; }
-        if (!(pq > 0.0)) { reason = 3; break; }
-        pq_hist[slot] = pq;
+        hipLaunchKernelGGL(k_fcgd_alpha, dim3(1), dim3(1024), 0, s, (const double *)part, nb, slot, st);
+        hipLaunchKernelGGL(k_fcg_xr<T>, grid, block, 0, s, n, (const double *)st, (const T *)p, (const T *)qk, x, r,
+                           part);
+        hipLaunchKernelGGL(k_fcgd_conv, dim3(1), dim3(1024), 0, s, (const double *)part, nb, st, bnorm, rtol, atol,
+                           patience);
+        read_state();
+        const int status = (int)hst[F_STATUS];
+        rr = hst[F_RR];
+        if (status == FS_NONFINITE) { reason = 2; break; }
+        if (status == FS_PQ) { reason = 3; break; }
         if (stored < (uint64_t)trunc) ++stored;
-        const double alpha = e2[1] / pq;
-        hipLaunchKernelGGL(k_fcg_xr<T>, grid, block, 0, s, n, alpha, (const T *)p, (const T *)qk, x, r, part);
-        fcg_fin_host(c, nb, 1, &rr);
-        rr = std::sqrt(rr);
-        if (!std::isfinite(rr)) { reason = 2; break; }
-        if (rr / bnorm < rtol || rr < atol) {
+        if (status == FS_STAG) { reason = 1; ++it; break; }
+        if (status == FS_CONV) {
             // The recurrence residual says converged: confirm with b - A x and,
             // if that is not yet below the tolerance, continue from it.
             rr = true_residual();
             if (rr / bnorm < rtol || rr < atol) {
                 info->fcg_iterations = it + 1;
+                finish_info();
                 res->converged = 1; res->iterations = it + 1; res->final_residual = rr / bnorm;
                 return FEMGPU_OK;
             }
             if (++replacements > 8) { reason = 1; ++it; break; }
+            double best = hst[F_BEST], since = hst[F_SINCE];
+            if (rr < best) { best = rr; since = 0.0; }
+            else if ((since += 1.0) >= patience) { reason = 1; ++it; break; }
+            hipLaunchKernelGGL(k_fcgd_resume, dim3(1), dim3(1), 0, s, st, rr, best, since);
         }
-        if (rr < best) { best = rr; since_best = 0; }
-        else if (++since_best >= patience) { reason = 1; ++it; break; }
         precond(r);
     }
     info->fcg_iterations = it;
+    finish_info();
     if (!reason) {
         res->converged = 0; res->iterations = it; res->final_residual = rr / bnorm;
         return FEMGPU_ERR_NOT_CONVERGED;

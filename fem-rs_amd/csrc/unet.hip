@@ -603,7 +603,13 @@ __global__ void k_flip_weights(int cin, int cout, int kpad, int kpadT, const uin
 // network input: in[v] = (r[v] * scale, aux[v], 0, 0)
 template <class T>
 __global__ void k_make_input(long long V, const T *__restrict__ r, double scale,
-                             const float *__restrict__ aux, uint16_t *__restrict__ in) {
+                             const float *__restrict__ aux, uint16_t *__restrict__ in,
+                             const double *nrm2 = nullptr) {
+    // nrm2: |r|^2 on the device; the scale is then sqrt(V / |r|^2) (0 when unusable)
+    if (nrm2) {
+        const double q = *nrm2;
+        scale = q > 0.0 && isfinite(q) ? sqrt((double)V / q) : 0.0;
+    }
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < V;
          v += (long long)gridDim.x * blockDim.x) {
         in[v * 4 + 0] = f2bf((float)((double)r[v] * scale));
@@ -724,7 +730,12 @@ __global__ void k_restrict_axis(long long outer, int nf, int nc, long long inner
 // (bmask: 0 on the boundary of the DOF grid, where every DOF is essential)
 template <class T>
 __global__ void k_prolong(Vol f, Vol c, int cf, const float *__restrict__ oc, double scale, int bmask,
-                          T *__restrict__ z) {
+                          T *__restrict__ z, const double *nrm2 = nullptr) {
+    // nrm2: |R r|^2 on the device; the scale is then sqrt(|R r|^2 / Vc)
+    if (nrm2) {
+        const double q = *nrm2;
+        scale = q > 0.0 && isfinite(q) ? sqrt(q / (double)c.n()) : 0.0;
+    }
     const long long V = f.n();
     for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < V;
          id += (long long)gridDim.x * blockDim.x) {
@@ -1064,6 +1075,29 @@ static bool unet_apply_t(femgpu_unet *u, const T *r, T *z, double rnorm) {
     forward(u);
     hipLaunchKernelGGL(k_out_scale<T>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->out.as<float>(), 1.0 / sc, z, 0);
     return true;
+}
+
+// The coarse placement's correction with its scale on the device: *nrm2 =
+// |R r|^2 is computed here and read by the input and prolongation kernels, so
+// nothing is read back (the flexible CG's per-iteration path).  An unusable
+// |R r| gives z = 0.
+template <class T>
+static void unet_apply_coarse_t(femgpu_unet *u, const T *r, T *z, double *nrm2) {
+    hipStream_t s = u->ctx->stream;
+    const long long V = u->d0.n();
+    restrict_to_coarse<T>(u, r);
+    dot<float>(u->ctx, (const float *)u->rc.as<float>(), (const float *)u->rc.as<float>(), (size_t)V, nrm2);
+    hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(), 0.0,
+                       (const float *)u->aux.as<float>(), u->in.as<uint16_t>(), (const double *)nrm2);
+    forward(u);
+    hipLaunchKernelGGL(k_prolong<T>, dim3(g1(u->fine.n())), dim3(256), 0, s, u->fine, u->d0, u->cf,
+                       (const float *)u->out.as<float>(), 0.0, u->bmask, z, (const double *)nrm2);
+}
+
+void unet_apply_coarse_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double *nrm2) {
+    FG_REQUIRE(u->cf > 1, FEMGPU_ERR_INVALID_ARG, "unet: device-scaled apply needs the coarse placement");
+    if (dt == FEMGPU_F64) unet_apply_coarse_t<double>(u, (const double *)r, (double *)z, nrm2);
+    else unet_apply_coarse_t<float>(u, (const float *)r, (float *)z, nrm2);
 }
 
 bool unet_apply_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double rnorm) {
