@@ -726,46 +726,73 @@ __global__ void k_restrict_axis(long long outer, int nf, int nc, long long inner
     }
 }
 
+// The x pass of R: out[line][i] = sum_t (1 - |t|/cf) in[line][cf i + t]
+// (|t| < cf, inside the line).  A workgroup stages `lines` consecutive
+// x-lines (one contiguous run of lines * nf values) in LDS with coalesced
+// loads, then each thread sums one output's window from LDS.
+template <class TI>
+__global__ __launch_bounds__(256) void k_restrict_x(long long nlines, int nf, int nc, int cf, int lines,
+                                                    const TI *__restrict__ in, float *__restrict__ out) {
+    extern __shared__ double sline[];  // lines * nf values
+    const long long l0 = (long long)blockIdx.x * lines;
+    const int nl = (int)min((long long)lines, nlines - l0);
+    const TI *src = in + l0 * nf;
+    for (int k = threadIdx.x; k < nl * nf; k += 256) sline[k] = (double)src[k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < nl * nc; k += 256) {
+        const int l = k / nc, i = k - l * nc;
+        const double *line = sline + l * nf;
+        double acc = 0.0;
+        for (int t = -(cf - 1); t <= cf - 1; ++t) {
+            const int g = cf * i + t;
+            if (g < 0 || g >= nf) continue;
+            acc += (1.0 - (double)abs(t) / cf) * line[g];
+        }
+        out[(l0 + l) * nc + i] = (float)acc;
+    }
+}
+
 // z[g] = scale * (P oc)[g] (trilinear interpolation of the coarse values)
-// (bmask: 0 on the boundary of the DOF grid, where every DOF is essential)
+// (bmask: 0 on the boundary of the DOF grid, where every DOF is essential).
+// One workgroup per fine x-line (gy, gz): the line's coarse row, interpolated
+// in y and z, is formed once in LDS; each fine node then interpolates along x.
 template <class T>
-__global__ void k_prolong(Vol f, Vol c, int cf, const float *__restrict__ oc, double scale, int bmask,
-                          T *__restrict__ z, const double *nrm2 = nullptr) {
+__global__ __launch_bounds__(256) void k_prolong(Vol f, Vol c, int cf, const float *__restrict__ oc, double scale,
+                                                 int bmask, T *__restrict__ z, const double *nrm2 = nullptr) {
     // nrm2: |R r|^2 on the device; the scale is then sqrt(|R r|^2 / Vc)
     if (nrm2) {
         const double q = *nrm2;
         scale = q > 0.0 && isfinite(q) ? sqrt(q / (double)c.n()) : 0.0;
     }
-    const long long V = f.n();
-    for (long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x; id < V;
-         id += (long long)gridDim.x * blockDim.x) {
-        const int gx = (int)(id % f.nx);
-        const long long r = id / f.nx;
-        const int gy = (int)(r % f.ny), gz = (int)(r / f.ny);
-        if (bmask && (gx == 0 || gy == 0 || gz == 0 || gx == f.nx - 1 || gy == f.ny - 1 || gz == f.nz - 1)) {
-            z[id] = T(0);
+    extern __shared__ float brow[];  // c.nx values
+    const unsigned line = blockIdx.x;
+    const int gy = (int)(line % (unsigned)f.ny), gz = (int)(line / (unsigned)f.ny);
+    T *zl = z + (size_t)line * f.nx;
+    if (bmask && (gy == 0 || gz == 0 || gy == f.ny - 1 || gz == f.nz - 1)) {
+        for (int gx = threadIdx.x; gx < f.nx; gx += 256) zl[gx] = T(0);
+        return;
+    }
+    // lower coarse node of the cell holding g (the last cell for g = Nf - 1);
+    // the upper one is clamped for a one-node axis, where t = 0
+    const float icf = 1.0f / (float)cf;
+    const int iy = min(gy / cf, max(c.ny - 2, 0)), iz = min(gz / cf, max(c.nz - 2, 0));
+    const int iy1 = min(iy + 1, c.ny - 1), iz1 = min(iz + 1, c.nz - 1);
+    const float ty = (float)(gy - cf * iy) * icf, tz = (float)(gz - cf * iz) * icf;
+    const float *p0 = oc + ((size_t)iz * c.ny + iy) * c.nx, *p1 = oc + ((size_t)iz * c.ny + iy1) * c.nx;
+    const float *p2 = oc + ((size_t)iz1 * c.ny + iy) * c.nx, *p3 = oc + ((size_t)iz1 * c.ny + iy1) * c.nx;
+    for (int i = threadIdx.x; i < c.nx; i += 256) {
+        const float b0 = p0[i] + ty * (p1[i] - p0[i]), b1 = p2[i] + ty * (p3[i] - p2[i]);
+        brow[i] = b0 + tz * (b1 - b0);
+    }
+    __syncthreads();
+    for (int gx = threadIdx.x; gx < f.nx; gx += 256) {
+        if (bmask && (gx == 0 || gx == f.nx - 1)) {
+            zl[gx] = T(0);
             continue;
         }
-        // lower coarse node of the cell holding g (the last cell for g = Nf - 1)
-        const int ix = min(gx / cf, max(c.nx - 2, 0)), iy = min(gy / cf, max(c.ny - 2, 0)),
-                  iz = min(gz / cf, max(c.nz - 2, 0));
-        const double tx = (double)(gx - cf * ix) / cf, ty = (double)(gy - cf * iy) / cf,
-                     tz = (double)(gz - cf * iz) / cf;
-        double acc = 0.0;
-        for (int dz = 0; dz < 2; ++dz) {
-            const double wz = dz ? tz : 1.0 - tz;
-            if (wz == 0.0 || iz + dz >= c.nz) continue;
-            for (int dy = 0; dy < 2; ++dy) {
-                const double wy = dy ? ty : 1.0 - ty;
-                if (wy == 0.0 || iy + dy >= c.ny) continue;
-                for (int dx = 0; dx < 2; ++dx) {
-                    const double wx = dx ? tx : 1.0 - tx;
-                    if (wx == 0.0 || ix + dx >= c.nx) continue;
-                    acc += wx * wy * wz * (double)oc[((long long)(iz + dz) * c.ny + (iy + dy)) * c.nx + ix + dx];
-                }
-            }
-        }
-        z[id] = (T)(acc * scale);
+        const int ix = min(gx / cf, max(c.nx - 2, 0)), ix1 = min(ix + 1, c.nx - 1);
+        const float tx = (float)(gx - cf * ix) * icf;
+        zl[gx] = (T)((double)(brow[ix] + tx * (brow[ix1] - brow[ix])) * scale);
     }
 }
 
@@ -1039,9 +1066,11 @@ template <class T>
 static void restrict_to_coarse(femgpu_unet *u, const T *v) {
     hipStream_t s = u->ctx->stream;
     const Vol f = u->fine, c = u->d0;
-    const long long o1 = (long long)f.nz * f.ny, n1 = o1 * c.nx;
-    hipLaunchKernelGGL(k_restrict_axis<T>, dim3(g1(n1)), dim3(256), 0, s, o1, f.nx, c.nx, 1ll, u->cf, v,
-                       u->rx.as<float>());
+    const long long o1 = (long long)f.nz * f.ny;
+    const int lines = std::max(1, std::min(4, 4096 / f.nx));  // at most 32 KB of LDS (f.nx <= 4096)
+    FG_REQUIRE(f.nx <= 4096, FEMGPU_ERR_UNSUPPORTED, "unet: DOF grid rows longer than 4096");
+    hipLaunchKernelGGL(k_restrict_x<T>, dim3((unsigned)((o1 + lines - 1) / lines)), dim3(256),
+                       sizeof(double) * lines * f.nx, s, o1, f.nx, c.nx, u->cf, lines, v, u->rx.as<float>());
     const long long n2 = (long long)f.nz * c.ny * c.nx;
     hipLaunchKernelGGL(k_restrict_axis<float>, dim3(g1(n2)), dim3(256), 0, s, (long long)f.nz, f.ny, c.ny,
                        (long long)c.nx, u->cf, (const float *)u->rx.as<float>(), u->rxy.as<float>());
@@ -1065,8 +1094,9 @@ static bool unet_apply_t(femgpu_unet *u, const T *r, T *z, double rnorm) {
         hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(),
                            scc, (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
         forward(u);
-        hipLaunchKernelGGL(k_prolong<T>, dim3(g1(u->fine.n())), dim3(256), 0, s, u->fine, u->d0, u->cf,
-                           (const float *)u->out.as<float>(), 1.0 / scc, u->bmask, z);
+        hipLaunchKernelGGL(k_prolong<T>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
+                           sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(),
+                           1.0 / scc, u->bmask, z);
         return true;
     }
     const double sc = std::sqrt((double)V) / rnorm;
@@ -1090,8 +1120,9 @@ static void unet_apply_coarse_t(femgpu_unet *u, const T *r, T *z, double *nrm2) 
     hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(), 0.0,
                        (const float *)u->aux.as<float>(), u->in.as<uint16_t>(), (const double *)nrm2);
     forward(u);
-    hipLaunchKernelGGL(k_prolong<T>, dim3(g1(u->fine.n())), dim3(256), 0, s, u->fine, u->d0, u->cf,
-                       (const float *)u->out.as<float>(), 0.0, u->bmask, z, (const double *)nrm2);
+    hipLaunchKernelGGL(k_prolong<T>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
+                       sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(), 0.0,
+                       u->bmask, z, (const double *)nrm2);
 }
 
 void unet_apply_coarse_dev(femgpu_unet *u, const void *r, void *z, femgpu_dtype dt, double *nrm2) {
@@ -1396,8 +1427,9 @@ int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r,
         // n on the DOF grid: the output itself, or its prolongation
         const float *nout = u->out.as<float>();
         if (coarse) {
-            hipLaunchKernelGGL(k_prolong<float>, dim3(g1(V)), dim3(256), 0, s, u->fine, u->d0, u->cf,
-                               (const float *)u->out.as<float>(), 1.0, u->bmask, u->nf.as<float>());
+            hipLaunchKernelGGL(k_prolong<float>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
+                               sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(),
+                               1.0, u->bmask, u->nf.as<float>());
             nout = u->nf.as<float>();
         }
         const int nb = red_blocks(V);
