@@ -296,3 +296,26 @@ def test_amg_staged_levels_bitwise(fg, fr, ctx, monkeypatch):
         A.close()
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("lanes", [None, "32"])
+def test_amg_long_rows(fg, fr, ctx, monkeypatch, lanes):
+    """AMG set-up takes any f64 CSR, whatever SpMV layout the matrix got:
+    the Q1 hex diffusion + mass matrix of a perturbed 10^3 mesh (27 nonzeros
+    per interior row, so the SpMV runs several lanes per row; FEMGPU_CSR_LANES
+    forces 32) solves to 1e-10 and agrees with a direct solve."""
+    if lanes:
+        monkeypatch.setenv("FEMGPU_CSR_LANES", lanes)
+    n = (10, 10, 10)
+    c = fr.hex_vertices(*n, 0.2, 4)
+    d = fr.hex_desc(1, 2, n, kind=fr.FR_DIFF_MASS, kappa=fr.coef_const(1.0), rho=1.0, dirichlet=False)
+    rp, ci, v = fr.hex_assemble_csr(d, c)
+    N = len(rp) - 1
+    assert v.size / N > 16
+    A = fg.CsrMatrix(rp, ci, v, ctx=ctx)
+    b = np.random.default_rng(3).standard_normal(N)
+    x = np.zeros(N)
+    res = fg.solve_amg_cg(A, b, x, fg.SolverConfig(rtol=1e-10, max_iter=500))
+    assert res.converged and res.final_residual < 1e-10
+    xd = spl.spsolve(_csr(rp, ci, v, (N, N)).tocsc(), b)
+    assert np.abs(x - xd).max() < 1e-8 * np.abs(xd).max()
