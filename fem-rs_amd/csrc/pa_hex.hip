@@ -1184,6 +1184,60 @@ __device__ inline double coef_eval(const CoefDev &c, const double xp[3], long lo
     }
 }
 
+// The trilinear (HexQ1) map of element (ex, ey, ez) at reference point xi:
+// J[i][d] = dX_i/dxi_d, the physical point xp; returns det J.  HexQ1 vertex
+// order (lagrange/hex.rs:27-36).
+__device__ __forceinline__ double hex_map(const HexGrid &g, const double *__restrict__ coords, int ex, int ey,
+                                          int ez, const double (&xi)[3], double (&J)[3][3], double (&xp)[3]) {
+    const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+              sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        xp[i] = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[i][d] = 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const long long vi = ex + (sx[v] > 0), vj = ey + (sy[v] > 0), vk = ez + (sz[v] > 0);
+        const size_t gv = (size_t)vi + (size_t)(g.nx + 1) * ((size_t)vj + (size_t)(g.ny + 1) * vk);
+        const double X0 = coords[3 * gv], X1 = coords[3 * gv + 1], X2 = coords[3 * gv + 2];
+        const double fx = 1.0 + sx[v] * xi[0], fy = 1.0 + sy[v] * xi[1], fz = 1.0 + sz[v] * xi[2];
+        const double phi = 0.125 * fx * fy * fz;
+        const double gr[3] = {0.125 * sx[v] * fy * fz, 0.125 * sy[v] * fx * fz, 0.125 * sz[v] * fx * fy};
+        const double X[3] = {X0, X1, X2};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            xp[i] += phi * X[i];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) J[i][d] += X[i] * gr[d];
+        }
+    }
+    return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+// D = s J^-1 J^-T (6 symmetric components: 00, 01, 02, 11, 12, 22)
+__device__ __forceinline__ void hex_metric(const double (&J)[3][3], double det, double s, double (&D)[6]) {
+    const double id = 1.0 / det;
+    double I[3][3];
+    I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
+    I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    D[0] = s * (I[0][0] * I[0][0] + I[0][1] * I[0][1] + I[0][2] * I[0][2]);
+    D[1] = s * (I[0][0] * I[1][0] + I[0][1] * I[1][1] + I[0][2] * I[1][2]);
+    D[2] = s * (I[0][0] * I[2][0] + I[0][1] * I[2][1] + I[0][2] * I[2][2]);
+    D[3] = s * (I[1][0] * I[1][0] + I[1][1] * I[1][1] + I[1][2] * I[1][2]);
+    D[4] = s * (I[1][0] * I[2][0] + I[1][1] * I[2][1] + I[1][2] * I[2][2]);
+    D[5] = s * (I[2][0] * I[2][0] + I[2][1] * I[2][1] + I[2][2] * I[2][2]);
+}
+
 template <class T>
 __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ xq,
                             const double *__restrict__ wq, const double *__restrict__ coords,
@@ -1199,29 +1253,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
         const int ex = (int)(e % g.nx);
         const int ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
         const double xi[3] = {xq[qx], xq[qy], xq[qz]};
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, xp[3] = {0, 0, 0};
-        // HexQ1 vertex order (lagrange/hex.rs:27-36)
-        const int sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
-                  sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const long long vi = ex + (sx[v] > 0), vj = ey + (sy[v] > 0), vk = ez + (sz[v] > 0);
-            const size_t gv = (size_t)vi + (size_t)(g.nx + 1) * ((size_t)vj + (size_t)(g.ny + 1) * vk);
-            const double X0 = coords[3 * gv], X1 = coords[3 * gv + 1], X2 = coords[3 * gv + 2];
-            const double fx = 1.0 + sx[v] * xi[0], fy = 1.0 + sy[v] * xi[1], fz = 1.0 + sz[v] * xi[2];
-            const double phi = 0.125 * fx * fy * fz;
-            const double gr[3] = {0.125 * sx[v] * fy * fz, 0.125 * sy[v] * fx * fz, 0.125 * sz[v] * fx * fy};
-            const double X[3] = {X0, X1, X2};
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                xp[i] += phi * X[i];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) J[i][d] += X[i] * gr[d];
-            }
-        }
-        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        double J[3][3], xp[3];
+        const double det = hex_map(g, coords, ex, ey, ez, xi, J, xp);
         const double wd = wq[qx] * wq[qy] * wq[qz] * fabs(det);
         const int t2 = qx + Q * qy;
         if ((form & 1) && qm) qm[qd_index(Q, 1, e, 0, qz, t2)] = (T)(mass_only_weight ? wd : wd * rho);
@@ -1232,25 +1265,8 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
             qd[tpe ? qd_index_tpe(Q, 1, e, 0, qp) : qd_index(Q, 1, e, 0, qz, t2)] =
                 (T)(wd * coef_eval(coef, xp, e, qp, Q3));
         } else if ((form & 2) && qd) {
-            const double id = 1.0 / det;
-            double I[3][3];
-            I[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
-            I[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-            I[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-            I[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-            I[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-            I[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-            I[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-            I[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-            I[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-            const double s = wd * coef_eval(coef, xp, e, qp, Q3);
-            const double D[6] = {
-                s * (I[0][0] * I[0][0] + I[0][1] * I[0][1] + I[0][2] * I[0][2]),
-                s * (I[0][0] * I[1][0] + I[0][1] * I[1][1] + I[0][2] * I[1][2]),
-                s * (I[0][0] * I[2][0] + I[0][1] * I[2][1] + I[0][2] * I[2][2]),
-                s * (I[1][0] * I[1][0] + I[1][1] * I[1][1] + I[1][2] * I[1][2]),
-                s * (I[1][0] * I[2][0] + I[1][1] * I[2][1] + I[1][2] * I[2][2]),
-                s * (I[2][0] * I[2][0] + I[2][1] * I[2][1] + I[2][2] * I[2][2])};
+            double D[6];
+            hex_metric(J, det, wd * coef_eval(coef, xp, e, qp, Q3), D);
 #pragma unroll
             for (int comp = 0; comp < 6; ++comp)
                 qd[tpe ? qd_index_tpe(Q, 6, e, comp, qp) : qd_index(Q, 6, e, comp, qz, t2)] = (T)D[comp];
@@ -1259,11 +1275,14 @@ __global__ void k_hex_setup(const HexGrid g, int Q, const double *__restrict__ x
 }
 
 // Assembled diagonal: one thread per (element, local DOF); setup-time only.
+// geo: qd holds w kappa per point (geometry on the fly); the metric and the
+// mass weight rho w |J| are formed here from the vertices (coords, xq, wq).
 template <class T>
 __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restrict__ B,
                            const double *__restrict__ G, const T *__restrict__ qd,
                            const T *__restrict__ qm, const T *__restrict__ cf, int cart, int tpe, int form,
-                           T *__restrict__ ye) {
+                           T *__restrict__ ye, int geo, const double *__restrict__ coords,
+                           const double *__restrict__ xq, const double *__restrict__ wq, double rho) {
     // Element diagonals into the element vector; k_hex_gather then sums them
     // per DOF in a fixed order.
     const int L = N1 * N1 * N1;
@@ -1281,11 +1300,21 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
                 for (int a = 0; a < Q; ++a) {
                     const int t2 = a + Q * b;
                     const double Bi = B[a * N1 + i], Bj = B[b * N1 + j], Bk = B[c * N1 + k];
+                    double gD[6] = {0, 0, 0, 0, 0, 0}, gwd = 0.0;  // geo: the metric and w |J| here
+                    if (geo) {
+                        const double xi[3] = {xq[a], xq[b], xq[c]};
+                        double J[3][3], xp[3];
+                        const double det = hex_map(g, coords, ex, ey, ez, xi, J, xp);
+                        gwd = wq[a] * wq[b] * wq[c] * fabs(det);
+                        if (form & 2) hex_metric(J, det, (double)qd[qd_index(Q, 1, e, 0, c, t2)] * fabs(det), gD);
+                    }
                     if (form & 2) {
                         const double g0 = G[a * N1 + i] * Bj * Bk, g1 = Bi * G[b * N1 + j] * Bk,
                                      g2 = Bi * Bj * G[c * N1 + k];
                         double d[6];
-                        if (cart) {
+                        if (geo) {
+                            for (int m = 0; m < 6; ++m) d[m] = gD[m];
+                        } else if (cart) {
                             const int qp = t2 + Q * Q * c;
                             const double sq = (double)qd[tpe ? qd_index_tpe(Q, 1, e, 0, qp) : qd_index(Q, 1, e, 0, c, t2)];
                             d[0] = sq * (double)cf[ex]; d[3] = sq * (double)cf[g.nx + ey];
@@ -1300,7 +1329,8 @@ __global__ void k_hex_diag(const HexGrid g, int N1, int Q, const double *__restr
                     }
                     if (form & 1) {
                         const double ph = Bi * Bj * Bk;
-                        s += (double)qm[qd_index(Q, 1, e, 0, c, t2)] * ph * ph;
+                        const double mw = geo ? gwd * rho : (double)qm[qd_index(Q, 1, e, 0, c, t2)];
+                        s += mw * ph * ph;
                     }
                 }
         ye[(size_t)e * L + l] = (T)s;
@@ -1549,20 +1579,10 @@ struct HexOp final : femgpu_op {
         const int n1 = desc.p + 1;
         const size_t L = (size_t)g.E * n1 * n1 * n1;
         DevBuf old(L * sizeof(T));
-        DevBuf gqd, gqm;  // geo: general qdata, built for the diagonal only
-        const T *dqd = qd.as<T>(), *dqm = qm.as<T>();
-        if (geo) {
-            const size_t nq = qd_count(g.E, desc.q);
-            gqd.alloc(6 * nq * sizeof(T));
-            if (desc.form & 1) gqm.alloc(nq * sizeof(T));
-            geo = false;  // setup_t builds the stored general form
-            setup_t<T>(desc.form, desc.rho, false, gqd.as<T>(), gqm.as<T>());
-            geo = true;
-            dqd = gqd.as<T>(); dqm = gqm.as<T>();
-        }
         hipLaunchKernelGGL(k_hex_diag<T>, dim3(grid1(L)), dim3(256), 0, s, g, n1, desc.q, Bd.as<double>(),
-                           Gd.as<double>(), dqd, dqm, (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
-                           desc.form, old.as<T>());
+                           Gd.as<double>(), qd.as<T>(), qm.as<T>(), (const T *)cf.ptr, cart ? 1 : 0, tpe ? 1 : 0,
+                           desc.form, old.as<T>(), geo ? 1 : 0, coords.as<double>(), xq.as<double>(),
+                           wq.as<double>(), desc.rho);
         HexGrid gd = g;
         gd.ymerge = 0;  // k_evec_merge builds the element vector without the y-merge
         hipLaunchKernelGGL(k_evec_merge<T>, dim3(grid1(seam_base(gd, n1))), dim3(256), 0, s, gd, n1,
