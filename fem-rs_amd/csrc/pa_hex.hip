@@ -268,7 +268,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     __shared__ T sBB[NE * LL.SB];  // Y out BB, later C_zeta+mass
     __shared__ T sBG[ND * LL.SB];  // Y out BG, later C_xi
     __shared__ T sGB[ND * LL.SB];  // Y out GB, later C_eta
-    __shared__ T sV[GEO ? NE * 24 : 1];  // GEO: element vertices, (i + 2j + 4k) * 3 + component
+    __shared__ T sC[GEO ? NE * 21 : 1];  // GEO: the map's monomial coefficients, component * 7 + m - 1
     auto iA = [](int e_, int k_, int j_, int a_) {
         return e_ * LL.SA + off3<LL.a0, LL.a1, LL.a2, N1, N1, Q>(k_, j_, a_);
     };
@@ -300,10 +300,27 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
 
     if constexpr (GEO) {
-        for (int u = t; valid && u < 24; u += T2) {
-            const int v = u / 3, comp = u - 3 * v;
-            const size_t vi = (size_t)ex + (v & 1), vj = (size_t)ey + ((v >> 1) & 1), vk = (size_t)ez + (v >> 2);
-            sV[le * 24 + u] = cf[3 * (vi + (size_t)(g.nx + 1) * (vj + (size_t)(g.ny + 1) * vk)) + comp];
+        // The trilinear map as a polynomial, X = sum_m c_m xi^m0 eta^m1 zeta^m2
+        // (m0, m1, m2 the bits of m): c_m = 1/8 sum_v s(v)^m V_v over the
+        // corners v = i + 2j + 4k with signs s(v) = (2i - 1, 2j - 1, 2k - 1),
+        // a Walsh-Hadamard transform of the 8 vertices; one lane per
+        // component.
+        if (valid && t < 3) {
+            const size_t sy = 3 * (size_t)(g.nx + 1), sz = sy * (size_t)(g.ny + 1);
+            const T *p0 = cf + 3 * (size_t)ex + sy * (size_t)ey + sz * (size_t)ez + t;
+            const T *p1 = p0 + sz;
+            T h[8] = {p0[0], p0[3], p0[sy], p0[sy + 3], p1[0], p1[3], p1[sy], p1[sy + 3]};
+#pragma unroll
+            for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+                for (int v = 0; v < 8; ++v)
+                    if (!(v & bit)) {
+                        const T lo = h[v], hi = h[v | bit];
+                        h[v] = hi + lo;
+                        h[v | bit] = hi - lo;
+                    }
+#pragma unroll
+            for (int m = 1; m < 8; ++m) sC[le * 21 + 7 * t + m - 1] = T(0.125) * h[m];
         }
     }
 
@@ -390,29 +407,19 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         T Pa[3] = {0, 0, 0}, Qa[3] = {0, 0, 0}, Pb[3] = {0, 0, 0}, Qb[3] = {0, 0, 0}, Hc[3] = {0, 0, 0};
         T wab = 0;
         if constexpr (GEO) {
-            // vertex V(i, j, k) at reference corner (2i - 1, 2j - 1, 2k - 1),
-            // shape functions L0(s) = (1 - s) / 2, L1(s) = (1 + s) / 2 per
-            // direction: dX/dxi = sum_jk (V_1jk - V_0jk) / 2 L_j(eta) L_k(zeta)
-            // = Pa + Qa zeta, and likewise for eta; dX/dzeta does not depend
-            // on zeta
+            // with X = sum_m c_m xi^m0 eta^m1 zeta^m2 (c_m = C[m - 1]):
+            // dX/dxi = (c1 + c3 eta) + (c5 + c7 eta) zeta, dX/deta = (c2 + c3 xi)
+            // + (c6 + c7 xi) zeta, dX/dzeta = c4 + c5 xi + (c6 + c7 xi) eta
             const T xa = Tab<T>::X(Q)[a], yb = Tab<T>::X(Q)[b];
-            const T la0 = T(0.5) * (T(1) - xa), la1 = T(0.5) * (T(1) + xa);
-            const T lb0 = T(0.5) * (T(1) - yb), lb1 = T(0.5) * (T(1) + yb);
-            const T *V = sV + le * 24;
+            const T *Cm = sC + le * 21;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                T E[2], F[2];
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const T v00 = V[(0 + 4 * k) * 3 + i], v10 = V[(1 + 4 * k) * 3 + i];
-                    const T v01 = V[(2 + 4 * k) * 3 + i], v11 = V[(3 + 4 * k) * 3 + i];
-                    E[k] = T(0.5) * ((v10 - v00) * lb0 + (v11 - v01) * lb1);
-                    F[k] = T(0.5) * ((v01 - v00) * la0 + (v11 - v10) * la1);
-                }
-                Pa[i] = T(0.5) * (E[0] + E[1]); Qa[i] = T(0.5) * (E[1] - E[0]);
-                Pb[i] = T(0.5) * (F[0] + F[1]); Qb[i] = T(0.5) * (F[1] - F[0]);
-                Hc[i] = T(0.5) * (la0 * (lb0 * (V[4 * 3 + i] - V[0 * 3 + i]) + lb1 * (V[6 * 3 + i] - V[2 * 3 + i])) +
-                                  la1 * (lb0 * (V[5 * 3 + i] - V[1 * 3 + i]) + lb1 * (V[7 * 3 + i] - V[3 * 3 + i])));
+                const T *c = Cm + 7 * i;
+                Pa[i] = c[0] + c[2] * yb;
+                Qa[i] = c[4] + c[6] * yb;
+                Pb[i] = c[1] + c[2] * xa;
+                Qb[i] = c[5] + c[6] * xa;
+                Hc[i] = c[3] + c[4] * xa + Qb[i] * yb;
             }
             wab = (rho * Tab<T>::W(Q)[a]) * Tab<T>::W(Q)[b];
         }
