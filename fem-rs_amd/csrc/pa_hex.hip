@@ -77,6 +77,111 @@ template <> struct Tab<float> {
     static __device__ __forceinline__ const float *W(int q) { return c_gauss32[q][1]; }
 };
 
+// Even-odd form of the 1-D tables.  With nodes and Gauss points symmetric
+// about the centre, an (R x Cc) table M (B or G, or their transposes) obeys
+// M[R-1-r][Cc-1-c] = s M[r][c] with s = +1 (B) or -1 (G).  With the mirrored
+// sums e_c = u_c + u_{Cc-1-c} and differences o_c = u_c - u_{Cc-1-c} (c <
+// Cc/2) and, for odd Cc, the middle input m, the products are
+//   Se_r = sum_c P[r][c] e_c + Mid[r] m,   So_r = sum_c Mm[r][c] o_c
+//   (M u)_r = Se_r + So_r,   (M u)_{R-1-r} = s (Se_r - So_r)
+// for r < (R+1)/2, with P = (M[r][c] + M[r][Cc-1-c]) / 2, Mm = (M[r][c] -
+// M[r][Cc-1-c]) / 2 and Mid[r] = M[r][Cc/2] (a middle output row keeps only
+// Se for s = +1 and only So for s = -1).  A 5 x 5 line product takes 13
+// multiply-adds and 8 additions instead of 25 multiply-adds (Kronbichler and
+// Kormann's even-odd decomposition).  Table [combo][m]: m = 0 B (Q x N1), 1
+// G, 2 B^T (N1 x Q), 3 G^T; P at [r (Cc/2) + c], Mid at [16 + r], Mm at
+// [24 + r (Cc/2) + c].
+__constant__ double c_eo64[kCombos][4][kTab];
+__constant__ float c_eo32[kCombos][4][kTab];
+template <class T> __device__ __forceinline__ const T *eo_tab(int c, int m);
+template <> __device__ __forceinline__ const double *eo_tab<double>(int c, int m) { return c_eo64[c][m]; }
+template <> __device__ __forceinline__ const float *eo_tab<float>(int c, int m) { return c_eo32[c][m]; }
+
+// mirrored sums / differences of an NI-vector and its middle entry
+template <int NI, class T> struct EOs {
+    T e[NI / 2 > 0 ? NI / 2 : 1], o[NI / 2 > 0 ? NI / 2 : 1];
+    T m;
+};
+template <int NI, class T> __device__ __forceinline__ EOs<NI, T> eo_split(const T (&u)[NI]) {
+    EOs<NI, T> r;
+#pragma unroll
+    for (int c = 0; c < NI / 2; ++c) {
+        r.e[c] = u[c] + u[NI - 1 - c];
+        r.o[c] = u[c] - u[NI - 1 - c];
+    }
+    r.m = (NI & 1) ? u[NI / 2] : T(0);
+    return r;
+}
+// Se_r (EVEN) or So_r (odd part) for r < (R+1)/2, added to acc; the middle
+// row (odd R) only when MIDROW
+template <int R, int NI, bool EVEN, bool MIDROW, class T>
+__device__ __forceinline__ void eo_part(const T *tab, const EOs<NI, T> &u, T (&acc)[(R + 1) / 2]) {
+    constexpr int HC = NI / 2, HR = R / 2;
+#pragma unroll
+    for (int r = 0; r < HR + ((R & 1) && MIDROW ? 1 : 0); ++r) {
+        T a = acc[r];
+        if constexpr (EVEN) {
+            if constexpr (NI & 1) a += tab[16 + r] * u.m;
+#pragma unroll
+            for (int c = 0; c < HC; ++c) a += tab[r * HC + c] * u.e[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < HC; ++c) a += tab[24 + r * HC + c] * u.o[c];
+        }
+        acc[r] = a;
+    }
+}
+// out = M u for a table of symmetry SIG
+template <int R, int NI, int SIG, class T>
+__device__ __forceinline__ void eo_contract(const T *tab, const EOs<NI, T> &u, T (&out)[R]) {
+    constexpr int HR = R / 2, HR1 = (R + 1) / 2;
+    T se[HR1], so[HR1];
+#pragma unroll
+    for (int r = 0; r < HR1; ++r) se[r] = so[r] = T(0);
+    eo_part<R, NI, true, SIG == 1>(tab, u, se);
+    eo_part<R, NI, false, SIG == -1>(tab, u, so);
+#pragma unroll
+    for (int r = 0; r < HR; ++r) {
+        out[r] = se[r] + so[r];
+        out[R - 1 - r] = SIG == 1 ? se[r] - so[r] : so[r] - se[r];
+    }
+    if constexpr (R & 1) out[HR] = SIG == 1 ? se[HR] : so[HR];
+}
+// out = M1 u1 + M2 u2 for tables of symmetry +1 (M1) and -1 (M2): A = Se1 +
+// So2 and Bv = So1 + Se2 give out_r = A + Bv, out_{R-1-r} = A - Bv
+template <int R, int NI, class T>
+__device__ __forceinline__ void eo_contract2(const T *tab1, const EOs<NI, T> &u1, const T *tab2,
+                                             const EOs<NI, T> &u2, T (&out)[R]) {
+    constexpr int HR = R / 2, HR1 = (R + 1) / 2;
+    T a[HR1], b[HR1];
+#pragma unroll
+    for (int r = 0; r < HR1; ++r) a[r] = b[r] = T(0);
+    eo_part<R, NI, true, true>(tab1, u1, a);
+    eo_part<R, NI, false, true>(tab2, u2, a);
+    eo_part<R, NI, false, false>(tab1, u1, b);
+    eo_part<R, NI, true, false>(tab2, u2, b);
+#pragma unroll
+    for (int r = 0; r < HR; ++r) {
+        out[r] = a[r] + b[r];
+        out[R - 1 - r] = a[r] - b[r];
+    }
+    if constexpr (R & 1) out[HR] = a[HR];
+}
+
+// the even-odd tables of an R x Cc table M (row-major, or its transpose when
+// TR: M[r][c] = src[c * R + r])
+static void eo_fill(const double *src, int R, int Cc, bool tr, double *dst) {
+    auto M = [&](int r, int c) { return tr ? src[c * R + r] : src[r * Cc + c]; };
+    const int HC = Cc / 2, HR1 = (R + 1) / 2;
+    for (int r = 0; r < HR1; ++r) {
+        for (int c = 0; c < HC; ++c) {
+            dst[r * HC + c] = 0.5 * (M(r, c) + M(r, Cc - 1 - c));
+            dst[24 + r * HC + c] = 0.5 * (M(r, c) - M(r, Cc - 1 - c));
+        }
+        dst[16 + r] = (Cc & 1) ? M(r, HC) : 0.0;
+    }
+}
+
 static std::mutex g_tab_mu;
 static bool g_tab_done[64];
 
@@ -86,8 +191,12 @@ static void ensure_tables(int device) {
     if (device < 64 && g_tab_done[device]) return;
     static double h64[kCombos][2][kTab];
     static float h32[kCombos][2][kTab];
+    static double e64[kCombos][4][kTab];
+    static float e32[kCombos][4][kTab];
     std::memset(h64, 0, sizeof h64);
     std::memset(h32, 0, sizeof h32);
+    std::memset(e64, 0, sizeof e64);
+    std::memset(e32, 0, sizeof e32);
     for (int n1 = 2; n1 <= 6; ++n1)
         for (int q = n1; q <= n1 + 2 && q <= 6; ++q) {
             double x[8], w[8], B[64], G[64];
@@ -98,9 +207,17 @@ static void ensure_tables(int device) {
                 h64[c][0][i] = B[i]; h64[c][1][i] = G[i];
                 h32[c][0][i] = (float)B[i]; h32[c][1][i] = (float)G[i];
             }
+            eo_fill(B, q, n1, false, e64[c][0]);
+            eo_fill(G, q, n1, false, e64[c][1]);
+            eo_fill(B, n1, q, true, e64[c][2]);
+            eo_fill(G, n1, q, true, e64[c][3]);
+            for (int m = 0; m < 4; ++m)
+                for (int i = 0; i < kTab; ++i) e32[c][m][i] = (float)e64[c][m][i];
         }
     FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab64), h64, sizeof h64));
     FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tab32), h32, sizeof h32));
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_eo64), e64, sizeof e64));
+    FG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_eo32), e32, sizeof e32));
     static double g64[kGaussMax][2][kGaussMax];
     static float g32[kGaussMax][2][kGaussMax];
     std::memset(g64, 0, sizeof g64);
@@ -263,11 +380,28 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     // index order and element stride of lds_layout (chosen against bank
     // conflicts of the five phases' access patterns).
     constexpr LdsLayout LL = lds_layout(N1, Q, (int)sizeof(T));
-    __shared__ T sA[NE * LL.SA];   // X out (B path), later F_B
-    __shared__ T sAg[ND * LL.SA];  // X out (G path), later F_G
-    __shared__ T sBB[NE * LL.SB];  // Y out BB, later C_zeta+mass
-    __shared__ T sBG[ND * LL.SB];  // Y out BG, later C_xi
-    __shared__ T sGB[ND * LL.SB];  // Y out GB, later C_eta
+    // Geometry on the fly (measured on Q4 128^3, FP64 apply 3.21 -> 2.84 ms):
+    // - ALIAS: the X-phase arrays (sA, sAg) and the Y-phase arrays (sBB, sBG,
+    //   sGB) share one buffer; the Y and Y^T phases read all their inputs
+    //   into registers and pass a barrier before they write, and the X^T phase
+    //   hands its face values over in sF.  33.7 instead of 51.7 KB per
+    //   workgroup at Q4 FP64, so 4 workgroups per CU instead of 3;
+    // - the Z phase in three passes (below), 126 instead of 150 VGPRs, so 4
+    //   waves per SIMD;
+    // - EO: the contractions in the even-odd form (eo_contract), 947 instead
+    //   of 1073 vector instructions per lane.
+    constexpr bool ALIAS = GEO;
+    // EO: the contractions in the even-odd form (eo_contract)
+    constexpr bool EO = GEO;
+    const T *E0 = eo_tab<T>(C, 0), *E1 = eo_tab<T>(C, 1), *E2 = eo_tab<T>(C, 2), *E3 = eo_tab<T>(C, 3);
+    constexpr int SZX = NE * LL.SA + ND * LL.SA, SZY = NE * LL.SB + 2 * ND * LL.SB;
+    __shared__ T sBuf[ALIAS ? (SZX > SZY ? SZX : SZY) : SZX + SZY];
+    __shared__ T sF[ALIAS ? NE * N1 * N1 : 1];
+    T *const sA = sBuf;                    // X out (B path), later F_B
+    T *const sAg = sBuf + NE * LL.SA;      // X out (G path), later F_G
+    T *const sBB = sBuf + (ALIAS ? 0 : SZX);  // Y out BB, later C_zeta+mass
+    T *const sBG = sBB + NE * LL.SB;       // Y out BG, later C_xi
+    T *const sGB = sBG + ND * LL.SB;       // Y out GB, later C_eta
     __shared__ T sC[GEO ? NE * 21 : 1];  // GEO: the map's monomial coefficients, component * 7 + m - 1
     auto iA = [](int e_, int k_, int j_, int a_) {
         return e_ * LL.SA + off3<LL.a0, LL.a1, LL.a2, N1, N1, Q>(k_, j_, a_);
@@ -297,6 +431,28 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         const unsigned r2 = r / nyu;
         ey = (int)(r - r2 * nyu);
         ez = (int)r2;
+    }
+
+    // One stored value per point (Cartesian: w kappa |J|; geometry on the fly:
+    // w kappa): the lane's Q values along its z-column, and (geometry) its
+    // point coordinates and weights, are loaded here so that their latency
+    // overlaps the X and Y phases instead of stalling the Z phase.
+    constexpr size_t plane = (size_t)NE * T2;
+    T sqv[(CART || GEO) && DIFF ? Q : 1];
+    T xa = 0, yb = 0, wa = 0, wb = 0;
+    if constexpr ((CART || GEO) && DIFF) {
+        if (valid) {
+            const size_t off1 = (size_t)lb * Q * plane + (size_t)le * T2 + t;
+#pragma unroll
+            for (int c = 0; c < Q; ++c) sqv[c] = qd[(size_t)c * plane + off1];
+            if constexpr (GEO) {
+                const int a = t % Q, b = t / Q;
+                xa = Tab<T>::X(Q)[a];
+                yb = Tab<T>::X(Q)[b];
+                wa = Tab<T>::W(Q)[a];
+                wb = Tab<T>::W(Q)[b];
+            }
+        }
     }
 
     if constexpr (GEO) {
@@ -337,6 +493,17 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             const bool b = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
             u[i] = b ? T(0) : x[base + i];
         }
+        if constexpr (EO) {
+            const EOs<N1, T> su = eo_split(u);
+            T s[Q], sg[Q];
+            eo_contract<Q, N1, 1>(E0, su, s);
+            if constexpr (DIFF) eo_contract<Q, N1, -1>(E1, su, sg);
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                sA[iA(le, k, j, a)] = s[a];
+                if constexpr (DIFF) sAg[iA(le, k, j, a)] = sg[a];
+            }
+        } else
 #pragma unroll
         for (int a = 0; a < Q; ++a) {
             T s = 0, sg = 0;
@@ -362,7 +529,23 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) Ag_[j] = sAg[iA(le, k, j, a)];
         }
     }
-    if (ylane) {
+    if constexpr (ALIAS) __syncthreads();
+    if (ylane && EO) {
+        const int a = t % Q, k = t / Q;
+        const EOs<N1, T> sa = eo_split(A_);
+        T bb[Q], bg[Q], gb[Q];
+        eo_contract<Q, N1, 1>(E0, sa, bb);
+        if constexpr (DIFF) {
+            const EOs<N1, T> sag = eo_split(Ag_);
+            eo_contract<Q, N1, -1>(E1, sa, gb);
+            eo_contract<Q, N1, 1>(E0, sag, bg);
+        }
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            sBB[iB(le, k, b, a)] = bb[b];
+            if constexpr (DIFF) { sBG[iB(le, k, b, a)] = bg[b]; sGB[iB(le, k, b, a)] = gb[b]; }
+        }
+    } else if (ylane) {
         const int a = t % Q, k = t / Q;
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
@@ -392,7 +575,6 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) { bg[k] = sBG[iB(le, k, b, a)]; gb[k] = sGB[iB(le, k, b, a)]; }
             c0[k] = c1[k] = c2[k] = 0;
         }
-        constexpr size_t plane = (size_t)NE * T2;
         const size_t offd = (size_t)lb * (CART || GEO ? 1 : 6) * Q * plane + (size_t)le * T2 + t;
         const size_t offm = (size_t)lb * Q * plane + (size_t)le * T2 + t;
         T fx = 0, fy = 0, fz = 0;  // Cartesian per-axis factors of this element
@@ -410,7 +592,6 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             // with X = sum_m c_m xi^m0 eta^m1 zeta^m2 (c_m = C[m - 1]):
             // dX/dxi = (c1 + c3 eta) + (c5 + c7 eta) zeta, dX/deta = (c2 + c3 xi)
             // + (c6 + c7 xi) zeta, dX/dzeta = c4 + c5 xi + (c6 + c7 xi) eta
-            const T xa = Tab<T>::X(Q)[a], yb = Tab<T>::X(Q)[b];
             const T *Cm = sC + le * 21;
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
@@ -421,8 +602,79 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                 Qb[i] = c[5] + c[6] * xa;
                 Hc[i] = c[3] + c[4] * xa + Qb[i] * yb;
             }
-            wab = (rho * Tab<T>::W(Q)[a]) * Tab<T>::W(Q)[b];
+            wab = (rho * wa) * wb;
         }
+        if constexpr (GEO) {
+            // Three passes over the column, so that the inputs bb, bg, gb and
+            // the accumulators c0, c1, c2 are never live together (register
+            // pressure sets the waves per SIMD here): the forward contraction
+            // at all Q points, the pointwise operator in place, the transposed
+            // contraction.  Each sum keeps the per-point loop's order.
+            T G0[Q], G1[Q], G2[Q], UU[Q];
+            if constexpr (EO) {
+                const EOs<N1, T> sbb = eo_split(bb);
+                eo_contract<Q, N1, 1>(E0, eo_split(bg), G0);
+                eo_contract<Q, N1, 1>(E0, eo_split(gb), G1);
+                eo_contract<Q, N1, -1>(E1, sbb, G2);
+                if constexpr (MASS) eo_contract<Q, N1, 1>(E0, sbb, UU);
+            } else
+#pragma unroll
+            for (int c = 0; c < Q; ++c) {
+                T g0 = 0, g1 = 0, g2 = 0, uu = 0;
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    g0 += Bt[c * N1 + k] * bg[k];
+                    g1 += Bt[c * N1 + k] * gb[k];
+                    g2 += Gt[c * N1 + k] * bb[k];
+                }
+                if constexpr (MASS) {
+#pragma unroll
+                    for (int k = 0; k < N1; ++k) uu += Bt[c * N1 + k] * bb[k];
+                }
+                G0[c] = g0;
+                G1[c] = g1;
+                G2[c] = g2;
+                UU[c] = uu;
+            }
+#pragma unroll
+            for (int c = 0; c < Q; ++c) {
+                const T zc = Tab<T>::X(Q)[c];
+                const T ax = Pa[0] + Qa[0] * zc, ay = Pa[1] + Qa[1] * zc, az = Pa[2] + Qa[2] * zc;
+                const T bx = Pb[0] + Qb[0] * zc, by = Pb[1] + Qb[1] * zc, bz = Pb[2] + Qb[2] * zc;
+                const T cx = Hc[0], cy = Hc[1], cz = Hc[2];
+                const T r0x = by * cz - bz * cy, r0y = bz * cx - bx * cz, r0z = bx * cy - by * cx;
+                const T r1x = cy * az - cz * ay, r1y = cz * ax - cx * az, r1z = cx * ay - cy * ax;
+                const T r2x = ay * bz - az * by, r2y = az * bx - ax * bz, r2z = ax * by - ay * bx;
+                const T det = ax * r0x + ay * r0y + az * r0z;
+                const T adet = fabs(det);
+                const T f = sqv[c] * recip(adet);
+                const T g0 = G0[c], g1 = G1[c], g2 = G2[c];
+                const T hx = g0 * r0x + g1 * r1x + g2 * r2x;
+                const T hy = g0 * r0y + g1 * r1y + g2 * r2y;
+                const T hz = g0 * r0z + g1 * r1z + g2 * r2z;
+                G0[c] = f * (r0x * hx + r0y * hy + r0z * hz);
+                G1[c] = f * (r1x * hx + r1y * hy + r1z * hz);
+                G2[c] = f * (r2x * hx + r2y * hy + r2z * hz);
+                if constexpr (MASS) UU[c] = (wab * Tab<T>::W(Q)[c] * adet) * UU[c];
+            }
+            if constexpr (EO) {
+                eo_contract<N1, Q, 1>(E2, eo_split(G0), c0);
+                eo_contract<N1, Q, 1>(E2, eo_split(G1), c1);
+                if constexpr (MASS) eo_contract2<N1, Q>(E2, eo_split(UU), E3, eo_split(G2), c2);
+                else eo_contract<N1, Q, -1>(E3, eo_split(G2), c2);
+            } else
+#pragma unroll
+            for (int c = 0; c < Q; ++c) {
+#pragma unroll
+                for (int k = 0; k < N1; ++k) {
+                    const T bz = Bt[c * N1 + k];
+                    c0[k] += bz * G0[c];
+                    c1[k] += bz * G1[c];
+                    c2[k] += Gt[c * N1 + k] * G2[c];
+                    if constexpr (MASS) c2[k] += bz * UU[c];
+                }
+            }
+        } else
 #pragma unroll
         for (int c = 0; c < Q; ++c) {
             T v0 = 0, v1 = 0, v2 = 0, vm = 0;
@@ -436,7 +688,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     g2 += Gt[c * N1 + k] * bb[k];
                 }
                 if constexpr (CART) {
-                    const T sq = qd[(size_t)c * plane + offd];
+                    const T sq = sqv[c];
                     v0 = (sq * fx) * g0;
                     v1 = (sq * fy) * g1;
                     v2 = (sq * fz) * g2;
@@ -453,7 +705,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
                     const T r2x = ay * bz - az * by, r2y = az * bx - ax * bz, r2z = ax * by - ay * bx;
                     const T det = ax * r0x + ay * r0y + az * r0z;
                     adet = fabs(det);
-                    const T f = qd[(size_t)c * plane + offd] * recip(adet);
+                    const T f = sqv[c] * recip(adet);
                     const T hx = g0 * r0x + g1 * r1x + g2 * r2x;
                     const T hy = g0 * r0y + g1 * r1y + g2 * r2y;
                     const T hz = g0 * r0z + g1 * r1z + g2 * r2z;
@@ -508,7 +760,22 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) { C0[b] = sBG[iB(le, k, b, a)]; C1[b] = sGB[iB(le, k, b, a)]; }
         }
     }
-    if (ylane) {
+    if constexpr (ALIAS) __syncthreads();
+    if (ylane && EO) {
+        const int a = t % Q, k = t / Q;
+        T fg[N1], fb[N1];
+        if constexpr (DIFF) {
+            eo_contract<N1, Q, 1>(E2, eo_split(C0), fg);
+            eo_contract2<N1, Q>(E2, eo_split(C2), E3, eo_split(C1), fb);
+        } else {
+            eo_contract<N1, Q, 1>(E2, eo_split(C2), fb);
+        }
+#pragma unroll
+        for (int j = 0; j < N1; ++j) {
+            sA[iA(le, k, j, a)] = fb[j];
+            if constexpr (DIFF) sAg[iA(le, k, j, a)] = fg[j];
+        }
+    } else if (ylane) {
         const int a = t % Q, k = t / Q;
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
@@ -540,6 +807,10 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             FB[a] = sA[iA(le, kx, jx, a)];
             if constexpr (DIFF) FG[a] = sAg[iA(le, kx, jx, a)];
         }
+        if constexpr (EO) {
+            if constexpr (DIFF) eo_contract2<N1, Q>(E2, eo_split(FB), E3, eo_split(FG), sv);
+            else eo_contract<N1, Q, 1>(E2, eo_split(FB), sv);
+        } else
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             T s = 0;
@@ -550,7 +821,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             }
             sv[i] = s;
         }
-        sBB[iB(le, kx, jx, 0)] = sv[0];
+        if constexpr (ALIAS) sF[le * N1 * N1 + t] = sv[0];
+        else sBB[iB(le, kx, jx, 0)] = sv[0];
     }
     __syncthreads();
     if (xt) {
@@ -561,7 +833,8 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
         for (int i = 1; i < P; ++i) dst[i] = sv[i];
         // the next element is in this workgroup (and row) unless this one ends
         // the row or the workgroup
-        if (ex < g.nx - 1 && le + 1 < NE) dst[P] = sv[P] + sBB[iB(le + 1, kx, jx, 0)];
+        if (ex < g.nx - 1 && le + 1 < NE)
+            dst[P] = sv[P] + (ALIAS ? sF[(le + 1) * N1 * N1 + t] : sBB[iB(le + 1, kx, jx, 0)]);
         else dst[P] = sv[P];
     }
 }
