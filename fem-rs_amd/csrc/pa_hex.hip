@@ -244,7 +244,24 @@ struct HexGrid {
     int ne;         // elements per element-kernel workgroup, ne_of(Q)
     long long Nxp;  // row stride of the x-merged element vector: Nx rounded up to even
     int ymerge;     // k_pa_hex_tpe with nx = 64 or 128: y-faces merged inside 256-element blocks
+    unsigned mx, my;  // n / nx and n / ny for n < 2^31 as udiv_m(n, mx, shx) (div_magic)
+    int shx, shy;
 };
+
+// Division by a mesh dimension d >= 1 without the division sequence: for
+// n < 2^31, n / d = umulhi(n, M) >> (s - 1) with s = ceil(log2 d) and M =
+// ceil(2^(31+s) / d) < 2^32 (the error n e / 2^(31+s), e = M d - 2^(31+s) <
+// d <= 2^s, stays below 1/d); M = 0 stands for d = 1.
+static inline void div_magic(long long d, unsigned &m, int &sh) {
+    if (d <= 1) { m = 0; sh = 0; return; }
+    int s = 0;
+    while ((1ll << s) < d) ++s;
+    m = (unsigned)(((1ull << (31 + s)) + (unsigned long long)d - 1) / (unsigned long long)d);
+    sh = s - 1;
+}
+__device__ __forceinline__ unsigned udiv_m(unsigned n, unsigned m, int sh) {
+    return m ? __umulhi(n, m) >> sh : n;
+}
 
 
 __device__ __forceinline__ bool on_bdr(const HexGrid &g, long long gx, long long gy, long long gz) {
@@ -418,6 +435,9 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const T *Gt = Tab<T>::G(C);
     const int le = threadIdx.x / T2;
     const int t = threadIdx.x - le * T2;
+    // the lane's coordinates in the (Q, Q) and (N1, N1) phase layouts
+    const int tdq = t / Q, tq = t - tdq * Q;
+    const int tdn = t / N1, tn = t - tdn * N1;
     // E < 2^31 (checked at creation), so 32-bit unsigned division is exact
     // and avoids the 64-bit division sequence
     const unsigned eu = lb * (unsigned)NE + (unsigned)le;
@@ -425,11 +445,10 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const bool valid = e < g.E;
     int ex = 0, ey = 0, ez = 0;
     if (valid) {
-        const unsigned nxu = (unsigned)g.nx, nyu = (unsigned)g.ny;
-        const unsigned r = eu / nxu;
-        ex = (int)(eu - r * nxu);
-        const unsigned r2 = r / nyu;
-        ey = (int)(r - r2 * nyu);
+        const unsigned r = udiv_m(eu, g.mx, g.shx);
+        ex = (int)(eu - r * (unsigned)g.nx);
+        const unsigned r2 = udiv_m(r, g.my, g.shy);
+        ey = (int)(r - r2 * (unsigned)g.ny);
         ez = (int)r2;
     }
 
@@ -446,7 +465,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
 #pragma unroll
             for (int c = 0; c < Q; ++c) sqv[c] = qd[(size_t)c * plane + off1];
             if constexpr (GEO) {
-                const int a = t % Q, b = t / Q;
+                const int a = tq, b = tdq;
                 xa = Tab<T>::X(Q)[a];
                 yb = Tab<T>::X(Q)[b];
                 wa = Tab<T>::W(Q)[a];
@@ -482,16 +501,21 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
 
     // ---- X: lane (j,k) owns an x-line
     if (valid && t < N1 * N1) {
-        const int j = t % N1, k = t / N1;
-        const long long gy = (long long)ey * P + j, gz = (long long)ez * P + k;
-        const long long base = g.Nx * (gy + g.Ny * gz) + (long long)ex * P;
-        const bool rowb = dirichlet && (gy == 0 || gy == g.Ny - 1 || gz == 0 || gz == g.Nz - 1);
+        const int j = tn, k = tdn;
+        // 32-bit row index gy + Ny gz (Ny Nz < 2^32, checked at creation)
+        const unsigned gy = (unsigned)(ey * P + j), gz = (unsigned)(ez * P + k);
+        const unsigned Nyu = (unsigned)g.Ny, Nzu = (unsigned)g.Nz;
+        const T *xr = x + (size_t)(gy + Nyu * gz) * (size_t)g.Nx + (size_t)(ex * P);
+        const bool rowb = dirichlet && (gy == 0 || gy == Nyu - 1 || gz == 0 || gz == Nzu - 1);
+        const bool lo = dirichlet && ex == 0, hi = dirichlet && ex == g.nx - 1;
         T u[N1];
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-            const long long gx = (long long)ex * P + i;
-            const bool b = rowb || (dirichlet && (gx == 0 || gx == g.Nx - 1));
-            u[i] = b ? T(0) : x[base + i];
+            // every x entry exists, so the load is unconditional and the
+            // essential DOFs are zeroed by a select
+            const T v = xr[i];
+            const bool b = rowb || (i == 0 && lo) || (i == P && hi);
+            u[i] = b ? T(0) : v;
         }
         if constexpr (EO) {
             const EOs<N1, T> su = eo_split(u);
@@ -522,7 +546,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const bool ylane = valid && t < Q * N1;
     T A_[N1], Ag_[N1];
     if (ylane) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
             A_[j] = sA[iA(le, k, j, a)];
@@ -531,7 +555,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
     if constexpr (ALIAS) __syncthreads();
     if (ylane && EO) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
         const EOs<N1, T> sa = eo_split(A_);
         T bb[Q], bg[Q], gb[Q];
         eo_contract<Q, N1, 1>(E0, sa, bb);
@@ -546,7 +570,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) { sBG[iB(le, k, b, a)] = bg[b]; sGB[iB(le, k, b, a)] = gb[b]; }
         }
     } else if (ylane) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
             T bb = 0, bg = 0, gb = 0;
@@ -566,7 +590,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
 
     // ---- Z + pointwise + Z^T: lane (qx,qy) owns a z-column
     if (valid) {
-        const int a = t % Q, b = t / Q;
+        const int a = tq, b = tdq;
         T bb[N1], bg[N1], gb[N1];
         T c0[N1], c1[N1], c2[N1];
 #pragma unroll
@@ -753,7 +777,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     // ---- Y^T: lane (qx,k)
     T C0[Q], C1[Q], C2[Q];
     if (ylane) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
 #pragma unroll
         for (int b = 0; b < Q; ++b) {
             C2[b] = sBB[iB(le, k, b, a)];
@@ -762,7 +786,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
     if constexpr (ALIAS) __syncthreads();
     if (ylane && EO) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
         T fg[N1], fb[N1];
         if constexpr (DIFF) {
             eo_contract<N1, Q, 1>(E2, eo_split(C0), fg);
@@ -776,7 +800,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
             if constexpr (DIFF) sAg[iA(le, k, j, a)] = fg[j];
         }
     } else if (ylane) {
-        const int a = t % Q, k = t / Q;
+        const int a = tq, k = tdq;
 #pragma unroll
         for (int j = 0; j < N1; ++j) {
             T fg = 0, fb = 0;
@@ -799,7 +823,7 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     // which writes the sum of the shared x-face DOF.
     T sv[N1];
     const bool xt = valid && t < N1 * N1;
-    const int jx = t % N1, kx = t / N1;
+    const int jx = tn, kx = tdn;
     if (xt) {
         T FG[Q], FB[Q];
 #pragma unroll
@@ -826,7 +850,9 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     }
     __syncthreads();
     if (xt) {
-        T *dst = y + mrow_index(g, N1, ey, ez, jx, kx) + (size_t)ex * P;
+        // 32-bit element-vector row (ny nz N1^2 < 2^32, checked at creation)
+        const unsigned mrow = (((unsigned)ez * (unsigned)g.ny + (unsigned)ey) * N1 + kx) * N1 + jx;
+        T *dst = y + (size_t)mrow * (size_t)g.Nxp + (size_t)(ex * P);
         if (ex == 0) dst[0] = sv[0];
         else if (le == 0) y[seam_index(g, N1, lb, jx, kx)] = sv[0];
 #pragma unroll
@@ -2036,8 +2062,13 @@ static HexOp *make_hex(femgpu_ctx *c, const femgpu_hex_desc &d, const double *co
         g.Nx = (long long)d.nx * d.p + 1; g.Ny = (long long)d.ny * d.p + 1; g.Nz = (long long)d.nz * d.p + 1;
         g.ne = ne_of(d.q);
         g.Nxp = g.Nx + (g.Nx & 1);
+        div_magic(g.nx, g.mx, g.shx);
+        div_magic(g.ny, g.my, g.shy);
         op->n = (uint64_t)g.Nx * g.Ny * g.Nz;
-        FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536 && g.E < (1ll << 31),
+        // the element kernels index DOF rows (gy + Ny gz) and element-vector
+        // rows in 32-bit arithmetic
+        FG_REQUIRE(op->n < (1ull << 40) && g.Nx * g.Ny < (1ll << 31) && g.Nz < 65536 && g.E < (1ll << 31) &&
+                       g.Ny * g.Nz < (1ll << 32) && (long long)d.ny * d.nz * (d.p + 1) * (d.p + 1) < (1ll << 32),
                    FEMGPU_ERR_INVALID_ARG, "pa_create_hex: mesh too large");
         const size_t nv = (size_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
         hipStream_t s = c->stream;
