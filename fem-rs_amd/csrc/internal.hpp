@@ -182,11 +182,13 @@ struct femgpu_op {
     // the results of that pair bit for bit (same per-entry arithmetic, and the
     // partial sums over the fixed red_blocks(n) x 256 grid-stride grid).
     //  apply_cheb_dev: t = A x; d = c1 d + c2 inv (r - t); z += d; with last,
-    //                  part[block] = sum r.z  (x may alias z)
+    //                  part[block] = sum r.z  (x may alias z).  With first,
+    //                  z equals d on entry and is not read (x = d): z = d_old + d
     //  apply_dot_dev:  y = A x; part[block] = sum x.y
     virtual bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d,
-                                double c1, double c2, int last, double *part, const int *skip) {
+                                double c1, double c2, int last, double *part, const int *skip, int first) {
         (void)x; (void)r; (void)inv; (void)z; (void)d; (void)c1; (void)c2; (void)last; (void)part; (void)skip;
+        (void)first;
         return false;
     }
     virtual bool apply_dot_dev(const void *x, void *y, double *part, const int *skip) {

@@ -1273,10 +1273,12 @@ template <> struct Pair<float> { using type = float2; };
 // divisions and essential-DOF tests are shared by the segment's DOFs.
 // Essential DOFs take x.  Returns the DOF count.
 // I: index type of the element vector (uint32_t when it has < 2^31 entries).
-template <class T, int P, class I>
+// XESS: the essential DOFs of the segment take x (y = x there); otherwise x is
+// not read and their bits are set in *ess instead.
+template <class T, int P, class I, bool XESS = true>
 __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restrict__ ye,
                                               const T *__restrict__ x, int dirichlet, int ex, int gy,
-                                              int gz, size_t row0, T (&out)[P + 1]) {
+                                              int gz, size_t row0, T (&out)[P + 1], unsigned *ess = nullptr) {
     constexpr int N1 = P + 1;
     const int cnt = ex == g.nx - 1 ? P + 1 : P;
     // y and z candidates: [0] = (e0 - 1, local P) on an element face, [1] = (e0, i0)
@@ -1335,7 +1337,10 @@ __device__ __forceinline__ int gather_segment(const HexGrid &g, const T *__restr
     for (int i = 0; i <= P; ++i) {
         if (i >= cnt) break;
         const int gx = ex * P + i;
-        if (rowbdr || (dirichlet && (gx == 0 || gx == g.Nx - 1))) out[i] = x[base + i];
+        if (rowbdr || (dirichlet && (gx == 0 || gx == g.Nx - 1))) {
+            if constexpr (XESS) out[i] = x[base + i];
+            else *ess |= 1u << i;
+        }
     }
     return cnt;
 }
@@ -1381,8 +1386,12 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
 
 // Gather fused with the Chebyshev step that consumes it (the solver's k_chebk):
 // t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z
-// over the fixed red_blocks(n) x 256 grid-stride grid of segments.
-template <class T, int P, class I, int BS>
+// over the fixed red_blocks(n) x 256 grid-stride grid of segments.  x may
+// alias z: each thread reads x only at its own segment's DOFs, before it
+// writes them.  FIRST: z equals d on entry (the element kernel applied A to
+// d): z is written, not read, and x is not read either (at the essential DOFs
+// t = d).
+template <class T, int P, class I, int BS, bool FIRST>
 __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
     const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
     const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2, int last,
@@ -1395,15 +1404,18 @@ __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
         seg_coords(g, sid, ex, gy, gz);
         const size_t row0 = (size_t)g.Nx * ((size_t)gy + (size_t)g.Ny * gz);
         T t[P + 1];
-        const int cnt = gather_segment<T, P, I>(g, ye, x, dirichlet, ex, gy, gz, row0, t);  // reads x of its DOFs
+        unsigned ess = 0;
+        const int cnt = gather_segment<T, P, I, !FIRST>(g, ye, x, dirichlet, ex, gy, gz, row0, t, &ess);
         const size_t base = row0 + (size_t)ex * P;
 #pragma unroll
         for (int i = 0; i <= P; ++i) {
             if (i >= cnt) break;
             const size_t k = base + i;
-            const T di = (T)c1 * d[k] + (T)c2 * (inv[k] * (r[k] - t[i]));
+            const T dk = d[k];
+            const T ti = (FIRST && ((ess >> i) & 1u)) ? dk : t[i];
+            const T di = (T)c1 * dk + (T)c2 * (inv[k] * (r[k] - ti));
             d[k] = di;
-            const T zi = z[k] + di;
+            const T zi = (FIRST ? dk : z[k]) + di;
             z[k] = zi;
             if (last) v[0] += (double)r[k] * (double)zi;
         }
@@ -1812,7 +1824,7 @@ struct HexOp final : femgpu_op {
     }
 
     template <class T> void fused_t(const T *x, const T *r, const T *inv, T *z, T *d, double c1, double c2,
-                                    int last, T *y, double *part, const int *skip) {
+                                    int last, T *y, double *part, const int *skip, int first = 0) {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
@@ -1845,8 +1857,10 @@ struct HexOp final : femgpu_op {
     do {                                                                                                    \
         const dim3 block(BS_);                                                                              \
         if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_, I_, BS_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
-        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, c2, \
-                                last, part, dir, skip);                                                     \
+        else if (first) hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, true>), grid, block, 0, s, g, ye, x, r,  \
+                                           inv, z, d, c1, c2, last, part, dir, skip);                                  \
+        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, false>), grid, block, 0, s, g, ye, x, r, inv, z, d,  \
+                                c1, c2, last, part, dir, skip);                                                     \
     } while (0)
         const bool small = evec_small(g, desc.p + 1);
         switch (desc.p) {
@@ -1862,13 +1876,13 @@ struct HexOp final : femgpu_op {
     }
 
     bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d, double c1, double c2,
-                        int last, double *part, const int *skip) override {
+                        int last, double *part, const int *skip, int first) override {
         if (dtype == FEMGPU_F64)
             fused_t<double>((const double *)x, (const double *)r, (const double *)inv, (double *)z, (double *)d,
-                            c1, c2, last, nullptr, part, skip);
+                            c1, c2, last, nullptr, part, skip, first);
         else
             fused_t<float>((const float *)x, (const float *)r, (const float *)inv, (float *)z, (float *)d, c1, c2,
-                           last, nullptr, part, skip);
+                           last, nullptr, part, skip, first);
         return true;
     }
 

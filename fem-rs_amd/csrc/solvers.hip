@@ -215,12 +215,12 @@ template <class T>
 __global__ __launch_bounds__(256) void k_cheb0(size_t n, const T *__restrict__ r,
                                                const T *__restrict__ inv, T *__restrict__ z,
                                                T *__restrict__ d, double inv_theta, int last,
-                                               double *__restrict__ part, const int *skip) {
+                                               double *__restrict__ part, const int *skip, int write_z) {
     if (skip && *skip) return;
     double v[1] = {0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         const T zi = (T)inv_theta * (inv[i] * r[i]);
-        z[i] = zi;
+        if (write_z) z[i] = zi;
         d[i] = zi;
         if (last) v[0] += (double)r[i] * (double)zi;
     }
@@ -230,19 +230,21 @@ __global__ __launch_bounds__(256) void k_cheb0(size_t n, const T *__restrict__ r
     }
 }
 
-// d = c1 d + c2 inv (r - t); z += d  [if last: partial r.z]
+// d = c1 d + c2 inv (r - t); z += d  [if last: partial r.z]; with first, z
+// equals d on entry and is not read
 template <class T>
 __global__ __launch_bounds__(256) void k_chebk(size_t n, const T *__restrict__ r,
                                                const T *__restrict__ t, const T *__restrict__ inv,
                                                T *__restrict__ z, T *__restrict__ d, double c1,
                                                double c2, int last, double *__restrict__ part,
-                                               const int *skip) {
+                                               const int *skip, int first) {
     if (skip && *skip) return;
     double v[1] = {0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const T di = (T)c1 * d[i] + (T)c2 * (inv[i] * (r[i] - t[i]));
+        const T dk = d[i];
+        const T di = (T)c1 * dk + (T)c2 * (inv[i] * (r[i] - t[i]));
         d[i] = di;
-        const T zi = z[i] + di;
+        const T zi = (first ? dk : z[i]) + di;
         z[i] = zi;
         if (last) v[0] += (double)r[i] * (double)zi;
     }
@@ -567,7 +569,8 @@ __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__
                                                       const T *__restrict__ ap, const T *__restrict__ inv,
                                                       T *__restrict__ z, T *__restrict__ d,
                                                       double *__restrict__ part, int nb, int *flags,
-                                                      int rz_in, double inv_theta, int rz_out, double brk) {
+                                                      int rz_in, double inv_theta, int rz_out, double brk,
+                                                      int write_z) {
     __shared__ double lds[2][16];
     __shared__ int stop;
     const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
@@ -604,7 +607,7 @@ __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__
         r[i] = ri;
         v[0] += (double)ri * (double)ri;
         const T zi = (T)inv_theta * (qi * ri);
-        z[i] = zi;
+        if (write_z) z[i] = zi;
         d[i] = zi;
         if (rz_out >= 0) v[1] += (double)ri * (double)zi;
     }
@@ -795,7 +798,10 @@ template <class T> struct Krylov {
     }
 
     // z = cheb(r) with partial r.z into part slot rz_slot (skippable).  With
-    // from1 the first step (z = d = inv r / theta) has been done by the caller.
+    // from1 the first step (z = d = inv r / theta) has been done by the caller,
+    // which for degree >= 2 wrote d only.  For degree >= 2 the first step
+    // leaves z unwritten and the next one applies A to d and forms z = d + d'
+    // (one vector write and one read fewer, the same values).
     void cheb_apply(const ChebParams &cp, const T *r, const T *inv, T *z, int rz_slot = 0,
                     bool from1 = false) {
         T *d = vec(W_D), *t = vec(W_T);
@@ -806,14 +812,17 @@ template <class T> struct Krylov {
         const unsigned nb = nblk(n);
         if (!from1)
             hipLaunchKernelGGL(k_cheb0<T>, dim3(nb), dim3(256), 0, s, n, r, inv, z, d, 1.0 / theta,
-                               cp.degree == 1 ? 1 : 0, pz, flags);
+                               cp.degree == 1 ? 1 : 0, pz, flags, cp.degree == 1 ? 1 : 0);
         for (int k = 1; k < cp.degree; ++k) {
             const double rho = 1.0 / (2.0 * sigma - rho_prev);
             const int last = k == cp.degree - 1 ? 1 : 0;
-            if (!op->apply_cheb_dev(z, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, pz, flags)) {
-                apply(z, t, true);
+            const int first = k == 1 ? 1 : 0;
+            const T *xin = first ? d : z;
+            if (!op->apply_cheb_dev(xin, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, pz, flags,
+                                    first)) {
+                apply(xin, t, true);
                 hipLaunchKernelGGL(k_chebk<T>, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
-                                   rho * rho_prev, 2.0 * rho / delta, last, pz, flags);
+                                   rho * rho_prev, 2.0 * rho / delta, last, pz, flags, first);
             }
             rho_prev = rho;
         }
@@ -1042,7 +1051,8 @@ template <class T> struct Krylov {
             }
             hipLaunchKernelGGL(k_update_cheb0<T>, dim3(nb), dim3(256), 0, s, n, S, x, (const T *)p, r,
                                (const T *)ap, inv, z, d, part, (int)nb, flags, rz_prev, inv_theta,
-                               cp.degree == 1 ? rz_new : -1, brk);
+                               cp.degree == 1 ? rz_new : -1, brk,
+                               (cp.degree == 1 || (segment && it + 1 == max_iter)) ? 1 : 0);
             hipLaunchKernelGGL(k_fin_conv, dim3(1), dim3(1024), 0, s, (const double *)(part + kRedBlocks),
                                (int)nb, S, flags, (int)MODE_PCG, 0, 0, rtol, atol, (double)(it + 1));
             record();
