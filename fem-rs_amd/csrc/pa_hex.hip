@@ -1386,15 +1386,16 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
 
 // Gather fused with the Chebyshev step that consumes it (the solver's k_chebk):
 // t = (A x)_i; d = c1 d + c2 inv (r - t); z += d; with last, partials of r.z
-// over the fixed red_blocks(n) x 256 grid-stride grid of segments.  x may
+// over the fixed red_blocks(n) x 256 grid-stride grid of segments (and d is
+// not stored: the polynomial ends there).  x may
 // alias z: each thread reads x only at its own segment's DOFs, before it
 // writes them.  FIRST: z equals d on entry (the element kernel applied A to
 // d): z is written, not read, and x is not read either (at the essential DOFs
 // t = d).
-template <class T, int P, class I, int BS, bool FIRST>
+template <class T, int P, class I, int BS, bool FIRST, bool LAST>
 __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
     const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
-    const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2, int last,
+    const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2,
     double *__restrict__ part, int dirichlet, const int *skip) {
     if (skip && *skip) return;
     const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
@@ -1414,13 +1415,13 @@ __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
             const T dk = d[k];
             const T ti = (FIRST && ((ess >> i) & 1u)) ? dk : t[i];
             const T di = (T)c1 * dk + (T)c2 * (inv[k] * (r[k] - ti));
-            d[k] = di;
+            if constexpr (!LAST) d[k] = di;  // the last step's d is not read again
             const T zi = (FIRST ? dk : z[k]) + di;
             z[k] = zi;
-            if (last) v[0] += (double)r[k] * (double)zi;
+            if constexpr (LAST) v[0] += (double)r[k] * (double)zi;
         }
     }
-    if (last) {
+    if constexpr (LAST) {
         block_sum<1, BS>(v);
         if (threadIdx.x == 0) part[blockIdx.x] = v[0];
     }
@@ -1853,14 +1854,17 @@ struct HexOp final : femgpu_op {
         else if (gbs == 1024) GF3(P_, I_, 1024);                                                           \
         else GF3(P_, I_, 256);                                                                             \
     } while (0)
+#define GFC(P_, I_, BS_, F_, L_)                                                                            \
+    hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, F_, L_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, \
+                       c2, part, dir, skip)
 #define GF3(P_, I_, BS_)                                                                                    \
     do {                                                                                                    \
         const dim3 block(BS_);                                                                              \
         if (y) hipLaunchKernelGGL((k_hex_gather_dot<T, P_, I_, BS_>), grid, block, 0, s, g, ye, x, y, part, dir, skip); \
-        else if (first) hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, true>), grid, block, 0, s, g, ye, x, r,  \
-                                           inv, z, d, c1, c2, last, part, dir, skip);                                  \
-        else hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, false>), grid, block, 0, s, g, ye, x, r, inv, z, d,  \
-                                c1, c2, last, part, dir, skip);                                                     \
+        else if (first && last) GFC(P_, I_, BS_, true, true);                                              \
+        else if (first) GFC(P_, I_, BS_, true, false);                                                      \
+        else if (last) GFC(P_, I_, BS_, false, true);                                                       \
+        else GFC(P_, I_, BS_, false, false);                                                     \
     } while (0)
         const bool small = evec_small(g, desc.p + 1);
         switch (desc.p) {
@@ -1870,6 +1874,7 @@ struct HexOp final : femgpu_op {
         case 4: GF(4); break;
         default: GF(5); break;
         }
+#undef GFC
 #undef GF3
 #undef GF2
 #undef GF

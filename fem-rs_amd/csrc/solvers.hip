@@ -230,25 +230,24 @@ __global__ __launch_bounds__(256) void k_cheb0(size_t n, const T *__restrict__ r
     }
 }
 
-// d = c1 d + c2 inv (r - t); z += d  [if last: partial r.z]; with first, z
-// equals d on entry and is not read
-template <class T>
+// d = c1 d + c2 inv (r - t); z += d  [if last: partial r.z, and d is not
+// stored]; with first, z equals d on entry and is not read
+template <class T, bool FIRST, bool LAST>
 __global__ __launch_bounds__(256) void k_chebk(size_t n, const T *__restrict__ r,
                                                const T *__restrict__ t, const T *__restrict__ inv,
                                                T *__restrict__ z, T *__restrict__ d, double c1,
-                                               double c2, int last, double *__restrict__ part,
-                                               const int *skip, int first) {
+                                               double c2, double *__restrict__ part, const int *skip) {
     if (skip && *skip) return;
     double v[1] = {0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         const T dk = d[i];
         const T di = (T)c1 * dk + (T)c2 * (inv[i] * (r[i] - t[i]));
-        d[i] = di;
-        const T zi = (first ? dk : z[i]) + di;
+        if constexpr (!LAST) d[i] = di;  // the last step's d is not read again
+        const T zi = (FIRST ? dk : z[i]) + di;
         z[i] = zi;
-        if (last) v[0] += (double)r[i] * (double)zi;
+        if constexpr (LAST) v[0] += (double)r[i] * (double)zi;
     }
-    if (last) {
+    if constexpr (LAST) {
         block_sum<1, 256>(v);
         if (threadIdx.x == 0) part[blockIdx.x] = v[0];
     }
@@ -821,8 +820,10 @@ template <class T> struct Krylov {
             if (!op->apply_cheb_dev(xin, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, pz, flags,
                                     first)) {
                 apply(xin, t, true);
-                hipLaunchKernelGGL(k_chebk<T>, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
-                                   rho * rho_prev, 2.0 * rho / delta, last, pz, flags, first);
+                auto kk = first ? (last ? k_chebk<T, true, true> : k_chebk<T, true, false>)
+                                : (last ? k_chebk<T, false, true> : k_chebk<T, false, false>);
+                hipLaunchKernelGGL(kk, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
+                                   rho * rho_prev, 2.0 * rho / delta, pz, flags);
             }
             rho_prev = rho;
         }
