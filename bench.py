@@ -161,6 +161,11 @@ def parse():
     ap.add_argument("--unet-passes", type=int, default=0,
                     help="time only the U-Net: this many forward + backward passes on a --unet-grid^3 grid")
     ap.add_argument("--unet-grid", type=int, default=129)
+    ap.add_argument("--sumfac-probe", action="store_true",
+                    help="time only the batched sum-factorisation sweep on the vector unit and on MFMA "
+                         "(femgpu_sumfac_probe) at the Q3 and Q4 shapes, FP64 and FP32")
+    ap.add_argument("--probe-blocks", type=int, default=4096)
+    ap.add_argument("--probe-reps", type=int, default=64)
     ap.add_argument("--coarsen", type=int, default=0, help="learned: network grid coarsening (0: the workload's, else the smallest "
                     "divisor leaving at most 64 intervals per direction)")
     return ap.parse_args()
@@ -819,6 +824,36 @@ def run_unet(a):
                       "backward_ms_incl_host_copies": round(1e3 * tb / k, 3)}))
 
 
+def run_sumfac_probe(a):
+    """One batched sweep Out = [B u | G u] (lines of p + 1 nodal values, q
+    Gauss points; 256 LDS-resident lines per workgroup, --probe-reps
+    repetitions per launch) on each engine: the vector unit with multiply-add
+    chains, the vector unit in the even-odd form (the production element
+    kernel's), and the matrix cores (v_mfma_f64_16x16x4f64 /
+    v_mfma_f32_16x16x4f32, 16 lines per tile).  Useful flops count 2 (p + 1)
+    2q per line and repetition; the padding an MFMA tile carries is not
+    counted."""
+    import femgpu as fg
+    ctx = fg.default_context()
+    rows = []
+    for p, q in ((3, 4), (4, 5)):
+        for name, dt, fdt in (("f64", np.float64, fg.F64), ("f32", np.float32, fg.F32)):
+            lines = a.probe_blocks * 256
+            x = fg.DeviceArray(ctx, lines * (p + 1), dt).upload(
+                np.random.default_rng(0).standard_normal(lines * (p + 1)).astype(dt))
+            y = fg.DeviceArray(ctx, lines * 2 * q, dt)
+            flops = 2.0 * (p + 1) * 2 * q * lines * a.probe_reps
+            for engine in fg.SUMFAC_ENGINES:
+                ms = fg.sumfac_probe(p, q, fdt, engine, a.probe_blocks, a.probe_reps, x, y, launches=5, ctx=ctx)
+                rows.append({"p": p, "q": q, "dtype": name, "engine": engine, "ms_per_launch": round(ms, 4),
+                             "useful_tflops": round(flops / (ms * 1e-3) / 1e12, 2)})
+                print(f"bench.py: probe p={p} q={q} {name} {engine}: {ms:.4f} ms, "
+                      f"{rows[-1]['useful_tflops']} TFLOP/s useful", file=sys.stderr, flush=True)
+            del x, y
+    print(json.dumps({"sumfac_probe": rows, "blocks": a.probe_blocks, "lines_per_block": 256,
+                      "reps_per_launch": a.probe_reps}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -829,6 +864,8 @@ def main():
 
     if a.unet_passes > 0:
         return run_unet(a)
+    if a.sumfac_probe:
+        return run_sumfac_probe(a)
     w = WORKLOADS[a.workload]
     if w["form"] == "csr":
         return run_csr(a, w)

@@ -2191,11 +2191,170 @@ bool hex_grid_dims(femgpu_op *op_, int *Nx, int *Ny, int *Nz) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Contraction probe: the vector unit against the matrix cores.  One batched
+// sweep of the element kernels, Out[line] = [B u | G u] for lines u of N1
+// nodal values (Q outputs for each table), on 256 LDS-resident lines per
+// workgroup, repeated reps times; each repetition reads the lines from LDS,
+// writes the 2Q outputs to LDS and feeds them back (u_k += 2^-10 (Out_k +
+// Out_{Q+k})), with the same two barriers for every engine, so the timings
+// compare the one-dimensional products and their LDS hand-off alone.
+//   ENGINE 0: vector unit, one line per lane, multiply-add chains (k_pa_hex's
+//             form without the even-odd tables);
+//   ENGINE 1: vector unit, even-odd form (k_pa_hex's geometry kernel);
+//   ENGINE 2: matrix cores, 16 lines per v_mfma_*_16x16x4 tile: A = 16 lines x
+//             4 nodal values (lane l: line l % 16, value l / 16 + 4 s), B = 4
+//             x 16 of [B^T | G^T] (zero-padded to 16 outputs and to whole
+//             K-steps of 4), D = 16 x 16 (lane l, result register v: output
+//             l % 16 of line MfmaTile<T>::row(l, v); measured on gfx950: rows
+//             4 (l / 16) + v in FP32, (l / 16) + 4 v in FP64).
+template <class T> struct MfmaTile;
+template <> struct MfmaTile<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
+};
+template <> struct MfmaTile<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
+};
+
+template <int N1, int Q, class T, int ENGINE>
+__global__ __launch_bounds__(256) void k_sumfac_probe(const T *__restrict__ in, T *__restrict__ out, int reps) {
+    constexpr int L = 256, NO = 2 * Q;
+    constexpr int SI = N1 | 1, SO = NO + 1;  // odd line strides
+    static_assert(NO <= 16, "one MFMA tile holds the outputs");
+    __shared__ T sI[L * SI];
+    __shared__ T sO[L * SO];
+    const int tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * L;
+    for (int i = tid; i < L * N1; i += L) sI[(i / N1) * SI + i % N1] = in[base * N1 + i];
+    __syncthreads();
+    constexpr int C = combo(N1, Q);
+    const T *Bt = Tab<T>::B(C), *Gt = Tab<T>::G(C);
+    constexpr int KS = (N1 + 3) / 4;
+    const int lane = tid & 63, w = tid >> 6;
+    T breg[KS];
+    if constexpr (ENGINE == 2) {
+#pragma unroll
+        for (int st = 0; st < KS; ++st) {
+            const int k = lane / 16 + 4 * st, n = lane % 16;
+            breg[st] = (k < N1 && n < Q) ? Bt[n * N1 + k] : (k < N1 && n < NO) ? Gt[(n - Q) * N1 + k] : T(0);
+        }
+    }
+    const T eps = T(1.0 / 1024.0);
+    for (int r = 0; r < reps; ++r) {
+        if constexpr (ENGINE == 2) {
+            using V = typename MfmaTile<T>::v4;
+#pragma unroll
+            for (int gq = 0; gq < 4; ++gq) {
+                const int g = w * 4 + gq;
+                V acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int st = 0; st < KS; ++st) {
+                    const int k = lane / 16 + 4 * st;
+                    const T a = k < N1 ? sI[(16 * g + lane % 16) * SI + k] : T(0);
+                    acc = MfmaTile<T>::mma(a, breg[st], acc);
+                }
+                const int n = lane % 16;
+                if (n < NO) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) sO[(16 * g + MfmaTile<T>::row(lane, i)) * SO + n] = acc[i];
+                }
+            }
+        } else {
+            T u[N1];
+#pragma unroll
+            for (int k = 0; k < N1; ++k) u[k] = sI[tid * SI + k];
+            T sb[Q], sg[Q];
+            if constexpr (ENGINE == 1) {
+                const EOs<N1, T> su = eo_split(u);
+                eo_contract<Q, N1, 1>(eo_tab<T>(C, 0), su, sb);
+                eo_contract<Q, N1, -1>(eo_tab<T>(C, 1), su, sg);
+            } else {
+#pragma unroll
+                for (int a = 0; a < Q; ++a) {
+                    T x0 = 0, x1 = 0;
+#pragma unroll
+                    for (int i = 0; i < N1; ++i) {
+                        x0 += Bt[a * N1 + i] * u[i];
+                        x1 += Gt[a * N1 + i] * u[i];
+                    }
+                    sb[a] = x0;
+                    sg[a] = x1;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                sO[tid * SO + a] = sb[a];
+                sO[tid * SO + Q + a] = sg[a];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < N1; ++k)
+            sI[tid * SI + k] += eps * (sO[tid * SO + k] + sO[tid * SO + Q + k]);
+        __syncthreads();
+    }
+    for (int a = 0; a < NO; ++a) out[(base + tid) * NO + a] = sO[tid * SO + a];
+}
+
 }  // namespace femgpu
 
 using namespace femgpu;
 
 extern "C" {
+
+int femgpu_sumfac_probe(femgpu_ctx *c, int p, int q, femgpu_dtype dtype, int engine, int nblocks, int reps,
+                        const void *in, void *out, int launches, double *ms) {
+    return guard([&] {
+        FG_REQUIRE(c && in && out && ms && nblocks > 0 && reps > 0 && launches > 0 && engine >= 0 && engine <= 2,
+                   FEMGPU_ERR_INVALID_ARG, "sumfac_probe: bad argument");
+        FG_REQUIRE((p == 3 && q == 4) || (p == 4 && q == 5), FEMGPU_ERR_UNSUPPORTED,
+                   "sumfac_probe: (p, q) must be (3, 4) or (4, 5)");
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        FG_HIP(hipSetDevice(c->device));
+        ensure_tables(c->device);
+        hipStream_t s = c->stream;
+        auto launch = [&]() {
+#define SP(N1_, Q_, T_, E_)                                                                                   \
+    hipLaunchKernelGGL((k_sumfac_probe<N1_, Q_, T_, E_>), dim3(nblocks), dim3(256), 0, s, (const T_ *)in, \
+                       (T_ *)out, reps)
+#define SPE(N1_, Q_, T_)                                                                                      \
+    do {                                                                                                      \
+        if (engine == 0) SP(N1_, Q_, T_, 0);                                                                  \
+        else if (engine == 1) SP(N1_, Q_, T_, 1);                                                             \
+        else SP(N1_, Q_, T_, 2);                                                                              \
+    } while (0)
+            if (p == 3) {
+                if (dtype == FEMGPU_F64) SPE(4, 4, double);
+                else SPE(4, 4, float);
+            } else {
+                if (dtype == FEMGPU_F64) SPE(5, 5, double);
+                else SPE(5, 5, float);
+            }
+#undef SPE
+#undef SP
+        };
+        launch();  // warm-up
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipEventRecord(c->t0, s));
+        for (int i = 0; i < launches; ++i) launch();
+        FG_HIP(hipEventRecord(c->t1, s));
+        FG_HIP(hipGetLastError());
+        FG_HIP(hipEventSynchronize(c->t1));
+        float tot = 0.f;
+        FG_HIP(hipEventElapsedTime(&tot, c->t0, c->t1));
+        *ms = tot / launches;
+        return 0;
+    });
+}
+
 
 int femgpu_pa_create_hex(femgpu_ctx *c, const femgpu_hex_desc *d, const double *coords,
                          femgpu_op **out) {

@@ -118,6 +118,7 @@ _sig("femgpu_op_dtype", [P, C.POINTER(I)])
 _sig("femgpu_op_diagonal", [P, P, I])
 _sig("femgpu_op_destroy", [P])
 _sig("femgpu_op_time_apply", [P, P, P, I, C.POINTER(D), C.POINTER(D)])
+_sig("femgpu_sumfac_probe", [P, I, I, I, I, I, I, P, P, I, C.POINTER(D)])
 _APPLY_FN = C.CFUNCTYPE(I, P, P, P, U64)
 _sig("femgpu_op_create_callback", [P, U64, I, I, _APPLY_FN, P, P, C.POINTER(P)])
 _sig("femgpu_residual_history", [P, P, U64, C.POINTER(U64)])
@@ -845,6 +846,25 @@ def _solve_vectors(op, b, x):
 
 def _len(v):
     return v.n if isinstance(v, DeviceArray) else len(v)
+
+
+SUMFAC_ENGINES = ("valu", "valu_even_odd", "mfma")
+
+
+def sumfac_probe(p, q, dtype, engine, nblocks, reps, x_in, y_out, launches=5, ctx=None):
+    """Time one batched sum-factorisation sweep (femgpu_sumfac_probe): 256
+    lines of p + 1 values per workgroup, Out = [B u | G u] at q Gauss points,
+    reps times per launch, on the vector unit ("valu", "valu_even_odd") or the
+    matrix cores ("mfma").  x_in: nblocks * 256 * (p + 1) device values,
+    y_out: nblocks * 256 * 2q.  Returns ms per launch."""
+    ctx = ctx or default_context()
+    eng = SUMFAC_ENGINES.index(engine)
+    if x_in.n != nblocks * 256 * (p + 1) or y_out.n != nblocks * 256 * 2 * q:
+        raise ValueError(f"sumfac_probe: sizes {x_in.n}, {y_out.n} do not fit {nblocks} blocks")
+    ms = D()
+    _check(lib.femgpu_sumfac_probe(ctx.h, p, q, dtype, eng, nblocks, reps, x_in.ptr, y_out.ptr, launches,
+                                   C.byref(ms)))
+    return ms.value
 
 
 def solve_cg_operator(nrows, ncols, op, b, x, cfg=None):

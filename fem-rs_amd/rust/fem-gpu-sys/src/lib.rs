@@ -187,6 +187,9 @@ extern "C" {
     pub fn femgpu_op_destroy(op: *mut femgpu_op) -> c_int;
     pub fn femgpu_op_time_apply(op: *mut femgpu_op, x: *const c_void, y: *mut c_void, reps: c_int,
                                 ms_apply: *mut f64, ms_kernel: *mut f64) -> c_int;
+    pub fn femgpu_sumfac_probe(ctx: *mut femgpu_ctx, p: c_int, q: c_int, dtype: femgpu_dtype, engine: c_int,
+                               nblocks: c_int, reps: c_int, input: *const c_void, output: *mut c_void,
+                               launches: c_int, ms: *mut f64) -> c_int;
     pub fn femgpu_op_create_callback(ctx: *mut femgpu_ctx, n: u64, dtype: femgpu_dtype, mem: femgpu_mem,
                                      apply: femgpu_apply_fn, user: *mut c_void, diag: *const c_void,
                                      out: *mut *mut femgpu_op) -> c_int;

@@ -104,6 +104,15 @@ int femgpu_op_destroy(femgpu_op *op);
  * around it on the context stream (-1 if not available). */
 int femgpu_op_time_apply(femgpu_op *op, const void *x, void *y, int reps, double *ms_apply,
                          double *ms_kernel);
+/* Contraction probe (measurement only; no reference counterpart): times one
+ * batched sum-factorisation sweep, Out = [B u | G u] over lines u of p + 1
+ * nodal values at q Gauss points, on 256 LDS-resident lines per workgroup,
+ * repeated reps times per launch, with engine 0 (vector unit, multiply-add
+ * chains), 1 (vector unit, even-odd form) or 2 (matrix cores, MFMA 16x16x4).
+ * in: nblocks * 256 * (p + 1) device values of dtype; out: nblocks * 256 * 2q.
+ * (p, q) = (3, 4) or (4, 5).  ms = mean time per launch over launches. */
+int femgpu_sumfac_probe(femgpu_ctx *ctx, int p, int q, femgpu_dtype dtype, int engine, int nblocks, int reps,
+                        const void *in, void *out, int launches, double *ms);
 
 /* An operator whose apply is the caller's function: the `apply: F` closure of
  * solve_cg_operator<F: Fn(&[f64], &mut [f64])> (solver/src/lib.rs:198-208),
