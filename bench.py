@@ -97,13 +97,41 @@ def geometry_flops_per_point(q):
     return 12 + 27 + 5 + 9 + 1 + 15 + 18 + 126.0 / q
 
 
+def sumfac_flops_even_odd(n1, q, mass):
+    """Flops (multiply-add = 2) of one element's diffusion (+ mass)
+    contractions in the even-odd form of the geometry kernel (eo_split,
+    eo_contract, eo_contract2): per line, the mirrored sums and differences of
+    its input, the even and odd half-products, and the recombination."""
+    def split(ni):
+        return ni // 2                                   # additions (and as many subtractions)
+    def eo(r, ni, sig):                                  # (multiply-adds, additions) of M u
+        hr, oddr, hc, oddc = r // 2, r % 2, ni // 2, ni % 2
+        ev = hr + (oddr if sig == 1 else 0)
+        od = hr + (oddr if sig == -1 else 0)
+        return ev * (hc + oddc) + od * hc, 2 * hr
+    def eo2(r, ni):                                      # M1 u1 + M2 u2, symmetries +1 / -1
+        hr, oddr, hc, oddc = r // 2, r % 2, ni // 2, ni % 2
+        return (hr + oddr) * (hc + oddc) + (hr + oddr) * hc + hr * hc + hr * (hc + oddc), 2 * hr
+    def tot(parts, nsplit_ni):
+        fma = sum(p[0] for p in parts)
+        add = sum(p[1] for p in parts) + sum(2 * split(ni) for ni in nsplit_ni)
+        return 2 * fma + add
+    x = n1 * n1 * tot([eo(q, n1, 1), eo(q, n1, -1)], [n1])
+    y = q * n1 * tot([eo(q, n1, 1), eo(q, n1, -1), eo(q, n1, 1)], [n1, n1])
+    zf = q * q * tot([eo(q, n1, 1), eo(q, n1, 1), eo(q, n1, -1)] + ([eo(q, n1, 1)] if mass else []), [n1] * 3)
+    zr = q * q * tot([eo(n1, q, 1), eo(n1, q, 1), eo2(n1, q) if mass else eo(n1, q, -1)], [q] * (4 if mass else 3))
+    yt = q * n1 * tot([eo(n1, q, 1), eo2(n1, q)], [q] * 3)
+    xt = n1 * n1 * tot([eo2(n1, q)], [q] * 2)
+    return float(x + y + zf + zr + yt + xt)
+
+
 def apply_flops_per_element(n1, q, mass, mode):
     """Flops of one element's diffusion (+ mass) apply in the given
-    quadrature-data mode ("general", "cartesian" or "geometry")."""
-    f = 2.0 * sumfac_fma_per_element(n1, q, True, mass, mode == "cartesian")
+    quadrature-data mode ("general", "cartesian" or "geometry"; the geometry
+    kernel contracts in the even-odd form)."""
     if mode == "geometry":
-        f += q ** 3 * (geometry_flops_per_point(q) - 18.0 + (2.0 if mass else 0.0))
-    return f
+        return sumfac_flops_even_odd(n1, q, mass) + q ** 3 * (geometry_flops_per_point(q) + (2.0 if mass else 0.0))
+    return 2.0 * sumfac_fma_per_element(n1, q, True, mass, mode == "cartesian")
 
 WORKLOADS = {
     "q3-96-learned": dict(p=3, n=96, form="learned", rtol=1e-8, coarsen=16,
