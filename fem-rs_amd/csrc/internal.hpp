@@ -167,9 +167,10 @@ struct femgpu_op {
         return ws[i].ptr;
     }
 
-    // Optional profiling: when set, apply_dev records these events around its
-    // main compute kernel (femgpu_op_time_apply).
-    hipEvent_t prof_begin = nullptr, prof_end = nullptr;
+    // Optional profiling (femgpu_op_time_apply): when set, apply_dev records
+    // prof_begin and prof_end around its kernels, and prof_mid right after its
+    // main compute kernel when it has a second one (the hex PA gather).
+    hipEvent_t prof_begin = nullptr, prof_mid = nullptr, prof_end = nullptr;
 
     virtual ~femgpu_op() = default;
     // y (+)= A x on device pointers of type dtype, enqueued on ctx->stream.

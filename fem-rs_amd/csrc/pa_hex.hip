@@ -1815,6 +1815,7 @@ struct HexOp final : femgpu_op {
         if (prof_begin) FG_HIP(hipEventRecord(prof_begin, s));
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
                      skip, s, tpe, geo, (T)desc.rho);
+        if (prof_mid) FG_HIP(hipEventRecord(prof_mid, s));
         launch_gather<T>(desc.p, g, evec.as<T>(), x, y, acc ? 1 : 0, dir, skip, s);
         if (prof_end) FG_HIP(hipEventRecord(prof_end, s));
     }

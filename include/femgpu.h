@@ -99,9 +99,10 @@ int femgpu_op_destroy(femgpu_op *op);
 /* Profiling (no reference counterpart; the reference only times with
  * std::time::Instant in benchmarks): runs reps overwrite applies on device
  * vectors and
- * returns the mean time per apply and, for operators with one main compute
- * kernel (hex PA), the mean duration of that kernel from HIP events recorded
- * around it on the context stream (-1 if not available). */
+ * returns the mean time per apply and, for operators that record it (hex
+ * PA), the mean duration of the main compute kernel from HIP events recorded
+ * around it on the context stream (hex PA: the element kernel, without the
+ * DOF gather; -1 if not available). */
 int femgpu_op_time_apply(femgpu_op *op, const void *x, void *y, int reps, double *ms_apply,
                          double *ms_kernel);
 /* Contraction probe (measurement only; no reference counterpart): times one
