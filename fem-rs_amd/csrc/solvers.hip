@@ -1603,37 +1603,97 @@ struct FcgSlots {
     int m, j0;
 };
 
+// The flexible-CG vector kernels stream up to 2 + 2 kFcgPack vectors.  With
+// every pointer 16-byte aligned (W4) each thread takes 4 consecutive entries
+// per grid-stride step, loaded as 16-byte vectors (twice as many bytes in
+// flight per wave as 4-byte loads); the n mod 4 tail goes to thread 0 of
+// block 0.  Either way every
+// thread sums its entries in a fixed order, so the partials are deterministic.
+template <class T> struct Q4 { T v[4]; };
+template <class T> __device__ __forceinline__ Q4<T> ld4(const T *p, size_t c) {
+    Q4<T> o;
+    if constexpr (sizeof(T) == 4) {
+        const float4 f = reinterpret_cast<const float4 *>(p)[c];
+        o.v[0] = f.x; o.v[1] = f.y; o.v[2] = f.z; o.v[3] = f.w;
+    } else {
+        const double2 a = reinterpret_cast<const double2 *>(p)[2 * c];
+        const double2 b = reinterpret_cast<const double2 *>(p)[2 * c + 1];
+        o.v[0] = a.x; o.v[1] = a.y; o.v[2] = b.x; o.v[3] = b.y;
+    }
+    return o;
+}
+template <class T> __device__ __forceinline__ void st4(T *p, size_t c, const Q4<T> &o) {
+    if constexpr (sizeof(T) == 4) {
+        reinterpret_cast<float4 *>(p)[c] = make_float4(o.v[0], o.v[1], o.v[2], o.v[3]);
+    } else {
+        reinterpret_cast<double2 *>(p)[2 * c] = make_double2(o.v[0], o.v[1]);
+        reinterpret_cast<double2 *>(p)[2 * c + 1] = make_double2(o.v[2], o.v[3]);
+    }
+}
+// Runs body(i) over [0, n): W4 in chunks of 4 (vector body4(c) for chunk c),
+// else one entry per step.
+template <bool W4, class F1, class F4>
+__device__ __forceinline__ void fcg_stride(size_t n, F1 &&body, F4 &&body4) {
+    const size_t t0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    if constexpr (W4) {
+        const size_t n4 = n / 4;
+        for (size_t c = t0; c < n4; c += stride) body4(c);
+        if (t0 == 0)
+            for (size_t i = 4 * n4; i < n; ++i) body(i);
+    } else {
+        for (size_t i = t0; i < n; i += stride) body(i);
+    }
+}
+
 // r1 = r - az1; partials of r1.r1
-template <class T>
+template <class T, bool W4>
 __global__ __launch_bounds__(256) void k_fcg_resid(size_t n, const T *__restrict__ r, const T *__restrict__ az1,
                                                    T *__restrict__ r1, double *__restrict__ part) {
     double v[1] = {0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    auto one = [&](size_t i) {
         const T d = r[i] - az1[i];
         r1[i] = d;
         v[0] += (double)d * (double)d;
-    }
+    };
+    fcg_stride<W4>(n, one, [&](size_t c) {
+        const Q4<T> a = ld4(r, c), b = ld4(az1, c);
+        Q4<T> d;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            d.v[e] = a.v[e] - b.v[e];
+            v[0] += (double)d.v[e] * (double)d.v[e];
+        }
+        st4(r1, c, d);
+    });
     block_sum<1, 256>(v);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
 // partials of nn.An (row 0) and nn.r1 (row 1)
-template <class T>
+template <class T, bool W4>
 __global__ __launch_bounds__(256) void k_fcg_dots2(size_t n, const T *__restrict__ nn, const T *__restrict__ An,
                                                    const T *__restrict__ r1, double *__restrict__ part) {
     double v[2] = {0.0, 0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    auto one = [&](size_t i) {
         const double a = (double)nn[i];
         v[0] += a * (double)An[i];
         v[1] += a * (double)r1[i];
-    }
+    };
+    fcg_stride<W4>(n, one, [&](size_t c) {
+        const Q4<T> a = ld4(nn, c), b = ld4(An, c), d = ld4(r1, c);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v[0] += (double)a.v[e] * (double)b.v[e];
+            v[1] += (double)a.v[e] * (double)d.v[e];
+        }
+    });
     block_sum<2, 256>(v);
     if (threadIdx.x == 0) { part[blockIdx.x] = v[0]; part[kRedBlocks + blockIdx.x] = v[1]; }
 }
 
 // partials of (z1 + g nn).q_j, j < m <= kPack (row j), g = st[F_GAMMA]; nn
 // unused when g == 0
-template <class T>
+template <class T, bool W4>
 __global__ __launch_bounds__(256) void k_fcg_zq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
                                                 const double *__restrict__ st, VecPack<T> qs,
                                                 double *__restrict__ part) {
@@ -1641,32 +1701,43 @@ __global__ __launch_bounds__(256) void k_fcg_zq(size_t n, const T *__restrict__ 
     double v[kPack];
 #pragma unroll
     for (int j = 0; j < kPack; ++j) v[j] = 0.0;
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    auto one = [&](size_t i) {
         const double z = g != 0.0 ? (double)(T)((double)z1[i] + g * (double)nn[i]) : (double)z1[i];
 #pragma unroll
         for (int j = 0; j < kPack; ++j)
             if (j < qs.m) v[j] += z * (double)qs.v[j][i];
-    }
+    };
+    fcg_stride<W4>(n, one, [&](size_t c) {
+        const Q4<T> a = ld4(z1, c);
+        double z[4];
+        if (g != 0.0) {
+            const Q4<T> b = ld4(nn, c);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) z[e] = (double)(T)((double)a.v[e] + g * (double)b.v[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) z[e] = (double)a.v[e];
+        }
+#pragma unroll
+        for (int j = 0; j < kPack; ++j)
+            if (j < qs.m) {
+                const Q4<T> q = ld4(qs.v[j], c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[j] += z[e] * (double)q.v[e];
+            }
+    });
     block_sum<kPack, 256>(v);
     if (threadIdx.x == 0)
         for (int j = 0; j < qs.m; ++j) part[(size_t)j * kRedBlocks + blockIdx.x] = v[j];
 }
 
-// p = base_p + sum_j beta_j p_j, q = base_q + sum_j beta_j q_j with base
-// (z1 + g nn, az1 + g An) on the first pass and (p, q) after; on the last pass
-// partials of p.q (row 0) and p.r (row 1)
+// one entry of k_fcg_pq
 template <class T>
-__global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
-                                                const T *__restrict__ az1, const T *__restrict__ An,
-                                                const double *__restrict__ st, const T *__restrict__ r,
-                                                FcgDirs<T> d, T *p, T *q, int first, int last,
-                                                double *__restrict__ part) {
-    const double g = st[F_GAMMA];
-    double b[kFcgPack];
-#pragma unroll
-    for (int j = 0; j < kFcgPack; ++j) b[j] = j < d.m ? st[F_BETA + d.j0 + j] : 0.0;
-    double v[2] = {0.0, 0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+__device__ __forceinline__ void fcg_pq_one(size_t i, const T *__restrict__ z1, const T *__restrict__ nn,
+                                           const T *__restrict__ az1, const T *__restrict__ An, double g,
+                                           const T *__restrict__ r, const FcgDirs<T> &d, const double (&b)[kFcgPack],
+                                           T *p, T *q, int first, int last, double (&v)[2]) {
+    {
         double sp, sq;
         if (first) {
             // the preconditioned residual z and A z, rounded to T as a stored
@@ -1691,6 +1762,67 @@ __global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ 
             v[1] += (double)pt * (double)r[i];
         }
     }
+}
+
+// p = base_p + sum_j beta_j p_j, q = base_q + sum_j beta_j q_j with base
+// (z1 + g nn, az1 + g An) on the first pass and (p, q) after; on the last pass
+// partials of p.q (row 0) and p.r (row 1)
+template <class T, bool W4>
+__global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ z1, const T *__restrict__ nn,
+                                                const T *__restrict__ az1, const T *__restrict__ An,
+                                                const double *__restrict__ st, const T *__restrict__ r,
+                                                FcgDirs<T> d, T *p, T *q, int first, int last,
+                                                double *__restrict__ part) {
+    const double g = st[F_GAMMA];
+    double b[kFcgPack];
+#pragma unroll
+    for (int j = 0; j < kFcgPack; ++j) b[j] = j < d.m ? st[F_BETA + d.j0 + j] : 0.0;
+    double v[2] = {0.0, 0.0};
+    fcg_stride<W4>(n, [&](size_t i) { fcg_pq_one(i, z1, nn, az1, An, g, r, d, b, p, q, first, last, v); },
+                   [&](size_t c) {
+        double sp[4], sq[4];
+        if (first) {
+            const Q4<T> a = ld4(z1, c), aa = ld4(az1, c);
+            if (g != 0.0) {
+                const Q4<T> bn = ld4(nn, c), bA = ld4(An, c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    sp[e] = (double)(T)((double)a.v[e] + g * (double)bn.v[e]);
+                    sq[e] = (double)(T)((double)aa.v[e] + g * (double)bA.v[e]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) { sp[e] = (double)a.v[e]; sq[e] = (double)aa.v[e]; }
+            }
+        } else {
+            const Q4<T> a = ld4((const T *)p, c), aa = ld4((const T *)q, c);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { sp[e] = (double)a.v[e]; sq[e] = (double)aa.v[e]; }
+        }
+#pragma unroll
+        for (int j = 0; j < kFcgPack; ++j)
+            if (j < d.m) {
+                const Q4<T> pj = ld4(d.p[j], c), qj = ld4(d.q[j], c);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    sp[e] += b[j] * (double)pj.v[e];
+                    sq[e] += b[j] * (double)qj.v[e];
+                }
+            }
+        Q4<T> pt, qt;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { pt.v[e] = (T)sp[e]; qt.v[e] = (T)sq[e]; }
+        st4(p, c, pt);
+        st4(q, c, qt);
+        if (last) {
+            const Q4<T> rv = ld4(r, c);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[0] += (double)pt.v[e] * (double)qt.v[e];
+                v[1] += (double)pt.v[e] * (double)rv.v[e];
+            }
+        }
+    });
     if (last) {
         block_sum<2, 256>(v);
         if (threadIdx.x == 0) { part[blockIdx.x] = v[0]; part[kRedBlocks + blockIdx.x] = v[1]; }
@@ -1699,19 +1831,31 @@ __global__ __launch_bounds__(256) void k_fcg_pq(size_t n, const T *__restrict__ 
 
 // x += alpha p, r -= alpha q (alpha = st[F_ALPHA]); partials of r.r.  Nothing
 // when the iteration has already stopped (p.q <= 0 or not finite).
-template <class T>
+template <class T, bool W4>
 __global__ __launch_bounds__(256) void k_fcg_xr(size_t n, const double *__restrict__ st, const T *__restrict__ p,
                                                 const T *__restrict__ q, T *__restrict__ x, T *__restrict__ r,
                                                 double *__restrict__ part) {
     if (st[F_STATUS] != FS_RUN) return;
     const double a = st[F_ALPHA];
     double v[1] = {0.0};
-    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    auto one = [&](size_t i) {
         x[i] = (T)((double)x[i] + a * (double)p[i]);
         const T ri = (T)((double)r[i] - a * (double)q[i]);
         r[i] = ri;
         v[0] += (double)ri * (double)ri;
-    }
+    };
+    fcg_stride<W4>(n, one, [&](size_t c) {
+        Q4<T> xv = ld4((const T *)x, c), rv = ld4((const T *)r, c);
+        const Q4<T> pv = ld4(p, c), qv = ld4(q, c);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            xv.v[e] = (T)((double)xv.v[e] + a * (double)pv.v[e]);
+            rv.v[e] = (T)((double)rv.v[e] - a * (double)qv.v[e]);
+            v[0] += (double)rv.v[e] * (double)rv.v[e];
+        }
+        st4(x, c, xv);
+        st4(r, c, rv);
+    });
     block_sum<1, 256>(v);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
@@ -1823,6 +1967,16 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     const dim3 grid((unsigned)nb), block(256);
     double *part = c->partials.as<double>();
     const bool coarse = unet_coarse(net);
+    // the vector kernels' 16-byte path needs every vector they touch aligned
+    // (the workspace always is; x is the caller's)
+    auto al16 = [](const void *q) { return ((uintptr_t)q & 15u) == 0; };
+    bool w4 = al16(x) && al16(r) && al16(nn) && al16(An) && al16(r1) && al16(z1) && al16(az1);
+    for (int j = 0; j < trunc; ++j) w4 = w4 && al16(P[j]) && al16(Q[j]);
+    auto kf_resid = w4 ? k_fcg_resid<T, true> : k_fcg_resid<T, false>;
+    auto kf_dots2 = w4 ? k_fcg_dots2<T, true> : k_fcg_dots2<T, false>;
+    auto kf_zq = w4 ? k_fcg_zq<T, true> : k_fcg_zq<T, false>;
+    auto kf_pq = w4 ? k_fcg_pq<T, true> : k_fcg_pq<T, false>;
+    auto kf_xr = w4 ? k_fcg_xr<T, true> : k_fcg_xr<T, false>;
     DevBuf stb(sizeof(double) * F_COUNT);
     double *st = stb.as<double>();
     double *hst = c->pinned + kMultiDotBase;  // read-back of the first 8 state slots
@@ -1841,7 +1995,7 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
     auto precond = [&](const T *rin) {
         K.cheb_apply(cp, rin, inv, z1);
         K.apply(z1, az1, false);
-        hipLaunchKernelGGL(k_fcg_resid<T>, grid, block, 0, s, n, rin, (const T *)az1, r1, part);
+        hipLaunchKernelGGL(kf_resid, grid, block, 0, s, n, rin, (const T *)az1, r1, part);
         if (coarse) {
             unet_apply_coarse_dev(net, r1, nn, dt, st + F_RC2);  // the scale stays on the device
         } else {
@@ -1853,7 +2007,7 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
             }
         }
         K.apply(nn, An, false);
-        hipLaunchKernelGGL(k_fcg_dots2<T>, grid, block, 0, s, n, (const T *)nn, (const T *)An, (const T *)r1, part);
+        hipLaunchKernelGGL(kf_dots2, grid, block, 0, s, n, (const T *)nn, (const T *)An, (const T *)r1, part);
         hipLaunchKernelGGL(k_fcgd_gamma, dim3(1), dim3(1024), 0, s, (const double *)part, nb, st, coarse ? 1 : 0);
     };
     info->fcg_iterations = 0; info->fallback_iterations = 0; info->fell_back = 0; info->fallback_reason = 0;
@@ -1894,7 +2048,7 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
             pk.m = sl.m = std::min(kPack, m - j0);
             sl.j0 = j0;
             for (int j = 0; j < pk.m; ++j) { pk.v[j] = Q[sj[j0 + j]]; sl.s[j] = sj[j0 + j]; }
-            hipLaunchKernelGGL(k_fcg_zq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const double *)st,
+            hipLaunchKernelGGL(kf_zq, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const double *)st,
                                pk, part);
             hipLaunchKernelGGL(k_fcgd_beta, dim3(1), dim3(1024), 0, s, (const double *)part, nb, sl, st);
         }
@@ -1904,12 +2058,12 @@ static int fcg_learned(Krylov<T> &K, femgpu_unet *net, const ChebParams &cp, con
             dd.j0 = ps * kFcgPack;
             dd.m = std::max(0, std::min(kFcgPack, m - dd.j0));
             for (int j = 0; j < dd.m; ++j) { dd.p[j] = P[sj[dd.j0 + j]]; dd.q[j] = Q[sj[dd.j0 + j]]; }
-            hipLaunchKernelGGL(k_fcg_pq<T>, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const T *)az1,
+            hipLaunchKernelGGL(kf_pq, grid, block, 0, s, n, (const T *)z1, (const T *)nn, (const T *)az1,
                                (const T *)An, (const double *)st, (const T *)r, dd, p, qk, ps == 0 ? 1 : 0,
                                ps == passes - 1 ? 1 : 0, part);
         }
         hipLaunchKernelGGL(k_fcgd_alpha, dim3(1), dim3(1024), 0, s, (const double *)part, nb, slot, st);
-        hipLaunchKernelGGL(k_fcg_xr<T>, grid, block, 0, s, n, (const double *)st, (const T *)p, (const T *)qk, x, r,
+        hipLaunchKernelGGL(kf_xr, grid, block, 0, s, n, (const double *)st, (const T *)p, (const T *)qk, x, r,
                            part);
         hipLaunchKernelGGL(k_fcgd_conv, dim3(1), dim3(1024), 0, s, (const double *)part, nb, st, bnorm, rtol, atol,
                            patience);
