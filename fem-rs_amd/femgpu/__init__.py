@@ -322,7 +322,23 @@ class DeviceArray:
         _check(lib.femgpu_vec_convert(self.ctx.h, self.n, self._dt(), self.ptr, out._dt(), out.ptr))
         return out
 
+    def view(self, offset, n):
+        """A non-owning DeviceArray over entries [offset, offset + n) of this
+        one, as a caller hands the library a sub-range of its own buffer (the
+        view keeps this array alive and frees nothing)."""
+        if offset < 0 or n < 0 or offset + n > self.n:
+            raise ValueError(f"view [{offset}, {offset + n}) outside an array of {self.n}")
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.n, v.dtype = self.ctx, int(n), self.dtype
+        v.nbytes = v.n * v.dtype.itemsize
+        v.ptr = P(self.ptr.value + offset * self.dtype.itemsize)
+        v._base = self
+        return v
+
     def free(self):
+        if getattr(self, "_base", None) is not None:  # a view owns nothing
+            self.ptr = None
+            return
         if getattr(self, "ptr", None):
             if not _EXITING:
                 lib.femgpu_free(self.ctx.h, self.ptr)

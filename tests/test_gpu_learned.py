@@ -121,6 +121,38 @@ def test_fcg_learned_and_fallback(fg, ctx):
     net.set_params(p)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fcg_vector_paths(fg, ctx, dtype):
+    """The flexible-CG vector kernels take 16-byte loads when every vector is
+    16-byte aligned and one entry per step otherwise: a solve whose x and b
+    start one entry into their buffers (misaligned) runs the second path, on
+    a mesh whose DOF count is not a multiple of 4 (the 16-byte path's tail).
+    Both must reach the same solution (their dot products round in different
+    orders, so not bitwise)."""
+    op64, b = _problem(fg, ctx, 8, 7)
+    op = op64.convert(fg.F32) if dtype == np.float32 else op64
+    N = op.n_dofs()
+    assert N % 4 != 0
+    net = fg.UNet.for_operator(op, seed=1)
+    bh = b.astype(dtype)
+    cfg = fg.SolverConfig(rtol=1e-10 if dtype == np.float64 else 1e-5, max_iter=3000)
+    sols = []
+    for off in (0, 1):
+        xb = fg.DeviceArray(ctx, N + off, dtype)
+        bb = fg.DeviceArray(ctx, N + off, dtype)
+        xv, bv = (xb, bb) if off == 0 else (xb.view(1, N), bb.view(1, N))
+        bv.upload(bh)
+        res, info = fg.solve_fcg_learned(op, net, bv, xv, cfg)
+        assert res.converged, (off, res)
+        x = xv.download().astype(np.float64)
+        r = b - op64.matvec(x)
+        assert np.linalg.norm(r) / np.linalg.norm(b) < (1e-10 if dtype == np.float64 else 2e-5), off
+        sols.append(x)
+    d = np.linalg.norm(sols[0] - sols[1]) / np.linalg.norm(sols[0])
+    print(f"{np.dtype(dtype).name}: aligned / misaligned solutions differ by {d:.2e}")
+    assert d < (1e-8 if dtype == np.float64 else 1e-3)
+
+
 def test_energy_loss_matches_definition(fg, ctx):
     """train_step_energy reports 1 - (n.r)^2 / ((n.A n)(e.r)) for the network
     output n before the step; recompute it from forward() and the operator."""
