@@ -152,6 +152,7 @@ WORKLOADS = {
                            "in [1e-2, 1e2] (seed 7), random nodal RHS (seed 11), Dirichlet, FP64 PA "
                            "({q}^3 Gauss points), PCG + Chebyshev-Jacobi(4) to 1e-10"),
     "q4-128-perturbed": dict(p=4, n=128, form="diff_mass", kappa=("grf", 7), amp=0.2, rhs=("random", 11),
+                             learned_degree=2,
                              rtol=1e-8, mass_apply=False, chunk=40,
                              desc="3-D diffusion + mass (rho = 1), {n}^3 hex Q{p} ({N} DOFs), vertices "
                                   "perturbed <= 0.2 h (seed 5), kappa = exp(GRF) in [1e-2, 1e2] (seed 7), "
@@ -181,6 +182,9 @@ def parse():
     ap.add_argument("--ref-rtol", type=float, default=1e-13, help="legs: tolerance of the FP64 reference")
     ap.add_argument("--train-steps", type=int, default=1200)
     ap.add_argument("--truncation", type=int, default=8, help="flexible CG: directions kept")
+    ap.add_argument("--learned-degree", type=int, default=0,
+                    help="learned legs: Chebyshev-Jacobi degree of the smoothing inside the flexible-CG "
+                         "preconditioner (0: the workload's; the network is trained on degree-4 smoothed pairs)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--train-fields", type=int, default=256, help="q3-96-learned: training fields (seeds 0, 1, ...)")
     ap.add_argument("--test-fields", type=int, default=32, help="q3-96-learned: held-out fields (seeds 256, ...)")
@@ -505,7 +509,8 @@ def run_learned(a, w):
         net.aux_from(K)
         x.copy_from(x0)
         ctx.synchronize(); t0 = time.perf_counter()
-        rl, info = fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax, truncation=a.truncation)
+        rl, info = fg.solve_fcg_learned(K, net, b, x, cfg, lmax=lmax, truncation=a.truncation,
+                                        degree=a.learned_degree)
         ctx.synchronize(); tl = time.perf_counter() - t0
         r = fg.DeviceArray(ctx, N, np.float64)
         K.apply_zero(x, r)
@@ -783,7 +788,7 @@ def legs(fg, ctx, K, b, x, lmax, rtol, a):
     say(f"harvested {len(pairs)} pairs in {t_harvest:.1f} s, trained {a.train_steps} steps in {t_train:.1f} s "
         f"(network grid {net.shape})")
     lrn = lambda tol: fg.solve_ir_learned(K, K32, net, b, x, cfg(tol), inner_rtol=1e-4, lmax=lmax,
-                                          truncation=a.truncation)
+                                          truncation=a.truncation, degree=a.learned_degree)
     (res_l, inner_l, info_l), t_l = timed(lambda: lrn(rtol))
     xl = fg.DeviceArray(ctx, N, np.float64); xl.copy_from(x)
     rrl, el = relres(), err(x)
@@ -794,7 +799,7 @@ def legs(fg, ctx, K, b, x, lmax, rtol, a):
     del xl
     out["mixed_learned"] = {
         "method": f"FP64 refinement (inner solves to 1e-4) around FP32 flexible CG ({a.truncation} directions) "
-                  f"whose preconditioner is Chebyshev-Jacobi(4) followed by the U-Net's Galerkin-scaled "
+                  f"whose preconditioner is Chebyshev-Jacobi({a.learned_degree}) followed by the U-Net's Galerkin-scaled "
                   f"correction; the U-Net (3 levels, 16/32/64 channels, BF16 MFMA) runs on every {cf}-th DOF "
                   f"node ({net.shape[0]}^3 grid), its input restricted and its output prolongated "
                   f"trilinearly; PCG-Chebyshev fallback on stagnation",
@@ -884,6 +889,12 @@ def run_sumfac_probe(a):
 
 def main():
     a = parse()
+    if a.learned_degree <= 0:
+        # measured per workload: the 65^3 network of the Q4 128^3 mesh leaves less
+        # to the smoother (degree 2: 8-12% less time than 4 over three network
+        # seeds); the 19^3 network of the Q3 96^3 family needs degree 4 (2.90x
+        # against 2.37x less time than PCG-Chebyshev)
+        a.learned_degree = WORKLOADS[a.workload].get("learned_degree", 4)
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
               "multi-GPU runs are not part of it", file=sys.stderr)
