@@ -716,84 +716,147 @@ __global__ void k_restrict_axis(long long outer, int nf, int nc, long long inner
         const long long sI = id % inner, t2 = id / inner;
         const int i = (int)(t2 % nc);
         const long long o = t2 / nc;
+        // weights (cf - |t|) / cf: integer taps, one scaling per output
+        const int t0 = max(-(cf - 1), -cf * i), t1 = min(cf - 1, nf - 1 - cf * i);
         double acc = 0.0;
-        for (int t = -(cf - 1); t <= cf - 1; ++t) {
-            const int g = cf * i + t;
-            if (g < 0 || g >= nf) continue;
-            acc += (1.0 - (double)abs(t) / cf) * (double)in[(o * nf + g) * inner + sI];
-        }
-        out[id] = (float)acc;
+        for (int t = t0; t <= t1; ++t)
+            acc += (double)(cf - abs(t)) * (double)in[(o * nf + cf * i + t) * inner + sI];
+        out[id] = (float)(acc * (1.0 / cf));
     }
 }
 
 // The x pass of R: out[line][i] = sum_t (1 - |t|/cf) in[line][cf i + t]
 // (|t| < cf, inside the line).  A workgroup stages `lines` consecutive
-// x-lines (one contiguous run of lines * nf values) in LDS with coalesced
-// loads, then each thread sums one output's window from LDS.
+// x-lines in LDS with coalesced loads, then each thread sums one output's
+// window from LDS.  A staged line is stored de-interleaved, value g at
+// (g mod cf) nc + g / cf, so that one tap's reads by consecutive outputs are
+// consecutive words (in line order they were cf words apart: for cf = 8, 16
+// lanes per bank).
 template <class TI>
 __global__ __launch_bounds__(256) void k_restrict_x(long long nlines, int nf, int nc, int cf, int lines,
                                                     const TI *__restrict__ in, float *__restrict__ out) {
-    extern __shared__ double sline[];  // lines * nf values
+    extern __shared__ double sline[];  // lines * cf * nc values
     const long long l0 = (long long)blockIdx.x * lines;
     const int nl = (int)min((long long)lines, nlines - l0);
     const TI *src = in + l0 * nf;
-    for (int k = threadIdx.x; k < nl * nf; k += 256) sline[k] = (double)src[k];
+    const int stride = cf * nc;
+    // k / nf and g / cf as floor((k + 1/2) / d) in float: exact for k <= 4096,
+    // any d (the relative error, ~2^-23, times (k + 1/2) / d stays below the
+    // distance 0.5 / d from the nearest integer)
+    const float icf = 1.0f / (float)cf, inf = 1.0f / (float)nf;
+    // every load of the thread is issued before any is used (lines nf <= 4096,
+    // at most 16 per thread): in a loop they would wait one after another
+    const int cnt = nl * nf;
+    TI v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int k = threadIdx.x + 256 * u;
+        v[u] = k < cnt ? src[k] : TI(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int k = threadIdx.x + 256 * u;
+        if (k < cnt) {
+            const int l = (int)(((float)k + 0.5f) * inf), g = k - l * nf;
+            const int q = (int)(((float)g + 0.5f) * icf), r = g - q * cf;
+            sline[l * stride + r * nc + q] = (double)v[u];
+        }
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < nl * nc; k += 256) {
         const int l = k / nc, i = k - l * nc;
-        const double *line = sline + l * nf;
+        const double *line = sline + l * stride;
+        // weights (cf - |t|) / cf: integer taps over the window inside the
+        // line, one scaling per output; g = cf i + t sits at (g mod cf) nc + g / cf
+        const int t0 = max(-(cf - 1), -cf * i), t1 = min(cf - 1, nf - 1 - cf * i);
         double acc = 0.0;
-        for (int t = -(cf - 1); t <= cf - 1; ++t) {
-            const int g = cf * i + t;
-            if (g < 0 || g >= nf) continue;
-            acc += (1.0 - (double)abs(t) / cf) * line[g];
-        }
-        out[(l0 + l) * nc + i] = (float)acc;
+        for (int t = t0; t <= t1; ++t)
+            acc += (double)(cf - abs(t)) * line[t >= 0 ? t * nc + i : (cf + t) * nc + i - 1];
+        out[(l0 + l) * nc + i] = (float)(acc * (1.0 / cf));
     }
 }
 
 // z[g] = scale * (P oc)[g] (trilinear interpolation of the coarse values)
 // (bmask: 0 on the boundary of the DOF grid, where every DOF is essential).
-// One workgroup per fine x-line (gy, gz): the line's coarse row, interpolated
-// in y and z, is formed once in LDS; each fine node then interpolates along x.
+// A workgroup takes lp consecutive fine x-lines (gy, gz) (prolong_lines: up
+// to 8, as LDS allows): each line's coarse row, interpolated in y and z, is
+// formed once in LDS; each fine node then interpolates along x.
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong(Vol f, Vol c, int cf, const float *__restrict__ oc, double scale,
-                                                 int bmask, T *__restrict__ z, const double *nrm2 = nullptr) {
+__global__ __launch_bounds__(256) void k_prolong(Vol f, Vol c, int cf, int lp, const float *__restrict__ oc,
+                                                 double scale, int bmask, T *__restrict__ z,
+                                                 const double *nrm2 = nullptr) {
     // nrm2: |R r|^2 on the device; the scale is then sqrt(|R r|^2 / Vc)
     if (nrm2) {
         const double q = *nrm2;
         scale = q > 0.0 && isfinite(q) ? sqrt(q / (double)c.n()) : 0.0;
     }
-    extern __shared__ float brow[];  // c.nx values
-    const unsigned line = blockIdx.x;
-    const int gy = (int)(line % (unsigned)f.ny), gz = (int)(line / (unsigned)f.ny);
-    T *zl = z + (size_t)line * f.nx;
-    if (bmask && (gy == 0 || gz == 0 || gy == f.ny - 1 || gz == f.nz - 1)) {
-        for (int gx = threadIdx.x; gx < f.nx; gx += 256) zl[gx] = T(0);
-        return;
-    }
-    // lower coarse node of the cell holding g (the last cell for g = Nf - 1);
-    // the upper one is clamped for a one-node axis, where t = 0
-    const float icf = 1.0f / (float)cf;
-    const int iy = min(gy / cf, max(c.ny - 2, 0)), iz = min(gz / cf, max(c.nz - 2, 0));
-    const int iy1 = min(iy + 1, c.ny - 1), iz1 = min(iz + 1, c.nz - 1);
-    const float ty = (float)(gy - cf * iy) * icf, tz = (float)(gz - cf * iz) * icf;
-    const float *p0 = oc + ((size_t)iz * c.ny + iy) * c.nx, *p1 = oc + ((size_t)iz * c.ny + iy1) * c.nx;
-    const float *p2 = oc + ((size_t)iz1 * c.ny + iy) * c.nx, *p3 = oc + ((size_t)iz1 * c.ny + iy1) * c.nx;
-    for (int i = threadIdx.x; i < c.nx; i += 256) {
-        const float b0 = p0[i] + ty * (p1[i] - p0[i]), b1 = p2[i] + ty * (p3[i] - p2[i]);
-        brow[i] = b0 + tz * (b1 - b0);
+    extern __shared__ float brow[];  // lp * c.nx values
+    const unsigned nlines = (unsigned)f.ny * (unsigned)f.nz;
+    const unsigned line0 = blockIdx.x * (unsigned)lp;
+    const float icf = 1.0f / (float)cf, inx = 1.0f / (float)c.nx;
+    // the lp c.nx coarse-row entries, in rounds of 8 per thread whose 4 loads
+    // each are issued before any is used
+    const int cnt = lp * c.nx;
+    for (int base = 0; base < cnt; base += 8 * 256) {
+        float a0[8], a1[8], a2[8], a3[8], tyv[8], tzv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + threadIdx.x + 256 * u;
+            a0[u] = a1[u] = a2[u] = a3[u] = tyv[u] = tzv[u] = 0.0f;
+            if (k >= cnt) continue;
+            // l = k / c.nx in float: exact for k <= 8192 (as for gx / cf below)
+            const int l = (int)(((float)k + 0.5f) * inx), i = k - l * c.nx;
+            const unsigned line = line0 + l;
+            if (line >= nlines) continue;
+            const int gy = (int)(line % (unsigned)f.ny), gz = (int)(line / (unsigned)f.ny);
+            // lower coarse node of the cell holding g (the last cell for g = Nf - 1);
+            // the upper one is clamped for a one-node axis, where t = 0
+            const int iy = min(gy / cf, max(c.ny - 2, 0)), iz = min(gz / cf, max(c.nz - 2, 0));
+            const int iy1 = min(iy + 1, c.ny - 1), iz1 = min(iz + 1, c.nz - 1);
+            tyv[u] = (float)(gy - cf * iy) * icf;
+            tzv[u] = (float)(gz - cf * iz) * icf;
+            a0[u] = oc[((size_t)iz * c.ny + iy) * c.nx + i];
+            a1[u] = oc[((size_t)iz * c.ny + iy1) * c.nx + i];
+            a2[u] = oc[((size_t)iz1 * c.ny + iy) * c.nx + i];
+            a3[u] = oc[((size_t)iz1 * c.ny + iy1) * c.nx + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = base + threadIdx.x + 256 * u;
+            if (k < cnt) {
+                const float b0 = a0[u] + tyv[u] * (a1[u] - a0[u]), b1 = a2[u] + tyv[u] * (a3[u] - a2[u]);
+                brow[k] = b0 + tzv[u] * (b1 - b0);
+            }
+        }
     }
     __syncthreads();
-    for (int gx = threadIdx.x; gx < f.nx; gx += 256) {
-        if (bmask && (gx == 0 || gx == f.nx - 1)) {
-            zl[gx] = T(0);
-            continue;
+    for (int l = 0; l < lp; ++l) {
+        const unsigned line = line0 + l;
+        if (line >= nlines) break;
+        const int gy = (int)(line % (unsigned)f.ny), gz = (int)(line / (unsigned)f.ny);
+        T *zl = z + (size_t)line * f.nx;
+        const float *br = brow + l * c.nx;
+        const bool zero_line = bmask && (gy == 0 || gz == 0 || gy == f.ny - 1 || gz == f.nz - 1);
+        for (int gx = threadIdx.x; gx < f.nx; gx += 256) {
+            if (zero_line || (bmask && (gx == 0 || gx == f.nx - 1))) {
+                zl[gx] = T(0);
+                continue;
+            }
+            // gx / cf as floor((gx + 1/2) / cf) in float: exact for gx <= 4096, any cf
+            // (the value's relative error, ~2^-23, times (gx + 1/2) / cf stays below
+            // its distance 0.5 / cf from the nearest integer)
+            const int ix = min((int)(((float)gx + 0.5f) * icf), max(c.nx - 2, 0)), ix1 = min(ix + 1, c.nx - 1);
+            const float tx = (float)(gx - cf * ix) * icf;
+            zl[gx] = (T)((double)(br[ix] + tx * (br[ix1] - br[ix])) * scale);
         }
-        const int ix = min(gx / cf, max(c.nx - 2, 0)), ix1 = min(ix + 1, c.nx - 1);
-        const float tx = (float)(gx - cf * ix) * icf;
-        zl[gx] = (T)((double)(brow[ix] + tx * (brow[ix1] - brow[ix])) * scale);
     }
+}
+
+// fine lines per k_prolong workgroup: up to 8, with at most 32 KB of LDS
+static inline int prolong_lines(const Vol &c) { return std::max(1, std::min(8, 8192 / std::max(c.nx, 1))); }
+static inline unsigned prolong_blocks(const Vol &f, const Vol &c) {
+    const int lp = prolong_lines(c);
+    return (unsigned)(((long long)f.ny * f.nz + lp - 1) / lp);
 }
 
 // ac[i] = af[cf i] (the coarse nodes are DOF nodes)
@@ -1070,7 +1133,8 @@ static void restrict_to_coarse(femgpu_unet *u, const T *v) {
     const int lines = std::max(1, std::min(4, 4096 / f.nx));  // at most 32 KB of LDS (f.nx <= 4096)
     FG_REQUIRE(f.nx <= 4096, FEMGPU_ERR_UNSUPPORTED, "unet: DOF grid rows longer than 4096");
     hipLaunchKernelGGL(k_restrict_x<T>, dim3((unsigned)((o1 + lines - 1) / lines)), dim3(256),
-                       sizeof(double) * lines * f.nx, s, o1, f.nx, c.nx, u->cf, lines, v, u->rx.as<float>());
+                       sizeof(double) * lines * u->cf * c.nx, s, o1, f.nx, c.nx, u->cf, lines, v,
+                       u->rx.as<float>());
     const long long n2 = (long long)f.nz * c.ny * c.nx;
     hipLaunchKernelGGL(k_restrict_axis<float>, dim3(g1(n2)), dim3(256), 0, s, (long long)f.nz, f.ny, c.ny,
                        (long long)c.nx, u->cf, (const float *)u->rx.as<float>(), u->rxy.as<float>());
@@ -1094,8 +1158,8 @@ static bool unet_apply_t(femgpu_unet *u, const T *r, T *z, double rnorm) {
         hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(),
                            scc, (const float *)u->aux.as<float>(), u->in.as<uint16_t>());
         forward(u);
-        hipLaunchKernelGGL(k_prolong<T>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
-                           sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(),
+        hipLaunchKernelGGL(k_prolong<T>, dim3(prolong_blocks(u->fine, u->d0)), dim3(256),
+                           sizeof(float) * prolong_lines(u->d0) * u->d0.nx, s, u->fine, u->d0, u->cf, prolong_lines(u->d0), (const float *)u->out.as<float>(),
                            1.0 / scc, u->bmask, z);
         return true;
     }
@@ -1120,8 +1184,8 @@ static void unet_apply_coarse_t(femgpu_unet *u, const T *r, T *z, double *nrm2) 
     hipLaunchKernelGGL(k_make_input<float>, dim3(g1(V)), dim3(256), 0, s, V, (const float *)u->rc.as<float>(), 0.0,
                        (const float *)u->aux.as<float>(), u->in.as<uint16_t>(), (const double *)nrm2);
     forward(u);
-    hipLaunchKernelGGL(k_prolong<T>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
-                       sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(), 0.0,
+    hipLaunchKernelGGL(k_prolong<T>, dim3(prolong_blocks(u->fine, u->d0)), dim3(256),
+                       sizeof(float) * prolong_lines(u->d0) * u->d0.nx, s, u->fine, u->d0, u->cf, prolong_lines(u->d0), (const float *)u->out.as<float>(), 0.0,
                        u->bmask, z, (const double *)nrm2);
 }
 
@@ -1427,8 +1491,8 @@ int femgpu_unet_train_step_energy(femgpu_unet *u, femgpu_op *op, const float *r,
         // n on the DOF grid: the output itself, or its prolongation
         const float *nout = u->out.as<float>();
         if (coarse) {
-            hipLaunchKernelGGL(k_prolong<float>, dim3((unsigned)((long long)u->fine.ny * u->fine.nz)), dim3(256),
-                               sizeof(float) * u->d0.nx, s, u->fine, u->d0, u->cf, (const float *)u->out.as<float>(),
+            hipLaunchKernelGGL(k_prolong<float>, dim3(prolong_blocks(u->fine, u->d0)), dim3(256),
+                               sizeof(float) * prolong_lines(u->d0) * u->d0.nx, s, u->fine, u->d0, u->cf, prolong_lines(u->d0), (const float *)u->out.as<float>(),
                                1.0, u->bmask, u->nf.as<float>());
             nout = u->nf.as<float>();
         }
