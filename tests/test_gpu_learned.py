@@ -254,26 +254,30 @@ def _R(shape_f, cf, r):
     return np.einsum("zk,yj,xi,zyx->kji", Pz, Py, Px, r.reshape(nz, ny, nx)).ravel()
 
 
-def _coarse_case(fg, ctx, cf=4):
-    op, b = _problem(fg, ctx, 8, 5, p=2)  # Q2 8^3: 17^3 DOF grid
+def _coarse_case(fg, ctx, cf=4, p=2):
+    op, b = _problem(fg, ctx, 8, 5, p=p)  # Q2 8^3: 17^3 DOF grid; Q3 8^3: 25^3
     net = fg.UNet.for_operator(op, seed=4, coarsen=cf)
     return op, b, net
 
 
-def test_coarse_unet_apply_is_prolongated_network(fg, ctx):
+@pytest.mark.parametrize("p,cf", [(2, 4), (3, 3), (3, 6)])
+def test_coarse_unet_apply_is_prolongated_network(fg, ctx, p, cf):
     """With the coarse placement the preconditioner correction is
     |Rr|/sqrt(Vc) P N(Rr sqrt(Vc)/|Rr|) with R = P^T (trilinear), zero on the
     essential boundary: checked against numpy restriction / prolongation and
-    the network's own forward pass."""
-    op, b, net = _coarse_case(fg, ctx)
-    assert net.shape == (5, 5, 5) and net.n_dofs == op.n_dofs()
-    shape_f = (17, 17, 17)
+    the network's own forward pass, for coarsening factors that are and are
+    not powers of two (the transfer kernels divide by cf in float)."""
+    op, b, net = _coarse_case(fg, ctx, cf=cf, p=p)
+    nd = 8 * p + 1
+    nc = (nd - 1) // cf + 1
+    assert net.shape == (nc, nc, nc) and net.n_dofs == op.n_dofs()
+    shape_f = (nd, nd, nd)
     r = np.random.default_rng(3).standard_normal(op.n_dofs())
-    rc = _R(shape_f, 4, r)
+    rc = _R(shape_f, cf, r)
     sc = np.sqrt(rc.size) / np.linalg.norm(rc)
     out = net.forward((rc * sc).astype(np.float32)).astype(np.float64)
-    want = _P(shape_f, 4, out) / sc
-    bnd = np.zeros((17, 17, 17), bool)
+    want = _P(shape_f, cf, out) / sc
+    bnd = np.zeros((nd, nd, nd), bool)
     bnd[0] = bnd[-1] = bnd[:, 0] = bnd[:, -1] = bnd[:, :, 0] = bnd[:, :, -1] = True
     want[bnd.ravel()] = 0.0
     z = net.apply(r)
