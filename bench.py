@@ -977,6 +977,16 @@ def main():
                                "algorithmic_bytes": bg,
                                "frac": round(bg / (msg_apply * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         Kg.close()
+    # the FP32 copy of the operator (the mixed-precision legs' inner apply)
+    K32 = K.convert(fg.F32)
+    xs = fg.DeviceArray(ctx, N, np.float32).upload(np.random.default_rng(0).standard_normal(N).astype(np.float32))
+    ys = fg.DeviceArray(ctx, N, np.float32)
+    K32.time_apply(xs, ys, 5)
+    ms32_apply, ms32_kern = K32.time_apply(xs, ys, a.apply_reps)
+    pa["fp32"] = {"operator_ms": ms32_apply, "operator_kernel_ms": ms32_kern,
+                  "operator_gdofs": round(N / (ms32_apply * 1e-3) / 1e9, 3)}
+    del xs, ys
+    K32.close()
     if w["mass_apply"]:
         M = fg.HexPAOperator(p, q, n, form=fg.MASS, rho=1.0, dirichlet=False, ctx=ctx)
         msm_apply, msm_kern = M.time_apply(xv, yv, a.apply_reps)

@@ -379,8 +379,16 @@ __device__ __forceinline__ float recip(float d) {
 // Cartesian (one value per point and per-axis factors cf), 2 geometry on the
 // fly (qd holds w kappa per point, cf the mesh vertices; J is rebuilt from the
 // element's 8 vertices at every point, and the mass weight is rho w |J|).
+// Waves per SIMD the compiler must allow (Q4 geometry kernels).  FP32 is
+// held to 72 VGPRs, 7 waves instead of 6 at its own 80, without spills: 1.22
+// -> 1.18 ms on 128^3; a cap of 8 waves spills and is slower.  FP64 is held
+// to 128 VGPRs (4 waves): left free, the compiler takes 146 (3 waves, 2.14 ->
+// 2.39 ms); 5 waves would spill 45 registers.
+template <int N1, int Q, class T, int QM> constexpr int pa_min_waves() {
+    return QM == 2 && N1 == 5 && Q == 5 ? (sizeof(T) == 4 ? 7 : 4) : 1;
+}
 template <int N1, int Q, class T, int FORM, int QM>
-__global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
+__global__ __launch_bounds__(ne_of(Q) * Q * Q, (pa_min_waves<N1, Q, T, QM>())) void k_pa_hex(
     const HexGrid g, const T *__restrict__ qd, const T *__restrict__ qm, const T *__restrict__ cf,
     const T *__restrict__ x, T *__restrict__ y, const int dirichlet, const int *skip, const T rho) {
     constexpr bool CART = QM == 1;
@@ -413,7 +421,12 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q) void k_pa_hex(
     const T *E0 = eo_tab<T>(C, 0), *E1 = eo_tab<T>(C, 1), *E2 = eo_tab<T>(C, 2), *E3 = eo_tab<T>(C, 3);
     constexpr int SZX = NE * LL.SA + ND * LL.SA, SZY = NE * LL.SB + 2 * ND * LL.SB;
     __shared__ T sBuf[ALIAS ? (SZX > SZY ? SZX : SZY) : SZX + SZY];
-    __shared__ T sF[ALIAS ? NE * N1 * N1 : 1];
+    // X^T's face values: in the part of sBuf beyond the X arrays (unused by
+    // X^T) when it fits
+    constexpr int SZF = NE * N1 * N1;
+    constexpr bool FIN = ALIAS && SZY >= SZX + SZF;
+    __shared__ T sF_[ALIAS && !FIN ? SZF : 1];
+    T *const sF = FIN ? sBuf + SZX : sF_;
     T *const sA = sBuf;                    // X out (B path), later F_B
     T *const sAg = sBuf + NE * LL.SA;      // X out (G path), later F_G
     T *const sBB = sBuf + (ALIAS ? 0 : SZX);  // Y out BB, later C_zeta+mass
