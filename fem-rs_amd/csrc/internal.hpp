@@ -184,12 +184,15 @@ struct femgpu_op {
     // partial sums over the fixed red_blocks(n) x 256 grid-stride grid).
     //  apply_cheb_dev: t = A x; d = c1 d + c2 inv (r - t); z += d; with last,
     //                  part[block] = sum r.z  (x may alias z).  With first,
-    //                  z equals d on entry and is not read (x = d): z = d_old + d
+    //                  z equals d on entry and is not read (x = d): z = d_old + d,
+    //                  and d_old = (1/theta) inv r (inv_theta = 1/theta) may be
+    //                  recomputed instead of read
     //  apply_dot_dev:  y = A x; part[block] = sum x.y
     virtual bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d,
-                                double c1, double c2, int last, double *part, const int *skip, int first) {
+                                double c1, double c2, int last, double *part, const int *skip, int first,
+                                double inv_theta) {
         (void)x; (void)r; (void)inv; (void)z; (void)d; (void)c1; (void)c2; (void)last; (void)part; (void)skip;
-        (void)first;
+        (void)first; (void)inv_theta;
         return false;
     }
     virtual bool apply_dot_dev(const void *x, void *y, double *part, const int *skip) {

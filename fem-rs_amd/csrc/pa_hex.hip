@@ -1402,14 +1402,15 @@ __global__ __launch_bounds__(256) void k_hex_gather(const HexGrid g, const T *__
 // over the fixed red_blocks(n) x 256 grid-stride grid of segments (and d is
 // not stored: the polynomial ends there).  x may
 // alias z: each thread reads x only at its own segment's DOFs, before it
-// writes them.  FIRST: z equals d on entry (the element kernel applied A to
-// d): z is written, not read, and x is not read either (at the essential DOFs
-// t = d).
+// writes them.  FIRST: z and d equal (1/theta) inv r on entry (the element
+// kernel applied A to d): that value is recomputed from r and inv instead of
+// read, z is written, not read, and x is not read either (at the essential
+// DOFs t = d).
 template <class T, int P, class I, int BS, bool FIRST, bool LAST>
 __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
     const HexGrid g, const T *__restrict__ ye, const T *x, const T *__restrict__ r,
     const T *__restrict__ inv, T *z, T *__restrict__ d, double c1, double c2,
-    double *__restrict__ part, int dirichlet, const int *skip) {
+    double *__restrict__ part, int dirichlet, const int *skip, double inv_theta) {
     if (skip && *skip) return;
     const size_t nseg = (size_t)g.nx * g.Ny * g.Nz;
     double v[1] = {0.0};
@@ -1425,7 +1426,7 @@ __global__ __launch_bounds__(BS) void k_hex_gather_cheb(
         for (int i = 0; i <= P; ++i) {
             if (i >= cnt) break;
             const size_t k = base + i;
-            const T dk = d[k];
+            const T dk = FIRST ? cheb_first<T>(inv_theta, inv[k], r[k]) : d[k];
             const T ti = (FIRST && ((ess >> i) & 1u)) ? dk : t[i];
             const T di = (T)c1 * dk + (T)c2 * (inv[k] * (r[k] - ti));
             if constexpr (!LAST) d[k] = di;  // the last step's d is not read again
@@ -1839,7 +1840,8 @@ struct HexOp final : femgpu_op {
     }
 
     template <class T> void fused_t(const T *x, const T *r, const T *inv, T *z, T *d, double c1, double c2,
-                                    int last, T *y, double *part, const int *skip, int first = 0) {
+                                    int last, T *y, double *part, const int *skip, int first = 0,
+                                    double inv_theta = 0.0) {
         hipStream_t s = ctx->stream;
         const int dir = desc.dirichlet ? 1 : 0;
         launch_pa<T>(desc.p + 1, desc.q, g, desc.form, qd.as<T>(), qm.as<T>(), cf_ptr<T>(), x, evec.as<T>(), dir,
@@ -1870,7 +1872,7 @@ struct HexOp final : femgpu_op {
     } while (0)
 #define GFC(P_, I_, BS_, F_, L_)                                                                            \
     hipLaunchKernelGGL((k_hex_gather_cheb<T, P_, I_, BS_, F_, L_>), grid, block, 0, s, g, ye, x, r, inv, z, d, c1, \
-                       c2, part, dir, skip)
+                       c2, part, dir, skip, inv_theta)
 #define GF3(P_, I_, BS_)                                                                                    \
     do {                                                                                                    \
         const dim3 block(BS_);                                                                              \
@@ -1895,13 +1897,13 @@ struct HexOp final : femgpu_op {
     }
 
     bool apply_cheb_dev(const void *x, const void *r, const void *inv, void *z, void *d, double c1, double c2,
-                        int last, double *part, const int *skip, int first) override {
+                        int last, double *part, const int *skip, int first, double inv_theta) override {
         if (dtype == FEMGPU_F64)
             fused_t<double>((const double *)x, (const double *)r, (const double *)inv, (double *)z, (double *)d,
-                            c1, c2, last, nullptr, part, skip, first);
+                            c1, c2, last, nullptr, part, skip, first, inv_theta);
         else
             fused_t<float>((const float *)x, (const float *)r, (const float *)inv, (float *)z, (float *)d, c1, c2,
-                           last, nullptr, part, skip, first);
+                           last, nullptr, part, skip, first, inv_theta);
         return true;
     }
 

@@ -7,6 +7,14 @@
 
 namespace femgpu {
 
+// The first Chebyshev iterate z0 = d0 = (1/theta) (inv r), as the kernels that
+// store it compute it (two rounded products); kept out of FMA contraction so
+// that a kernel recomputing it gets the stored value bit for bit.
+template <class T> __device__ __forceinline__ T cheb_first(double inv_theta, T inv, T r) {
+#pragma clang fp contract(off)
+    return (T)inv_theta * (inv * r);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -231,16 +231,18 @@ __global__ __launch_bounds__(256) void k_cheb0(size_t n, const T *__restrict__ r
 }
 
 // d = c1 d + c2 inv (r - t); z += d  [if last: partial r.z, and d is not
-// stored]; with first, z equals d on entry and is not read
+// stored]; with first, z and d equal (1/theta) inv r on entry: neither is read,
+// the value is recomputed from r and inv (read here anyway)
 template <class T, bool FIRST, bool LAST>
 __global__ __launch_bounds__(256) void k_chebk(size_t n, const T *__restrict__ r,
                                                const T *__restrict__ t, const T *__restrict__ inv,
                                                T *__restrict__ z, T *__restrict__ d, double c1,
-                                               double c2, double *__restrict__ part, const int *skip) {
+                                               double c2, double *__restrict__ part, const int *skip,
+                                               double inv_theta) {
     if (skip && *skip) return;
     double v[1] = {0.0};
     for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
-        const T dk = d[i];
+        const T dk = FIRST ? cheb_first<T>(inv_theta, inv[i], r[i]) : d[i];
         const T di = (T)c1 * dk + (T)c2 * (inv[i] * (r[i] - t[i]));
         if constexpr (!LAST) d[i] = di;  // the last step's d is not read again
         const T zi = (FIRST ? dk : z[i]) + di;
@@ -818,12 +820,12 @@ template <class T> struct Krylov {
             const int first = k == 1 ? 1 : 0;
             const T *xin = first ? d : z;
             if (!op->apply_cheb_dev(xin, r, inv, z, d, rho * rho_prev, 2.0 * rho / delta, last, pz, flags,
-                                    first)) {
+                                    first, 1.0 / theta)) {
                 apply(xin, t, true);
                 auto kk = first ? (last ? k_chebk<T, true, true> : k_chebk<T, true, false>)
                                 : (last ? k_chebk<T, false, true> : k_chebk<T, false, false>);
                 hipLaunchKernelGGL(kk, dim3(nb), dim3(256), 0, s, n, r, (const T *)t, inv, z, d,
-                                   rho * rho_prev, 2.0 * rho / delta, pz, flags);
+                                   rho * rho_prev, 2.0 * rho / delta, pz, flags, 1.0 / theta);
             }
             rho_prev = rho;
         }
