@@ -29,7 +29,7 @@ namespace femgpu {
 
 enum Scal {
     S_RZ = 0, S_RZNEW, S_PAP, S_RR, S_ALPHA, S_BETA, S_BNORM, S_TOL, S_RES, S_ITERS, S_STATUS,
-    S_BB, S_RR0, S_COUNT
+    S_BB, S_RR0, S_XPEND, S_COUNT
 };
 enum Status { ST_RUNNING = 0, ST_CONVERGED = 1, ST_BREAKDOWN = 2 };
 enum Mode { MODE_CG = 0, MODE_PCG = 1 };
@@ -141,6 +141,7 @@ __global__ __launch_bounds__(1024) void k_fin_init(const double *part, int nb, d
     S[S_RR] = o[2];
     S[S_ITERS] = 0;
     S[S_STATUS] = ST_RUNNING;
+    S[S_XPEND] = 0;
     flags[0] = 0;
     if (mode == MODE_CG) {
         // tol = max(atol, rtol * max(|b|, 1e-32)); return if |r0| <= tol  (lib.rs:229-240)
@@ -327,6 +328,7 @@ __global__ void k_set_cont(double *S, int *flags, double bnorm) {
     S[S_BNORM] = bnorm;
     S[S_STATUS] = ST_RUNNING;
     S[S_ITERS] = 0;
+    S[S_XPEND] = 0;
     flags[0] = 0;
 }
 
@@ -558,13 +560,18 @@ __global__ __launch_bounds__(256) void k_pcgj_update(size_t n, T *__restrict__ x
 // PCG: k_update_cheb0 takes alpha from the p.Ap partials (slot 0) and the
 // previous r.z partials, updates x and r (partial r.r into slot 1) and starts
 // the Chebyshev polynomial (z = d = inv r / theta); k_pupdate_beta takes beta
-// from the r.z partials of this and the previous iteration and forms p.  The
+// from the r.z partials of this and the previous iteration and forms p.  With
+// DEFER the update of x moves from k_update_cheb0 to k_pupdate_beta, which
+// reads the old p anyway (one vector read fewer per iteration; the same
+// x += alpha p_old, bit for bit): S_XPEND marks an x update that is pending,
+// and k_x_fixup applies it after the loop when the iteration that set it was
+// the last (a stop skips k_pupdate_beta).  The
 // r.z partials alternate between slots kFcRz[0] / kFcRz[1] by iteration.  The
 // one-block k_fin_conv between them keeps the stopping test ahead of the
 // Chebyshev applies.  Iterates are bitwise those of the separate kernels.
 constexpr int kFcRz[2] = {2, 3};
 
-template <class T>
+template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__restrict__ x,
                                                       const T *__restrict__ p, T *__restrict__ r,
                                                       const T *__restrict__ ap, const T *__restrict__ inv,
@@ -577,8 +584,10 @@ __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__
     const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
     T x0 = 0, p0 = 0, r0 = 0, a0 = 0, q0 = 0;
     if (i0 < n) {
-        x0 = x[i0];
-        p0 = p[i0];
+        if constexpr (!DEFER) {
+            x0 = x[i0];
+            p0 = p[i0];
+        }
         r0 = r[i0];
         a0 = ap[i0];
         q0 = inv[i0];
@@ -594,16 +603,20 @@ __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__
             flags[0] = 1;
         } else {
             S[S_ALPHA] = rz / pap;
+            if (DEFER) S[S_XPEND] = 1;
         }
     }
     if (brkdn) return;
     const T alpha = (T)(rz / pap);
     double v[2] = {0, 0};
     for (size_t i = i0; i < n; i += stride) {
-        T xi, pi, ri, ai, qi;
-        if (i == i0) { xi = x0; pi = p0; ri = r0; ai = a0; qi = q0; }
-        else { xi = x[i]; pi = p[i]; ri = r[i]; ai = ap[i]; qi = inv[i]; }
-        x[i] = fmad<T>(alpha, pi, xi);
+        T ri, ai, qi;
+        if (i == i0) { ri = r0; ai = a0; qi = q0; }
+        else { ri = r[i]; ai = ap[i]; qi = inv[i]; }
+        if constexpr (!DEFER) {
+            const T xi = i == i0 ? x0 : x[i], pi = i == i0 ? p0 : p[i];
+            x[i] = fmad<T>(alpha, pi, xi);
+        }
         ri = fmad<T>(-alpha, ai, ri);
         r[i] = ri;
         v[0] += (double)ri * (double)ri;
@@ -619,31 +632,48 @@ __global__ __launch_bounds__(256) void k_update_cheb0(size_t n, double *S, T *__
     }
 }
 
-template <class T>
+template <class T, bool DEFER>
 __global__ __launch_bounds__(256) void k_pupdate_beta(size_t n, double *S, const T *__restrict__ z,
                                                       T *__restrict__ p, const double *__restrict__ part,
-                                                      int nb, int *flags, int rz_new, int rz_old) {
+                                                      int nb, int *flags, int rz_new, int rz_old,
+                                                      T *__restrict__ x) {
     __shared__ double lds[2][16];
     __shared__ int stop;
     const size_t i0 = blockIdx.x * (size_t)256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
-    T z0 = 0, p0 = 0;
+    T z0 = 0, p0 = 0, x0 = 0;
     if (i0 < n) {
         z0 = z[i0];
         p0 = p[i0];
+        if constexpr (DEFER) x0 = x[i0];
     }
     double o[2];
     if (rsum256<2>(part, nb, {rz_new, rz_old}, o, lds, flags, &stop)) return;
     const double b = o[0] / o[1];
+    // DEFER: alpha as k_update_cheb0 rounded it, (T)(rz / pAp)
+    const T alpha = DEFER ? (T)S[S_ALPHA] : T(0);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S[S_BETA] = b;
         S[S_RZ] = o[0];
+        if (DEFER) S[S_XPEND] = 0;  // read only by k_x_fixup, after the loop
     }
     const T beta = (T)b;
     for (size_t i = i0; i < n; i += stride) {
         const T zi = i == i0 ? z0 : z[i];
         const T pi = i == i0 ? p0 : p[i];
+        if constexpr (DEFER) x[i] = fmad<T>(alpha, pi, i == i0 ? x0 : x[i]);
         p[i] = fmad<T>(beta, pi, zi);
     }
+}
+
+// The x update of the last iteration, when k_pupdate_beta did not run (see
+// k_update_cheb0); not skipped by the stop flag.
+template <class T>
+__global__ __launch_bounds__(256) void k_x_fixup(size_t n, const double *S, T *__restrict__ x,
+                                                 const T *__restrict__ p) {
+    if (S[S_XPEND] == 0.0) return;
+    const T alpha = (T)S[S_ALPHA];
+    for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        x[i] = fmad<T>(alpha, p[i], x[i]);
 }
 
 // After the last enqueued iteration: records it and applies the stopping test.
@@ -1052,7 +1082,7 @@ template <class T> struct Krylov {
                 hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, s, n, (const T *)p, (const T *)ap,
                                    part, 0, flags);
             }
-            hipLaunchKernelGGL(k_update_cheb0<T>, dim3(nb), dim3(256), 0, s, n, S, x, (const T *)p, r,
+            hipLaunchKernelGGL((k_update_cheb0<T, true>), dim3(nb), dim3(256), 0, s, n, S, x, (const T *)p, r,
                                (const T *)ap, inv, z, d, part, (int)nb, flags, rz_prev, inv_theta,
                                cp.degree == 1 ? rz_new : -1, brk,
                                (cp.degree == 1 || (segment && it + 1 == max_iter)) ? 1 : 0);
@@ -1064,12 +1094,13 @@ template <class T> struct Krylov {
                 break;
             }
             cheb_apply(cp, r, inv, z, rz_new, true);
-            hipLaunchKernelGGL(k_pupdate_beta<T>, dim3(nb), dim3(256), 0, s, n, S, (const T *)z, p,
-                               (const double *)part, (int)nb, flags, rz_new, rz_prev);
+            hipLaunchKernelGGL((k_pupdate_beta<T, true>), dim3(nb), dim3(256), 0, s, n, S, (const T *)z, p,
+                               (const double *)part, (int)nb, flags, rz_new, rz_prev, x);
             FG_HIP(hipGetLastError());
             ++it;
             if (it % check == 0 || it == max_iter) stopped = read_flag() != 0;
         }
+        hipLaunchKernelGGL(k_x_fixup<T>, dim3(nb), dim3(256), 0, s, n, (const double *)S, x, (const T *)p);
         return finish(MODE_PCG, max_iter, res);
     }
 
