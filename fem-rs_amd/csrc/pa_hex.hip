@@ -441,9 +441,13 @@ __global__ __launch_bounds__(ne_of(Q) * Q * Q, (pa_min_waves<N1, Q, T, QM>())) v
     };
 
     if (skip && *skip) return;
-    // Hardware order (consecutive workgroups on different XCDs): measured
-    // faster here than the XCD-contiguous order, which helps only the gather.
-    const unsigned lb = blockIdx.x;
+    // Element group of this workgroup.  Geometry on the fly: XCD-contiguous
+    // order (workgroup b runs on XCD b mod 8, and XCD k takes the groups
+    // [k C, (k + 1) C) in order, C = gridDim.x / 8, the grid a multiple of 8):
+    // neighbouring groups share x in one XCD's L2.  Q4 128^3, bitwise the
+    // same output: FP64 2.12 -> 2.10-2.12 ms, FP32 1.18 -> 1.13 ms.  With
+    // stored qdata the hardware order was faster (Q2 64^3: 76 against 81.5 us).
+    const unsigned lb = GEO ? (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3) : blockIdx.x;
     const T *Bt = Tab<T>::B(C);
     const T *Gt = Tab<T>::G(C);
     const int le = threadIdx.x / T2;
@@ -1727,7 +1731,10 @@ template <int N1, int Q, class T>
 static void launch_form(const HexGrid &g, int form, int qm_mode, const T *qd, const T *qm, const T *cf,
                         const T *x, T *y, int dir, const int *skip, T rho, hipStream_t s) {
     constexpr int NE = ne_of(Q);
-    const dim3 grid((unsigned)((g.E + NE - 1) / NE)), block(NE * Q * Q);
+    // geometry on the fly: a multiple of 8 workgroups (the XCD-contiguous
+    // order); the extra ones hold no element
+    const long long ng = (g.E + NE - 1) / NE;
+    const dim3 grid((unsigned)(qm_mode == 2 ? (ng + 7) / 8 * 8 : ng)), block(NE * Q * Q);
 #define L_(F_, M_) hipLaunchKernelGGL((k_pa_hex<N1, Q, T, F_, M_>), grid, block, 0, s, g, qd, qm, cf, x, y, dir, skip, rho)
     if (form == 1) L_(1, 0);
     else if (form == 2 && qm_mode == 1) L_(2, 1);
